@@ -1,0 +1,12 @@
+"""MI355X-native NLOS time-of-flight backprojection (drop-in for mjgalindo/backprojector's
+GPU path).  Compute lives in libbpx.so (hand-written gfx950 HIP kernels behind the C ABI of
+include/bpx.h); this package is its Python front.
+"""
+from . import nlosbpy
+from ._native import BpxError, InvalidArgument, device_count
+from .core import (Plan, as_pairs, backprojection, phasor_pulse, render_transient,
+                   window_backprojection, PAIR_DTYPE)
+
+__all__ = ["nlosbpy", "Plan", "backprojection", "window_backprojection", "phasor_pulse",
+           "render_transient", "as_pairs", "PAIR_DTYPE", "BpxError", "InvalidArgument",
+           "device_count"]

@@ -1,0 +1,130 @@
+"""ctypes binding of libbpx.so (the C ABI in include/bpx.h).
+
+The library is loaded from this package directory (built in-tree by
+backprojector_amd.build).  There is no fallback: if the library is missing or no gfx950
+device is visible, calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbpx.so")
+
+PAIR_DTYPE = np.dtype(
+    [("camera_point", np.float32, (3,)), ("laser_point", np.float32, (3,)),
+     ("camera_wall", np.float32), ("laser_wall", np.float32)]
+)
+assert PAIR_DTYPE.itemsize == 32
+
+STATUS = {0: "BPX_OK", 1: "BPX_ERR_INVALID_ARGUMENT", 2: "BPX_ERR_OUT_OF_MEMORY",
+          3: "BPX_ERR_DEVICE", 4: "BPX_ERR_UNSUPPORTED", 5: "BPX_ERR_NO_DEVICE"}
+
+STORAGE = {"f32": 0, "fp32": 0, "float32": 0, "f16": 1, "fp16": 1, "float16": 1,
+           "bf16": 2, "bfloat16": 2}
+
+
+class BpxError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{STATUS.get(status, status)}: {message}")
+        self.status = status
+
+
+class InvalidArgument(BpxError, ValueError):
+    pass
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("storage", ctypes.c_uint32), ("kahan_block", ctypes.c_uint32),
+                ("flags", ctypes.c_uint32), ("variant", ctypes.c_int32)]
+
+
+class Timing(ctypes.Structure):
+    _fields_ = [("total_ms", ctypes.c_double), ("per_exec_ms", ctypes.c_double),
+                ("table_kernel_ms", ctypes.c_double), ("gather_kernel_ms", ctypes.c_double),
+                ("iterations", ctypes.c_int32)]
+
+
+FLAG_REFERENCE_PAIR_TRUNCATION = 0x1
+
+_lib = None
+
+# Every symbol include/bpx.h declares (checked by the CPU test suite).
+EXPORTED = [
+    "bpx_last_error", "bpx_version", "bpx_abi_version", "bpx_device_count",
+    "bpx_backprojection", "bpx_complex_backprojection", "bpx_backprojection_ex",
+    "bpx_window_backprojection", "bpx_phasor_pulse", "bpx_nlos_backproject", "bpx_nlos_phasor",
+    "bpx_plan_create", "bpx_nlos_plan_create", "bpx_plan_execute", "bpx_plan_synchronize", "bpx_plan_read_volume",
+    "bpx_plan_info", "bpx_plan_benchmark", "bpx_plan_destroy", "bpx_render_transient",
+]
+
+
+def lib():
+    """Loads libbpx.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: run backprojector_amd.build.build() first")
+    L = ctypes.CDLL(LIB_PATH)
+    c_p = ctypes.c_void_p
+    u32, u64, i32, f32 = ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int32, ctypes.c_float
+    L.bpx_last_error.restype = ctypes.c_char_p
+    L.bpx_version.restype = ctypes.c_char_p
+    L.bpx_abi_version.restype = ctypes.c_int
+    L.bpx_device_count.restype = ctypes.c_int
+    for name in ("bpx_backprojection", "bpx_complex_backprojection"):
+        getattr(L, name).argtypes = [c_p, u64, u32, c_p, u32, c_p, c_p, c_p, c_p, f32, f32]
+    L.bpx_backprojection_ex.argtypes = [c_p, u64, u32, c_p, u32, c_p, c_p, c_p, c_p, f32, f32,
+                                        ctypes.c_int, c_p]
+    L.bpx_window_backprojection.argtypes = [c_p, u64, u32, c_p, u32, c_p, c_p, c_p, c_p, f32,
+                                            f32, ctypes.c_int]
+    L.bpx_phasor_pulse.argtypes = [c_p, u64, u32, f32, f32, u32, f32, c_p]
+    L.bpx_nlos_backproject.argtypes = [c_p, c_p, ctypes.c_int, c_p, c_p, c_p, c_p, c_p, c_p, f32,
+                                       f32, ctypes.c_int, c_p, c_p, c_p, c_p, ctypes.c_int,
+                                       ctypes.c_int, c_p]
+    L.bpx_nlos_phasor.argtypes = [c_p, c_p, ctypes.c_int, c_p, c_p, c_p, c_p, c_p, c_p, f32, f32,
+                                  ctypes.c_int, c_p, c_p, c_p, f32, ctypes.c_int, ctypes.c_int,
+                                  c_p]
+    L.bpx_plan_create.argtypes = [c_p, u64, u32, c_p, u32, c_p, c_p, c_p, f32, f32, ctypes.c_int,
+                                  c_p, ctypes.POINTER(c_p)]
+    L.bpx_nlos_plan_create.argtypes = [c_p, c_p, ctypes.c_int, c_p, c_p, c_p, c_p, c_p, c_p, f32,
+                                       f32, ctypes.c_int, c_p, c_p, c_p, c_p, ctypes.c_int, c_p,
+                                       ctypes.POINTER(c_p)]
+    L.bpx_plan_execute.argtypes = [c_p, c_p]
+    L.bpx_plan_synchronize.argtypes = [c_p]
+    L.bpx_plan_read_volume.argtypes = [c_p, c_p]
+    L.bpx_plan_info.argtypes = [c_p, ctypes.POINTER(i32), ctypes.POINTER(i32),
+                                ctypes.POINTER(u32), ctypes.POINTER(u32), ctypes.POINTER(f32)]
+    L.bpx_plan_benchmark.argtypes = [c_p, i32, i32, ctypes.POINTER(Timing)]
+    L.bpx_plan_destroy.argtypes = [c_p]
+    L.bpx_plan_destroy.restype = None
+    L.bpx_render_transient.argtypes = [c_p, u32, u32, f32, f32, c_p, u64, f32, ctypes.c_int,
+                                       u64, c_p]
+    _lib = L
+    return L
+
+
+def check(status: int) -> None:
+    if status != 0:
+        msg = lib().bpx_last_error().decode(errors="replace")
+        if status == 1:
+            raise InvalidArgument(status, msg)
+        raise BpxError(status, msg)
+
+
+def ptr(a: np.ndarray):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def device_count() -> int:
+    return int(lib().bpx_device_count())
+
+
+def make_options(storage="f32", kahan_block=0, flags=0, variant=-1) -> Options:
+    if isinstance(storage, str):
+        storage = STORAGE[storage.lower()]
+    return Options(int(storage), int(kahan_block), int(flags), int(variant))

@@ -1,0 +1,197 @@
+"""Python front of the C ABI: numpy in, numpy out.  All compute runs in libbpx.so on the GPU.
+
+Function names follow the reference's C++ interface (backproject_cuda.hpp:19-49,
+backproject.hpp:236-247, 430-479); each docstring names the reference it replaces.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _native as nat
+from ._native import PAIR_DTYPE, check, lib, ptr
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _u32(a):
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+def as_pairs(pairs) -> np.ndarray:
+    """Coerces to the CameraLaserPair record layout (backproject_cuda.hpp:9-13)."""
+    pr = np.ascontiguousarray(pairs)
+    if pr.dtype != PAIR_DTYPE:
+        if pr.dtype.names and set(pr.dtype.names) >= set(PAIR_DTYPE.names):
+            out = np.zeros(len(pr), PAIR_DTYPE)
+            for n in PAIR_DTYPE.names:
+                out[n] = pr[n]
+            return out
+        raise TypeError("pairs must be a structured array with the CameraLaserPair fields")
+    return pr
+
+
+def _samples(chunk, complex_samples):
+    if complex_samples:
+        c = np.ascontiguousarray(chunk, dtype=np.complex64)
+        return c, c.size
+    c = _f32(chunk)
+    return c, c.size
+
+
+def backprojection(transient_chunk, pairs, voxels_per_side, volume_zero_pos, voxel_inc, t0,
+                   deltaT, storage="f32", kahan_block=0, reference_pair_truncation=False,
+                   variant=-1):
+    """call_cuda_backprojection / call_cuda_complex_backprojection
+    (backproject_cuda.cu:370-500, 671-801): single-tap GPU-path volume [N0][N1][N2].
+    Complex input selects the complex path (complex64 volume)."""
+    cplx = np.iscomplexobj(transient_chunk)
+    H, n = _samples(transient_chunk, cplx)
+    pr = as_pairs(pairs)
+    P = len(pr)
+    T = (n // P) if P else (H.shape[-1] if H.ndim else 0)
+    N = _u32(voxels_per_side).reshape(3)
+    out = np.zeros(tuple(int(x) for x in N), np.complex64 if cplx else np.float32)
+    flags = nat.FLAG_REFERENCE_PAIR_TRUNCATION if reference_pair_truncation else 0
+    opt = nat.make_options(storage, kahan_block, flags, variant)
+    check(lib().bpx_backprojection_ex(ptr(H), n, T, ptr(pr), P, ptr(out), ptr(N),
+                                      ptr(_f32(volume_zero_pos)), ptr(_f32(voxel_inc).reshape(9)),
+                                      float(t0), float(deltaT), 1 if cplx else 0,
+                                      ctypes.byref(opt)))
+    return out
+
+
+def window_backprojection(transient_chunk, pairs, voxels_per_side, volume_position, volume_size,
+                          t0, deltaT):
+    """classic_backprojection (backproject.hpp:236-247): the CPU path's semantics (grid
+    spacing size/N, voxel-diagonal window, double distances), computed on the GPU."""
+    cplx = np.iscomplexobj(transient_chunk)
+    H, n = _samples(transient_chunk, cplx)
+    pr = as_pairs(pairs)
+    P = len(pr)
+    T = (n // P) if P else 0
+    N = _u32(voxels_per_side).reshape(3)
+    out = np.zeros(tuple(int(x) for x in N), np.complex64 if cplx else np.float32)
+    check(lib().bpx_window_backprojection(ptr(H), n, T, ptr(pr), P, ptr(out), ptr(N),
+                                          ptr(_f32(volume_position)), ptr(_f32(volume_size)),
+                                          float(t0), float(deltaT), 1 if cplx else 0))
+    return out
+
+
+def phasor_pulse(transient_data, wavelength, deltaT, times=4, sigma=0.0):
+    """phasor_pulse (backproject.hpp:430-479): complex FIR along the last axis on the GPU.
+    sigma > 0 selects the explicit-sigma extension."""
+    x = _f32(transient_data)
+    T = x.shape[-1]
+    rows = x.size // T if T else 0
+    out = np.zeros(x.shape, np.complex64)
+    check(lib().bpx_phasor_pulse(ptr(x), rows, T, float(wavelength), float(deltaT), int(times),
+                                 float(sigma), ptr(out)))
+    return out
+
+
+def render_transient(pairs, T, t0, deltaT, samples, peak_count=1000.0, poisson=True, seed=0):
+    """Synthetic capture [P][T] of hidden samples [n][7] = (x, y, z, nx, ny, nz, area)."""
+    pr = as_pairs(pairs)
+    s = _f32(samples).reshape(-1, 7)
+    out = np.zeros((len(pr), int(T)), np.float32)
+    check(lib().bpx_render_transient(ptr(pr), len(pr), int(T), float(t0), float(deltaT), ptr(s),
+                                     len(s), float(peak_count), 1 if poisson else 0,
+                                     int(seed) & ((1 << 64) - 1), ptr(out)))
+    return out
+
+
+class Plan:
+    """Device-resident capture + volume; execute() runs the whole reconstruction on the GPU."""
+
+    def __init__(self, transient_chunk, pairs, voxels_per_side, volume_zero_pos, voxel_inc, t0,
+                 deltaT, storage="f32", kahan_block=0, reference_pair_truncation=False,
+                 variant=-1):
+        cplx = np.iscomplexobj(transient_chunk)
+        H, n = _samples(transient_chunk, cplx)
+        pr = as_pairs(pairs)
+        P = len(pr)
+        T = (n // P) if P else 0
+        self.N = tuple(int(x) for x in _u32(voxels_per_side).reshape(3))
+        self.complex = cplx
+        flags = nat.FLAG_REFERENCE_PAIR_TRUNCATION if reference_pair_truncation else 0
+        opt = nat.make_options(storage, kahan_block, flags, variant)
+        h = ctypes.c_void_p()
+        check(lib().bpx_plan_create(ptr(H), n, T, ptr(pr), P, ptr(_u32(self.N)),
+                                    ptr(_f32(volume_zero_pos)), ptr(_f32(voxel_inc).reshape(9)),
+                                    float(t0), float(deltaT), 1 if cplx else 0, ctypes.byref(opt),
+                                    ctypes.byref(h)))
+        self._h = h
+
+    @classmethod
+    def from_nlos(cls, transient_data, camera_grid_positions, laser_grid_positions,
+                  camera_position, laser_position, t0, deltaT, is_confocal, volume_position,
+                  volume_size, voxels_per_side, wall_normal=(0, 0, 0), assume_row_major=True,
+                  storage="f32", kahan_block=0, variant=-1):
+        """Plan of nlosbpy.backproject's GPU path (bp::backproject, backproject.hpp:762-891)."""
+        H = _f32(transient_data)
+        cg, lg = _f32(camera_grid_positions), _f32(laser_grid_positions)
+        N = _u32(voxels_per_side).reshape(3)
+        sh = lambda a: np.ascontiguousarray(np.asarray(a.shape, np.int64))
+        ds, cs, ls = sh(H), sh(cg), sh(lg)
+        opt = nat.make_options(storage, kahan_block, 0, variant)
+        self = cls.__new__(cls)
+        self.N = tuple(int(x) for x in N)
+        self.complex = False
+        h = ctypes.c_void_p()
+        check(lib().bpx_nlos_plan_create(ptr(H), ptr(ds), H.ndim, ptr(cg), ptr(cs), ptr(lg), ptr(ls),
+                                         ptr(_f32(camera_position)), ptr(_f32(laser_position)),
+                                         float(t0), float(deltaT), 1 if is_confocal else 0,
+                                         ptr(_f32(volume_position)), ptr(_f32(volume_size)),
+                                         ptr(N), ptr(_f32(wall_normal)),
+                                         1 if assume_row_major else 0, ctypes.byref(opt),
+                                         ctypes.byref(h)))
+        self._h = h
+        return self
+
+    def execute(self, stream=None):
+        check(lib().bpx_plan_execute(self._h, stream))
+
+    def synchronize(self):
+        check(lib().bpx_plan_synchronize(self._h))
+
+    def volume(self):
+        out = np.zeros(self.N, np.complex64 if self.complex else np.float32)
+        check(lib().bpx_plan_read_volume(self._h, ptr(out)))
+        return out
+
+    def info(self):
+        path, k = ctypes.c_int32(), ctypes.c_int32()
+        used, tc, sc = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_float()
+        check(lib().bpx_plan_info(self._h, ctypes.byref(path), ctypes.byref(k), ctypes.byref(used),
+                                  ctypes.byref(tc), ctypes.byref(sc)))
+        names = {0: "confocal", 1: "grid", 2: "pairs", 3: "window"}
+        return dict(path=names.get(path.value, path.value), lane_voxels=k.value,
+                    pairs_used=used.value, Tc=tc.value, scale=sc.value)
+
+    def benchmark(self, warmup=2, iterations=10):
+        t = nat.Timing()
+        check(lib().bpx_plan_benchmark(self._h, int(warmup), int(iterations), ctypes.byref(t)))
+        return dict(total_ms=t.total_ms, per_exec_ms=t.per_exec_ms,
+                    table_kernel_ms=t.table_kernel_ms, gather_kernel_ms=t.gather_kernel_ms,
+                    iterations=t.iterations)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().bpx_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

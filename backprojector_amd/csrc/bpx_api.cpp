@@ -1,0 +1,262 @@
+// extern "C" entry points of libbpx (declared in include/bpx.h).  Every entry point catches
+// all failures and converts them to a bpx_status plus a thread-local message.
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+
+#include "bpx_host.hpp"
+
+namespace bpx {
+
+static thread_local std::string g_error;
+
+bpx_status set_error(bpx_status s, const std::string& msg) {
+    g_error = msg;
+    return s;
+}
+void clear_error() { g_error.clear(); }
+void fail(bpx_status s, const std::string& msg) { throw Failure{s, msg}; }
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess)
+        fail(BPX_ERR_DEVICE, std::string(what) + ": " + hipGetErrorName(e) + " (" +
+                                 hipGetErrorString(e) + ")");
+}
+
+static int g_device = -2;  // -2 unknown, -1 none
+static std::mutex g_dev_mu;
+
+static int find_device() {
+    std::lock_guard<std::mutex> lk(g_dev_mu);
+    if (g_device != -2) return g_device;
+    g_device = -1;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) {
+        (void)hipGetLastError();
+        return g_device;
+    }
+    for (int d = 0; d < n; ++d) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, d) != hipSuccess) continue;
+        if (std::strncmp(prop.gcnArchName, "gfx950", 6) == 0) {
+            g_device = d;
+            break;
+        }
+    }
+    return g_device;
+}
+
+bool device_ready() {
+    const int d = find_device();
+    if (d < 0) return false;
+    return hipSetDevice(d) == hipSuccess;
+}
+
+template <class F>
+static bpx_status guarded(F&& f) {
+    clear_error();
+    try {
+        f();
+        return BPX_OK;
+    } catch (const Failure& e) {
+        return set_error(e.status, e.msg);
+    } catch (const std::bad_alloc&) {
+        return set_error(BPX_ERR_OUT_OF_MEMORY, "host allocation failed");
+    } catch (const std::exception& e) {
+        return set_error(BPX_ERR_INVALID_ARGUMENT, e.what());
+    }
+}
+
+static void run_plan_to_host(PlanImpl* raw, float* out) {
+    std::unique_ptr<PlanImpl> pl(raw);
+    pl->execute(pl->stream);
+    const size_t nvox = (size_t)pl->g.n[0] * pl->g.n[1] * pl->g.n[2];
+    hip_check(hipMemcpyAsync(out, pl->vol.ptr, nvox * (pl->cplx ? 8 : 4), hipMemcpyDeviceToHost,
+                             pl->stream),
+              "hipMemcpyAsync(volume)");
+    hip_check(hipStreamSynchronize(pl->stream), "hipStreamSynchronize");
+}
+
+}  // namespace bpx
+
+using namespace bpx;
+
+extern "C" {
+
+const char* bpx_last_error(void) { return g_error.c_str(); }
+const char* bpx_version(void) { return "bpx 0.1.0 (gfx950)"; }
+int bpx_abi_version(void) { return BPX_ABI_VERSION; }
+int bpx_device_count(void) { return find_device() >= 0 ? 1 : 0; }
+
+bpx_status bpx_backprojection_ex(const float* transient_chunk, uint64_t transient_size, uint32_t T,
+                                 const bpx_pair* scanned_pairs, uint32_t num_pairs,
+                                 float* voxel_volume, const uint32_t voxels_per_side[3],
+                                 const float volume_zero_pos[3], const float voxel_inc[9],
+                                 float t0, float deltaT, int complex_samples,
+                                 const bpx_options* options) {
+    return guarded([&] {
+        if (!voxel_volume || !voxels_per_side || !volume_zero_pos || !voxel_inc)
+            fail(BPX_ERR_INVALID_ARGUMENT, "NULL volume/grid argument");
+        CaptureIn c{transient_chunk, transient_size, T, scanned_pairs, num_pairs, t0, deltaT,
+                    complex_samples != 0};
+        run_plan_to_host(make_plan_single_tap(c, voxels_per_side, volume_zero_pos, voxel_inc, options),
+                         voxel_volume);
+    });
+}
+
+bpx_status bpx_backprojection(const float* transient_chunk, uint64_t transient_size, uint32_t T,
+                              const bpx_pair* scanned_pairs, uint32_t num_pairs,
+                              float* voxel_volume, const uint32_t voxels_per_side[3],
+                              const float volume_zero_pos[3], const float voxel_inc[9], float t0,
+                              float deltaT) {
+    return bpx_backprojection_ex(transient_chunk, transient_size, T, scanned_pairs, num_pairs,
+                                 voxel_volume, voxels_per_side, volume_zero_pos, voxel_inc, t0,
+                                 deltaT, 0, nullptr);
+}
+
+bpx_status bpx_complex_backprojection(const float* transient_chunk, uint64_t transient_size,
+                                      uint32_t T, const bpx_pair* scanned_pairs,
+                                      uint32_t num_pairs, float* voxel_volume,
+                                      const uint32_t voxels_per_side[3],
+                                      const float volume_zero_pos[3], const float voxel_inc[9],
+                                      float t0, float deltaT) {
+    return bpx_backprojection_ex(transient_chunk, transient_size, T, scanned_pairs, num_pairs,
+                                 voxel_volume, voxels_per_side, volume_zero_pos, voxel_inc, t0,
+                                 deltaT, 1, nullptr);
+}
+
+bpx_status bpx_window_backprojection(const float* transient_chunk, uint64_t transient_size,
+                                     uint32_t T, const bpx_pair* scanned_pairs,
+                                     uint32_t num_pairs, float* voxel_volume,
+                                     const uint32_t voxels_per_side[3],
+                                     const float volume_position[3], const float volume_size[3],
+                                     float t0, float deltaT, int complex_samples) {
+    return guarded([&] {
+        if (!voxel_volume || !voxels_per_side || !volume_position || !volume_size)
+            fail(BPX_ERR_INVALID_ARGUMENT, "NULL volume/grid argument");
+        CaptureIn c{transient_chunk, transient_size, T, scanned_pairs, num_pairs, t0, deltaT,
+                    complex_samples != 0};
+        run_plan_to_host(make_plan_window(c, voxels_per_side, volume_position, volume_size),
+                         voxel_volume);
+    });
+}
+
+bpx_status bpx_phasor_pulse(const float* transient_in, uint64_t rows, uint32_t T, float wavelength,
+                            float deltaT, uint32_t times, float sigma, float* complex_out) {
+    return guarded([&] {
+        if ((!transient_in || !complex_out) && rows) fail(BPX_ERR_INVALID_ARGUMENT, "NULL buffer");
+        if (!(deltaT > 0.0f) || !std::isfinite(deltaT)) fail(BPX_ERR_INVALID_ARGUMENT, "deltaT must be > 0");
+        if (!(wavelength > 0.0f) || !std::isfinite(wavelength))
+            fail(BPX_ERR_INVALID_ARGUMENT, "wavelength must be > 0");
+        std::vector<float> c, s;
+        if (pipeline::phasor_taps(wavelength, deltaT, times, sigma, c, s) <= 0)
+            fail(BPX_ERR_INVALID_ARGUMENT, "the phasor pulse has no taps for these parameters");
+        if (!device_ready()) fail(BPX_ERR_NO_DEVICE, "no gfx950 device available");
+        phasor_filter_host(transient_in, rows, T, c, s, complex_out);
+    });
+}
+
+bpx_status bpx_plan_create(const float* transient_chunk, uint64_t transient_size, uint32_t T,
+                           const bpx_pair* scanned_pairs, uint32_t num_pairs,
+                           const uint32_t voxels_per_side[3], const float volume_zero_pos[3],
+                           const float voxel_inc[9], float t0, float deltaT, int complex_samples,
+                           const bpx_options* options, bpx_plan* plan_out) {
+    return guarded([&] {
+        if (!plan_out || !voxels_per_side || !volume_zero_pos || !voxel_inc)
+            fail(BPX_ERR_INVALID_ARGUMENT, "NULL argument");
+        *plan_out = nullptr;
+        CaptureIn c{transient_chunk, transient_size, T, scanned_pairs, num_pairs, t0, deltaT,
+                    complex_samples != 0};
+        auto p = std::make_unique<bpx_plan_s>();
+        p->impl.reset(make_plan_single_tap(c, voxels_per_side, volume_zero_pos, voxel_inc, options));
+        *plan_out = p.release();
+    });
+}
+
+bpx_status bpx_plan_execute(bpx_plan plan, void* hip_stream) {
+    return guarded([&] {
+        if (!plan) fail(BPX_ERR_INVALID_ARGUMENT, "NULL plan");
+        plan->impl->execute(hip_stream ? (hipStream_t)hip_stream : plan->impl->stream);
+    });
+}
+
+bpx_status bpx_plan_synchronize(bpx_plan plan) {
+    return guarded([&] {
+        if (!plan) fail(BPX_ERR_INVALID_ARGUMENT, "NULL plan");
+        hip_check(hipStreamSynchronize(plan->impl->stream), "hipStreamSynchronize");
+    });
+}
+
+bpx_status bpx_plan_read_volume(bpx_plan plan, float* voxel_volume) {
+    return guarded([&] {
+        if (!plan || !voxel_volume) fail(BPX_ERR_INVALID_ARGUMENT, "NULL argument");
+        PlanImpl& pl = *plan->impl;
+        const size_t nvox = (size_t)pl.g.n[0] * pl.g.n[1] * pl.g.n[2];
+        hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        hip_check(hipMemcpy(voxel_volume, pl.vol.ptr, nvox * (pl.cplx ? 8 : 4), hipMemcpyDeviceToHost),
+                  "hipMemcpy(volume)");
+    });
+}
+
+bpx_status bpx_plan_info(bpx_plan plan, int32_t* path, int32_t* lane_voxels, uint32_t* pairs_used,
+                         uint32_t* Tc, float* scale) {
+    return guarded([&] {
+        if (!plan) fail(BPX_ERR_INVALID_ARGUMENT, "NULL plan");
+        const PlanImpl& pl = *plan->impl;
+        if (path) *path = pl.path;
+        if (lane_voxels) *lane_voxels = pl.K;
+        if (pairs_used) *pairs_used = pl.P;
+        if (Tc) *Tc = pl.T;
+        if (scale) *scale = pl.scale;
+    });
+}
+
+bpx_status bpx_plan_benchmark(bpx_plan plan, int32_t warmup, int32_t iterations, bpx_timing* out) {
+    return guarded([&] {
+        if (!plan || !out || iterations <= 0 || warmup < 0) fail(BPX_ERR_INVALID_ARGUMENT, "bad arguments");
+        PlanImpl& pl = *plan->impl;
+        for (int i = 0; i < warmup; ++i) pl.execute(pl.stream);
+        hip_check(hipStreamSynchronize(pl.stream), "hipStreamSynchronize");
+        double table_ms = 0, gather_ms = 0;
+        const auto t_start = std::chrono::steady_clock::now();
+        for (int i = 0; i < iterations; ++i) {
+            pl.execute(pl.stream);
+            if (pl.P == 0) continue;
+            hip_check(hipEventSynchronize(pl.ev[3]), "hipEventSynchronize");
+            float a = 0, b = 0;
+            hip_check(hipEventElapsedTime(&a, pl.ev[1], pl.ev[2]), "hipEventElapsedTime");
+            hip_check(hipEventElapsedTime(&b, pl.ev[2], pl.ev[3]), "hipEventElapsedTime");
+            table_ms += a;
+            gather_ms += b;
+        }
+        hip_check(hipStreamSynchronize(pl.stream), "hipStreamSynchronize");
+        const auto t_end = std::chrono::steady_clock::now();
+        out->iterations = iterations;
+        out->total_ms = std::chrono::duration<double, std::milli>(t_end - t_start).count();
+        out->per_exec_ms = out->total_ms / iterations;
+        out->table_kernel_ms = table_ms / iterations;
+        out->gather_kernel_ms = gather_ms / iterations;
+    });
+}
+
+void bpx_plan_destroy(bpx_plan plan) {
+    if (!plan) return;
+    (void)hipStreamSynchronize(plan->impl->stream);
+    delete plan;
+}
+
+bpx_status bpx_render_transient(const bpx_pair* pairs, uint32_t num_pairs, uint32_t T, float t0,
+                                float deltaT, const float* samples, uint64_t num_samples,
+                                float peak_count, int poisson, uint64_t seed, float* out) {
+    return guarded([&] {
+        if ((!pairs && num_pairs) || (!samples && num_samples) || !out)
+            fail(BPX_ERR_INVALID_ARGUMENT, "NULL buffer");
+        if (!(deltaT > 0.0f)) fail(BPX_ERR_INVALID_ARGUMENT, "deltaT must be > 0");
+        render_transient(pairs, num_pairs, T, t0, deltaT, samples, num_samples, peak_count,
+                         poisson != 0, seed, out);
+    });
+}
+
+}  // extern "C"

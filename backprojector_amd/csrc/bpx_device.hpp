@@ -1,0 +1,186 @@
+// Device-side building blocks shared by the gfx950 kernels: exact IEEE arithmetic of the
+// reference path-length formula, monotone-threshold search, buffer-resource sample loads.
+//
+// The translation units that include this header are compiled with -ffp-contract=off: every
+// product and sum of the reference formula rounds separately.  sqrtf and __fdiv_rn lower to
+// correctly rounded instruction sequences (HIP's default
+// -fhip-fp32-correctly-rounded-divide-sqrt); fused multiply-adds appear only where written
+// as __builtin_fmaf, and only in approximations whose error is bounded on the host.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace bpx {
+
+// ------------------------------------------------------------------ exact arithmetic
+__device__ __forceinline__ float exact_sqrt(float x) { return __builtin_sqrtf(x); }
+__device__ __forceinline__ float exact_div(float a, float b) { return __fdiv_rn(a, b); }
+// C99 round(): half away from zero (the reference's CUDA round(float) and C++ round()).
+__device__ __forceinline__ float round_away(float x) { return __builtin_roundf(x); }
+
+// Squared distance ((a*a + b*b) + c*c) with every operation rounded (the expression of the
+// reference's distance(), backproject_cuda.cu:25-29, with norm3df replaced by it).
+__device__ __forceinline__ float sq3(float a, float b, float c) {
+    return (a * a + b * b) + c * c;
+}
+
+// The single-tap bin of the GPU path, backproject_cuda.cu:155 with footprint 0:
+// round((total - t0) / deltaT), total = ((lw + d1) + d2) + cw (:45).
+__device__ __forceinline__ float tap_round(float total, float t0, float dT) {
+    return round_away(exact_div(total - t0, dT));
+}
+
+// ------------------------------------------------------------------ ordered float keys
+// key(a) < key(b) iff a < b for non-NaN floats; -0 and +0 are adjacent keys.
+__device__ __forceinline__ uint32_t fkey(float x) {
+    uint32_t u = __float_as_uint(x);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float fkey_inv(uint32_t k) {
+    uint32_t u = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+    return __uint_as_float(u);
+}
+
+// Smallest key k in [klo, khi] with pred(k) true, for a predicate that is monotone
+// (false ... false true ... true) on that range and true at khi.  Gallops outward from a
+// guess, then bisects; at most ~64 predicate evaluations, typically 3-6 for a good guess.
+template <class Pred>
+__device__ uint32_t min_true_key(const Pred& pred, uint32_t kguess, uint32_t klo, uint32_t khi) {
+    kguess = kguess < klo ? klo : (kguess > khi ? khi : kguess);
+    uint32_t lo_false, hi_true;
+    if (pred(kguess)) {
+        hi_true = kguess;
+        uint32_t step = 1;
+        for (;;) {
+            if (hi_true == klo) return klo;
+            const uint32_t c = (hi_true - klo > step) ? hi_true - step : klo;
+            if (pred(c)) {
+                hi_true = c;
+                step <<= 1;
+            } else {
+                lo_false = c;
+                break;
+            }
+        }
+    } else {
+        lo_false = kguess;
+        uint32_t step = 1;
+        for (;;) {
+            const uint32_t c = (khi - lo_false > step) ? lo_false + step : khi;
+            if (c == khi || pred(c)) {
+                hi_true = c;
+                break;
+            }
+            lo_false = c;
+            step <<= 1;
+        }
+    }
+    while (hi_true - lo_false > 1u) {
+        const uint32_t mid = lo_false + (hi_true - lo_false) / 2u;
+        if (pred(mid)) hi_true = mid;
+        else lo_false = mid;
+    }
+    return hi_true;
+}
+
+// ------------------------------------------------------------------ buffer resources
+// Raw buffer descriptor (gfx9 dword3 = 0x00020000: 32-bit data format, no swizzle).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* base, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ float buf_f32(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+}
+
+// Transient sample formats.  `idx` is an element index inside a row that starts `soff`
+// bytes into the buffer.  Real formats return float, complex formats float2.
+struct SmpF32 {
+    static constexpr int kBytes = 4;
+    static constexpr bool kComplex = false;
+    using acc_t = float;
+    __device__ static __forceinline__ float load(__amdgpu_buffer_rsrc_t r, uint32_t idx, uint32_t soff) {
+        return buf_f32(r, idx * 4u, soff);
+    }
+};
+struct SmpF16 {
+    static constexpr int kBytes = 2;
+    static constexpr bool kComplex = false;
+    using acc_t = float;
+    __device__ static __forceinline__ float load(__amdgpu_buffer_rsrc_t r, uint32_t idx, uint32_t soff) {
+        const unsigned short b = __builtin_amdgcn_raw_buffer_load_b16(r, (int)(idx * 2u), (int)soff, 0);
+        return (float)__builtin_bit_cast(_Float16, b);
+    }
+};
+struct SmpBF16 {
+    static constexpr int kBytes = 2;
+    static constexpr bool kComplex = false;
+    using acc_t = float;
+    __device__ static __forceinline__ float load(__amdgpu_buffer_rsrc_t r, uint32_t idx, uint32_t soff) {
+        const unsigned short b = __builtin_amdgcn_raw_buffer_load_b16(r, (int)(idx * 2u), (int)soff, 0);
+        return __uint_as_float((uint32_t)b << 16);
+    }
+};
+struct SmpC64 {
+    static constexpr int kBytes = 8;
+    static constexpr bool kComplex = true;
+    using acc_t = float2;
+    __device__ static __forceinline__ float2 load(__amdgpu_buffer_rsrc_t r, uint32_t idx, uint32_t soff) {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)(idx * 8u), (int)soff, 0);
+        return make_float2(__uint_as_float(v[0]), __uint_as_float(v[1]));
+    }
+};
+struct SmpCF16 {
+    static constexpr int kBytes = 4;
+    static constexpr bool kComplex = true;
+    using acc_t = float2;
+    __device__ static __forceinline__ float2 load(__amdgpu_buffer_rsrc_t r, uint32_t idx, uint32_t soff) {
+        const uint32_t b = __builtin_amdgcn_raw_buffer_load_b32(r, (int)(idx * 4u), (int)soff, 0);
+        return make_float2((float)__builtin_bit_cast(_Float16, (unsigned short)(b & 0xffffu)),
+                           (float)__builtin_bit_cast(_Float16, (unsigned short)(b >> 16)));
+    }
+};
+struct SmpCBF16 {
+    static constexpr int kBytes = 4;
+    static constexpr bool kComplex = true;
+    using acc_t = float2;
+    __device__ static __forceinline__ float2 load(__amdgpu_buffer_rsrc_t r, uint32_t idx, uint32_t soff) {
+        const uint32_t b = __builtin_amdgcn_raw_buffer_load_b32(r, (int)(idx * 4u), (int)soff, 0);
+        return make_float2(__uint_as_float(b << 16), __uint_as_float(b & 0xffff0000u));
+    }
+};
+
+// ------------------------------------------------------------------ accumulators
+__device__ __forceinline__ void acc_add(float& a, float x) { a = a + x; }
+__device__ __forceinline__ void acc_add(float2& a, float2 x) { a.x = a.x + x.x; a.y = a.y + x.y; }
+__device__ __forceinline__ void acc_zero(float& a) { a = 0.0f; }
+__device__ __forceinline__ void acc_zero(float2& a) { a = make_float2(0.0f, 0.0f); }
+
+// Kahan-compensated add of a block partial `x` into (sum, comp); comp holds the negated
+// low-order part lost so far.
+__device__ __forceinline__ void kahan_add(float& sum, float& comp, float x) {
+    const float y = x - comp;
+    const float t = sum + y;
+    comp = (t - sum) - y;
+    sum = t;
+}
+__device__ __forceinline__ void kahan_add(float2& sum, float2& comp, float2 x) {
+    kahan_add(sum.x, comp.x, x.x);
+    kahan_add(sum.y, comp.y, x.y);
+}
+__device__ __forceinline__ float kahan_result(float sum, float comp) { return sum - comp; }
+__device__ __forceinline__ float2 kahan_result(float2 sum, float2 comp) {
+    return make_float2(sum.x - comp.x, sum.y - comp.y);
+}
+
+// Work-group index remap so that consecutive tiles land on the same XCD (dispatch places
+// work-group i on XCD i mod 8): neighbouring voxel tiles read the same transient-row
+// segments and share that XCD's L2.
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t bid, uint32_t nblocks) {
+    constexpr uint32_t kXcd = 8;
+    const uint32_t per = nblocks / kXcd, rem = nblocks % kXcd;
+    const uint32_t xcd = bid % kXcd, pos = bid / kXcd;
+    // XCDs with index < rem receive one extra block.
+    return (xcd < rem ? xcd * (per + 1) : rem * (per + 1) + (xcd - rem) * per) + pos;
+}
+
+}  // namespace bpx

@@ -1,0 +1,132 @@
+// Host-side internals of libbpx: error reporting, HIP checks, the plan object, and the
+// host pipeline restating the reference's bp::backproject / bp::phasor_reconstruction.
+#pragma once
+#include <hip/hip_runtime_api.h>
+#include <stdint.h>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/bpx.h"
+#include "bpx_internal.hpp"
+
+namespace bpx {
+
+// ------------------------------------------------------------------ errors
+bpx_status set_error(bpx_status s, const std::string& msg);
+void clear_error();
+struct Failure {
+    bpx_status status;
+    std::string msg;
+};
+[[noreturn]] void fail(bpx_status s, const std::string& msg);
+void hip_check(hipError_t e, const char* what);
+bool device_ready();  // a gfx950 device is visible and selected
+
+// ------------------------------------------------------------------ plan
+enum Path : int { kPathConfocal = 0, kPathGrid = 1, kPathPairs = 2, kPathWindow = 3 };
+
+struct DeviceBuffer {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    void alloc(size_t n);
+    void release();
+    ~DeviceBuffer() { release(); }
+};
+
+struct PlanImpl {
+    hipStream_t stream = nullptr;
+    int path = kPathPairs;
+    int format = kF32;
+    bool cplx = false;
+    int variant = 0, K = 4;
+    uint32_t P = 0, T = 0, ldh = 0, lds = 0, L = 0, C = 0;
+    float t0 = 0, dT = 1, scale = 1;
+    Grid g{};
+    Tiles tiles{};
+    bool sep = false, flat = false;
+    float lz = 0;
+    uint32_t kahan_block = 32;
+    float inv2 = 0, inv = 0, c1 = 0;
+    // window path
+    float vpos[3]{}, vsize_in[3]{}, wzero[3]{}, wvs[3]{}, half_width = 0, c1_lo = 0, c1_hi = 0;
+    bool all_confocal = false;
+    DeviceBuffer raw, H, S, Z, Z2, pair8, pair4, lasers, spads, vol;
+    hipEvent_t ev[8] = {};
+    ~PlanImpl();
+    void execute(hipStream_t s);
+};
+
+// Describes a capture given to a plan (host buffers).
+struct CaptureIn {
+    const float* chunk;      // [P][T] real, or interleaved complex
+    uint64_t size;           // elements (complex elements when cplx)
+    uint32_t T;
+    const bpx_pair* pairs;
+    uint32_t P;
+    float t0, dT;
+    bool cplx;
+};
+
+PlanImpl* make_plan_single_tap(const CaptureIn& c, const uint32_t n[3], const float zero[3],
+                               const float inc[9], const bpx_options* opt);
+PlanImpl* make_plan_window(const CaptureIn& c, const uint32_t n[3], const float vpos[3],
+                           const float vsize[3]);
+
+}  // namespace bpx
+
+struct bpx_plan_s {
+    std::unique_ptr<bpx::PlanImpl> impl;
+};
+
+namespace bpx {
+
+// ------------------------------------------------------------------ host pipeline
+namespace pipeline {
+
+struct Arrays {
+    // row-major copies owned here when a transposition was needed
+    std::vector<float> data, cgrid, lgrid;
+    const float* data_p = nullptr;
+    const float* cgrid_p = nullptr;
+    const float* lgrid_p = nullptr;
+    std::vector<int64_t> data_shape, cgrid_shape, lgrid_shape;
+};
+
+// Normalises inputs to row-major (transposing column-major arrays) and validates shapes.
+Arrays normalise(const float* data, const int64_t* data_shape, int data_ndim, const float* cgrid,
+                 const int64_t* cgrid_shape, const float* lgrid, const int64_t* lgrid_shape,
+                 bool row_major, bool confocal);
+
+// backproject.hpp:356-418
+std::vector<bpx_pair> precompute_distances(const float laser_position[3], const float* lgrid,
+                                           uint32_t Lx, uint32_t Ly,
+                                           const float camera_position[3], const float* cgrid,
+                                           uint32_t Cx, uint32_t Cy, bool confocal);
+// backproject.hpp:617-674 (row-major); returns (min_T_index - 100, max_T_index + 100)
+void time_limits(const float laser_position[3], const float* lgrid, uint32_t Lx, uint32_t Ly,
+                 const float camera_position[3], const float* cgrid, uint32_t Cx, uint32_t Cy,
+                 const float volume_size[3], const float volume_position[3], float t0, float dT,
+                 float* min_idx, float* max_idx);
+// backproject.hpp:820-839
+void basis(const float volume_position[3], const float volume_size[3], const uint32_t n[3],
+           const float wall_normal[3], float zero[3], float inc[9]);
+// backproject.hpp:689-740 (row-major); returns the chunk [rows][Tc]
+std::vector<float> transient_chunk(const float* data, uint64_t rows, uint32_t T, float min_idx,
+                                   float max_idx, uint32_t* Tc);
+// backproject.hpp:430-451 taps (reference parametrisation when sigma <= 0)
+int phasor_taps(float wavelength, float dT, uint32_t times, float sigma, std::vector<float>& c,
+                std::vector<float>& s);
+
+}  // namespace pipeline
+
+// Phasor FIR of host rows on the device: out [rows][T] interleaved complex (host).
+void phasor_filter_host(const float* in, uint64_t rows, uint32_t T, const std::vector<float>& c,
+                        const std::vector<float>& s, float* out);
+
+// Synthetic renderer (host, multithreaded).
+void render_transient(const bpx_pair* pairs, uint32_t P, uint32_t T, float t0, float dT,
+                      const float* samples, uint64_t n, float peak, bool poisson, uint64_t seed,
+                      float* out);
+
+}  // namespace bpx

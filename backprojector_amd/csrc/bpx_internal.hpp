@@ -1,0 +1,129 @@
+// Internal declarations shared by the C-ABI layer (host) and the kernel translation unit.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+
+namespace bpx {
+
+// Sample formats of the device-resident transient rows.
+enum SampleFormat : int {
+    kF32 = 0, kF16 = 1, kBF16 = 2,     // real
+    kC64 = 3, kCF16 = 4, kCBF16 = 5,   // complex (interleaved re, im)
+};
+__host__ __device__ inline bool format_complex(int f) { return f >= kC64; }
+inline int format_bytes(int f) {
+    switch (f) {
+        case kF32: return 4;
+        case kF16: case kBF16: return 2;
+        case kC64: return 8;
+        default: return 4;
+    }
+}
+
+// Volume grid of the GPU path: v[k] = ((zero[k] + inc[k]x) + inc[3+k]y) + inc[6+k]z.
+struct Grid {
+    uint32_t n[3];
+    float zero[3];
+    float inc[9];
+};
+
+// Tile decomposition shared by the gather kernels: a work-group of 256 lanes (4 waves)
+// covers 8 x 8 x (4K) voxels; each wave a 4 x 4 x 4 lane cube; lane r-th voxel at
+// z = tile_z0 + lane_z + 4r.
+struct Tiles {
+    uint32_t tx, ty, tz, count;
+};
+
+// Confocal single-tap kernel parameters (threshold table in squared-distance space).
+struct ConfocalArgs {
+    const void* H;         // padded rows [P][ldh]: H[p][0] = H[p][Tc+1] = 0, H[p][1+b] = sample b
+    const float* S;        // thresholds [P][lds]: S[p][j] = min s with round-bin(s) >= j, j in [0, Tc]
+    const float4* pairs;   // (lx, ly, lz, c1): c1 ~ (lw + cw - t0)/dT + 1
+    void* out;             // volume, float or float2 per voxel
+    uint32_t P, Tc, ldh, lds;
+    uint32_t h_bytes, s_bytes;   // buffer sizes for the resource descriptors
+    Grid g;
+    Tiles t;
+    float inv2;            // ~2/deltaT
+    float scale;           // per-capture sample scale (power of two)
+    float lz;              // common wall z when every pair shares it
+    uint32_t kahan_block;
+};
+
+// Factorised exhaustive-grid kernel parameters (threshold table in path-length space).
+struct GridArgs {
+    const void* H;          // padded rows [L*C][ldh], pair p = l*C + c
+    const float4* lasers;   // (x, y, z, laser_wall) per laser point
+    const float4* spads;    // (x, y, z, camera_wall) per SPAD point
+    const float* Z;         // thresholds [Tc+1] in path-length space
+    void* out;
+    uint32_t L, C, Tc, ldh;
+    uint32_t h_bytes;
+    Grid g;
+    Tiles t;
+    float inv;              // ~1/deltaT
+    float c1;               // ~1 - t0/deltaT
+    float scale;
+    uint32_t kahan_block;
+};
+
+// Arbitrary pair list, single tap (no structure assumed).
+struct PairsArgs {
+    const void* H;
+    const float* pairs;     // 8 floats per pair: camera_point[3], laser_point[3], camera_wall, laser_wall
+    const float* Z;
+    void* out;
+    uint32_t P, Tc, ldh;
+    uint32_t h_bytes;
+    Grid g;
+    Tiles t;
+    float inv, c1, scale;
+    uint32_t kahan_block;
+};
+
+// Window (reference CPU path) kernel parameters.
+struct WindowArgs {
+    const void* PS;         // exclusive prefix sums in double [P][ldp] (complex: double2)
+    const float* pairs;     // 8 floats per pair
+    const float* Zlo;       // thresholds [T] for the window start bin
+    const float* Zhi;       // thresholds [T] for the window end bin
+    void* out;
+    uint32_t P, T, ldp;
+    uint64_t ps_bytes;
+    uint32_t n[3];
+    float zero[3], vsize[3];
+    float inv, c1_lo, c1_hi;
+    uint32_t confocal;      // every pair has laser point == camera point
+    Tiles t;
+};
+
+// ------------------------------------------------------------------ kernel launchers
+// All launchers enqueue on `stream` and return the hipError_t of the launch.
+hipError_t launch_stable(const float* pair8, uint32_t P, uint32_t Tc, float t0, float dT,
+                         float* S, uint32_t lds, hipStream_t stream);
+hipError_t launch_ztable(int kind, float t0, float dT, float half_width, uint32_t n, float* Z,
+                         hipStream_t stream);
+hipError_t launch_pad_rows(const float* src, uint32_t rows, uint32_t Tc, uint32_t src_pitch,
+                           void* dst, uint32_t ldh, int format, float inv_scale,
+                           hipStream_t stream);
+hipError_t launch_prefix(const float* src, uint32_t rows, uint32_t T, int complex_samples,
+                         void* dst, uint32_t ldp, hipStream_t stream);
+hipError_t launch_phasor_fir(const float* src, uint32_t rows, uint32_t Tc, const float* taps_c,
+                             const float* taps_s, int pulse, void* dst, uint32_t ldh, int format,
+                             float inv_scale, hipStream_t stream);
+hipError_t launch_confocal(const ConfocalArgs& a, int format, int variant, bool sep, bool flat,
+                           hipStream_t stream);
+hipError_t launch_grid(const GridArgs& a, int format, int variant, hipStream_t stream);
+hipError_t launch_pairs(const PairsArgs& a, int format, int variant, hipStream_t stream);
+hipError_t launch_window(const WindowArgs& a, int complex_samples, hipStream_t stream);
+hipError_t launch_abs_max(const float* src, uint64_t n, float* result_dev, hipStream_t stream);
+
+// Voxels per lane (K) of each tuned variant; variant -1 picks the default.
+int variant_k(int variant);
+int variant_count();
+
+// Tiles of a volume for a given K.
+Tiles make_tiles(const uint32_t n[3], int K);
+
+}  // namespace bpx

@@ -1,0 +1,705 @@
+// gfx950 kernels of the NLOS backprojection: threshold tables, row staging, phasor FIR and
+// the voxel gather-accumulate kernels.  Compiled with -ffp-contract=off (see bpx_device.hpp).
+//
+// Bin exactness.  The reference GPU path (backproject_cuda.cu:25-46, 144-162) assigns pair p
+// and voxel v the single bin r = round(((lw + d1) + d2 + cw - t0) / deltaT) computed in fp32.
+// For fixed pair, r is a monotone step function of the path length (and, for a confocal pair
+// where d1 == d2, of the squared distance s).  The kernels therefore never divide: a table
+// holds, for every bin j, the smallest input whose rounded bin is >= j (computed once per
+// capture with the exact correctly-rounded formula, `min_true_key`), an approximate quotient
+// picks the two candidate bins {m, m+1}, and one comparison against the table decides.  The
+// host guarantees |approximate - exact quotient| < 0.5 (bpx_api.cpp, tap_error_bound), so the
+// selected bin is bit-identical to the fp32 formula for every voxel and pair.
+#include "bpx_device.hpp"
+#include "bpx_internal.hpp"
+
+namespace bpx {
+
+// =================================================================== threshold tables
+// Confocal pair: input s = squared wall-point-to-voxel distance; d1 = d2 = sqrt(s).
+struct StablePred {
+    float lw, cw, t0, dT, jf;
+    __device__ bool operator()(uint32_t key) const {
+        const float s = fkey_inv(key);
+        const float d = exact_sqrt(s);
+        float total = lw + d;
+        total = total + d;
+        total = total + cw;
+        return tap_round(total, t0, dT) >= jf;
+    }
+};
+
+__global__ __launch_bounds__(256) void stable_kernel(const float* __restrict__ pair8, uint32_t P,
+                                                     uint32_t Tc, float t0, float dT,
+                                                     float* __restrict__ S, uint32_t lds) {
+    const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = Tc + 1u;
+    if (gid >= (uint64_t)P * n) return;
+    const uint32_t p = (uint32_t)(gid / n), j = (uint32_t)(gid % n);
+    const float cw = pair8[8u * p + 6u], lw = pair8[8u * p + 7u];
+    const StablePred pred{lw, cw, t0, dT, (float)j};
+    const float d = 0.5f * (t0 + ((float)j - 0.5f) * dT - lw - cw);
+    const float guess = d > 0.0f ? d * d : 0.0f;
+    const uint32_t k = min_true_key(pred, fkey(guess), fkey(0.0f), fkey(__builtin_inff()));
+    S[(size_t)p * lds + j] = fkey_inv(k);
+}
+
+// Path-length space: kind 0 = single tap (backproject_cuda.cu:155), kind 1 / 2 = start / end
+// of the CPU path's window, round(((t -/+ w/2) - t0)/deltaT) (backproject.hpp:88-89).
+struct ZPred {
+    int kind;
+    float t0, dT, hw, jf;
+    __device__ bool operator()(uint32_t key) const {
+        const float t = fkey_inv(key);
+        float y;
+        if (kind == 0) y = t - t0;
+        else if (kind == 1) y = (t - hw) - t0;
+        else y = (t + hw) - t0;
+        return round_away(exact_div(y, dT)) >= jf;
+    }
+};
+
+__global__ __launch_bounds__(256) void ztable_kernel(int kind, float t0, float dT, float hw,
+                                                     uint32_t n, float* __restrict__ Z) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const ZPred pred{kind, t0, dT, hw, (float)j};
+    float g = t0 + ((float)j - 0.5f) * dT;
+    if (kind == 1) g = g + hw;
+    if (kind == 2) g = g - hw;
+    const uint32_t k = min_true_key(pred, fkey(g), fkey(-__builtin_inff()), fkey(__builtin_inff()));
+    Z[j] = fkey_inv(k);
+}
+
+hipError_t launch_stable(const float* pair8, uint32_t P, uint32_t Tc, float t0, float dT,
+                         float* S, uint32_t lds, hipStream_t stream) {
+    const uint64_t n = (uint64_t)P * (Tc + 1u);
+    const uint32_t blocks = (uint32_t)((n + 255u) / 256u);
+    hipLaunchKernelGGL(stable_kernel, dim3(blocks), dim3(256), 0, stream, pair8, P, Tc, t0, dT, S,
+                       lds);
+    return hipGetLastError();
+}
+
+hipError_t launch_ztable(int kind, float t0, float dT, float half_width, uint32_t n, float* Z,
+                         hipStream_t stream) {
+    hipLaunchKernelGGL(ztable_kernel, dim3((n + 255u) / 256u), dim3(256), 0, stream, kind, t0, dT,
+                       half_width, n, Z);
+    return hipGetLastError();
+}
+
+// =================================================================== row staging
+__device__ __forceinline__ uint16_t f32_to_bf16_rne(float x) {
+    const uint32_t u = __float_as_uint(x);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);  // quiet NaN
+    const uint32_t r = u + 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(r >> 16);
+}
+__device__ __forceinline__ uint16_t f32_to_f16(float x) {
+    return __builtin_bit_cast(uint16_t, (_Float16)x);
+}
+
+// Writes one sample (real: re only) into a padded destination row element.
+__device__ __forceinline__ void store_sample(void* dst, size_t idx, int format, float re, float im) {
+    switch (format) {
+        case kF32: static_cast<float*>(dst)[idx] = re; break;
+        case kF16: static_cast<uint16_t*>(dst)[idx] = f32_to_f16(re); break;
+        case kBF16: static_cast<uint16_t*>(dst)[idx] = f32_to_bf16_rne(re); break;
+        case kC64: static_cast<float2*>(dst)[idx] = make_float2(re, im); break;
+        case kCF16:
+            static_cast<uint32_t*>(dst)[idx] = (uint32_t)f32_to_f16(re) | ((uint32_t)f32_to_f16(im) << 16);
+            break;
+        default:
+            static_cast<uint32_t*>(dst)[idx] =
+                (uint32_t)f32_to_bf16_rne(re) | ((uint32_t)f32_to_bf16_rne(im) << 16);
+            break;
+    }
+}
+
+// dst[p][0] = dst[p][Tc+1..ldh) = 0, dst[p][1+b] = src[p][b] * inv_scale.  `src` holds real
+// samples, or interleaved complex samples when the destination format is complex.
+__global__ __launch_bounds__(256) void pad_rows_kernel(const float* __restrict__ src, uint32_t rows,
+                                                       uint32_t Tc, uint32_t src_pitch, void* dst,
+                                                       uint32_t ldh, int format, float inv_scale) {
+    const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= (uint64_t)rows * ldh) return;
+    const uint32_t p = (uint32_t)(gid / ldh), i = (uint32_t)(gid % ldh);
+    float re = 0.0f, im = 0.0f;
+    if (i >= 1u && i <= Tc) {
+        const uint32_t b = i - 1u;
+        if (format_complex(format)) {
+            re = src[2 * ((size_t)p * src_pitch + b)] * inv_scale;
+            im = src[2 * ((size_t)p * src_pitch + b) + 1] * inv_scale;
+        } else {
+            re = src[(size_t)p * src_pitch + b] * inv_scale;
+        }
+    }
+    store_sample(dst, gid, format, re, im);
+}
+
+hipError_t launch_pad_rows(const float* src, uint32_t rows, uint32_t Tc, uint32_t src_pitch,
+                           void* dst, uint32_t ldh, int format, float inv_scale,
+                           hipStream_t stream) {
+    const uint64_t n = (uint64_t)rows * ldh;
+    hipLaunchKernelGGL(pad_rows_kernel, dim3((uint32_t)((n + 255u) / 256u)), dim3(256), 0, stream,
+                       src, rows, Tc, src_pitch, dst, ldh, format, inv_scale);
+    return hipGetLastError();
+}
+
+// Exclusive prefix sums in double, one wave per row: PS[p][k] = sum_{i<k} H[p][i], k < T.
+__global__ __launch_bounds__(256) void prefix_kernel(const float* __restrict__ src, uint32_t rows,
+                                                     uint32_t T, int cplx, void* dst, uint32_t ldp) {
+    const uint32_t row = blockIdx.x * 4u + (threadIdx.x >> 6);
+    const uint32_t lane = threadIdx.x & 63u;
+    if (row >= rows) return;
+    double carry_re = 0.0, carry_im = 0.0;
+    for (uint32_t c0 = 0; c0 < T; c0 += 64u) {
+        const uint32_t i = c0 + lane;
+        double xr = 0.0, xi = 0.0;
+        if (i < T) {
+            if (cplx) {
+                xr = src[2 * ((size_t)row * T + i)];
+                xi = src[2 * ((size_t)row * T + i) + 1];
+            } else {
+                xr = src[(size_t)row * T + i];
+            }
+        }
+        // inclusive wave scan
+        double sr = xr, si = xi;
+        for (int off = 1; off < 64; off <<= 1) {
+            const double ur = __shfl_up(sr, off, 64);
+            const double ui = __shfl_up(si, off, 64);
+            if ((int)lane >= off) {
+                sr += ur;
+                si += ui;
+            }
+        }
+        if (i < T) {
+            const double er = carry_re + sr - xr, ei = carry_im + si - xi;
+            if (cplx) static_cast<double2*>(dst)[(size_t)row * ldp + i] = make_double2(er, ei);
+            else static_cast<double*>(dst)[(size_t)row * ldp + i] = er;
+        }
+        carry_re += __shfl(sr, 63, 64);
+        carry_im += __shfl(si, 63, 64);
+    }
+}
+
+hipError_t launch_prefix(const float* src, uint32_t rows, uint32_t T, int complex_samples,
+                         void* dst, uint32_t ldp, hipStream_t stream) {
+    hipLaunchKernelGGL(prefix_kernel, dim3((rows + 3u) / 4u), dim3(256), 0, stream, src, rows, T,
+                       complex_samples, dst, ldp);
+    return hipGetLastError();
+}
+
+// Phasor-field filter (backproject.hpp:459-476): out[t] = sum_{j<pulse} in[t+j-pulse/2] *
+// (c_j + i s_j) over 0 < idx < Tc, products rounded, summed in j order; written straight
+// into a padded complex destination row (fused staging/quantisation).
+__global__ __launch_bounds__(256) void phasor_fir_kernel(const float* __restrict__ src, uint32_t rows,
+                                                         uint32_t Tc, const float* __restrict__ tc,
+                                                         const float* __restrict__ ts, int pulse,
+                                                         void* dst, uint32_t ldh, int format,
+                                                         float inv_scale) {
+    extern __shared__ float taps[];
+    for (int j = threadIdx.x; j < pulse; j += blockDim.x) {
+        taps[2 * j] = tc[j];
+        taps[2 * j + 1] = ts[j];
+    }
+    __syncthreads();
+    const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= (uint64_t)rows * ldh) return;
+    const uint32_t p = (uint32_t)(gid / ldh), i = (uint32_t)(gid % ldh);
+    float re = 0.0f, im = 0.0f;
+    if (i >= 1u && i <= Tc) {
+        const int64_t t = (int64_t)i - 1;
+        const float* row = src + (size_t)p * Tc;
+        const int half = pulse / 2;
+        for (int j = 0; j < pulse; ++j) {
+            const int64_t idx = t + j - half;
+            if (idx > 0 && idx < (int64_t)Tc) {
+                const float x = row[idx];
+                re = re + x * taps[2 * j];
+                im = im + x * taps[2 * j + 1];
+            }
+        }
+        re = re * inv_scale;
+        im = im * inv_scale;
+    }
+    store_sample(dst, gid, format, re, im);
+}
+
+hipError_t launch_phasor_fir(const float* src, uint32_t rows, uint32_t Tc, const float* taps_c,
+                             const float* taps_s, int pulse, void* dst, uint32_t ldh, int format,
+                             float inv_scale, hipStream_t stream) {
+    const uint64_t n = (uint64_t)rows * ldh;
+    hipLaunchKernelGGL(phasor_fir_kernel, dim3((uint32_t)((n + 255u) / 256u)), dim3(256),
+                       sizeof(float) * 2 * pulse, stream, src, rows, Tc, taps_c, taps_s, pulse, dst,
+                       ldh, format, inv_scale);
+    return hipGetLastError();
+}
+
+// max |x| over n floats, as the bit pattern of a non-negative float (atomic max on uint).
+__global__ __launch_bounds__(256) void abs_max_kernel(const float* __restrict__ src, uint64_t n,
+                                                      unsigned int* result) {
+    float m = 0.0f;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const float a = fabsf(src[i]);
+        m = (a > m || a != a) ? a : m;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const float o = __shfl_xor(m, off, 64);
+        m = (o > m || o != o) ? o : m;
+    }
+    if ((threadIdx.x & 63u) == 0u) atomicMax(result, __float_as_uint(m));
+}
+
+hipError_t launch_abs_max(const float* src, uint64_t n, float* result_dev, hipStream_t stream) {
+    uint64_t blocks = (n + 255u) / 256u;
+    if (blocks > 4096u) blocks = 4096u;
+    if (blocks == 0u) blocks = 1u;
+    hipLaunchKernelGGL(abs_max_kernel, dim3((uint32_t)blocks), dim3(256), 0, stream, src, n,
+                       reinterpret_cast<unsigned int*>(result_dev));
+    return hipGetLastError();
+}
+
+// =================================================================== gather kernels
+Tiles make_tiles(const uint32_t n[3], int K) {
+    Tiles t;
+    t.tx = (n[0] + 7u) / 8u;
+    t.ty = (n[1] + 7u) / 8u;
+    t.tz = (n[2] + 4u * K - 1u) / (4u * K);
+    t.count = t.tx * t.ty * t.tz;
+    return t;
+}
+
+// Lane -> voxel mapping of a 256-lane work-group covering 8 x 8 x 4K voxels.
+struct LaneVoxels {
+    uint32_t ix, iy, iz0;
+};
+__device__ __forceinline__ LaneVoxels lane_voxels(const Tiles& t, int K) {
+    const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t tz = tile % t.tz, txy = tile / t.tz;
+    const uint32_t ty = txy % t.ty, tx = txy / t.ty;
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    LaneVoxels lv;
+    lv.ix = tx * 8u + (w & 1u) * 4u + (lane & 3u);
+    lv.iy = ty * 8u + (w >> 1) * 4u + ((lane >> 2) & 3u);
+    lv.iz0 = tz * (4u * (uint32_t)K) + (lane >> 4);
+    return lv;
+}
+
+// A.1 voxel position, backproject_cuda.cu:144-148.
+__device__ __forceinline__ void voxel_position(const Grid& g, uint32_t x, uint32_t y, uint32_t z,
+                                               float v[3]) {
+    const float fx = (float)x, fy = (float)y, fz = (float)z;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        float a = g.zero[k] + g.inc[k] * fx;
+        a = a + g.inc[3 + k] * fy;
+        v[k] = a + g.inc[6 + k] * fz;
+    }
+}
+
+template <class T>
+__device__ __forceinline__ void store_voxel(void* out, size_t id, T val);
+template <>
+__device__ __forceinline__ void store_voxel<float>(void* out, size_t id, float val) {
+    static_cast<float*>(out)[id] = val;
+}
+template <>
+__device__ __forceinline__ void store_voxel<float2>(void* out, size_t id, float2 val) {
+    static_cast<float2*>(out)[id] = val;
+}
+__device__ __forceinline__ float scaled(float v, float s) { return v * s; }
+__device__ __forceinline__ float2 scaled(float2 v, float s) { return make_float2(v.x * s, v.y * s); }
+
+// ------------------------------------------------------------------- confocal single tap
+// Every pair has laser point == camera point, so d1 == d2 == sqrt(s) with
+// s = ((dx*dx + dy*dy) + dz*dz).  Per voxel-pair update: s (one add when the wall is flat),
+// one hardware sqrt for the approximate quotient, a clamp, one table read, one compare and
+// the sample read.  SEP: voxels of a lane share x and y (inc[6] == inc[7] == 0).
+// FLAT: every wall point has the same z.
+template <class Smp, int K, bool SEP, bool FLAT>
+__global__ __launch_bounds__(256) void confocal_kernel(ConfocalArgs a) {
+    using acc_t = typename Smp::acc_t;
+    const LaneVoxels lv = lane_voxels(a.t, K);
+    const uint32_t cx = min(lv.ix, a.g.n[0] - 1u), cy = min(lv.iy, a.g.n[1] - 1u);
+    float vx[SEP ? 1 : K], vy[SEP ? 1 : K], vz[K], dz2[K];
+#pragma unroll
+    for (int r = 0; r < K; ++r) {
+        const uint32_t cz = min(lv.iz0 + 4u * r, a.g.n[2] - 1u);
+        float v[3];
+        voxel_position(a.g, cx, cy, cz, v);
+        if (SEP) {
+            if (r == 0) {
+                vx[0] = v[0];
+                vy[0] = v[1];
+            }
+        } else {
+            vx[r] = v[0];
+            vy[r] = v[1];
+        }
+        vz[r] = v[2];
+        const float dz = a.lz - v[2];
+        dz2[r] = dz * dz;
+    }
+    const auto hr = make_rsrc(a.H, a.h_bytes);
+    const auto sr = make_rsrc(a.S, a.s_bytes);
+    const float fTc = (float)a.Tc;
+    acc_t sum[K], comp[K], blk[K];
+#pragma unroll
+    for (int r = 0; r < K; ++r) {
+        acc_zero(sum[r]);
+        acc_zero(comp[r]);
+        acc_zero(blk[r]);
+    }
+    uint32_t p = 0;
+    while (p < a.P) {
+        const uint32_t pend = min(p + a.kahan_block, a.P);
+        for (; p < pend; ++p) {
+            const float4 pr = a.pairs[p];
+            const uint32_t soff_s = p * a.lds * 4u;
+            const uint32_t soff_h = p * a.ldh * (uint32_t)Smp::kBytes;
+            float axy = 0.0f;
+            if (SEP) {
+                const float dx = pr.x - vx[0], dy = pr.y - vy[0];
+                axy = dx * dx + dy * dy;
+            }
+#pragma unroll
+            for (int r = 0; r < K; ++r) {
+                float s;
+                if (SEP) {
+                    if (FLAT) {
+                        s = axy + dz2[r];
+                    } else {
+                        const float dz = pr.z - vz[r];
+                        s = axy + dz * dz;
+                    }
+                } else {
+                    s = sq3(pr.x - vx[r], pr.y - vy[r], pr.z - vz[r]);
+                }
+                const float d = __builtin_amdgcn_sqrtf(s);
+                const float q1 = __builtin_fmaf(d, a.inv2, pr.w);
+                const uint32_t j = (uint32_t)(int)__builtin_amdgcn_fmed3f(q1, 0.0f, fTc);
+                const float th = buf_f32(sr, j * 4u, soff_s);
+                const uint32_t hj = j + (s >= th ? 1u : 0u);
+                acc_add(blk[r], Smp::load(hr, hj, soff_h));
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < K; ++r) {
+            kahan_add(sum[r], comp[r], blk[r]);
+            acc_zero(blk[r]);
+        }
+    }
+    if (lv.ix < a.g.n[0] && lv.iy < a.g.n[1]) {
+#pragma unroll
+        for (int r = 0; r < K; ++r) {
+            const uint32_t iz = lv.iz0 + 4u * r;
+            if (iz < a.g.n[2]) {
+                const size_t id = ((size_t)lv.ix * a.g.n[1] + lv.iy) * a.g.n[2] + iz;
+                store_voxel(a.out, id, scaled(kahan_result(sum[r], comp[r]), a.scale));
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------- exhaustive grid
+// Pairs p = l*C + c of an L-laser x C-SPAD capture.  d1 (laser leg) is computed once per
+// (voxel, laser) and d2 (SPAD leg) once per (voxel, SPAD) per CS-chunk, both with the exact
+// correctly-rounded sqrt; the total path ((lw + d1) + d2) + cw is then exact and its bin is
+// decided against the path-length thresholds Z held in LDS.
+template <class Smp, int K, int CS>
+__global__ __launch_bounds__(256) void grid_kernel(GridArgs a) {
+    using acc_t = typename Smp::acc_t;
+    extern __shared__ float zs[];
+    for (uint32_t j = threadIdx.x; j <= a.Tc; j += blockDim.x) zs[j] = a.Z[j];
+    __syncthreads();
+    const LaneVoxels lv = lane_voxels(a.t, K);
+    const uint32_t cx = min(lv.ix, a.g.n[0] - 1u), cy = min(lv.iy, a.g.n[1] - 1u);
+    float v[K][3];
+#pragma unroll
+    for (int r = 0; r < K; ++r) voxel_position(a.g, cx, cy, min(lv.iz0 + 4u * r, a.g.n[2] - 1u), v[r]);
+    const auto hr = make_rsrc(a.H, a.h_bytes);
+    const float fTc = (float)a.Tc;
+    acc_t sum[K], comp[K], blk[K];
+#pragma unroll
+    for (int r = 0; r < K; ++r) {
+        acc_zero(sum[r]);
+        acc_zero(comp[r]);
+        acc_zero(blk[r]);
+    }
+    uint32_t since = 0;
+    for (uint32_t c0 = 0; c0 < a.C; c0 += CS) {
+        const uint32_t ncs = min((uint32_t)CS, a.C - c0);
+        float d2[CS][K], cw[CS];
+#pragma unroll
+        for (int cs = 0; cs < CS; ++cs) {
+            const float4 sp = a.spads[min(c0 + cs, a.C - 1u)];
+            cw[cs] = sp.w;
+#pragma unroll
+            for (int r = 0; r < K; ++r)
+                d2[cs][r] = exact_sqrt(sq3(v[r][0] - sp.x, v[r][1] - sp.y, v[r][2] - sp.z));
+        }
+        for (uint32_t l = 0; l < a.L; ++l) {
+            const float4 lp = a.lasers[l];
+            float d1[K];
+#pragma unroll
+            for (int r = 0; r < K; ++r)
+                d1[r] = exact_sqrt(sq3(lp.x - v[r][0], lp.y - v[r][1], lp.z - v[r][2]));
+#pragma unroll
+            for (int cs = 0; cs < CS; ++cs) {
+                if ((uint32_t)cs < ncs) {
+                    const uint32_t p = l * a.C + c0 + cs;
+                    const uint32_t soff_h = p * a.ldh * (uint32_t)Smp::kBytes;
+#pragma unroll
+                    for (int r = 0; r < K; ++r) {
+                        float total = lp.w + d1[r];
+                        total = total + d2[cs][r];
+                        total = total + cw[cs];
+                        const float q1 = __builtin_fmaf(total, a.inv, a.c1);
+                        const uint32_t j = (uint32_t)(int)__builtin_amdgcn_fmed3f(q1, 0.0f, fTc);
+                        const uint32_t hj = j + (total >= zs[j] ? 1u : 0u);
+                        acc_add(blk[r], Smp::load(hr, hj, soff_h));
+                    }
+                }
+            }
+            since += ncs;
+            if (since >= a.kahan_block) {
+                since = 0;
+#pragma unroll
+                for (int r = 0; r < K; ++r) {
+                    kahan_add(sum[r], comp[r], blk[r]);
+                    acc_zero(blk[r]);
+                }
+            }
+        }
+    }
+    if (lv.ix < a.g.n[0] && lv.iy < a.g.n[1]) {
+#pragma unroll
+        for (int r = 0; r < K; ++r) {
+            kahan_add(sum[r], comp[r], blk[r]);
+            const uint32_t iz = lv.iz0 + 4u * r;
+            if (iz < a.g.n[2]) {
+                const size_t id = ((size_t)lv.ix * a.g.n[1] + lv.iy) * a.g.n[2] + iz;
+                store_voxel(a.out, id, scaled(kahan_result(sum[r], comp[r]), a.scale));
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------- arbitrary pairs
+template <class Smp, int K>
+__global__ __launch_bounds__(256) void pairs_kernel(PairsArgs a) {
+    using acc_t = typename Smp::acc_t;
+    extern __shared__ float zs[];
+    for (uint32_t j = threadIdx.x; j <= a.Tc; j += blockDim.x) zs[j] = a.Z[j];
+    __syncthreads();
+    const LaneVoxels lv = lane_voxels(a.t, K);
+    const uint32_t cx = min(lv.ix, a.g.n[0] - 1u), cy = min(lv.iy, a.g.n[1] - 1u);
+    float v[K][3];
+#pragma unroll
+    for (int r = 0; r < K; ++r) voxel_position(a.g, cx, cy, min(lv.iz0 + 4u * r, a.g.n[2] - 1u), v[r]);
+    const auto hr = make_rsrc(a.H, a.h_bytes);
+    const float fTc = (float)a.Tc;
+    acc_t sum[K], comp[K], blk[K];
+#pragma unroll
+    for (int r = 0; r < K; ++r) {
+        acc_zero(sum[r]);
+        acc_zero(comp[r]);
+        acc_zero(blk[r]);
+    }
+    uint32_t p = 0;
+    while (p < a.P) {
+        const uint32_t pend = min(p + a.kahan_block, a.P);
+        for (; p < pend; ++p) {
+            const float* pr = a.pairs + 8u * p;
+            const float cpx = pr[0], cpy = pr[1], cpz = pr[2];
+            const float lpx = pr[3], lpy = pr[4], lpz = pr[5];
+            const float cw = pr[6], lw = pr[7];
+            const uint32_t soff_h = p * a.ldh * (uint32_t)Smp::kBytes;
+#pragma unroll
+            for (int r = 0; r < K; ++r) {
+                const float d1 = exact_sqrt(sq3(lpx - v[r][0], lpy - v[r][1], lpz - v[r][2]));
+                const float d2 = exact_sqrt(sq3(v[r][0] - cpx, v[r][1] - cpy, v[r][2] - cpz));
+                float total = lw + d1;
+                total = total + d2;
+                total = total + cw;
+                const float q1 = __builtin_fmaf(total, a.inv, a.c1);
+                const uint32_t j = (uint32_t)(int)__builtin_amdgcn_fmed3f(q1, 0.0f, fTc);
+                const uint32_t hj = j + (total >= zs[j] ? 1u : 0u);
+                acc_add(blk[r], Smp::load(hr, hj, soff_h));
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < K; ++r) {
+            kahan_add(sum[r], comp[r], blk[r]);
+            acc_zero(blk[r]);
+        }
+    }
+    if (lv.ix < a.g.n[0] && lv.iy < a.g.n[1]) {
+#pragma unroll
+        for (int r = 0; r < K; ++r) {
+            const uint32_t iz = lv.iz0 + 4u * r;
+            if (iz < a.g.n[2]) {
+                const size_t id = ((size_t)lv.ix * a.g.n[1] + lv.iy) * a.g.n[2] + iz;
+                store_voxel(a.out, id, scaled(kahan_result(sum[r], comp[r]), a.scale));
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------- window (CPU path)
+// backproject.hpp:75-96 on the CPU grid (octree_volume.hpp:147-167): float differences,
+// double squares/sum/sqrt (:42-53), wvw = float(d1 + d2), total = (lw + wvw) + cw; window
+// [max(0, round(((t - w/2) - t0)/dT)), min(T-1, round(((t + w/2) - t0)/dT))) summed through
+// double prefix sums.
+__device__ __forceinline__ double a2_dist(float ax, float ay, float az, float bx, float by, float bz) {
+    double x = (double)(ax - bx), y = (double)(ay - by), z = (double)(az - bz);
+    x = x * x;
+    y = y * y;
+    z = z * z;
+    return __builtin_sqrt((x + y) + z);
+}
+
+// clamp(round-bin(t), 0, T-1) from an approximate quotient and the thresholds Z.
+__device__ __forceinline__ uint32_t clamped_bin(float t, float inv, float c1, const float* Z, float hi) {
+    const float q1 = __builtin_fmaf(t, inv, c1);
+    const uint32_t j = (uint32_t)(int)__builtin_amdgcn_fmed3f(q1, 1.0f, hi);
+    return j - 1u + (t >= Z[j] ? 1u : 0u);
+}
+
+template <bool CPLX>
+__global__ __launch_bounds__(256) void window_kernel(WindowArgs a) {
+    const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
+    (void)tile;
+    const uint64_t id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t nvox = (uint64_t)a.n[0] * a.n[1] * a.n[2];
+    if (id >= nvox) return;
+    const uint32_t z = (uint32_t)(id % a.n[2]);
+    const uint32_t y = (uint32_t)((id / a.n[2]) % a.n[1]);
+    const uint32_t x = (uint32_t)(id / ((uint64_t)a.n[2] * a.n[1]));
+    const float vx = a.zero[0] + (float)x * a.vsize[0];
+    const float vy = a.zero[1] + (float)y * a.vsize[1];
+    const float vz = a.zero[2] + (float)z * a.vsize[2];
+    const float hiT = (float)(a.T - 1u);
+    double sre = 0.0, sim = 0.0;
+    for (uint32_t p = 0; p < a.P; ++p) {
+        const float* pr = a.pairs + 8u * p;
+        const double d1 = a2_dist(pr[3], pr[4], pr[5], vx, vy, vz);
+        const double d2 = a.confocal ? d1 : a2_dist(vx, vy, vz, pr[0], pr[1], pr[2]);
+        const float wvw = (float)(d1 + d2);
+        float total = pr[7] + wvw;
+        total = total + pr[6];
+        const uint32_t first = clamped_bin(total, a.inv, a.c1_lo, a.Zlo, hiT);
+        const uint32_t last = clamped_bin(total, a.inv, a.c1_hi, a.Zhi, hiT);
+        if (last > first) {
+            if (CPLX) {
+                const double2* row = static_cast<const double2*>(a.PS) + (size_t)p * a.ldp;
+                sre += row[last].x - row[first].x;
+                sim += row[last].y - row[first].y;
+            } else {
+                const double* row = static_cast<const double*>(a.PS) + (size_t)p * a.ldp;
+                sre += row[last] - row[first];
+            }
+        }
+    }
+    if (CPLX) static_cast<float2*>(a.out)[id] = make_float2((float)sre, (float)sim);
+    else static_cast<float*>(a.out)[id] = (float)sre;
+}
+
+hipError_t launch_window(const WindowArgs& a, int complex_samples, hipStream_t stream) {
+    const uint64_t nvox = (uint64_t)a.n[0] * a.n[1] * a.n[2];
+    const uint32_t blocks = (uint32_t)((nvox + 255u) / 256u);
+    if (complex_samples)
+        hipLaunchKernelGGL(window_kernel<true>, dim3(blocks), dim3(256), 0, stream, a);
+    else
+        hipLaunchKernelGGL(window_kernel<false>, dim3(blocks), dim3(256), 0, stream, a);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------- dispatch
+// Variants: 0 -> K = 4, 1 -> K = 8, 2 -> K = 2 voxels per lane.
+int variant_k(int variant) {
+    switch (variant) {
+        case 1: return 8;
+        case 2: return 2;
+        default: return 4;
+    }
+}
+int variant_count() { return 3; }
+
+template <class Smp, int K>
+static hipError_t confocal_dispatch_k(const ConfocalArgs& a, bool sep, bool flat, hipStream_t s) {
+    const dim3 grid(a.t.count), block(256);
+    if (sep && flat) hipLaunchKernelGGL((confocal_kernel<Smp, K, true, true>), grid, block, 0, s, a);
+    else if (sep) hipLaunchKernelGGL((confocal_kernel<Smp, K, true, false>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((confocal_kernel<Smp, K, false, false>), grid, block, 0, s, a);
+    return hipGetLastError();
+}
+
+template <class Smp>
+static hipError_t confocal_dispatch(const ConfocalArgs& a, int variant, bool sep, bool flat,
+                                    hipStream_t s) {
+    switch (variant_k(variant)) {
+        case 8: return confocal_dispatch_k<Smp, 8>(a, sep, flat, s);
+        case 2: return confocal_dispatch_k<Smp, 2>(a, sep, flat, s);
+        default: return confocal_dispatch_k<Smp, 4>(a, sep, flat, s);
+    }
+}
+
+hipError_t launch_confocal(const ConfocalArgs& a, int format, int variant, bool sep, bool flat,
+                           hipStream_t stream) {
+    switch (format) {
+        case kF32: return confocal_dispatch<SmpF32>(a, variant, sep, flat, stream);
+        case kF16: return confocal_dispatch<SmpF16>(a, variant, sep, flat, stream);
+        case kBF16: return confocal_dispatch<SmpBF16>(a, variant, sep, flat, stream);
+        case kC64: return confocal_dispatch<SmpC64>(a, variant, sep, flat, stream);
+        case kCF16: return confocal_dispatch<SmpCF16>(a, variant, sep, flat, stream);
+        case kCBF16: return confocal_dispatch<SmpCBF16>(a, variant, sep, flat, stream);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <class Smp>
+static hipError_t grid_dispatch(const GridArgs& a, int variant, hipStream_t s) {
+    const size_t lds = sizeof(float) * (a.Tc + 1u);
+    if (variant_k(variant) == 2)
+        hipLaunchKernelGGL((grid_kernel<Smp, 2, 8>), dim3(a.t.count), dim3(256), lds, s, a);
+    else
+        hipLaunchKernelGGL((grid_kernel<Smp, 4, 8>), dim3(a.t.count), dim3(256), lds, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_grid(const GridArgs& a, int format, int variant, hipStream_t stream) {
+    switch (format) {
+        case kF32: return grid_dispatch<SmpF32>(a, variant, stream);
+        case kF16: return grid_dispatch<SmpF16>(a, variant, stream);
+        case kBF16: return grid_dispatch<SmpBF16>(a, variant, stream);
+        case kC64: return grid_dispatch<SmpC64>(a, variant, stream);
+        case kCF16: return grid_dispatch<SmpCF16>(a, variant, stream);
+        case kCBF16: return grid_dispatch<SmpCBF16>(a, variant, stream);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <class Smp>
+static hipError_t pairs_dispatch(const PairsArgs& a, hipStream_t s) {
+    const size_t lds = sizeof(float) * (a.Tc + 1u);
+    hipLaunchKernelGGL((pairs_kernel<Smp, 4>), dim3(a.t.count), dim3(256), lds, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_pairs(const PairsArgs& a, int format, int variant, hipStream_t stream) {
+    (void)variant;
+    switch (format) {
+        case kF32: return pairs_dispatch<SmpF32>(a, stream);
+        case kF16: return pairs_dispatch<SmpF16>(a, stream);
+        case kBF16: return pairs_dispatch<SmpBF16>(a, stream);
+        case kC64: return pairs_dispatch<SmpC64>(a, stream);
+        case kCF16: return pairs_dispatch<SmpCF16>(a, stream);
+        case kCBF16: return pairs_dispatch<SmpCBF16>(a, stream);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace bpx

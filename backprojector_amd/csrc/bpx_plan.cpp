@@ -1,0 +1,474 @@
+// Plans: validation, structure detection, device staging and execution of one capture.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <memory>
+
+#include "bpx_host.hpp"
+
+namespace bpx {
+
+// ------------------------------------------------------------------ device buffers
+void DeviceBuffer::alloc(size_t n) {
+    release();
+    if (n == 0) n = 256;
+    const hipError_t e = hipMalloc(&ptr, n);
+    if (e != hipSuccess) {
+        ptr = nullptr;
+        fail(BPX_ERR_OUT_OF_MEMORY, "hipMalloc(" + std::to_string(n) + " bytes) failed: " +
+                                        hipGetErrorString(e));
+    }
+    bytes = n;
+}
+void DeviceBuffer::release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+}
+
+PlanImpl::~PlanImpl() {
+    for (auto& e : ev)
+        if (e) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+static uint32_t round_up(uint64_t x, uint32_t m) { return (uint32_t)((x + m - 1) / m * m); }
+
+static bool finite_pair(const bpx_pair& p) {
+    for (int k = 0; k < 3; ++k)
+        if (!std::isfinite(p.camera_point[k]) || !std::isfinite(p.laser_point[k])) return false;
+    return std::isfinite(p.camera_wall) && std::isfinite(p.laser_wall);
+}
+
+static bool same3(const float* a, const float* b) { return std::memcmp(a, b, 12) == 0; }
+
+// Bound on |approximate quotient - exact quotient| of the table lookup.  The approximation
+// q1 = fma(d', 2/dT, c1) (confocal) or fma(t, 1/dT, c1) differs from the exact fp32 chain
+// (<= 3 rounded adds, 1 sqrt of <= 2 ulp error in the hardware sqrt, 1 rounded subtract,
+// 1 rounded division) by a few units of 2^-24 times the magnitudes involved; 16 * 2^-24 * Q
+// with Q = (max |total| + |t0|)/dT + Tc + 2 covers every term with a wide margin.
+static double tap_error_bound(double max_total, double t0, double dT, double Tc) {
+    const double Q = (std::fabs(max_total) + std::fabs(t0)) / dT + Tc + 2.0;
+    return 16.0 * std::ldexp(1.0, -24) * Q;
+}
+
+static double dist3(const float* a, const double* b) {
+    const double x = a[0] - b[0], y = a[1] - b[1], z = a[2] - b[2];
+    return std::sqrt(x * x + y * y + z * z);
+}
+
+// Voxel-grid corners of the A.1 grid (convex hull of all voxel positions).
+static void grid_corners(const uint32_t n[3], const float zero[3], const float inc[9],
+                         double corners[8][3]) {
+    for (int m = 0; m < 8; ++m) {
+        const double i[3] = {(m & 4) ? (double)(n[0] - 1) : 0.0, (m & 2) ? (double)(n[1] - 1) : 0.0,
+                             (m & 1) ? (double)(n[2] - 1) : 0.0};
+        for (int k = 0; k < 3; ++k)
+            corners[m][k] = (double)zero[k] + inc[k] * i[0] + inc[3 + k] * i[1] + inc[6 + k] * i[2];
+    }
+}
+
+static double max_total_path(const std::vector<bpx_pair>& pr, const double corners[8][3]) {
+    double mx = 0.0;
+    for (const auto& p : pr) {
+        double dl = 0, dc = 0;
+        for (int m = 0; m < 8; ++m) {
+            dl = std::max(dl, dist3(p.laser_point, corners[m]));
+            dc = std::max(dc, dist3(p.camera_point, corners[m]));
+        }
+        mx = std::max(mx, std::fabs((double)p.laser_wall) + std::fabs((double)p.camera_wall) + dl + dc);
+    }
+    return mx;
+}
+
+// Power-of-two scale s so that max|x| / s lies in [2^14, 2^15) (exact rescaling).
+static float choose_scale(const float* x, uint64_t n) {
+    float m = 0.0f;
+    for (uint64_t i = 0; i < n; ++i) m = std::max(m, std::fabs(x[i]));
+    if (!(m > 0.0f) || !std::isfinite(m)) return 1.0f;
+    int e;
+    std::frexp(m, &e);  // m in [2^(e-1), 2^e)
+    return std::ldexp(1.0f, e - 1 - 14);
+}
+
+static void validate_common(const CaptureIn& c, const uint32_t n[3]) {
+    if (!c.chunk && c.size) fail(BPX_ERR_INVALID_ARGUMENT, "transient_chunk is NULL");
+    if (!c.pairs && c.P) fail(BPX_ERR_INVALID_ARGUMENT, "scanned_pairs is NULL");
+    if (c.T == 0) fail(BPX_ERR_INVALID_ARGUMENT, "T must be > 0");
+    if ((uint64_t)c.P * c.T != c.size)
+        fail(BPX_ERR_INVALID_ARGUMENT, "transient_size (" + std::to_string(c.size) +
+                                           ") != num_pairs * T (" +
+                                           std::to_string((uint64_t)c.P * c.T) + ")");
+    if (!std::isfinite(c.t0)) fail(BPX_ERR_INVALID_ARGUMENT, "t0 must be finite");
+    if (!(c.dT > 0.0f) || !std::isfinite(c.dT) || !std::isfinite(1.0f / c.dT) ||
+        c.dT < std::numeric_limits<float>::min())
+        fail(BPX_ERR_INVALID_ARGUMENT, "deltaT must be a positive normal float");
+    for (int k = 0; k < 3; ++k)
+        if (n[k] == 0) fail(BPX_ERR_INVALID_ARGUMENT, "voxels_per_side must be > 0");
+    if ((uint64_t)n[0] * n[1] * n[2] > (1ull << 32))
+        fail(BPX_ERR_UNSUPPORTED, "volumes above 2^32 voxels are not supported");
+    if (c.T > (1u << 22)) fail(BPX_ERR_UNSUPPORTED, "T above 2^22 bins is not supported");
+}
+
+// Pairs that take part: finite ones, minus the reference launcher's dropped tail if asked.
+static std::vector<uint32_t> used_pairs(const CaptureIn& c, uint32_t flags) {
+    uint32_t P = c.P;
+    if (flags & BPX_FLAG_REFERENCE_PAIR_TRUNCATION) {
+        const uint32_t B = std::min<uint32_t>(256u, std::max<uint32_t>(P, 1u));
+        P = (P / B) * B;
+    }
+    std::vector<uint32_t> idx;
+    idx.reserve(P);
+    for (uint32_t p = 0; p < P; ++p)
+        if (finite_pair(c.pairs[p])) idx.push_back(p);
+    return idx;
+}
+
+// Stages the raw rows of the used pairs on the device (compacting when pairs were dropped).
+static void stage_raw(PlanImpl& pl, const CaptureIn& c, const std::vector<uint32_t>& idx) {
+    const size_t comp = c.cplx ? 2 : 1;
+    const size_t row = (size_t)c.T * comp;
+    pl.raw.alloc(sizeof(float) * row * std::max<size_t>(idx.size(), 1));
+    if (idx.size() == c.P) {
+        hip_check(hipMemcpy(pl.raw.ptr, c.chunk, sizeof(float) * row * c.P, hipMemcpyHostToDevice),
+                  "hipMemcpy(transient)");
+    } else {
+        std::vector<float> tmp(row * idx.size());
+        for (size_t i = 0; i < idx.size(); ++i)
+            std::memcpy(&tmp[i * row], c.chunk + (size_t)idx[i] * row, sizeof(float) * row);
+        if (!tmp.empty())
+            hip_check(hipMemcpy(pl.raw.ptr, tmp.data(), sizeof(float) * tmp.size(),
+                                hipMemcpyHostToDevice),
+                      "hipMemcpy(transient)");
+    }
+}
+
+static void make_events(PlanImpl& pl) {
+    for (auto& e : pl.ev) hip_check(hipEventCreate(&e), "hipEventCreate");
+}
+
+// Exhaustive-grid structure: p = l*C + c with laser data depending only on l and camera
+// data only on c.  Returns C (0 when the list is not such a product).
+static uint32_t grid_structure(const std::vector<bpx_pair>& pr) {
+    const uint32_t P = (uint32_t)pr.size();
+    if (P < 2) return 0;
+    uint32_t C = 1;
+    while (C < P && same3(pr[C].laser_point, pr[0].laser_point) &&
+           pr[C].laser_wall == pr[0].laser_wall)
+        ++C;
+    if (C == P || P % C != 0 || C < 2) return 0;
+    for (uint32_t p = 0; p < P; ++p) {
+        const uint32_t l = p / C, c = p % C;
+        const bpx_pair& a = pr[p];
+        const bpx_pair& lb = pr[(size_t)l * C];
+        const bpx_pair& cb = pr[c];
+        if (!same3(a.laser_point, lb.laser_point) || a.laser_wall != lb.laser_wall) return 0;
+        if (!same3(a.camera_point, cb.camera_point) || a.camera_wall != cb.camera_wall) return 0;
+    }
+    return C;
+}
+
+PlanImpl* make_plan_single_tap(const CaptureIn& c, const uint32_t n[3], const float zero[3],
+                               const float inc[9], const bpx_options* opt) {
+    validate_common(c, n);
+    for (int k = 0; k < 3; ++k)
+        if (!std::isfinite(zero[k])) fail(BPX_ERR_INVALID_ARGUMENT, "volume_zero_pos must be finite");
+    for (int k = 0; k < 9; ++k)
+        if (!std::isfinite(inc[k])) fail(BPX_ERR_INVALID_ARGUMENT, "voxel_inc must be finite");
+    if (!device_ready()) fail(BPX_ERR_NO_DEVICE, "no gfx950 device available");
+
+    bpx_options o{BPX_STORAGE_F32, 32u, 0u, -1};
+    if (opt) o = *opt;
+    if (o.kahan_block == 0) o.kahan_block = 32;
+    if (o.storage > BPX_STORAGE_BF16) fail(BPX_ERR_INVALID_ARGUMENT, "unknown storage format");
+    if (o.variant >= variant_count()) fail(BPX_ERR_INVALID_ARGUMENT, "unknown kernel variant");
+
+    auto pl = std::make_unique<PlanImpl>();
+    hip_check(hipStreamCreateWithFlags(&pl->stream, hipStreamNonBlocking), "hipStreamCreate");
+    make_events(*pl);
+    pl->cplx = c.cplx;
+    pl->format = c.cplx ? (o.storage == BPX_STORAGE_F16 ? kCF16 : o.storage == BPX_STORAGE_BF16 ? kCBF16 : kC64)
+                        : (o.storage == BPX_STORAGE_F16 ? kF16 : o.storage == BPX_STORAGE_BF16 ? kBF16 : kF32);
+    pl->kahan_block = o.kahan_block;
+    pl->t0 = c.t0;
+    pl->dT = c.dT;
+    pl->T = c.T;
+    std::memcpy(pl->g.n, n, sizeof(uint32_t) * 3);
+    std::memcpy(pl->g.zero, zero, sizeof(float) * 3);
+    std::memcpy(pl->g.inc, inc, sizeof(float) * 9);
+
+    const std::vector<uint32_t> idx = used_pairs(c, o.flags);
+    std::vector<bpx_pair> pr(idx.size());
+    for (size_t i = 0; i < idx.size(); ++i) pr[i] = c.pairs[idx[i]];
+    pl->P = (uint32_t)pr.size();
+
+    const size_t nvox = (size_t)n[0] * n[1] * n[2];
+    pl->vol.alloc(nvox * (c.cplx ? 8 : 4));
+
+    // structure
+    bool confocal = !pr.empty();
+    for (const auto& p : pr) confocal = confocal && same3(p.laser_point, p.camera_point);
+    pl->C = confocal ? 0 : grid_structure(pr);
+    pl->L = pl->C ? pl->P / pl->C : 0;
+    pl->path = confocal ? kPathConfocal : (pl->C ? kPathGrid : kPathPairs);
+
+    // exactness precondition of the table lookup
+    double corners[8][3];
+    grid_corners(n, zero, inc, corners);
+    const double bound = tap_error_bound(max_total_path(pr, corners), c.t0, c.dT, c.T);
+    if (bound >= 0.25)
+        fail(BPX_ERR_UNSUPPORTED,
+             "path lengths / t0 too large relative to deltaT for the exact bin lookup (bound " +
+                 std::to_string(bound) + ")");
+
+    pl->variant = o.variant < 0 ? 0 : o.variant;
+    pl->K = variant_k(pl->variant);
+    if (pl->path != kPathConfocal && pl->K == 8) pl->K = 4;
+    pl->tiles = make_tiles(n, pl->K);
+
+    if (pl->P == 0) return pl.release();
+
+    // samples: raw rows resident on the device; scale for 16-bit storage
+    pl->scale = (pl->format == kF32 || pl->format == kC64) ? 1.0f : choose_scale(c.chunk, c.size * (c.cplx ? 2 : 1));
+    stage_raw(*pl, c, idx);
+    pl->ldh = round_up((uint64_t)c.T + 2, 64);
+    const uint64_t hbytes = (uint64_t)pl->P * pl->ldh * format_bytes(pl->format);
+    if (hbytes >= (1ull << 32)) fail(BPX_ERR_UNSUPPORTED, "transient rows above 4 GiB per plan");
+    pl->H.alloc(hbytes);
+
+    std::vector<float> p8((size_t)pl->P * 8);
+    std::memcpy(p8.data(), pr.data(), sizeof(float) * p8.size());
+    pl->pair8.alloc(sizeof(float) * p8.size());
+    hip_check(hipMemcpy(pl->pair8.ptr, p8.data(), sizeof(float) * p8.size(), hipMemcpyHostToDevice),
+              "hipMemcpy(pairs)");
+
+    if (pl->path == kPathConfocal) {
+        pl->lds = round_up((uint64_t)c.T + 1, 64);
+        const uint64_t sbytes = (uint64_t)pl->P * pl->lds * 4;
+        if (sbytes >= (1ull << 32)) fail(BPX_ERR_UNSUPPORTED, "threshold table above 4 GiB");
+        pl->S.alloc(sbytes);
+        pl->inv2 = (float)(2.0 / (double)c.dT);
+        std::vector<float> p4((size_t)pl->P * 4);
+        bool flat = true;
+        for (uint32_t p = 0; p < pl->P; ++p) {
+            p4[4 * p + 0] = pr[p].laser_point[0];
+            p4[4 * p + 1] = pr[p].laser_point[1];
+            p4[4 * p + 2] = pr[p].laser_point[2];
+            p4[4 * p + 3] = (float)(((double)pr[p].laser_wall + (double)pr[p].camera_wall - (double)c.t0) /
+                                        (double)c.dT + 1.0);
+            flat = flat && pr[p].laser_point[2] == pr[0].laser_point[2];
+        }
+        pl->flat = flat;
+        pl->lz = pr[0].laser_point[2];
+        pl->sep = inc[6] == 0.0f && inc[7] == 0.0f;
+        if (!pl->sep) pl->flat = false;
+        pl->pair4.alloc(sizeof(float) * p4.size());
+        hip_check(hipMemcpy(pl->pair4.ptr, p4.data(), sizeof(float) * p4.size(), hipMemcpyHostToDevice),
+                  "hipMemcpy(pair table)");
+    } else {
+        if ((uint64_t)(c.T + 1) * 4 > 64 * 1024)
+            fail(BPX_ERR_UNSUPPORTED, "non-confocal captures need T <= 16383 after cropping");
+        pl->Z.alloc(sizeof(float) * (c.T + 1));
+        pl->inv = (float)(1.0 / (double)c.dT);
+        pl->c1 = (float)(1.0 - (double)c.t0 / (double)c.dT);
+        if (pl->path == kPathGrid) {
+            std::vector<float> las((size_t)pl->L * 4), spd((size_t)pl->C * 4);
+            for (uint32_t l = 0; l < pl->L; ++l) {
+                const bpx_pair& a = pr[(size_t)l * pl->C];
+                las[4 * l + 0] = a.laser_point[0];
+                las[4 * l + 1] = a.laser_point[1];
+                las[4 * l + 2] = a.laser_point[2];
+                las[4 * l + 3] = a.laser_wall;
+            }
+            for (uint32_t s = 0; s < pl->C; ++s) {
+                spd[4 * s + 0] = pr[s].camera_point[0];
+                spd[4 * s + 1] = pr[s].camera_point[1];
+                spd[4 * s + 2] = pr[s].camera_point[2];
+                spd[4 * s + 3] = pr[s].camera_wall;
+            }
+            pl->lasers.alloc(sizeof(float) * las.size());
+            pl->spads.alloc(sizeof(float) * spd.size());
+            hip_check(hipMemcpy(pl->lasers.ptr, las.data(), sizeof(float) * las.size(), hipMemcpyHostToDevice),
+                      "hipMemcpy(lasers)");
+            hip_check(hipMemcpy(pl->spads.ptr, spd.data(), sizeof(float) * spd.size(), hipMemcpyHostToDevice),
+                      "hipMemcpy(spads)");
+        }
+    }
+    return pl.release();
+}
+
+void PlanImpl::execute(hipStream_t s) {
+    const size_t nvox = (size_t)g.n[0] * g.n[1] * g.n[2];
+    if (path == kPathWindow) {
+        if (P == 0 || T < 2) {
+            hip_check(hipMemsetAsync(vol.ptr, 0, nvox * (cplx ? 8 : 4), s), "hipMemsetAsync");
+            return;
+        }
+        hip_check(hipEventRecord(ev[0], s), "hipEventRecord");
+        hip_check(launch_prefix(static_cast<const float*>(raw.ptr), P, T, cplx ? 1 : 0, H.ptr, ldh, s),
+                  "prefix kernel");
+        hip_check(hipEventRecord(ev[1], s), "hipEventRecord");
+        hip_check(launch_ztable(1, t0, dT, half_width, T, static_cast<float*>(Z.ptr), s), "window table (start)");
+        hip_check(launch_ztable(2, t0, dT, half_width, T, static_cast<float*>(Z2.ptr), s), "window table (end)");
+        hip_check(hipEventRecord(ev[2], s), "hipEventRecord");
+        WindowArgs a{};
+        a.PS = H.ptr;
+        a.pairs = static_cast<const float*>(pair8.ptr);
+        a.Zlo = static_cast<const float*>(Z.ptr);
+        a.Zhi = static_cast<const float*>(Z2.ptr);
+        a.out = vol.ptr;
+        a.P = P;
+        a.T = T;
+        a.ldp = ldh;
+        a.ps_bytes = H.bytes;
+        std::memcpy(a.n, g.n, sizeof(a.n));
+        std::memcpy(a.zero, wzero, sizeof(a.zero));
+        std::memcpy(a.vsize, wvs, sizeof(a.vsize));
+        a.inv = inv;
+        a.c1_lo = c1_lo;
+        a.c1_hi = c1_hi;
+        a.confocal = all_confocal ? 1u : 0u;
+        hip_check(launch_window(a, cplx ? 1 : 0, s), "window gather kernel");
+        hip_check(hipEventRecord(ev[3], s), "hipEventRecord");
+        return;
+    }
+    if (P == 0) {
+        hip_check(hipMemsetAsync(vol.ptr, 0, nvox * (cplx ? 8 : 4), s), "hipMemsetAsync");
+        return;
+    }
+    hip_check(hipEventRecord(ev[0], s), "hipEventRecord");
+    hip_check(launch_pad_rows(static_cast<const float*>(raw.ptr), P, T, T, H.ptr, ldh, format,
+                              1.0f / scale, s),
+              "row staging kernel");
+    hip_check(hipEventRecord(ev[1], s), "hipEventRecord");
+    if (path == kPathConfocal) {
+        hip_check(launch_stable(static_cast<const float*>(pair8.ptr), P, T, t0, dT,
+                                static_cast<float*>(S.ptr), lds, s),
+                  "threshold table kernel");
+        hip_check(hipEventRecord(ev[2], s), "hipEventRecord");
+        ConfocalArgs a{};
+        a.H = H.ptr;
+        a.S = static_cast<const float*>(S.ptr);
+        a.pairs = static_cast<const float4*>(pair4.ptr);
+        a.out = vol.ptr;
+        a.P = P;
+        a.Tc = T;
+        a.ldh = ldh;
+        a.lds = lds;
+        a.h_bytes = (uint32_t)H.bytes;
+        a.s_bytes = (uint32_t)S.bytes;
+        a.g = g;
+        a.t = tiles;
+        a.inv2 = inv2;
+        a.scale = scale;
+        a.lz = lz;
+        a.kahan_block = kahan_block;
+        hip_check(launch_confocal(a, format, variant, sep, flat, s), "confocal gather kernel");
+    } else {
+        hip_check(launch_ztable(0, t0, dT, 0.0f, T + 1, static_cast<float*>(Z.ptr), s),
+                  "threshold table kernel");
+        hip_check(hipEventRecord(ev[2], s), "hipEventRecord");
+        if (path == kPathGrid) {
+            GridArgs a{};
+            a.H = H.ptr;
+            a.lasers = static_cast<const float4*>(lasers.ptr);
+            a.spads = static_cast<const float4*>(spads.ptr);
+            a.Z = static_cast<const float*>(Z.ptr);
+            a.out = vol.ptr;
+            a.L = L;
+            a.C = C;
+            a.Tc = T;
+            a.ldh = ldh;
+            a.h_bytes = (uint32_t)H.bytes;
+            a.g = g;
+            a.t = tiles;
+            a.inv = inv;
+            a.c1 = c1;
+            a.scale = scale;
+            a.kahan_block = kahan_block;
+            hip_check(launch_grid(a, format, variant, s), "grid gather kernel");
+        } else {
+            PairsArgs a{};
+            a.H = H.ptr;
+            a.pairs = static_cast<const float*>(pair8.ptr);
+            a.Z = static_cast<const float*>(Z.ptr);
+            a.out = vol.ptr;
+            a.P = P;
+            a.Tc = T;
+            a.ldh = ldh;
+            a.h_bytes = (uint32_t)H.bytes;
+            a.g = g;
+            a.t = tiles;
+            a.inv = inv;
+            a.c1 = c1;
+            a.scale = scale;
+            a.kahan_block = kahan_block;
+            hip_check(launch_pairs(a, format, variant, s), "pairs gather kernel");
+        }
+    }
+    hip_check(hipEventRecord(ev[3], s), "hipEventRecord");
+}
+
+PlanImpl* make_plan_window(const CaptureIn& c, const uint32_t n[3], const float vpos[3],
+                           const float vsize[3]) {
+    validate_common(c, n);
+    for (int k = 0; k < 3; ++k)
+        if (!std::isfinite(vpos[k]) || !std::isfinite(vsize[k]))
+            fail(BPX_ERR_INVALID_ARGUMENT, "volume position/size must be finite");
+    if (!device_ready()) fail(BPX_ERR_NO_DEVICE, "no gfx950 device available");
+    auto pl = std::make_unique<PlanImpl>();
+    hip_check(hipStreamCreateWithFlags(&pl->stream, hipStreamNonBlocking), "hipStreamCreate");
+    make_events(*pl);
+    pl->path = kPathWindow;
+    pl->cplx = c.cplx;
+    pl->format = c.cplx ? kC64 : kF32;
+    pl->t0 = c.t0;
+    pl->dT = c.dT;
+    pl->T = c.T;
+    std::memcpy(pl->g.n, n, sizeof(uint32_t) * 3);
+    // CPU grid (octree_volume.hpp:42, 147-167) and window width (backproject.hpp:147)
+    for (int k = 0; k < 3; ++k) {
+        pl->wzero[k] = vpos[k] - vsize[k] / 2.0f;
+        pl->wvs[k] = vsize[k] / (float)n[k];
+    }
+    const float a2 = pl->wvs[0] * pl->wvs[0], b2 = pl->wvs[1] * pl->wvs[1], c2 = pl->wvs[2] * pl->wvs[2];
+    const float width = (float)std::sqrt(((double)a2 + (double)b2) + (double)c2);
+    pl->half_width = width / 2.0f;
+    const std::vector<uint32_t> idx = used_pairs(c, 0);
+    std::vector<bpx_pair> pr(idx.size());
+    for (size_t i = 0; i < idx.size(); ++i) pr[i] = c.pairs[idx[i]];
+    pl->P = (uint32_t)pr.size();
+    bool confocal = true;
+    for (const auto& p : pr) confocal = confocal && same3(p.laser_point, p.camera_point);
+    pl->all_confocal = confocal;
+    const size_t nvox = (size_t)n[0] * n[1] * n[2];
+    pl->vol.alloc(nvox * (c.cplx ? 8 : 4));
+    // exactness precondition: CPU-grid corners
+    double corners[8][3];
+    for (int m = 0; m < 8; ++m) {
+        const double i[3] = {(m & 4) ? (double)(n[0] - 1) : 0.0, (m & 2) ? (double)(n[1] - 1) : 0.0,
+                             (m & 1) ? (double)(n[2] - 1) : 0.0};
+        for (int k = 0; k < 3; ++k) corners[m][k] = (double)pl->wzero[k] + (double)pl->wvs[k] * i[k];
+    }
+    const double bound = tap_error_bound(max_total_path(pr, corners) + width, c.t0, c.dT, c.T);
+    if (bound >= 0.25)
+        fail(BPX_ERR_UNSUPPORTED, "path lengths / t0 too large relative to deltaT for the exact bin lookup");
+    pl->inv = (float)(1.0 / (double)c.dT);
+    pl->c1_lo = (float)(1.0 - ((double)c.t0 + (double)pl->half_width) / (double)c.dT);
+    pl->c1_hi = (float)(1.0 - ((double)c.t0 - (double)pl->half_width) / (double)c.dT);
+    if (pl->P == 0 || c.T < 2) return pl.release();
+    stage_raw(*pl, c, idx);
+    pl->ldh = round_up(c.T, 32);
+    pl->H.alloc((size_t)pl->P * pl->ldh * (c.cplx ? 16 : 8));
+    pl->Z.alloc(sizeof(float) * c.T);
+    pl->Z2.alloc(sizeof(float) * c.T);
+    std::vector<float> p8((size_t)pl->P * 8);
+    std::memcpy(p8.data(), pr.data(), sizeof(float) * p8.size());
+    pl->pair8.alloc(sizeof(float) * p8.size());
+    hip_check(hipMemcpy(pl->pair8.ptr, p8.data(), sizeof(float) * p8.size(), hipMemcpyHostToDevice),
+              "hipMemcpy(pairs)");
+    return pl.release();
+}
+
+}  // namespace bpx

@@ -1,0 +1,93 @@
+"""Drop-in for the reference's Python module `nlosbpy` (nlosbpy.cpp:81-120).
+
+Same functions, argument names, order and defaults:
+
+    backproject(transient_data, camera_grid_positions, laser_grid_positions, camera_position,
+                laser_position, t0, deltaT, is_confocal, volume_position, volume_size,
+                voxels_per_side, wall_normal, assume_row_major=False, use_cpu=False)
+        -> float32 [N0, N1, N2]
+    phasor(transient_data, camera_grid_positions, laser_grid_positions, camera_position,
+           laser_position, t0, deltaT, is_confocal, volume_position, volume_size,
+           voxels_per_side, wavelength, assume_row_major=False, use_cpu=False)
+        -> complex64 [N0, N1, N2]
+
+Every computation runs on the GPU through libbpx.so.  `use_cpu=True` selects the reference
+CPU path's *semantics* (grid spacing size/N, voxel-diagonal window, double-precision
+distances, backproject.hpp:138-216), still computed on the GPU.  `assume_row_major=False`
+means the arrays are column-major (Fortran-order) transposes of the row-major layout; they
+are transposed before the pipeline runs (the reference's column-major branch reads out of
+bounds, backproject.hpp:384-385, 405).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from ._native import check, lib, ptr
+
+__all__ = ["backproject", "phasor"]
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _shape(a):
+    return np.ascontiguousarray(np.asarray(a.shape, dtype=np.int64))
+
+
+def _vec3(a, name):
+    v = _f32(a).reshape(-1)
+    if v.size != 3:
+        raise ValueError(f"{name} must have 3 elements")
+    return v
+
+
+def _prep(transient_data, camera_grid_positions, laser_grid_positions, camera_position,
+          laser_position, volume_position, volume_size, voxels_per_side):
+    H = _f32(transient_data)
+    cg = _f32(camera_grid_positions)
+    lg = _f32(laser_grid_positions)
+    if cg.ndim != 3 or lg.ndim != 3:
+        raise ValueError("grid positions must be 3-D arrays")
+    N = np.ascontiguousarray(np.asarray(voxels_per_side, dtype=np.uint32).reshape(-1))
+    if N.size != 3:
+        raise ValueError("voxels_per_side must have 3 elements")
+    return (H, cg, lg, _vec3(camera_position, "camera_position"),
+            _vec3(laser_position, "laser_position"), _vec3(volume_position, "volume_position"),
+            _vec3(volume_size, "volume_size"), N)
+
+
+def backproject(transient_data, camera_grid_positions, laser_grid_positions, camera_position,
+                laser_position, t0, deltaT, is_confocal, volume_position, volume_size,
+                voxels_per_side, wall_normal, assume_row_major=False, use_cpu=False):
+    """bp::backproject (backproject.hpp:762-909) -> float32 volume [N0, N1, N2]."""
+    H, cg, lg, cp, lp, vp, vs, N = _prep(transient_data, camera_grid_positions,
+                                         laser_grid_positions, camera_position, laser_position,
+                                         volume_position, volume_size, voxels_per_side)
+    wn = _vec3(wall_normal, "wall_normal")
+    out = np.zeros(tuple(int(x) for x in N), np.float32)
+    dshape, cshape, lshape = _shape(H), _shape(cg), _shape(lg)
+    check(lib().bpx_nlos_backproject(ptr(H), ptr(dshape), H.ndim, ptr(cg), ptr(cshape), ptr(lg),
+                                     ptr(lshape), ptr(cp), ptr(lp), float(t0), float(deltaT),
+                                     1 if is_confocal else 0, ptr(vp), ptr(vs), ptr(N), ptr(wn),
+                                     1 if assume_row_major else 0, 1 if use_cpu else 0, ptr(out)))
+    return out
+
+
+def phasor(transient_data, camera_grid_positions, laser_grid_positions, camera_position,
+           laser_position, t0, deltaT, is_confocal, volume_position, volume_size,
+           voxels_per_side, wavelength, assume_row_major=False, use_cpu=False):
+    """bp::phasor_reconstruction (backproject.hpp:932-1039) -> complex64 volume."""
+    H, cg, lg, cp, lp, vp, vs, N = _prep(transient_data, camera_grid_positions,
+                                         laser_grid_positions, camera_position, laser_position,
+                                         volume_position, volume_size, voxels_per_side)
+    out = np.zeros(tuple(int(x) for x in N), np.complex64)
+    dshape, cshape, lshape = _shape(H), _shape(cg), _shape(lg)
+    check(lib().bpx_nlos_phasor(ptr(H), ptr(dshape), H.ndim, ptr(cg), ptr(cshape), ptr(lg),
+                                ptr(lshape), ptr(cp), ptr(lp), float(t0), float(deltaT),
+                                1 if is_confocal else 0, ptr(vp), ptr(vs), ptr(N),
+                                float(wavelength), 1 if assume_row_major else 0,
+                                1 if use_cpu else 0, ptr(out)))
+    return out

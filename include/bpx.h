@@ -1,0 +1,215 @@
+/*
+ * bpx.h -- C ABI of the MI355X-native NLOS time-of-flight backprojector (libbpx.so).
+ *
+ * Plain C: POD structs, pointers and sizes, status codes.  No function calls exit(); every
+ * failure returns a bpx_status and leaves a message for bpx_last_error().  Host-buffer entry
+ * points are synchronous; device-buffer entry points (plans, batches) enqueue on the stream
+ * given (NULL = the library's own stream) and synchronise only where documented.
+ *
+ * Reference interfaces replaced (mjgalindo/backprojector):
+ *   bpx_backprojection          <- call_cuda_backprojection          backproject_cuda.hpp:19-28
+ *   bpx_octree_backprojection   <- call_cuda_octree_backprojection   backproject_cuda.hpp:30-39
+ *   bpx_complex_backprojection  <- call_cuda_complex_backprojection  backproject_cuda.hpp:41-49
+ *   bpx_window_backprojection   <- classic_backprojection (CPU path) backproject.hpp:236-247
+ *   bpx_phasor_pulse            <- phasor_pulse                      backproject.hpp:430-479
+ *   bpx_nlos_backproject        <- bp::backproject via nlosbpy.backproject
+ *                                  backproject.hpp:762-909, nlosbpy.cpp:15-45
+ *   bpx_nlos_phasor             <- bp::phasor_reconstruction via nlosbpy.phasor
+ *                                  backproject.hpp:932-1039, nlosbpy.cpp:47-77
+ *   bpx_pair                    <- struct CameraLaserPair            backproject_cuda.hpp:9-13
+ */
+#ifndef BPX_H
+#define BPX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BPX_ABI_VERSION 1
+
+typedef enum bpx_status {
+    BPX_OK = 0,
+    BPX_ERR_INVALID_ARGUMENT = 1, /* shapes, sizes, non-finite geometry, bad enum values */
+    BPX_ERR_OUT_OF_MEMORY = 2,    /* device or host allocation failed */
+    BPX_ERR_DEVICE = 3,           /* a HIP call failed (message names it) */
+    BPX_ERR_UNSUPPORTED = 4,      /* valid request outside what this build implements */
+    BPX_ERR_NO_DEVICE = 5         /* no gfx950 device visible */
+} bpx_status;
+
+/* Same layout and meaning as the reference's CameraLaserPair (32 bytes). */
+typedef struct bpx_pair {
+    float camera_point[3];
+    float laser_point[3];
+    float camera_wall;
+    float laser_wall;
+} bpx_pair;
+
+/* Device storage of transient samples. Real captures: F32/F16/BF16; complex (phasor)
+ * captures are stored as the complex counterpart (c64 / 2xf16 / 2xbf16). 16-bit storage
+ * uses a per-capture power-of-two scale chosen from max|H| (exact rescaling). */
+typedef enum bpx_storage {
+    BPX_STORAGE_F32 = 0,
+    BPX_STORAGE_F16 = 1,
+    BPX_STORAGE_BF16 = 2
+} bpx_storage;
+
+/* Semantics of a reconstruction.
+ * SINGLE_TAP: the reference GPU path (backproject_cuda.cu:117-175): one bin per pair,
+ *   grid spacing size/(N-1) with the wall-normal basis, fp32 path length.
+ * VOXEL_WINDOW: the reference CPU path (backproject.hpp:75-96, 138-216): bins in the
+ *   voxel-diagonal window, grid spacing size/N, double-precision distances. */
+typedef enum bpx_semantics {
+    BPX_SEMANTICS_SINGLE_TAP = 0,
+    BPX_SEMANTICS_VOXEL_WINDOW = 1
+} bpx_semantics;
+
+/* Options of the extended entry points; a NULL pointer means all defaults. */
+typedef struct bpx_options {
+    uint32_t storage;     /* bpx_storage (default F32)                                       */
+    uint32_t kahan_block; /* pairs summed plainly between compensated adds (0 -> 32)          */
+    uint32_t flags;       /* BPX_FLAG_*                                                       */
+    int32_t variant;      /* gather-kernel variant (-1 -> autotuned/default)                  */
+} bpx_options;
+
+/* Drop the last P mod min(256, P) pairs exactly as the reference launcher does
+ * (backproject_cuda.cu:150, 418); off by default (every pair is used). */
+#define BPX_FLAG_REFERENCE_PAIR_TRUNCATION 0x1u
+
+/* ------------------------------------------------------------------ diagnostics */
+const char* bpx_last_error(void);   /* thread-local; "" when the last call succeeded */
+const char* bpx_version(void);
+int bpx_abi_version(void);
+/* Number of usable gfx950 devices (0 when none); never fails. */
+int bpx_device_count(void);
+
+/* ------------------------------------------------------------------ reference-equivalent
+ * Host buffers in, host volume out (overwritten), synchronous.
+ * transient_chunk: [num_pairs][T] row-major samples, transient_size = num_pairs * T.
+ * voxel_volume: [N0][N1][N2] row-major; voxel_inc: 3x3 row-major (row r = step of index r). */
+bpx_status bpx_backprojection(const float* transient_chunk, uint64_t transient_size, uint32_t T,
+                              const bpx_pair* scanned_pairs, uint32_t num_pairs,
+                              float* voxel_volume, const uint32_t voxels_per_side[3],
+                              const float volume_zero_pos[3], const float voxel_inc[9],
+                              float t0, float deltaT);
+
+/* Complex samples and volume as interleaved (re, im) float pairs; transient_size counts
+ * complex elements. */
+bpx_status bpx_complex_backprojection(const float* transient_chunk, uint64_t transient_size,
+                                      uint32_t T, const bpx_pair* scanned_pairs,
+                                      uint32_t num_pairs, float* voxel_volume,
+                                      const uint32_t voxels_per_side[3],
+                                      const float volume_zero_pos[3], const float voxel_inc[9],
+                                      float t0, float deltaT);
+
+bpx_status bpx_backprojection_ex(const float* transient_chunk, uint64_t transient_size,
+                                 uint32_t T, const bpx_pair* scanned_pairs, uint32_t num_pairs,
+                                 float* voxel_volume, const uint32_t voxels_per_side[3],
+                                 const float volume_zero_pos[3], const float voxel_inc[9],
+                                 float t0, float deltaT, int complex_samples,
+                                 const bpx_options* options);
+
+/* CPU-path semantics on the GPU: the grid is given as the reference CPU path takes it
+ * (volume_position = centre, volume_size), spacing size/N, voxel-diagonal window. */
+bpx_status bpx_window_backprojection(const float* transient_chunk, uint64_t transient_size,
+                                     uint32_t T, const bpx_pair* scanned_pairs,
+                                     uint32_t num_pairs, float* voxel_volume,
+                                     const uint32_t voxels_per_side[3],
+                                     const float volume_position[3], const float volume_size[3],
+                                     float t0, float deltaT, int complex_samples);
+
+/* Phasor-field pulse filter.  in: [rows][T] real; out: [rows][T] interleaved complex.
+ * sigma <= 0 selects the reference's sigma = times*wavelength/6 and pulse length
+ * (int)(times*wavelength/deltaT); sigma > 0 is an extension: Gaussian window of that sigma
+ * truncated at +-3 sigma (pulse = 2*ceil(3 sigma/deltaT) + 1 taps). */
+bpx_status bpx_phasor_pulse(const float* transient_in, uint64_t rows, uint32_t T,
+                            float wavelength, float deltaT, uint32_t times, float sigma,
+                            float* complex_out);
+
+/* ------------------------------------------------------------------ reference pipeline
+ * Row-major inputs as nlosbpy passes them with assume_row_major=True:
+ *   confocal: transient [X][Y][T], grids [X][Y][3]; exhaustive: transient [Lx][Ly][Cx][Cy][T].
+ * shape arrays give the dimensions of each array (ndim entries).  assume_row_major = 0 means
+ * the arrays are the column-major (Fortran-order) transposes; they are transposed to row-major
+ * first.  Output: float [N0][N1][N2] (backproject) or interleaved complex (phasor). */
+bpx_status bpx_nlos_backproject(const float* transient_data, const int64_t* data_shape, int data_ndim,
+                                const float* camera_grid_positions, const int64_t* camera_grid_shape,
+                                const float* laser_grid_positions, const int64_t* laser_grid_shape,
+                                const float camera_position[3], const float laser_position[3],
+                                float t0, float deltaT, int is_confocal,
+                                const float volume_position[3], const float volume_size[3],
+                                const uint32_t voxels_per_side[3], const float wall_normal[3],
+                                int assume_row_major, int use_cpu, float* volume_out);
+
+bpx_status bpx_nlos_phasor(const float* transient_data, const int64_t* data_shape, int data_ndim,
+                           const float* camera_grid_positions, const int64_t* camera_grid_shape,
+                           const float* laser_grid_positions, const int64_t* laser_grid_shape,
+                           const float camera_position[3], const float laser_position[3],
+                           float t0, float deltaT, int is_confocal,
+                           const float volume_position[3], const float volume_size[3],
+                           const uint32_t voxels_per_side[3], float wavelength,
+                           int assume_row_major, int use_cpu, float* complex_volume_out);
+
+/* ------------------------------------------------------------------ device-resident plans
+ * A plan owns the device copy of one capture (samples staged once into the padded layout
+ * and storage format) and its output volume; bpx_plan_execute runs the whole per-capture
+ * reconstruction (threshold tables + gather) on the device. */
+typedef struct bpx_plan_s* bpx_plan;
+
+bpx_status bpx_plan_create(const float* transient_chunk, uint64_t transient_size, uint32_t T,
+                           const bpx_pair* scanned_pairs, uint32_t num_pairs,
+                           const uint32_t voxels_per_side[3], const float volume_zero_pos[3],
+                           const float voxel_inc[9], float t0, float deltaT, int complex_samples,
+                           const bpx_options* options, bpx_plan* plan_out);
+/* Plan of the GPU path built by the reference pipeline (bp::backproject up to the
+ * call_cuda_backprojection call, backproject.hpp:779-891): time limits, transient chunk,
+ * pair table, basis and chunked t0 from nlosbpy-style inputs (see bpx_nlos_backproject). */
+bpx_status bpx_nlos_plan_create(const float* transient_data, const int64_t* data_shape, int data_ndim,
+                                const float* camera_grid_positions, const int64_t* camera_grid_shape,
+                                const float* laser_grid_positions, const int64_t* laser_grid_shape,
+                                const float camera_position[3], const float laser_position[3],
+                                float t0, float deltaT, int is_confocal,
+                                const float volume_position[3], const float volume_size[3],
+                                const uint32_t voxels_per_side[3], const float wall_normal[3],
+                                int assume_row_major, const bpx_options* options,
+                                bpx_plan* plan_out);
+bpx_status bpx_plan_execute(bpx_plan plan, void* hip_stream);
+bpx_status bpx_plan_synchronize(bpx_plan plan);
+/* Copies the device volume to host (float or interleaved complex). */
+bpx_status bpx_plan_read_volume(bpx_plan plan, float* voxel_volume);
+/* Describes the plan: kernel path (0 confocal, 1 grid, 2 pairs), voxels per lane, number of
+ * pairs used, chunk length, per-capture scale. */
+bpx_status bpx_plan_info(bpx_plan plan, int32_t* path, int32_t* lane_voxels, uint32_t* pairs_used,
+                         uint32_t* Tc, float* scale);
+
+typedef struct bpx_timing {
+    double total_ms;        /* host wall time of all timed executions                    */
+    double per_exec_ms;     /* total_ms / iterations                                      */
+    double table_kernel_ms; /* mean device time of the threshold-table kernel per execution */
+    double gather_kernel_ms;/* mean device time of the gather kernel per execution         */
+    int32_t iterations;
+} bpx_timing;
+/* Runs `warmup` untimed then `iterations` timed executions on the plan's stream, timing
+ * each kernel with HIP events recorded on that stream. */
+bpx_status bpx_plan_benchmark(bpx_plan plan, int32_t warmup, int32_t iterations, bpx_timing* out);
+void bpx_plan_destroy(bpx_plan plan);
+
+/* ------------------------------------------------------------------ synthetic captures
+ * Renders a transient capture of hidden Lambertian geometry (no reference counterpart).
+ * samples: [n][7] = (x, y, z, nx, ny, nz, area) surface samples; normal (0,0,0) = isotropic
+ * point.  Pairs follow bpx_pair; path L = lw + |w_l - q| + |q - w_s| + cw, bin
+ * floor((L - t0)/deltaT), weight area * cos_l * cos_s / (|w_l - q|^2 |q - w_s|^2) with the
+ * cosines taken at the hidden sample (1 for isotropic points).  The rendered expectation is
+ * scaled so its maximum bin equals peak_count; if poisson != 0 each bin is replaced by a
+ * Poisson draw from a counter-based generator keyed by (seed, pair, bin).  out: [P][T]. */
+bpx_status bpx_render_transient(const bpx_pair* pairs, uint32_t num_pairs, uint32_t T, float t0,
+                                float deltaT, const float* samples, uint64_t num_samples,
+                                float peak_count, int poisson, uint64_t seed, float* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BPX_H */

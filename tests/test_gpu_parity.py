@@ -1,0 +1,357 @@
+"""GPU parity: the HIP kernels (through the C ABI) against the CPU oracle.
+
+Bin exactness is tested at scale with integer-valued samples: H[p][b] is a small positive
+integer that varies with b, so every voxel sum is an integer below 2^24 and any float
+summation order is exact.  A single bin off by one changes a voxel's sum, so the GPU volume
+must EQUAL the oracle's exactly-accumulated volume.  Real-valued captures are compared
+within 1e-5 relative L2 and each states its error against the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+from backprojector_amd import scenes
+
+pytestmark = pytest.mark.gpu
+f32 = np.float32
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.complex128 if np.iscomplexobj(a) else np.float64)
+    b = np.asarray(b, a.dtype)
+    n = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / (n if n > 0 else 1.0))
+
+
+def coded_samples(P, T, seed=0):
+    """Integer sample codes in [1, 61] varying along the bin axis and across pairs."""
+    b = np.arange(T, dtype=np.int64)[None, :]
+    p = np.arange(P, dtype=np.int64)[:, None]
+    return (((7 * b + 13 * p + seed) % 61) + 1).astype(f32)
+
+
+def grid_basis(N, pos=(0, 0, 0.8), size=(1, 1, 1)):
+    pos, size = np.asarray(pos, f32), np.asarray(size, f32)
+    zero = pos - size / f32(2)
+    inc = np.diag([size[k] / f32(N[k] - 1) for k in range(3)]).astype(f32).ravel()
+    return zero, inc
+
+
+def confocal_pairs(side, jitter=0.0, seed=0):
+    cg = scenes.wall_grid(side)
+    if jitter:
+        cg = cg + np.random.default_rng(seed).uniform(-jitter, jitter, cg.shape).astype(f32) * [1, 1, 0]
+        cg = cg.astype(f32)
+    return scenes.wall_pairs(cg, cg, True)
+
+
+def sample_voxels(N, n, seed=0):
+    rng = np.random.default_rng(seed)
+    return np.stack([rng.integers(0, N[k], n) for k in range(3)], 1).astype(np.uint32)
+
+
+# ----------------------------------------------------------------- bit-exact bins
+@pytest.mark.parametrize("side,N,t0", [(16, (20, 24, 28), 2.0), (32, (32, 32, 32), 2.3419),
+                                       (8, (7, 5, 3), 1.75)])
+def test_confocal_bins_bit_exact(gpu, oracle, side, N, t0):
+    pr = confocal_pairs(side, jitter=0.01, seed=side)
+    T = 2048
+    H = coded_samples(len(pr), T)
+    zero, inc = grid_basis(N)
+    vol = gpu.backprojection(H, pr, N, zero, inc, t0, 0.001)
+    ref = oracle.a1_volume_f64acc(H, pr, N, zero, inc, t0, 0.001)
+    assert np.all(ref == np.round(ref)) and ref.max() < 2 ** 24
+    mism = np.count_nonzero(vol != ref.astype(f32))
+    assert mism == 0, f"{mism} voxels differ (bin mismatch)"
+
+
+def test_confocal_bins_bit_exact_all_variants_and_non_separable_basis(gpu, oracle):
+    pr = confocal_pairs(12, jitter=0.02, seed=3)
+    T = 2500
+    H = coded_samples(len(pr), T, seed=5)
+    N = (18, 17, 19)
+    zero, inc = grid_basis(N)
+    ref = oracle.a1_volume_f64acc(H, pr, N, zero, inc, 2.05, 0.001)
+    for v in (0, 1, 2):
+        vol = gpu.backprojection(H, pr, N, zero, inc, 2.05, 0.001, variant=v)
+        assert np.array_equal(vol, ref.astype(f32)), f"variant {v}"
+    # a rotated basis (wall normal) makes the grid non-separable and the wall non-flat
+    zero2, inc2 = oracle.basis([0, 0.1, 0.8], [1, 1, 1], N, wall_normal=[0.3, -1, 0.2])
+    pr2 = pr.copy()
+    pr2["laser_point"][:, 2] = np.linspace(0, 0.05, len(pr2)).astype(f32)
+    pr2["camera_point"] = pr2["laser_point"]
+    ref2 = oracle.a1_volume_f64acc(H, pr2, N, zero2, inc2, 2.05, 0.001)
+    vol2 = gpu.backprojection(H, pr2, N, zero2, inc2, 2.05, 0.001)
+    assert np.array_equal(vol2, ref2.astype(f32))
+
+
+def test_round_half_away_ties_on_gpu(gpu, oracle):
+    # total = 4 exactly (walls 1.25, voxel 0.75 above the wall point); t0 places q at k + 0.5
+    pr = np.zeros(1, gpu.PAIR_DTYPE)
+    pr["laser_wall"] = pr["camera_wall"] = 1.25
+    zero = np.array([0, 0, 0.75], f32)
+    inc = np.eye(3, dtype=f32).ravel()
+    T = 512
+    H = np.arange(1, T + 1, dtype=f32)[None, :]
+    for k in (100, 101, 37, 0, -1, 510, 511):
+        t0 = f32(4.0) - f32((k + 0.5) / 256)
+        vol = gpu.backprojection(H, pr, (1, 1, 1), zero, inc, float(t0), 2.0 ** -8)
+        ref = oracle.a1_volume_f64acc(H, pr, (1, 1, 1), zero, inc, float(t0), 2.0 ** -8)
+        # round half away: k + 0.5 -> k + 1 for k >= 0, -0.5 -> -1 (no tap); code = bin + 1
+        want = k + 2 if (k >= 0 and k + 1 <= T - 1) else 0
+        assert vol[0, 0, 0] == ref[0, 0, 0] == want, k
+
+
+def test_exhaustive_grid_bins_bit_exact(gpu, oracle):
+    lg = scenes.wall_grid(8)
+    cg = (scenes.wall_grid(6) * f32(0.9)).astype(f32)
+    pr = oracle.precompute_pairs(scenes.ORIGIN, lg, scenes.ORIGIN, cg, confocal=False)
+    T = 3000
+    H = coded_samples(len(pr), T, seed=2)
+    N = (16, 12, 20)
+    zero, inc = grid_basis(N)
+    vol = gpu.backprojection(H, pr, N, zero, inc, 2.2, 0.001)
+    ref = oracle.a1_volume_f64acc(H, pr, N, zero, inc, 2.2, 0.001)
+    assert np.array_equal(vol, ref.astype(f32))
+
+
+def test_arbitrary_pairs_bins_bit_exact(gpu, oracle):
+    rng = np.random.default_rng(11)
+    P = 300
+    pr = np.zeros(P, gpu.PAIR_DTYPE)
+    pr["laser_point"] = rng.uniform(-1, 1, (P, 3)) * [1, 1, 0.05]
+    pr["camera_point"] = rng.uniform(-1, 1, (P, 3)) * [1, 1, 0.05]
+    pr["laser_wall"] = rng.uniform(0.5, 1.5, P)
+    pr["camera_wall"] = rng.uniform(0.5, 1.5, P)
+    T = 4000
+    H = coded_samples(P, T, seed=9)
+    N = (13, 11, 9)
+    zero, inc = grid_basis(N)
+    vol = gpu.backprojection(H, pr, N, zero, inc, 2.0, 0.001)
+    ref = oracle.a1_volume_f64acc(H, pr, N, zero, inc, 2.0, 0.001)
+    assert np.array_equal(vol, ref.astype(f32))
+
+
+def test_metric_size_bins_on_sampled_voxels(gpu, oracle):
+    # 128 x 128 confocal pairs into 128^3 with coded samples; 2048 random voxels checked
+    pr = confocal_pairs(128)
+    T = 3600
+    H = coded_samples(len(pr), T, seed=1)
+    N = (128, 128, 128)
+    zero, inc = grid_basis(N)
+    vol = gpu.backprojection(H, pr, N, zero, inc, 2.1, 0.001)
+    vox = sample_voxels(N, 2048, seed=4)
+    ref = oracle.a1_voxels_f64acc(H, pr, vox, zero, inc, 2.1, 0.001)
+    got = vol[vox[:, 0], vox[:, 1], vox[:, 2]]
+    assert np.array_equal(got, ref.astype(f32))
+
+
+# ----------------------------------------------------------------- real-valued volumes
+def test_confocal_volume_parity_rendered(gpu, oracle):
+    cap = scenes.render(scenes.config_scene(1), 32, 4096, t0=2.0, seed=1001)
+    pr = oracle.precompute_pairs(cap.laser_position, cap.laser_grid, cap.camera_position,
+                                 cap.camera_grid, True)
+    H = cap.transient.reshape(len(pr), -1)
+    N = (32, 32, 32)
+    zero, inc = grid_basis(N)
+    vol = gpu.backprojection(H, pr, N, zero, inc, 2.0, 0.001)
+    exact = oracle.a1_volume_f64acc(H, pr, N, zero, inc, 2.0, 0.001)
+    refo = oracle.a1_volume(H, pr, N, zero, inc, 2.0, 0.001)
+    fp64 = oracle.fp64_volume(H, pr, N, zero, inc, 2.0, 0.001)
+    assert rel_l2(vol, exact) <= 1e-6
+    assert rel_l2(vol, refo) <= 1e-5
+    err64 = rel_l2(vol, fp64)
+    print(f"confocal 32^2 -> 32^3: rel L2 vs reference order {rel_l2(vol, refo):.2e}, "
+          f"vs FP64 oracle {err64:.2e}")
+    assert err64 < 5e-2
+
+
+def test_complex_bins_bit_exact(gpu, oracle):
+    pr = confocal_pairs(16, jitter=0.01, seed=8)
+    T = 2048
+    H = coded_samples(len(pr), T) + 1j * coded_samples(len(pr), T, seed=17)
+    N = (12, 14, 16)
+    zero, inc = grid_basis(N)
+    vol = gpu.backprojection(H.astype(np.complex64), pr, N, zero, inc, 2.0, 0.001)
+    re = oracle.a1_volume_f64acc(H.real.astype(f32), pr, N, zero, inc, 2.0, 0.001)
+    im = oracle.a1_volume_f64acc(H.imag.astype(f32), pr, N, zero, inc, 2.0, 0.001)
+    assert vol.dtype == np.complex64
+    assert np.array_equal(vol.real, re.astype(f32)) and np.array_equal(vol.imag, im.astype(f32))
+
+
+@pytest.mark.parametrize("storage", ["f16", "bf16"])
+def test_half_storage_error_vs_fp64(gpu, oracle, storage):
+    cap = scenes.render(scenes.config_scene(1), 32, 4096, t0=2.0, seed=1001)
+    pr = oracle.precompute_pairs(cap.laser_position, cap.laser_grid, cap.camera_position,
+                                 cap.camera_grid, True)
+    H = cap.transient.reshape(len(pr), -1)
+    N = (24, 24, 24)
+    zero, inc = grid_basis(N)
+    vol = gpu.backprojection(H, pr, N, zero, inc, 2.0, 0.001, storage=storage)
+    v32 = gpu.backprojection(H, pr, N, zero, inc, 2.0, 0.001)
+    fp64 = oracle.fp64_volume(H, pr, N, zero, inc, 2.0, 0.001)
+    e16, e32 = rel_l2(vol, fp64), rel_l2(v32, fp64)
+    print(f"{storage}: rel L2 vs FP64 oracle {e16:.2e} (fp32 storage {e32:.2e}), "
+          f"vs fp32 storage {rel_l2(vol, v32):.2e}")
+    if storage == "f16" and H.max() <= 2048:
+        assert np.array_equal(vol, v32)      # integer counts <= 2048 are exact in fp16
+    assert rel_l2(vol, v32) < 4e-3
+
+
+# ----------------------------------------------------------------- CPU-path semantics
+@pytest.mark.parametrize("confocal", [True, False])
+def test_window_parity(gpu, oracle, confocal):
+    rng = np.random.default_rng(21)
+    if confocal:
+        pr = confocal_pairs(16)
+    else:
+        pr = oracle.precompute_pairs(scenes.ORIGIN, scenes.wall_grid(6), scenes.ORIGIN,
+                                     scenes.wall_grid(5), confocal=False)
+    T = 2600
+    H = rng.poisson(2.0, (len(pr), T)).astype(f32)
+    N = (20, 18, 16)
+    vp, vs = np.array([0, 0, 0.8], f32), np.array([1, 1, 1], f32)
+    vol = gpu.window_backprojection(H, pr, N, vp, vs, 2.0, 0.001)
+    ref = oracle.a2_volume(H, pr, N, vp, vs, 2.0, 0.001)
+    fp64 = oracle.a2_volume_fp64(H, pr, N, vp, vs, 2.0, 0.001)
+    print(f"window confocal={confocal}: rel L2 vs CPU path {rel_l2(vol, ref):.2e}, "
+          f"vs FP64 {rel_l2(vol, fp64):.2e}")
+    assert rel_l2(vol, ref) <= 1e-5
+
+
+# ----------------------------------------------------------------- phasor
+@pytest.mark.parametrize("wl", [0.004, 0.005, 0.04])
+def test_phasor_filter_bit_exact(gpu, oracle, wl):
+    rng = np.random.default_rng(3)
+    x = rng.poisson(4.0, (37, 1500)).astype(f32)
+    got = gpu.phasor_pulse(x, wl, 0.001)
+    want = oracle.phasor_filter(x, wl, 0.001)
+    assert np.array_equal(got, want)
+
+
+# ----------------------------------------------------------------- reference pipeline
+def test_nlosbpy_backproject_matches_oracle_pipeline(gpu, oracle):
+    cap = scenes.render(scenes.config_scene(1), 32, 2048, t0=2.0, seed=1001)
+    N = (24, 24, 24)
+    args = cap.nlos_args(N)
+    vol = gpu.nlosbpy.backproject(*args, np.zeros(3, f32), assume_row_major=True)
+    ref, inter = oracle.backproject_pipeline(cap.transient, cap.camera_grid, cap.laser_grid,
+                                             cap.camera_position, cap.laser_position, cap.t0,
+                                             cap.deltaT, True, cap.volume_position,
+                                             cap.volume_size, N)
+    exact = oracle.a1_volume_f64acc(inter["H"], inter["pairs"], N, inter["zero"], inter["inc"],
+                                    float(inter["t0"]), cap.deltaT)
+    assert rel_l2(vol, exact) <= 1e-6 and rel_l2(vol, ref) <= 1e-5
+
+
+def test_nlosbpy_column_major_equals_row_major(gpu):
+    cap = scenes.render(scenes.config_scene(1), 16, 2048, t0=2.0, seed=3)
+    N = (12, 12, 12)
+    a = gpu.nlosbpy.backproject(*cap.nlos_args(N), np.zeros(3), assume_row_major=True)
+    cm = (np.asfortranarray(cap.transient).T, cap.camera_grid.T, cap.laser_grid.T)
+    args = list(cap.nlos_args(N))
+    args[0], args[1], args[2] = np.ascontiguousarray(cm[0]), np.ascontiguousarray(cm[1]), np.ascontiguousarray(cm[2])
+    b = gpu.nlosbpy.backproject(*args, np.zeros(3), assume_row_major=False)
+    assert np.array_equal(a, b)
+
+
+def test_nlosbpy_use_cpu_semantics(gpu, oracle):
+    cap = scenes.render(scenes.config_scene(1), 16, 2048, t0=2.0, seed=7)
+    N = (16, 16, 16)
+    vol = gpu.nlosbpy.backproject(*cap.nlos_args(N), np.zeros(3), assume_row_major=True,
+                                  use_cpu=True)
+    ref, _ = oracle.backproject_pipeline(cap.transient, cap.camera_grid, cap.laser_grid,
+                                         cap.camera_position, cap.laser_position, cap.t0,
+                                         cap.deltaT, True, cap.volume_position, cap.volume_size,
+                                         N, use_cpu=True)
+    assert rel_l2(vol, ref) <= 1e-5
+
+
+def test_nlosbpy_exhaustive(gpu, oracle):
+    cap = scenes.render(scenes.config_scene(2), 6, 2048, t0=2.0, confocal=False, laser_side=5,
+                        seed=1002)
+    N = (16, 16, 16)
+    vol = gpu.nlosbpy.backproject(*cap.nlos_args(N), np.zeros(3), assume_row_major=True)
+    ref, inter = oracle.backproject_pipeline(cap.transient, cap.camera_grid, cap.laser_grid,
+                                             cap.camera_position, cap.laser_position, cap.t0,
+                                             cap.deltaT, False, cap.volume_position,
+                                             cap.volume_size, N)
+    assert rel_l2(vol, ref) <= 1e-5
+
+
+@pytest.mark.parametrize("use_cpu", [False, True])
+def test_nlosbpy_phasor(gpu, oracle, use_cpu):
+    cap = scenes.render(scenes.config_scene(1), 16, 2048, t0=2.0, seed=5)
+    N = (12, 12, 12)
+    args = list(cap.nlos_args(N))
+    vol = gpu.nlosbpy.phasor(*args, 0.005, assume_row_major=True, use_cpu=use_cpu)
+    ref, _ = oracle.phasor_pipeline(cap.transient, cap.camera_grid, cap.laser_grid,
+                                    cap.camera_position, cap.laser_position, cap.t0, cap.deltaT,
+                                    True, cap.volume_position, cap.volume_size, N, 0.005,
+                                    use_cpu=use_cpu)
+    assert vol.dtype == np.complex64
+    assert rel_l2(vol, ref) <= 1e-5
+
+
+# ----------------------------------------------------------------- edge cases
+def test_edge_cases(gpu, oracle):
+    pr = confocal_pairs(8)
+    T = 1024
+    H = coded_samples(len(pr), T)
+    N = (6, 6, 6)
+    zero, inc = grid_basis(N)
+    # all-zero samples
+    assert not gpu.backprojection(np.zeros_like(H), pr, N, zero, inc, 2.0, 0.001).any()
+    # volume whose paths all fall beyond the chunk -> zeros
+    assert not gpu.backprojection(H, pr, N, zero + f32(50), inc, 2.0, 0.001).any()
+    # non-finite pairs contribute nothing
+    bad = pr.copy()
+    bad["laser_point"][3] = np.nan
+    bad["camera_point"][3] = np.nan
+    bad["camera_wall"][5] = np.inf
+    keep = np.ones(len(pr), bool)
+    keep[[3, 5]] = False
+    vol = gpu.backprojection(H, bad, N, zero, inc, 2.0, 0.001)
+    ref = oracle.a1_volume_f64acc(H[keep], pr[keep], N, zero, inc, 2.0, 0.001)
+    assert np.array_equal(vol, ref.astype(f32))
+    # T = 1 and a single pair
+    v1 = gpu.backprojection(H[:1, :1], pr[:1], N, zero, inc, 2.0, 0.001)
+    r1 = oracle.a1_volume_f64acc(H[:1, :1], pr[:1], N, zero, inc, 2.0, 0.001)
+    assert np.array_equal(v1, r1.astype(f32))
+    # a 1-voxel volume
+    v2 = gpu.backprojection(H, pr, (1, 1, 1), zero, inc, 2.0, 0.001)
+    r2 = oracle.a1_volume_f64acc(H, pr, (1, 1, 1), zero, inc, 2.0, 0.001)
+    assert np.array_equal(v2, r2.astype(f32))
+
+
+def test_reference_pair_truncation_flag(gpu, oracle):
+    rng = np.random.default_rng(2)
+    pr = confocal_pairs(18)          # 324 pairs -> the reference uses 256
+    T = 1024
+    H = coded_samples(len(pr), T)
+    N = (8, 8, 8)
+    zero, inc = grid_basis(N)
+    vol = gpu.backprojection(H, pr, N, zero, inc, 2.0, 0.001, reference_pair_truncation=True)
+    ref = oracle.a1_volume(H, pr, N, zero, inc, 2.0, 0.001)
+    assert np.array_equal(vol, ref)
+    full = gpu.backprojection(H, pr, N, zero, inc, 2.0, 0.001)
+    assert not np.array_equal(full, vol)
+
+
+def test_determinism_and_plan_reuse(gpu):
+    pr = confocal_pairs(32)
+    rng = np.random.default_rng(0)
+    H = rng.poisson(3.0, (len(pr), 2048)).astype(f32)
+    N = (32, 32, 32)
+    zero, inc = grid_basis(N)
+    a = gpu.backprojection(H, pr, N, zero, inc, 2.0, 0.001)
+    with gpu.Plan(H, pr, N, zero, inc, 2.0, 0.001) as plan:
+        plan.execute()
+        b = plan.volume()
+        plan.execute()
+        plan.execute()
+        c = plan.volume()
+        info = plan.info()
+        t = plan.benchmark(1, 3)
+    assert np.array_equal(a, b) and np.array_equal(b, c)
+    assert info["path"] == "confocal" and info["pairs_used"] == len(pr)
+    assert t["gather_kernel_ms"] > 0 and t["table_kernel_ms"] > 0
