@@ -149,6 +149,62 @@ struct SmpCBF16 {
     }
 };
 
+// Fused layouts: sample and threshold interleaved so that ONE load at element j returns
+// the threshold S[j] and both candidate samples Hp[j] (bin j-1) and Hp[j+1] (bin j).
+//   F32      element 8 B  (Hp[j], S[j])                 load 12 B at 8j: Hp[j], S[j], Hp[j+1]
+//   F16/BF16 element 8 B  (S[j], Hp[j] | Hp[j+1] << 16) load  8 B at 8j
+//   C64      element 16 B (re[j], im[j], S[j], 0)        loads 16 B at 16j and 8 B at 16(j+1)
+//   CF16/CBF16 element 16 B (S[j], c[j], c[j+1], 0)      load 12 B at 16j
+// `pick` returns the sample of bin j-1+[s >= S[j]].
+struct FusedF32 {
+    static constexpr int kElemBytes = 8;
+    static constexpr int kSOffset = 1;  // float slot of S inside the element
+    using acc_t = float;
+    __device__ static __forceinline__ float pick(__amdgpu_buffer_rsrc_t r, uint32_t j, uint32_t soff, float s) {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b96(r, (int)(j * 8u), (int)soff, 0);
+        return s >= __uint_as_float(v[1]) ? __uint_as_float(v[2]) : __uint_as_float(v[0]);
+    }
+};
+template <bool BF>
+struct FusedH16 {
+    static constexpr int kElemBytes = 8;
+    static constexpr int kSOffset = 0;
+    using acc_t = float;
+    __device__ static __forceinline__ float cvt(uint32_t b) {
+        return BF ? __uint_as_float(b << 16) : (float)__builtin_bit_cast(_Float16, (unsigned short)b);
+    }
+    __device__ static __forceinline__ float pick(__amdgpu_buffer_rsrc_t r, uint32_t j, uint32_t soff, float s) {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)(j * 8u), (int)soff, 0);
+        const uint32_t h = s >= __uint_as_float(v[0]) ? (v[1] >> 16) : (v[1] & 0xffffu);
+        return cvt(h);
+    }
+};
+struct FusedC64 {
+    static constexpr int kElemBytes = 16;
+    static constexpr int kSOffset = 2;
+    using acc_t = float2;
+    __device__ static __forceinline__ float2 pick(__amdgpu_buffer_rsrc_t r, uint32_t j, uint32_t soff, float s) {
+        const auto a = __builtin_amdgcn_raw_buffer_load_b128(r, (int)(j * 16u), (int)soff, 0);
+        const auto b = __builtin_amdgcn_raw_buffer_load_b64(r, (int)(j * 16u + 16u), (int)soff, 0);
+        const bool up = s >= __uint_as_float(a[2]);
+        return make_float2(__uint_as_float(up ? b[0] : a[0]), __uint_as_float(up ? b[1] : a[1]));
+    }
+};
+template <bool BF>
+struct FusedC16 {
+    static constexpr int kElemBytes = 16;
+    static constexpr int kSOffset = 0;
+    using acc_t = float2;
+    __device__ static __forceinline__ float cvt(uint32_t b) {
+        return BF ? __uint_as_float(b << 16) : (float)__builtin_bit_cast(_Float16, (unsigned short)b);
+    }
+    __device__ static __forceinline__ float2 pick(__amdgpu_buffer_rsrc_t r, uint32_t j, uint32_t soff, float s) {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b96(r, (int)(j * 16u), (int)soff, 0);
+        const uint32_t c = s >= __uint_as_float(v[0]) ? v[2] : v[1];
+        return make_float2(cvt(c & 0xffffu), cvt(c >> 16));
+    }
+};
+
 // ------------------------------------------------------------------ accumulators
 __device__ __forceinline__ void acc_add(float& a, float x) { a = a + x; }
 __device__ __forceinline__ void acc_add(float2& a, float2 x) { a.x = a.x + x.x; a.y = a.y + x.y; }

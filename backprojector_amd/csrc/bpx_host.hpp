@@ -44,7 +44,7 @@ struct PlanImpl {
     float t0 = 0, dT = 1, scale = 1;
     Grid g{};
     Tiles tiles{};
-    bool sep = false, flat = false;
+    bool sep = false, flat = false, fused = false;
     float lz = 0;
     uint32_t kahan_block = 32;
     float inv2 = 0, inv = 0, c1 = 0;

@@ -101,7 +101,13 @@ struct WindowArgs {
 // ------------------------------------------------------------------ kernel launchers
 // All launchers enqueue on `stream` and return the hipError_t of the launch.
 hipError_t launch_stable(const float* pair8, uint32_t P, uint32_t Tc, float t0, float dT,
-                         float* S, uint32_t lds, hipStream_t stream);
+                         float* S, uint32_t pitch, uint32_t stride, uint32_t offset,
+                         hipStream_t stream);
+hipError_t launch_fused_pad(const float* src, uint32_t rows, uint32_t Tc, uint32_t src_pitch,
+                            void* dst, uint32_t lde, int format, float inv_scale, hipStream_t stream);
+bool variant_fused(int variant);
+int fused_elem_bytes(int format);
+int fused_s_offset(int format);
 hipError_t launch_ztable(int kind, float t0, float dT, float half_width, uint32_t n, float* Z,
                          hipStream_t stream);
 hipError_t launch_pad_rows(const float* src, uint32_t rows, uint32_t Tc, uint32_t src_pitch,
@@ -120,6 +126,7 @@ hipError_t launch_window(const WindowArgs& a, int complex_samples, hipStream_t s
 hipError_t launch_abs_max(const float* src, uint64_t n, float* result_dev, hipStream_t stream);
 
 // Voxels per lane (K) of each tuned variant; variant -1 picks the default.
+constexpr int kDefaultVariant = 5;  // fused layout, 2 voxels per lane (fastest measured)
 int variant_k(int variant);
 int variant_count();
 

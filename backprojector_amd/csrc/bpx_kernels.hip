@@ -29,9 +29,12 @@ struct StablePred {
     }
 };
 
+// S[p][j] is written at float index p*pitch + j*stride + offset (stride 1, offset 0 for a
+// plain table; the element stride/slot of a fused layout otherwise).
 __global__ __launch_bounds__(256) void stable_kernel(const float* __restrict__ pair8, uint32_t P,
                                                      uint32_t Tc, float t0, float dT,
-                                                     float* __restrict__ S, uint32_t lds) {
+                                                     float* __restrict__ S, uint32_t pitch,
+                                                     uint32_t stride, uint32_t offset) {
     const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t n = Tc + 1u;
     if (gid >= (uint64_t)P * n) return;
@@ -41,7 +44,7 @@ __global__ __launch_bounds__(256) void stable_kernel(const float* __restrict__ p
     const float d = 0.5f * (t0 + ((float)j - 0.5f) * dT - lw - cw);
     const float guess = d > 0.0f ? d * d : 0.0f;
     const uint32_t k = min_true_key(pred, fkey(guess), fkey(0.0f), fkey(__builtin_inff()));
-    S[(size_t)p * lds + j] = fkey_inv(k);
+    S[(size_t)p * pitch + (size_t)j * stride + offset] = fkey_inv(k);
 }
 
 // Path-length space: kind 0 = single tap (backproject_cuda.cu:155), kind 1 / 2 = start / end
@@ -72,11 +75,12 @@ __global__ __launch_bounds__(256) void ztable_kernel(int kind, float t0, float d
 }
 
 hipError_t launch_stable(const float* pair8, uint32_t P, uint32_t Tc, float t0, float dT,
-                         float* S, uint32_t lds, hipStream_t stream) {
+                         float* S, uint32_t pitch, uint32_t stride, uint32_t offset,
+                         hipStream_t stream) {
     const uint64_t n = (uint64_t)P * (Tc + 1u);
     const uint32_t blocks = (uint32_t)((n + 255u) / 256u);
     hipLaunchKernelGGL(stable_kernel, dim3(blocks), dim3(256), 0, stream, pair8, P, Tc, t0, dT, S,
-                       lds);
+                       pitch, stride, offset);
     return hipGetLastError();
 }
 
@@ -134,6 +138,60 @@ __global__ __launch_bounds__(256) void pad_rows_kernel(const float* __restrict__
         }
     }
     store_sample(dst, gid, format, re, im);
+}
+
+// Sample slots of the fused layouts (see FusedF32 etc.); threshold slots are zeroed here and
+// written afterwards by stable_kernel.
+__device__ __forceinline__ float padded_sample(const float* row, uint32_t i, uint32_t Tc, float inv_scale,
+                                               int comp, int cplx) {
+    if (i < 1u || i > Tc) return 0.0f;
+    return cplx ? row[2 * (i - 1u) + comp] * inv_scale : row[i - 1u] * inv_scale;
+}
+
+__global__ __launch_bounds__(256) void fused_pad_kernel(const float* __restrict__ src, uint32_t rows,
+                                                        uint32_t Tc, uint32_t src_pitch, void* dst,
+                                                        uint32_t lde, int format, float inv_scale) {
+    const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= (uint64_t)rows * lde) return;
+    const uint32_t p = (uint32_t)(gid / lde), j = (uint32_t)(gid % lde);
+    const int cplx = format_complex(format) ? 1 : 0;
+    const float* row = src + (size_t)p * src_pitch * (cplx ? 2u : 1u);
+    switch (format) {
+        case kF32:
+            static_cast<float2*>(dst)[gid] = make_float2(padded_sample(row, j, Tc, inv_scale, 0, 0), 0.0f);
+            break;
+        case kF16:
+        case kBF16: {
+            const float a = padded_sample(row, j, Tc, inv_scale, 0, 0);
+            const float b = padded_sample(row, j + 1u, Tc, inv_scale, 0, 0);
+            const uint32_t ha = format == kF16 ? f32_to_f16(a) : f32_to_bf16_rne(a);
+            const uint32_t hb = format == kF16 ? f32_to_f16(b) : f32_to_bf16_rne(b);
+            static_cast<uint2*>(dst)[gid] = make_uint2(0u, ha | (hb << 16));
+            break;
+        }
+        case kC64:
+            static_cast<float4*>(dst)[gid] = make_float4(padded_sample(row, j, Tc, inv_scale, 0, 1),
+                                                         padded_sample(row, j, Tc, inv_scale, 1, 1), 0.0f, 0.0f);
+            break;
+        default: {
+            const bool bf = format == kCBF16;
+            auto q = [&](float x) -> uint32_t { return bf ? f32_to_bf16_rne(x) : f32_to_f16(x); };
+            const uint32_t c0 = q(padded_sample(row, j, Tc, inv_scale, 0, 1)) |
+                                (q(padded_sample(row, j, Tc, inv_scale, 1, 1)) << 16);
+            const uint32_t c1 = q(padded_sample(row, j + 1u, Tc, inv_scale, 0, 1)) |
+                                (q(padded_sample(row, j + 1u, Tc, inv_scale, 1, 1)) << 16);
+            static_cast<uint4*>(dst)[gid] = make_uint4(0u, c0, c1, 0u);
+            break;
+        }
+    }
+}
+
+hipError_t launch_fused_pad(const float* src, uint32_t rows, uint32_t Tc, uint32_t src_pitch,
+                            void* dst, uint32_t lde, int format, float inv_scale, hipStream_t stream) {
+    const uint64_t n = (uint64_t)rows * lde;
+    hipLaunchKernelGGL(fused_pad_kernel, dim3((uint32_t)((n + 255u) / 256u)), dim3(256), 0, stream,
+                       src, rows, Tc, src_pitch, dst, lde, format, inv_scale);
+    return hipGetLastError();
 }
 
 hipError_t launch_pad_rows(const float* src, uint32_t rows, uint32_t Tc, uint32_t src_pitch,
@@ -318,9 +376,38 @@ __device__ __forceinline__ float2 scaled(float2 v, float s) { return make_float2
 // one hardware sqrt for the approximate quotient, a clamp, one table read, one compare and
 // the sample read.  SEP: voxels of a lane share x and y (inc[6] == inc[7] == 0).
 // FLAT: every wall point has the same z.
-template <class Smp, int K, bool SEP, bool FLAT>
-__global__ __launch_bounds__(256) void confocal_kernel(ConfocalArgs a) {
+// Sample sources of the confocal kernel.  Split: threshold table and padded rows are two
+// arrays (two dependent loads).  Fused: one interleaved array (one load, see FusedF32).
+template <class Smp>
+struct SplitSource {
     using acc_t = typename Smp::acc_t;
+    __amdgpu_buffer_rsrc_t hr, sr;
+    uint32_t ldh, lds, soff_h = 0, soff_s = 0;
+    __device__ explicit SplitSource(const ConfocalArgs& a)
+        : hr(make_rsrc(a.H, a.h_bytes)), sr(make_rsrc(a.S, a.s_bytes)), ldh(a.ldh), lds(a.lds) {}
+    __device__ __forceinline__ void set_pair(uint32_t p) {
+        soff_s = p * lds * 4u;
+        soff_h = p * ldh * (uint32_t)Smp::kBytes;
+    }
+    __device__ __forceinline__ acc_t fetch(uint32_t j, float s) const {
+        const float th = buf_f32(sr, j * 4u, soff_s);
+        const uint32_t hj = j + (s >= th ? 1u : 0u);
+        return Smp::load(hr, hj, soff_h);
+    }
+};
+template <class Fz>
+struct FusedSource {
+    using acc_t = typename Fz::acc_t;
+    __amdgpu_buffer_rsrc_t er;
+    uint32_t lde, soff = 0;
+    __device__ explicit FusedSource(const ConfocalArgs& a) : er(make_rsrc(a.H, a.h_bytes)), lde(a.ldh) {}
+    __device__ __forceinline__ void set_pair(uint32_t p) { soff = p * lde * (uint32_t)Fz::kElemBytes; }
+    __device__ __forceinline__ acc_t fetch(uint32_t j, float s) const { return Fz::pick(er, j, soff, s); }
+};
+
+template <class Src, int K, bool SEP, bool FLAT>
+__global__ __launch_bounds__(256) void confocal_kernel(ConfocalArgs a) {
+    using acc_t = typename Src::acc_t;
     const LaneVoxels lv = lane_voxels(a.t, K);
     const uint32_t cx = min(lv.ix, a.g.n[0] - 1u), cy = min(lv.iy, a.g.n[1] - 1u);
     float vx[SEP ? 1 : K], vy[SEP ? 1 : K], vz[K], dz2[K];
@@ -342,8 +429,7 @@ __global__ __launch_bounds__(256) void confocal_kernel(ConfocalArgs a) {
         const float dz = a.lz - v[2];
         dz2[r] = dz * dz;
     }
-    const auto hr = make_rsrc(a.H, a.h_bytes);
-    const auto sr = make_rsrc(a.S, a.s_bytes);
+    Src src(a);
     const float fTc = (float)a.Tc;
     acc_t sum[K], comp[K], blk[K];
 #pragma unroll
@@ -357,8 +443,7 @@ __global__ __launch_bounds__(256) void confocal_kernel(ConfocalArgs a) {
         const uint32_t pend = min(p + a.kahan_block, a.P);
         for (; p < pend; ++p) {
             const float4 pr = a.pairs[p];
-            const uint32_t soff_s = p * a.lds * 4u;
-            const uint32_t soff_h = p * a.ldh * (uint32_t)Smp::kBytes;
+            src.set_pair(p);
             float axy = 0.0f;
             if (SEP) {
                 const float dx = pr.x - vx[0], dy = pr.y - vy[0];
@@ -380,9 +465,7 @@ __global__ __launch_bounds__(256) void confocal_kernel(ConfocalArgs a) {
                 const float d = __builtin_amdgcn_sqrtf(s);
                 const float q1 = __builtin_fmaf(d, a.inv2, pr.w);
                 const uint32_t j = (uint32_t)(int)__builtin_amdgcn_fmed3f(q1, 0.0f, fTc);
-                const float th = buf_f32(sr, j * 4u, soff_s);
-                const uint32_t hj = j + (s >= th ? 1u : 0u);
-                acc_add(blk[r], Smp::load(hr, hj, soff_h));
+                acc_add(blk[r], src.fetch(j, s));
             }
         }
 #pragma unroll
@@ -618,44 +701,70 @@ hipError_t launch_window(const WindowArgs& a, int complex_samples, hipStream_t s
 }
 
 // ------------------------------------------------------------------- dispatch
-// Variants: 0 -> K = 4, 1 -> K = 8, 2 -> K = 2 voxels per lane.
+// Variants v = 3*layout + k: layout 0 = split tables, 1 = fused layout; k: 0 -> K = 4,
+// 1 -> K = 8, 2 -> K = 2 voxels per lane.
 int variant_k(int variant) {
-    switch (variant) {
+    switch (variant % 3) {
         case 1: return 8;
         case 2: return 2;
         default: return 4;
     }
 }
-int variant_count() { return 3; }
+bool variant_fused(int variant) { return variant >= 3; }
+int variant_count() { return 6; }
+int fused_elem_bytes(int format) {
+    switch (format) {
+        case kF32: case kF16: case kBF16: return 8;
+        default: return 16;
+    }
+}
+int fused_s_offset(int format) {
+    switch (format) {
+        case kF32: return FusedF32::kSOffset;
+        case kC64: return FusedC64::kSOffset;
+        default: return 0;
+    }
+}
 
-template <class Smp, int K>
+template <class Src, int K>
 static hipError_t confocal_dispatch_k(const ConfocalArgs& a, bool sep, bool flat, hipStream_t s) {
     const dim3 grid(a.t.count), block(256);
-    if (sep && flat) hipLaunchKernelGGL((confocal_kernel<Smp, K, true, true>), grid, block, 0, s, a);
-    else if (sep) hipLaunchKernelGGL((confocal_kernel<Smp, K, true, false>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((confocal_kernel<Smp, K, false, false>), grid, block, 0, s, a);
+    if (sep && flat) hipLaunchKernelGGL((confocal_kernel<Src, K, true, true>), grid, block, 0, s, a);
+    else if (sep) hipLaunchKernelGGL((confocal_kernel<Src, K, true, false>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((confocal_kernel<Src, K, false, false>), grid, block, 0, s, a);
     return hipGetLastError();
 }
 
-template <class Smp>
+template <class Src>
 static hipError_t confocal_dispatch(const ConfocalArgs& a, int variant, bool sep, bool flat,
                                     hipStream_t s) {
     switch (variant_k(variant)) {
-        case 8: return confocal_dispatch_k<Smp, 8>(a, sep, flat, s);
-        case 2: return confocal_dispatch_k<Smp, 2>(a, sep, flat, s);
-        default: return confocal_dispatch_k<Smp, 4>(a, sep, flat, s);
+        case 8: return confocal_dispatch_k<Src, 8>(a, sep, flat, s);
+        case 2: return confocal_dispatch_k<Src, 2>(a, sep, flat, s);
+        default: return confocal_dispatch_k<Src, 4>(a, sep, flat, s);
     }
 }
 
 hipError_t launch_confocal(const ConfocalArgs& a, int format, int variant, bool sep, bool flat,
                            hipStream_t stream) {
+    if (variant_fused(variant)) {
+        switch (format) {
+            case kF32: return confocal_dispatch<FusedSource<FusedF32>>(a, variant, sep, flat, stream);
+            case kF16: return confocal_dispatch<FusedSource<FusedH16<false>>>(a, variant, sep, flat, stream);
+            case kBF16: return confocal_dispatch<FusedSource<FusedH16<true>>>(a, variant, sep, flat, stream);
+            case kC64: return confocal_dispatch<FusedSource<FusedC64>>(a, variant, sep, flat, stream);
+            case kCF16: return confocal_dispatch<FusedSource<FusedC16<false>>>(a, variant, sep, flat, stream);
+            case kCBF16: return confocal_dispatch<FusedSource<FusedC16<true>>>(a, variant, sep, flat, stream);
+            default: return hipErrorInvalidValue;
+        }
+    }
     switch (format) {
-        case kF32: return confocal_dispatch<SmpF32>(a, variant, sep, flat, stream);
-        case kF16: return confocal_dispatch<SmpF16>(a, variant, sep, flat, stream);
-        case kBF16: return confocal_dispatch<SmpBF16>(a, variant, sep, flat, stream);
-        case kC64: return confocal_dispatch<SmpC64>(a, variant, sep, flat, stream);
-        case kCF16: return confocal_dispatch<SmpCF16>(a, variant, sep, flat, stream);
-        case kCBF16: return confocal_dispatch<SmpCBF16>(a, variant, sep, flat, stream);
+        case kF32: return confocal_dispatch<SplitSource<SmpF32>>(a, variant, sep, flat, stream);
+        case kF16: return confocal_dispatch<SplitSource<SmpF16>>(a, variant, sep, flat, stream);
+        case kBF16: return confocal_dispatch<SplitSource<SmpBF16>>(a, variant, sep, flat, stream);
+        case kC64: return confocal_dispatch<SplitSource<SmpC64>>(a, variant, sep, flat, stream);
+        case kCF16: return confocal_dispatch<SplitSource<SmpCF16>>(a, variant, sep, flat, stream);
+        case kCBF16: return confocal_dispatch<SplitSource<SmpCBF16>>(a, variant, sep, flat, stream);
         default: return hipErrorInvalidValue;
     }
 }

@@ -222,9 +222,11 @@ PlanImpl* make_plan_single_tap(const CaptureIn& c, const uint32_t n[3], const fl
              "path lengths / t0 too large relative to deltaT for the exact bin lookup (bound " +
                  std::to_string(bound) + ")");
 
-    pl->variant = o.variant < 0 ? 0 : o.variant;
+    pl->variant = o.variant < 0 ? kDefaultVariant : o.variant;
+    pl->fused = pl->path == kPathConfocal && variant_fused(pl->variant);
     pl->K = variant_k(pl->variant);
-    if (pl->path != kPathConfocal && pl->K == 8) pl->K = 4;
+    if (pl->path == kPathPairs) pl->K = 4;
+    if (pl->path == kPathGrid && pl->K == 8) pl->K = 4;
     pl->tiles = make_tiles(n, pl->K);
 
     if (pl->P == 0) return pl.release();
@@ -232,8 +234,10 @@ PlanImpl* make_plan_single_tap(const CaptureIn& c, const uint32_t n[3], const fl
     // samples: raw rows resident on the device; scale for 16-bit storage
     pl->scale = (pl->format == kF32 || pl->format == kC64) ? 1.0f : choose_scale(c.chunk, c.size * (c.cplx ? 2 : 1));
     stage_raw(*pl, c, idx);
-    pl->ldh = round_up((uint64_t)c.T + 2, 64);
-    const uint64_t hbytes = (uint64_t)pl->P * pl->ldh * format_bytes(pl->format);
+    // padded rows (split) or the fused sample+threshold layout, [P][ldh] elements
+    pl->ldh = round_up((uint64_t)c.T + 2, pl->fused ? 16 : 64);
+    const uint64_t hbytes = (uint64_t)pl->P * pl->ldh *
+                            (pl->fused ? fused_elem_bytes(pl->format) : format_bytes(pl->format));
     if (hbytes >= (1ull << 32)) fail(BPX_ERR_UNSUPPORTED, "transient rows above 4 GiB per plan");
     pl->H.alloc(hbytes);
 
@@ -244,10 +248,12 @@ PlanImpl* make_plan_single_tap(const CaptureIn& c, const uint32_t n[3], const fl
               "hipMemcpy(pairs)");
 
     if (pl->path == kPathConfocal) {
-        pl->lds = round_up((uint64_t)c.T + 1, 64);
-        const uint64_t sbytes = (uint64_t)pl->P * pl->lds * 4;
-        if (sbytes >= (1ull << 32)) fail(BPX_ERR_UNSUPPORTED, "threshold table above 4 GiB");
-        pl->S.alloc(sbytes);
+        if (!pl->fused) {
+            pl->lds = round_up((uint64_t)c.T + 1, 64);
+            const uint64_t sbytes = (uint64_t)pl->P * pl->lds * 4;
+            if (sbytes >= (1ull << 32)) fail(BPX_ERR_UNSUPPORTED, "threshold table above 4 GiB");
+            pl->S.alloc(sbytes);
+        }
         pl->inv2 = (float)(2.0 / (double)c.dT);
         std::vector<float> p4((size_t)pl->P * 4);
         bool flat = true;
@@ -338,14 +344,27 @@ void PlanImpl::execute(hipStream_t s) {
         return;
     }
     hip_check(hipEventRecord(ev[0], s), "hipEventRecord");
-    hip_check(launch_pad_rows(static_cast<const float*>(raw.ptr), P, T, T, H.ptr, ldh, format,
-                              1.0f / scale, s),
-              "row staging kernel");
+    if (fused)
+        hip_check(launch_fused_pad(static_cast<const float*>(raw.ptr), P, T, T, H.ptr, ldh, format,
+                                   1.0f / scale, s),
+                  "row staging kernel");
+    else
+        hip_check(launch_pad_rows(static_cast<const float*>(raw.ptr), P, T, T, H.ptr, ldh, format,
+                                  1.0f / scale, s),
+                  "row staging kernel");
     hip_check(hipEventRecord(ev[1], s), "hipEventRecord");
     if (path == kPathConfocal) {
-        hip_check(launch_stable(static_cast<const float*>(pair8.ptr), P, T, t0, dT,
-                                static_cast<float*>(S.ptr), lds, s),
-                  "threshold table kernel");
+        if (fused) {
+            const uint32_t stride = (uint32_t)fused_elem_bytes(format) / 4u;
+            hip_check(launch_stable(static_cast<const float*>(pair8.ptr), P, T, t0, dT,
+                                    static_cast<float*>(H.ptr), ldh * stride, stride,
+                                    (uint32_t)fused_s_offset(format), s),
+                      "threshold table kernel");
+        } else {
+            hip_check(launch_stable(static_cast<const float*>(pair8.ptr), P, T, t0, dT,
+                                    static_cast<float*>(S.ptr), lds, 1u, 0u, s),
+                      "threshold table kernel");
+        }
         hip_check(hipEventRecord(ev[2], s), "hipEventRecord");
         ConfocalArgs a{};
         a.H = H.ptr;
@@ -357,7 +376,7 @@ void PlanImpl::execute(hipStream_t s) {
         a.ldh = ldh;
         a.lds = lds;
         a.h_bytes = (uint32_t)H.bytes;
-        a.s_bytes = (uint32_t)S.bytes;
+        a.s_bytes = (uint32_t)(fused ? 0 : S.bytes);
         a.g = g;
         a.t = tiles;
         a.inv2 = inv2;

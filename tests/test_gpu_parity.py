@@ -71,7 +71,7 @@ def test_confocal_bins_bit_exact_all_variants_and_non_separable_basis(gpu, oracl
     N = (18, 17, 19)
     zero, inc = grid_basis(N)
     ref = oracle.a1_volume_f64acc(H, pr, N, zero, inc, 2.05, 0.001)
-    for v in (0, 1, 2):
+    for v in range(6):
         vol = gpu.backprojection(H, pr, N, zero, inc, 2.05, 0.001, variant=v)
         assert np.array_equal(vol, ref.astype(f32)), f"variant {v}"
     # a rotated basis (wall normal) makes the grid non-separable and the wall non-flat
@@ -80,8 +80,9 @@ def test_confocal_bins_bit_exact_all_variants_and_non_separable_basis(gpu, oracl
     pr2["laser_point"][:, 2] = np.linspace(0, 0.05, len(pr2)).astype(f32)
     pr2["camera_point"] = pr2["laser_point"]
     ref2 = oracle.a1_volume_f64acc(H, pr2, N, zero2, inc2, 2.05, 0.001)
-    vol2 = gpu.backprojection(H, pr2, N, zero2, inc2, 2.05, 0.001)
-    assert np.array_equal(vol2, ref2.astype(f32))
+    for v in (2, 5):
+        vol2 = gpu.backprojection(H, pr2, N, zero2, inc2, 2.05, 0.001, variant=v)
+        assert np.array_equal(vol2, ref2.astype(f32)), f"variant {v}"
 
 
 def test_round_half_away_ties_on_gpu(gpu, oracle):
@@ -171,23 +172,28 @@ def test_complex_bins_bit_exact(gpu, oracle):
     H = coded_samples(len(pr), T) + 1j * coded_samples(len(pr), T, seed=17)
     N = (12, 14, 16)
     zero, inc = grid_basis(N)
-    vol = gpu.backprojection(H.astype(np.complex64), pr, N, zero, inc, 2.0, 0.001)
     re = oracle.a1_volume_f64acc(H.real.astype(f32), pr, N, zero, inc, 2.0, 0.001)
     im = oracle.a1_volume_f64acc(H.imag.astype(f32), pr, N, zero, inc, 2.0, 0.001)
-    assert vol.dtype == np.complex64
-    assert np.array_equal(vol.real, re.astype(f32)) and np.array_equal(vol.imag, im.astype(f32))
+    for v in (2, 5):
+        for storage in ("f32", "f16", "bf16"):   # codes <= 61 are exact in every format
+            vol = gpu.backprojection(H.astype(np.complex64), pr, N, zero, inc, 2.0, 0.001,
+                                     variant=v, storage=storage)
+            assert vol.dtype == np.complex64
+            assert np.array_equal(vol.real, re.astype(f32)), (v, storage)
+            assert np.array_equal(vol.imag, im.astype(f32)), (v, storage)
 
 
 @pytest.mark.parametrize("storage", ["f16", "bf16"])
-def test_half_storage_error_vs_fp64(gpu, oracle, storage):
+@pytest.mark.parametrize("variant", [2, 5])
+def test_half_storage_error_vs_fp64(gpu, oracle, storage, variant):
     cap = scenes.render(scenes.config_scene(1), 32, 4096, t0=2.0, seed=1001)
     pr = oracle.precompute_pairs(cap.laser_position, cap.laser_grid, cap.camera_position,
                                  cap.camera_grid, True)
     H = cap.transient.reshape(len(pr), -1)
     N = (24, 24, 24)
     zero, inc = grid_basis(N)
-    vol = gpu.backprojection(H, pr, N, zero, inc, 2.0, 0.001, storage=storage)
-    v32 = gpu.backprojection(H, pr, N, zero, inc, 2.0, 0.001)
+    vol = gpu.backprojection(H, pr, N, zero, inc, 2.0, 0.001, storage=storage, variant=variant)
+    v32 = gpu.backprojection(H, pr, N, zero, inc, 2.0, 0.001, variant=variant)
     fp64 = oracle.fp64_volume(H, pr, N, zero, inc, 2.0, 0.001)
     e16, e32 = rel_l2(vol, fp64), rel_l2(v32, fp64)
     print(f"{storage}: rel L2 vs FP64 oracle {e16:.2e} (fp32 storage {e32:.2e}), "
