@@ -59,6 +59,7 @@ EXPORTED = [
     "bpx_window_backprojection", "bpx_phasor_pulse", "bpx_nlos_backproject", "bpx_nlos_phasor",
     "bpx_plan_create", "bpx_nlos_plan_create", "bpx_plan_execute", "bpx_plan_synchronize", "bpx_plan_read_volume",
     "bpx_plan_info", "bpx_plan_benchmark", "bpx_plan_destroy", "bpx_render_transient",
+    "bpx_copy_benchmark",
 ]
 
 
@@ -104,6 +105,7 @@ def lib():
     L.bpx_plan_destroy.restype = None
     L.bpx_render_transient.argtypes = [c_p, u32, u32, f32, f32, c_p, u64, f32, ctypes.c_int,
                                        u64, c_p]
+    L.bpx_copy_benchmark.argtypes = [u64, i32, ctypes.POINTER(ctypes.c_double)]
     _lib = L
     return L
 

@@ -26,6 +26,7 @@ SOURCES = [
     "bpx_api.cpp",
     "bpx_pipeline.cpp",
     "bpx_render.cpp",
+    "bpx_compat.cpp",
 ]
 
 COMMON = [

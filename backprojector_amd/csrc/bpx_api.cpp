@@ -247,6 +247,35 @@ void bpx_plan_destroy(bpx_plan plan) {
     delete plan;
 }
 
+bpx_status bpx_copy_benchmark(uint64_t bytes, int32_t iterations, double* mean_ms) {
+    return guarded([&] {
+        if (!mean_ms || iterations <= 0 || bytes < 16) fail(BPX_ERR_INVALID_ARGUMENT, "bad arguments");
+        if (!device_ready()) fail(BPX_ERR_NO_DEVICE, "no gfx950 device available");
+        bytes &= ~(uint64_t)15;
+        DeviceBuffer a, b;
+        a.alloc(bytes);
+        b.alloc(bytes);
+        hip_check(hipMemset(a.ptr, 0, bytes), "hipMemset");
+        hipEvent_t e0, e1;
+        hip_check(hipEventCreate(&e0), "hipEventCreate");
+        hip_check(hipEventCreate(&e1), "hipEventCreate");
+        hip_check(launch_copy(b.ptr, a.ptr, bytes, nullptr), "copy kernel");  // warm-up
+        double total = 0;
+        for (int i = 0; i < iterations; ++i) {
+            hip_check(hipEventRecord(e0, nullptr), "hipEventRecord");
+            hip_check(launch_copy(b.ptr, a.ptr, bytes, nullptr), "copy kernel");
+            hip_check(hipEventRecord(e1, nullptr), "hipEventRecord");
+            hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+            float ms = 0;
+            hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+            total += ms;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        *mean_ms = total / iterations;
+    });
+}
+
 bpx_status bpx_render_transient(const bpx_pair* pairs, uint32_t num_pairs, uint32_t T, float t0,
                                 float deltaT, const float* samples, uint64_t num_samples,
                                 float peak_count, int poisson, uint64_t seed, float* out) {

@@ -124,6 +124,7 @@ hipError_t launch_grid(const GridArgs& a, int format, int variant, hipStream_t s
 hipError_t launch_pairs(const PairsArgs& a, int format, int variant, hipStream_t stream);
 hipError_t launch_window(const WindowArgs& a, int complex_samples, hipStream_t stream);
 hipError_t launch_abs_max(const float* src, uint64_t n, float* result_dev, hipStream_t stream);
+hipError_t launch_copy(void* dst, const void* src, uint64_t bytes, hipStream_t stream);
 
 // Voxels per lane (K) of each tuned variant; variant -1 picks the default.
 constexpr int kDefaultVariant = 5;  // fused layout, 2 voxels per lane (fastest measured)

@@ -294,6 +294,22 @@ hipError_t launch_phasor_fir(const float* src, uint32_t rows, uint32_t Tc, const
     return hipGetLastError();
 }
 
+// Plain streaming copy (16 B per lane, grid-stride): the HBM bandwidth reference and the
+// calibration kernel of the FETCH_SIZE / WRITE_SIZE counters.
+__global__ __launch_bounds__(256) void copy_kernel(float4* __restrict__ dst, const float4* __restrict__ src,
+                                                   uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+hipError_t launch_copy(void* dst, const void* src, uint64_t bytes, hipStream_t stream) {
+    const uint64_t n = bytes / 16u;
+    hipLaunchKernelGGL(copy_kernel, dim3(256u * 32u), dim3(256), 0, stream, static_cast<float4*>(dst),
+                       static_cast<const float4*>(src), n);
+    return hipGetLastError();
+}
+
 // max |x| over n floats, as the bit pattern of a non-negative float (atomic max on uint).
 __global__ __launch_bounds__(256) void abs_max_kernel(const float* __restrict__ src, uint64_t n,
                                                       unsigned int* result) {

@@ -126,6 +126,16 @@ def main():
                          f"{dt:.1f} s on {nth} threads"}
         log(f"cpu baseline: {cpu}")
 
+    # ---- HBM traffic per launch of the gather kernel, from the committed rocprofv3 --pmc
+    # summary (FETCH_SIZE / WRITE_SIZE scaled by the copy-kernel calibration)
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "roofline_traffic.json")
+    key = f"confocal_{info['path']}_v{args.variant}_{args.storage}_{args.side}_{N[0]}_{args.T}"
+    if os.path.exists(tfile):
+        tab = json.load(open(tfile))
+        if key in tab:
+            traffic = tab[key]["hbm_bytes_per_launch"]
+
     out = {
         "metric": METRIC,
         "value": value,
@@ -145,7 +155,8 @@ def main():
                    "storage": args.storage, "kernel_path": info["path"],
                    "lane_voxels": info["lane_voxels"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_key": key,
                      "kernel": "confocal_kernel (gather)", "kernel_ms": gather_ms,
                      "table_kernel_ms": timing["table_kernel_ms"],
                      "bytes_per_update": sample_bytes},

@@ -196,6 +196,12 @@ typedef struct bpx_timing {
 bpx_status bpx_plan_benchmark(bpx_plan plan, int32_t warmup, int32_t iterations, bpx_timing* out);
 void bpx_plan_destroy(bpx_plan plan);
 
+/* Copies `bytes` (rounded down to 16) between two device buffers with a plain streaming copy
+ * kernel `iterations` times; reports the mean kernel time.  The measured HBM bandwidth
+ * reference and the calibration of the FETCH_SIZE / WRITE_SIZE counters (no reference
+ * counterpart). */
+bpx_status bpx_copy_benchmark(uint64_t bytes, int32_t iterations, double* mean_ms);
+
 /* ------------------------------------------------------------------ synthetic captures
  * Renders a transient capture of hidden Lambertian geometry (no reference counterpart).
  * samples: [n][7] = (x, y, z, nx, ny, nz, area) surface samples; normal (0,0,0) = isotropic
