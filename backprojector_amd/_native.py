@@ -48,6 +48,38 @@ class Timing(ctypes.Structure):
                 ("iterations", ctypes.c_int32)]
 
 
+class Pair(ctypes.Structure):
+    _fields_ = [("camera_point", ctypes.c_float * 3), ("laser_point", ctypes.c_float * 3),
+                ("camera_wall", ctypes.c_float), ("laser_wall", ctypes.c_float)]
+
+
+class NlosPrepared(ctypes.Structure):
+    _fields_ = [("chunk", ctypes.POINTER(ctypes.c_float)), ("rows", ctypes.c_uint64),
+                ("Tc", ctypes.c_uint32), ("pairs", ctypes.POINTER(Pair)),
+                ("num_pairs", ctypes.c_uint32), ("zero", ctypes.c_float * 3),
+                ("inc", ctypes.c_float * 9), ("chunked_t0", ctypes.c_float),
+                ("min_index", ctypes.c_float), ("max_index", ctypes.c_float)]
+
+
+class OctreeOptions(ctypes.Structure):
+    _fields_ = [("levels", ctypes.c_uint32), ("threshold", ctypes.c_float),
+                ("proposal_threshold", ctypes.c_float), ("flags", ctypes.c_uint32)]
+
+
+class OctreeStats(ctypes.Structure):
+    _fields_ = [("levels", ctypes.c_uint32), ("active", ctypes.c_uint32 * 9),
+                ("running_max", ctypes.c_double * 8), ("bound_max", ctypes.c_double * 8),
+                ("bound_norm", ctypes.c_double * 8), ("threshold", ctypes.c_double * 8),
+                ("fine_updates", ctypes.c_uint64), ("bound_updates", ctypes.c_uint64)]
+
+    def to_dict(self):
+        L = self.levels
+        return dict(levels=L, active=list(self.active[:L + 1]),
+                    running_max=list(self.running_max[:L]), bound_max=list(self.bound_max[:L]),
+                    bound_norm=list(self.bound_norm[:L]), threshold=list(self.threshold[:L]),
+                    fine_updates=int(self.fine_updates), bound_updates=int(self.bound_updates))
+
+
 FLAG_REFERENCE_PAIR_TRUNCATION = 0x1
 
 _lib = None
@@ -59,7 +91,8 @@ EXPORTED = [
     "bpx_window_backprojection", "bpx_phasor_pulse", "bpx_nlos_backproject", "bpx_nlos_phasor",
     "bpx_plan_create", "bpx_nlos_plan_create", "bpx_plan_execute", "bpx_plan_synchronize", "bpx_plan_read_volume",
     "bpx_plan_info", "bpx_plan_benchmark", "bpx_plan_destroy", "bpx_render_transient",
-    "bpx_copy_benchmark",
+    "bpx_copy_benchmark", "bpx_octree_backprojection", "bpx_octree_plan_create",
+    "bpx_octree_plan_stats", "bpx_nlos_prepare", "bpx_nlos_prepared_free",
 ]
 
 
@@ -106,6 +139,16 @@ def lib():
     L.bpx_render_transient.argtypes = [c_p, u32, u32, f32, f32, c_p, u64, f32, ctypes.c_int,
                                        u64, c_p]
     L.bpx_copy_benchmark.argtypes = [u64, i32, ctypes.POINTER(ctypes.c_double)]
+    L.bpx_octree_backprojection.argtypes = [c_p, u64, u32, c_p, u32, c_p, c_p, c_p, c_p, f32,
+                                            f32, c_p, c_p]
+    L.bpx_octree_plan_create.argtypes = [c_p, u64, u32, c_p, u32, c_p, c_p, c_p, f32, f32, c_p,
+                                         ctypes.POINTER(c_p)]
+    L.bpx_octree_plan_stats.argtypes = [c_p, c_p]
+    L.bpx_nlos_prepare.argtypes = [c_p, c_p, ctypes.c_int, c_p, c_p, c_p, c_p, c_p, c_p, f32, f32,
+                                   ctypes.c_int, c_p, c_p, c_p, c_p, ctypes.c_int,
+                                   ctypes.POINTER(NlosPrepared)]
+    L.bpx_nlos_prepared_free.argtypes = [ctypes.POINTER(NlosPrepared)]
+    L.bpx_nlos_prepared_free.restype = None
     _lib = L
     return L
 

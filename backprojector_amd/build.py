@@ -27,6 +27,8 @@ SOURCES = [
     "bpx_pipeline.cpp",
     "bpx_render.cpp",
     "bpx_compat.cpp",
+    "bpx_octree.hip",
+    "bpx_octree_plan.cpp",
 ]
 
 COMMON = [

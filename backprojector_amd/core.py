@@ -104,6 +104,25 @@ def render_transient(pairs, T, t0, deltaT, samples, peak_count=1000.0, poisson=T
     return out
 
 
+def octree_backprojection(transient_chunk, pairs, voxels_per_side, volume_zero_pos, voxel_inc,
+                          t0, deltaT, levels=4, threshold=0.25, proposal_threshold=0.5):
+    """Coarse-to-fine single-tap reconstruction (replaces the reference's non-functional
+    call_cuda_octree_backprojection, backproject_cuda.cu:503-669).  Returns (volume, stats)."""
+    H, n = _samples(transient_chunk, False)
+    pr = as_pairs(pairs)
+    P = len(pr)
+    T = (n // P) if P else 0
+    N = _u32(voxels_per_side).reshape(3)
+    out = np.zeros(tuple(int(x) for x in N), np.float32)
+    o = nat.OctreeOptions(int(levels), float(threshold), float(proposal_threshold), 0)
+    st = nat.OctreeStats()
+    check(lib().bpx_octree_backprojection(ptr(H), n, T, ptr(pr), P, ptr(out), ptr(N),
+                                          ptr(_f32(volume_zero_pos)),
+                                          ptr(_f32(voxel_inc).reshape(9)), float(t0),
+                                          float(deltaT), ctypes.byref(o), ctypes.byref(st)))
+    return out, st.to_dict()
+
+
 class Plan:
     """Device-resident capture + volume; execute() runs the whole reconstruction on the GPU."""
 
@@ -151,6 +170,31 @@ class Plan:
                                          ctypes.byref(h)))
         self._h = h
         return self
+
+    @classmethod
+    def octree(cls, transient_chunk, pairs, voxels_per_side, volume_zero_pos, voxel_inc, t0,
+               deltaT, levels=4, threshold=0.25, proposal_threshold=0.5):
+        """Device-resident octree plan (see octree_backprojection)."""
+        H, n = _samples(transient_chunk, False)
+        pr = as_pairs(pairs)
+        P = len(pr)
+        T = (n // P) if P else 0
+        self = cls.__new__(cls)
+        self.N = tuple(int(x) for x in _u32(voxels_per_side).reshape(3))
+        self.complex = False
+        o = nat.OctreeOptions(int(levels), float(threshold), float(proposal_threshold), 0)
+        h = ctypes.c_void_p()
+        check(lib().bpx_octree_plan_create(ptr(H), n, T, ptr(pr), P, ptr(_u32(self.N)),
+                                           ptr(_f32(volume_zero_pos)),
+                                           ptr(_f32(voxel_inc).reshape(9)), float(t0),
+                                           float(deltaT), ctypes.byref(o), ctypes.byref(h)))
+        self._h = h
+        return self
+
+    def octree_stats(self):
+        st = nat.OctreeStats()
+        check(lib().bpx_octree_plan_stats(self._h, ctypes.byref(st)))
+        return st.to_dict()
 
     def execute(self, stream=None):
         check(lib().bpx_plan_execute(self._h, stream))

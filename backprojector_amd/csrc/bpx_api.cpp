@@ -175,6 +175,69 @@ bpx_status bpx_plan_create(const float* transient_chunk, uint64_t transient_size
     });
 }
 
+static void fill_octree_stats(const PlanImpl& pl, bpx_octree_stats* s) {
+    std::memset(s, 0, sizeof(*s));
+    if (!pl.oct) return;
+    const OctreeState& st = *pl.oct;
+    s->levels = st.L;
+    for (uint32_t l = 0; l <= st.L && l < st.level_active.size(); ++l) s->active[l] = st.level_active[l];
+    for (uint32_t l = 0; l < st.L && 5 * l + 4 < st.level_stats.size(); ++l) {
+        s->running_max[l] = st.level_stats[5 * l + 0];
+        s->bound_max[l] = st.level_stats[5 * l + 1];
+        s->bound_norm[l] = std::sqrt(st.level_stats[5 * l + 2]);
+        s->threshold[l] = st.level_stats[5 * l + 4];
+    }
+    uint64_t b = 0;
+    for (uint32_t l = 0; l < st.L && l < st.level_active.size(); ++l) b += (uint64_t)st.level_active[l] * pl.P;
+    s->bound_updates = b;
+    s->fine_updates = st.level_active.size() > st.L ? (uint64_t)st.level_active[st.L] * pl.P : 0;
+}
+
+bpx_status bpx_octree_backprojection(const float* transient_chunk, uint64_t transient_size,
+                                     uint32_t T, const bpx_pair* scanned_pairs, uint32_t num_pairs,
+                                     float* voxel_volume, const uint32_t voxels_per_side[3],
+                                     const float volume_zero_pos[3], const float voxel_inc[9],
+                                     float t0, float deltaT, const bpx_octree_options* options,
+                                     bpx_octree_stats* stats) {
+    return guarded([&] {
+        if (!voxel_volume || !voxels_per_side || !volume_zero_pos || !voxel_inc)
+            fail(BPX_ERR_INVALID_ARGUMENT, "NULL volume/grid argument");
+        CaptureIn c{transient_chunk, transient_size, T, scanned_pairs, num_pairs, t0, deltaT, false};
+        std::unique_ptr<PlanImpl> pl(make_plan_octree(c, voxels_per_side, volume_zero_pos, voxel_inc, options));
+        pl->execute(pl->stream);
+        const size_t nvox = (size_t)pl->g.n[0] * pl->g.n[1] * pl->g.n[2];
+        hip_check(hipMemcpyAsync(voxel_volume, pl->vol.ptr, nvox * 4, hipMemcpyDeviceToHost, pl->stream),
+                  "hipMemcpyAsync(volume)");
+        hip_check(hipStreamSynchronize(pl->stream), "hipStreamSynchronize");
+        if (stats) fill_octree_stats(*pl, stats);
+    });
+}
+
+bpx_status bpx_octree_plan_create(const float* transient_chunk, uint64_t transient_size, uint32_t T,
+                                  const bpx_pair* scanned_pairs, uint32_t num_pairs,
+                                  const uint32_t voxels_per_side[3], const float volume_zero_pos[3],
+                                  const float voxel_inc[9], float t0, float deltaT,
+                                  const bpx_octree_options* options, bpx_plan* plan_out) {
+    return guarded([&] {
+        if (!plan_out || !voxels_per_side || !volume_zero_pos || !voxel_inc)
+            fail(BPX_ERR_INVALID_ARGUMENT, "NULL argument");
+        *plan_out = nullptr;
+        CaptureIn c{transient_chunk, transient_size, T, scanned_pairs, num_pairs, t0, deltaT, false};
+        auto p = std::make_unique<bpx_plan_s>();
+        p->impl.reset(make_plan_octree(c, voxels_per_side, volume_zero_pos, voxel_inc, options));
+        *plan_out = p.release();
+    });
+}
+
+bpx_status bpx_octree_plan_stats(bpx_plan plan, bpx_octree_stats* stats) {
+    return guarded([&] {
+        if (!plan || !stats) fail(BPX_ERR_INVALID_ARGUMENT, "NULL argument");
+        if (!plan->impl->oct) fail(BPX_ERR_INVALID_ARGUMENT, "not an octree plan");
+        hip_check(hipStreamSynchronize(plan->impl->stream), "hipStreamSynchronize");
+        fill_octree_stats(*plan->impl, stats);
+    });
+}
+
 bpx_status bpx_plan_execute(bpx_plan plan, void* hip_stream) {
     return guarded([&] {
         if (!plan) fail(BPX_ERR_INVALID_ARGUMENT, "NULL plan");

@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime_api.h>
 #include <stdint.h>
+#include <functional>
 #include <memory>
 #include <string>
 #include <vector>
@@ -24,14 +25,42 @@ void hip_check(hipError_t e, const char* what);
 bool device_ready();  // a gfx950 device is visible and selected
 
 // ------------------------------------------------------------------ plan
-enum Path : int { kPathConfocal = 0, kPathGrid = 1, kPathPairs = 2, kPathWindow = 3 };
+enum Path : int { kPathConfocal = 0, kPathGrid = 1, kPathPairs = 2, kPathWindow = 3, kPathOctree = 4 };
 
 struct DeviceBuffer {
     void* ptr = nullptr;
     size_t bytes = 0;
+    DeviceBuffer() = default;
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+    DeviceBuffer(DeviceBuffer&& o) noexcept : ptr(o.ptr), bytes(o.bytes) { o.ptr = nullptr; o.bytes = 0; }
     void alloc(size_t n);
     void release();
     ~DeviceBuffer() { release(); }
+};
+
+struct PlanImpl;
+
+// Octree refinement state (bpx_octree.hip kernels, bpx_octree_plan.cpp orchestration).
+struct OctreeState {
+    struct Level {
+        uint32_t B = 1, n[3] = {0, 0, 0}, W = 0;
+        double R = 0;
+        float cR = 0;
+        DeviceBuffer M;  // window-max table [P][ldm]
+    };
+    uint32_t L = 4, ldm = 0;
+    float tau = 0.25f, prob_lo = 0.5f;
+    std::vector<Level> levels;
+    DeviceBuffer ping, pong, list_a, list_b, U, V, flags, offs, sums, total, stats, partial;
+    uint32_t* host_total = nullptr;
+    double* host_stats = nullptr;
+    std::vector<uint32_t> level_active;   // blocks evaluated per level (+ fine voxels last)
+    std::vector<double> level_stats;      // 5 FP64 values per level (see oct_finish_kernel)
+    // Learned proposals: returns a device probability per active block, or nullptr.
+    std::function<const float*(PlanImpl&, uint32_t level, const uint32_t* blocks, uint32_t n,
+                               hipStream_t)> proposer;
+    ~OctreeState();
 };
 
 struct PlanImpl {
@@ -53,9 +82,12 @@ struct PlanImpl {
     bool all_confocal = false;
     DeviceBuffer raw, H, S, Z, Z2, pair8, pair4, lasers, spads, vol;
     hipEvent_t ev[8] = {};
+    std::unique_ptr<OctreeState> oct;
     ~PlanImpl();
     void execute(hipStream_t s);
 };
+
+void execute_octree(PlanImpl& pl, hipStream_t s);
 
 // Describes a capture given to a plan (host buffers).
 struct CaptureIn {
@@ -72,6 +104,8 @@ PlanImpl* make_plan_single_tap(const CaptureIn& c, const uint32_t n[3], const fl
                                const float inc[9], const bpx_options* opt);
 PlanImpl* make_plan_window(const CaptureIn& c, const uint32_t n[3], const float vpos[3],
                            const float vsize[3]);
+PlanImpl* make_plan_octree(const CaptureIn& c, const uint32_t n[3], const float zero[3],
+                           const float inc[9], const bpx_octree_options* o);
 
 }  // namespace bpx
 

@@ -98,7 +98,46 @@ struct WindowArgs {
     Tiles t;
 };
 
+// Octree level bound: one lane per block (packed x | y << 10 | z << 20).
+struct OctBoundArgs {
+    const float* M;         // window-max table of the level [P][ldm]
+    const void* E;          // fused fp32 sample+threshold rows [P][lde]
+    const float4* pairs;    // (x, y, z, c1)
+    const uint32_t* blocks;
+    uint32_t nblocks;
+    float* U;               // upper bound per block
+    float* V;               // exact value of the block's representative fine voxel
+    uint32_t P, Tc, ldm, lde, m_bytes, e_bytes;
+    Grid g;                 // fine grid
+    uint32_t B;             // block edge in fine voxels
+    float cR;               // window-start offset: -3 - 2R/deltaT
+    float inv2;
+};
+struct OctFineArgs {
+    const void* E;
+    const float4* pairs;
+    const uint32_t* voxels;  // packed fine voxel coordinates
+    uint32_t n;
+    float* out;
+    uint32_t P, Tc, lde, e_bytes, kahan_block;
+    Grid g;
+    float inv2;
+};
+
 // ------------------------------------------------------------------ kernel launchers
+hipError_t launch_winmax_double(const float* in, uint32_t in_pitch, float* out, uint32_t out_pitch,
+                                uint32_t rows, uint32_t Tc, uint32_t half, hipStream_t s);
+hipError_t launch_winmax_combine(const float* t2, uint32_t pitch2, float* out, uint32_t out_pitch,
+                                 uint32_t rows, uint32_t Tc, uint32_t shift, hipStream_t s);
+hipError_t launch_oct_bound(const OctBoundArgs& a, hipStream_t s);
+hipError_t launch_oct_reduce(const float* U, const float* V, uint32_t n, double* partial,
+                             uint32_t max_parts, double* stats, double tau, hipStream_t s);
+hipError_t launch_oct_select(const float* U, uint32_t n, const double* stats, const float* prob,
+                             float lo, uint32_t* flags, uint32_t* offs, uint32_t* sums, uint32_t* total,
+                             const uint32_t* blocks, uint32_t* children, hipStream_t s);
+hipError_t launch_oct_fine(const OctFineArgs& a, hipStream_t s);
+hipError_t launch_fill(float* p, uint64_t n, float v, hipStream_t s);
+hipError_t launch_oct_all_blocks(uint32_t nx, uint32_t ny, uint32_t nz, uint32_t* out, hipStream_t s);
 // All launchers enqueue on `stream` and return the hipError_t of the launch.
 hipError_t launch_stable(const float* pair8, uint32_t P, uint32_t Tc, float t0, float dT,
                          float* S, uint32_t pitch, uint32_t stride, uint32_t offset,

@@ -1,0 +1,382 @@
+// Octree coarse-to-fine backprojection (confocal captures, fp32 samples).
+//
+// Level l of L refinements holds blocks of B = 2^(L-l) fine voxels per axis.  For a block the
+// kernels compute
+//   U = sum_p max{ H[p][b] : b in the block's bin window for pair p }   (>= every fine voxel's
+//       value inside the block when samples are non-negative: a sound upper bound), and
+//   V = the exact single-tap value of one fine voxel of the block (a lower bound of the max).
+// The bin window comes from the block's bounding sphere (centre m, radius R): every fine voxel
+// v in the block has |w - v| in [|w - m| - R, |w - m| + R], so its bin lies in a window of
+// fixed width W_l starting at floor(2(|w - m| - R)/dT + c) - margin.  A per-level sliding
+// window maximum table M_l[p][b] = max(H[p][b .. b + W_l - 1], 0) turns the window max into one
+// read.  Running max and L2 norm of each level are reduced in FP64; blocks with
+// U >= tau * max_so_far are refined.  The finest level is computed exactly (same fused
+// sample+threshold layout and the same per-update arithmetic as the dense kernel).
+#include "bpx_device.hpp"
+#include "bpx_internal.hpp"
+
+namespace bpx {
+
+// ------------------------------------------------------------------ sliding-window max
+// out[p][b] = max(0, max(in[p][b .. b + w - 1])) for b < Tc, out[p][Tc] = 0; samples beyond
+// the row count as 0 (an out-of-range bin contributes nothing).  in_pitch / out_pitch in
+// floats.  Built by doubling: width 2w from two width-w tables.
+__global__ __launch_bounds__(256) void winmax_double_kernel(const float* __restrict__ in, uint32_t in_pitch,
+                                                            float* __restrict__ out, uint32_t out_pitch,
+                                                            uint32_t rows, uint32_t Tc, uint32_t half) {
+    const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = Tc + 1u;
+    if (gid >= (uint64_t)rows * n) return;
+    const uint32_t p = (uint32_t)(gid / n), b = (uint32_t)(gid % n);
+    const float* r = in + (size_t)p * in_pitch;
+    float m = 0.0f;
+    if (b < Tc) {
+        m = fmaxf(m, r[b]);
+        if (b + half < Tc) m = fmaxf(m, r[b + half]);
+    }
+    out[(size_t)p * out_pitch + b] = m;
+}
+
+// out[b] = max(t2[b], t2[b + w - p2]) where t2 is the width-p2 table (p2 <= w < 2 p2).
+__global__ __launch_bounds__(256) void winmax_combine_kernel(const float* __restrict__ t2, uint32_t pitch2,
+                                                             float* __restrict__ out, uint32_t out_pitch,
+                                                             uint32_t rows, uint32_t Tc, uint32_t shift) {
+    const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = Tc + 1u;
+    if (gid >= (uint64_t)rows * n) return;
+    const uint32_t p = (uint32_t)(gid / n), b = (uint32_t)(gid % n);
+    const float* r = t2 + (size_t)p * pitch2;
+    float m = r[b];
+    const uint32_t b2 = b + shift;
+    if (b2 < Tc) m = fmaxf(m, r[b2]);
+    out[(size_t)p * out_pitch + b] = m;
+}
+
+hipError_t launch_winmax_double(const float* in, uint32_t in_pitch, float* out, uint32_t out_pitch,
+                                uint32_t rows, uint32_t Tc, uint32_t half, hipStream_t s) {
+    const uint64_t n = (uint64_t)rows * (Tc + 1u);
+    hipLaunchKernelGGL(winmax_double_kernel, dim3((uint32_t)((n + 255u) / 256u)), dim3(256), 0, s, in,
+                       in_pitch, out, out_pitch, rows, Tc, half);
+    return hipGetLastError();
+}
+
+hipError_t launch_winmax_combine(const float* t2, uint32_t pitch2, float* out, uint32_t out_pitch,
+                                 uint32_t rows, uint32_t Tc, uint32_t shift, hipStream_t s) {
+    const uint64_t n = (uint64_t)rows * (Tc + 1u);
+    hipLaunchKernelGGL(winmax_combine_kernel, dim3((uint32_t)((n + 255u) / 256u)), dim3(256), 0, s, t2,
+                       pitch2, out, out_pitch, rows, Tc, shift);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ level bounds
+__device__ __forceinline__ void unpack_block(uint32_t c, uint32_t& x, uint32_t& y, uint32_t& z) {
+    x = c & 1023u;
+    y = (c >> 10) & 1023u;
+    z = c >> 20;
+}
+
+// One lane per block; pairs in the wave-uniform loop (pair data in SGPRs).
+__global__ __launch_bounds__(256) void oct_bound_kernel(OctBoundArgs a) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t ii = min(i, a.nblocks - 1u);
+    uint32_t bx, by, bz;
+    unpack_block(a.blocks[ii], bx, by, bz);
+    // block centre (double precision, then float) and representative fine voxel
+    const double h = 0.5 * (double)(a.B - 1u);
+    float m[3], v[3];
+    for (int k = 0; k < 3; ++k)
+        m[k] = (float)((double)a.g.zero[k] + a.g.inc[k] * ((double)bx * a.B + h) +
+                       a.g.inc[3 + k] * ((double)by * a.B + h) + a.g.inc[6 + k] * ((double)bz * a.B + h));
+    const uint32_t rep = a.B / 2u;
+    {
+        const float fx = (float)(bx * a.B + rep), fy = (float)(by * a.B + rep), fz = (float)(bz * a.B + rep);
+        for (int k = 0; k < 3; ++k) {
+            float t = a.g.zero[k] + a.g.inc[k] * fx;
+            t = t + a.g.inc[3 + k] * fy;
+            v[k] = t + a.g.inc[6 + k] * fz;
+        }
+    }
+    const auto mr = make_rsrc(a.M, a.m_bytes);
+    const auto er = make_rsrc(a.E, a.e_bytes);
+    const float fTc = (float)a.Tc;
+    float U = 0.0f, Vs = 0.0f;
+    for (uint32_t p = 0; p < a.P; ++p) {
+        const float4 pr = a.pairs[p];
+        // window start for the bounding sphere
+        const float sm = sq3(pr.x - m[0], pr.y - m[1], pr.z - m[2]);
+        const float dm = __builtin_amdgcn_sqrtf(sm);
+        const float qlo = __builtin_fmaf(dm, a.inv2, pr.w + a.cR);
+        const uint32_t blo = (uint32_t)(int)__builtin_amdgcn_fmed3f(qlo, 0.0f, fTc);
+        U = U + buf_f32(mr, blo * 4u, p * a.ldm * 4u);
+        // exact tap of the representative fine voxel
+        const float s = sq3(pr.x - v[0], pr.y - v[1], pr.z - v[2]);
+        const float q1 = __builtin_fmaf(__builtin_amdgcn_sqrtf(s), a.inv2, pr.w);
+        const uint32_t j = (uint32_t)(int)__builtin_amdgcn_fmed3f(q1, 0.0f, fTc);
+        Vs = Vs + FusedF32::pick(er, j, p * a.lde * 8u, s);
+    }
+    if (i < a.nblocks) {
+        a.U[i] = U;
+        a.V[i] = Vs;
+    }
+}
+
+hipError_t launch_oct_bound(const OctBoundArgs& a, hipStream_t s) {
+    if (a.nblocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(oct_bound_kernel, dim3((a.nblocks + 255u) / 256u), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ FP64 reductions
+// stats[0] = running max of V (kept across levels), stats[1] = max of U, stats[2] = sum U^2,
+// stats[3] = sum V^2, stats[4] = threshold tau * stats[0] (as used by the selection).
+__global__ __launch_bounds__(256) void oct_reduce_kernel(const float* __restrict__ U, const float* __restrict__ V,
+                                                         uint32_t n, double* __restrict__ partial) {
+    __shared__ double sh[4][256];
+    double mv = -INFINITY, mu = -INFINITY, su = 0.0, sv = 0.0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double u = U[i], v = V[i];
+        mv = fmax(mv, v);
+        mu = fmax(mu, u);
+        su += u * u;
+        sv += v * v;
+    }
+    sh[0][threadIdx.x] = mv;
+    sh[1][threadIdx.x] = mu;
+    sh[2][threadIdx.x] = su;
+    sh[3][threadIdx.x] = sv;
+    __syncthreads();
+    for (uint32_t o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            sh[0][threadIdx.x] = fmax(sh[0][threadIdx.x], sh[0][threadIdx.x + o]);
+            sh[1][threadIdx.x] = fmax(sh[1][threadIdx.x], sh[1][threadIdx.x + o]);
+            sh[2][threadIdx.x] += sh[2][threadIdx.x + o];
+            sh[3][threadIdx.x] += sh[3][threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 4; ++k) partial[4 * blockIdx.x + k] = sh[k][0];
+}
+
+__global__ __launch_bounds__(256) void oct_finish_kernel(const double* __restrict__ partial, uint32_t nparts,
+                                                         double* __restrict__ stats, double tau) {
+    __shared__ double sh[4][256];
+    double mv = -INFINITY, mu = -INFINITY, su = 0.0, sv = 0.0;
+    for (uint32_t i = threadIdx.x; i < nparts; i += blockDim.x) {
+        mv = fmax(mv, partial[4 * i]);
+        mu = fmax(mu, partial[4 * i + 1]);
+        su += partial[4 * i + 2];
+        sv += partial[4 * i + 3];
+    }
+    sh[0][threadIdx.x] = mv;
+    sh[1][threadIdx.x] = mu;
+    sh[2][threadIdx.x] = su;
+    sh[3][threadIdx.x] = sv;
+    __syncthreads();
+    for (uint32_t o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            sh[0][threadIdx.x] = fmax(sh[0][threadIdx.x], sh[0][threadIdx.x + o]);
+            sh[1][threadIdx.x] = fmax(sh[1][threadIdx.x], sh[1][threadIdx.x + o]);
+            sh[2][threadIdx.x] += sh[2][threadIdx.x + o];
+            sh[3][threadIdx.x] += sh[3][threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        stats[0] = fmax(stats[0], sh[0][0]);
+        stats[1] = sh[1][0];
+        stats[2] = sh[2][0];
+        stats[3] = sh[3][0];
+        stats[4] = tau * stats[0];
+    }
+}
+
+hipError_t launch_oct_reduce(const float* U, const float* V, uint32_t n, double* partial,
+                             uint32_t max_parts, double* stats, double tau, hipStream_t s) {
+    uint32_t parts = (n + 255u) / 256u;
+    if (parts > max_parts) parts = max_parts;
+    if (parts == 0) parts = 1;
+    hipLaunchKernelGGL(oct_reduce_kernel, dim3(parts), dim3(256), 0, s, U, V, n, partial);
+    hipLaunchKernelGGL(oct_finish_kernel, dim3(1), dim3(256), 0, s, partial, parts, stats, tau);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ selection
+// flags[i] = refine block i: U >= threshold (fixed rule) or, with a proposal map, the learned
+// decision (prob >= hi) or low confidence (lo < prob < hi) falls back to refinement.
+__global__ __launch_bounds__(256) void oct_flag_kernel(const float* __restrict__ U, uint32_t n,
+                                                       const double* __restrict__ stats,
+                                                       const float* __restrict__ prob, float lo,
+                                                       uint32_t* __restrict__ flags) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    // the sound bound always applies; a learned proposal can only prune further
+    uint32_t f = (double)U[i] >= stats[4] ? 1u : 0u;
+    if (prob && !(prob[i] > lo)) f = 0u;
+    flags[i] = f;
+}
+
+// Exclusive scan of flags in chunks of 1024 (one work-group each), chunk totals in sums.
+__global__ __launch_bounds__(1024) void scan_chunk_kernel(const uint32_t* __restrict__ flags, uint32_t n,
+                                                          uint32_t* __restrict__ offs, uint32_t* __restrict__ sums) {
+    __shared__ uint32_t sh[1024];
+    const uint32_t i = blockIdx.x * 1024u + threadIdx.x;
+    const uint32_t f = i < n ? flags[i] : 0u;
+    sh[threadIdx.x] = f;
+    __syncthreads();
+    for (uint32_t o = 1; o < 1024u; o <<= 1) {
+        const uint32_t t = threadIdx.x >= o ? sh[threadIdx.x - o] : 0u;
+        __syncthreads();
+        sh[threadIdx.x] += t;
+        __syncthreads();
+    }
+    if (i < n) offs[i] = sh[threadIdx.x] - f;
+    if (threadIdx.x == 1023u) sums[blockIdx.x] = sh[1023];
+}
+
+__global__ __launch_bounds__(1024) void scan_sums_kernel(uint32_t* __restrict__ sums, uint32_t nchunks,
+                                                         uint32_t* __restrict__ total) {
+    __shared__ uint32_t carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    __shared__ uint32_t sh[1024];
+    for (uint32_t base = 0; base < nchunks; base += 1024u) {
+        const uint32_t i = base + threadIdx.x;
+        const uint32_t v = i < nchunks ? sums[i] : 0u;
+        sh[threadIdx.x] = v;
+        __syncthreads();
+        for (uint32_t o = 1; o < 1024u; o <<= 1) {
+            const uint32_t t = threadIdx.x >= o ? sh[threadIdx.x - o] : 0u;
+            __syncthreads();
+            sh[threadIdx.x] += t;
+            __syncthreads();
+        }
+        if (i < nchunks) sums[i] = carry + sh[threadIdx.x] - v;
+        __syncthreads();
+        if (threadIdx.x == 1023u) carry += sh[1023];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *total = carry;
+}
+
+// Children of flagged blocks, in parent order (8 per parent, z fastest).
+__global__ __launch_bounds__(256) void oct_children_kernel(const uint32_t* __restrict__ blocks,
+                                                           const uint32_t* __restrict__ flags,
+                                                           const uint32_t* __restrict__ offs,
+                                                           const uint32_t* __restrict__ sums, uint32_t n,
+                                                           uint32_t* __restrict__ children) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !flags[i]) return;
+    const uint32_t o = sums[i / 1024u] + offs[i];
+    uint32_t x, y, z;
+    unpack_block(blocks[i], x, y, z);
+    for (uint32_t c = 0; c < 8u; ++c) {
+        const uint32_t cx = 2u * x + ((c >> 2) & 1u), cy = 2u * y + ((c >> 1) & 1u), cz = 2u * z + (c & 1u);
+        children[8u * o + c] = cx | (cy << 10) | (cz << 20);
+    }
+}
+
+hipError_t launch_oct_select(const float* U, uint32_t n, const double* stats, const float* prob,
+                             float lo, uint32_t* flags, uint32_t* offs, uint32_t* sums, uint32_t* total,
+                             const uint32_t* blocks, uint32_t* children, hipStream_t s) {
+    if (n == 0) return hipMemsetAsync(total, 0, sizeof(uint32_t), s);
+    const uint32_t chunks = (n + 1023u) / 1024u;
+    hipLaunchKernelGGL(oct_flag_kernel, dim3((n + 255u) / 256u), dim3(256), 0, s, U, n, stats, prob, lo, flags);
+    hipLaunchKernelGGL(scan_chunk_kernel, dim3(chunks), dim3(1024), 0, s, flags, n, offs, sums);
+    hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(1024), 0, s, sums, chunks, total);
+    hipLaunchKernelGGL(oct_children_kernel, dim3((n + 255u) / 256u), dim3(256), 0, s, blocks, flags, offs, sums,
+                       n, children);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ finest level
+// Exact single-tap values of a list of fine voxels (K per lane), scattered into the volume.
+template <int K>
+__global__ __launch_bounds__(256) void oct_fine_kernel(OctFineArgs a) {
+    const uint32_t base = (blockIdx.x * blockDim.x + threadIdx.x) * K;
+    float vx[K], vy[K], vz[K];
+    uint32_t ids[K];
+#pragma unroll
+    for (int r = 0; r < K; ++r) {
+        const uint32_t idx = min(base + r, a.n - 1u);
+        uint32_t x, y, z;
+        unpack_block(a.voxels[idx], x, y, z);
+        float v[3];
+        const float fx = (float)x, fy = (float)y, fz = (float)z;
+        for (int k = 0; k < 3; ++k) {
+            float t = a.g.zero[k] + a.g.inc[k] * fx;
+            t = t + a.g.inc[3 + k] * fy;
+            v[k] = t + a.g.inc[6 + k] * fz;
+        }
+        vx[r] = v[0];
+        vy[r] = v[1];
+        vz[r] = v[2];
+        ids[r] = (x * a.g.n[1] + y) * a.g.n[2] + z;
+    }
+    const auto er = make_rsrc(a.E, a.e_bytes);
+    const float fTc = (float)a.Tc;
+    float sum[K], comp[K], blk[K];
+#pragma unroll
+    for (int r = 0; r < K; ++r) sum[r] = comp[r] = blk[r] = 0.0f;
+    uint32_t p = 0;
+    while (p < a.P) {
+        const uint32_t pend = min(p + a.kahan_block, a.P);
+        for (; p < pend; ++p) {
+            const float4 pr = a.pairs[p];
+            const uint32_t soff = p * a.lde * 8u;
+#pragma unroll
+            for (int r = 0; r < K; ++r) {
+                const float s = sq3(pr.x - vx[r], pr.y - vy[r], pr.z - vz[r]);
+                const float q1 = __builtin_fmaf(__builtin_amdgcn_sqrtf(s), a.inv2, pr.w);
+                const uint32_t j = (uint32_t)(int)__builtin_amdgcn_fmed3f(q1, 0.0f, fTc);
+                blk[r] = blk[r] + FusedF32::pick(er, j, soff, s);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < K; ++r) {
+            kahan_add(sum[r], comp[r], blk[r]);
+            blk[r] = 0.0f;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < K; ++r)
+        if (base + r < a.n) a.out[ids[r]] = kahan_result(sum[r], comp[r]);
+}
+
+hipError_t launch_oct_fine(const OctFineArgs& a, hipStream_t s) {
+    if (a.n == 0) return hipSuccess;
+    constexpr int K = 2;
+    const uint32_t lanes = (a.n + K - 1) / K;
+    hipLaunchKernelGGL(oct_fine_kernel<K>, dim3((lanes + 255u) / 256u), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void fill_kernel(float* __restrict__ p, uint64_t n, float v) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+hipError_t launch_fill(float* p, uint64_t n, float v, hipStream_t s) {
+    uint64_t blocks = (n + 255u) / 256u;
+    if (blocks > 8192u) blocks = 8192u;
+    if (blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(fill_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
+// Dense list of every block of a level grid (row-major, z fastest).
+__global__ __launch_bounds__(256) void oct_all_blocks_kernel(uint32_t nx, uint32_t ny, uint32_t nz,
+                                                             uint32_t* __restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nx * ny * nz) return;
+    const uint32_t z = i % nz, y = (i / nz) % ny, x = i / (nz * ny);
+    out[i] = x | (y << 10) | (z << 20);
+}
+
+hipError_t launch_oct_all_blocks(uint32_t nx, uint32_t ny, uint32_t nz, uint32_t* out, hipStream_t s) {
+    const uint32_t n = nx * ny * nz;
+    hipLaunchKernelGGL(oct_all_blocks_kernel, dim3((n + 255u) / 256u), dim3(256), 0, s, nx, ny, nz, out);
+    return hipGetLastError();
+}
+
+}  // namespace bpx

@@ -1,0 +1,197 @@
+// Host side of the octree plan: level geometry, window-max tables, per-level bound ->
+// FP64 reduction -> selection -> children, then the exact finest level.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+
+#include "bpx_host.hpp"
+
+namespace bpx {
+
+static uint32_t round_up32(uint64_t x, uint32_t m) { return (uint32_t)((x + m - 1) / m * m); }
+
+OctreeState::~OctreeState() {
+    if (host_total) (void)hipHostFree(host_total);
+    if (host_stats) (void)hipHostFree(host_stats);
+}
+
+// Half-diagonal of a block of B fine voxels: max over corner sign patterns of
+// |0.5 (B-1) (+-r0 +- r1 +- r2)| with r_k the rows of the grid increment matrix.
+static double block_radius(const Grid& g, uint32_t B) {
+    double best = 0.0;
+    for (int m = 0; m < 4; ++m) {
+        const double s1 = (m & 1) ? -1.0 : 1.0, s2 = (m & 2) ? -1.0 : 1.0;
+        double v[3];
+        for (int k = 0; k < 3; ++k) v[k] = (double)g.inc[k] + s1 * g.inc[3 + k] + s2 * g.inc[6 + k];
+        best = std::max(best, std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]));
+    }
+    return 0.5 * (double)(B - 1u) * best;
+}
+
+PlanImpl* make_plan_octree(const CaptureIn& c, const uint32_t n[3], const float zero[3],
+                           const float inc[9], const bpx_octree_options* oo) {
+    bpx_octree_options o{4u, 0.25f, 0.5f, 0u};
+    if (oo) o = *oo;
+    if (o.levels == 0 || o.levels > 8) fail(BPX_ERR_INVALID_ARGUMENT, "octree levels must be 1..8");
+    if (!(o.threshold >= 0.0f) || !std::isfinite(o.threshold))
+        fail(BPX_ERR_INVALID_ARGUMENT, "octree threshold must be finite and >= 0");
+    for (int k = 0; k < 3; ++k) {
+        if (n[k] == 0 || n[k] % (1u << o.levels) != 0)
+            fail(BPX_ERR_INVALID_ARGUMENT, "voxels_per_side must be divisible by 2^levels");
+        if (n[k] > 1024u) fail(BPX_ERR_UNSUPPORTED, "octree volumes up to 1024 voxels per side");
+    }
+    if (c.cplx) fail(BPX_ERR_UNSUPPORTED, "the octree path takes real samples");
+    for (uint32_t p = 0; p < c.P; ++p)
+        if (c.pairs && std::memcmp(c.pairs[p].laser_point, c.pairs[p].camera_point, 12) != 0)
+            fail(BPX_ERR_UNSUPPORTED, "the octree path takes confocal captures");
+    bpx_options po{BPX_STORAGE_F32, 32u, 0u, 5};  // fused layout, 2 voxels per lane
+    std::unique_ptr<PlanImpl> pl(make_plan_single_tap(c, n, zero, inc, &po));
+    auto st = std::make_unique<OctreeState>();
+    st->L = o.levels;
+    st->tau = o.threshold;
+    st->prob_lo = o.proposal_threshold;
+    for (uint32_t l = 0; l < st->L; ++l) {
+        OctreeState::Level lv;
+        lv.B = 1u << (st->L - l);
+        for (int k = 0; k < 3; ++k) lv.n[k] = n[k] / lv.B;
+        lv.R = block_radius(pl->g, lv.B);
+        lv.W = (uint32_t)std::ceil(4.0 * lv.R / (double)c.dT) + 8u;
+        lv.cR = (float)(-3.0 - 2.0 * lv.R / (double)c.dT);
+        st->levels.push_back(std::move(lv));
+    }
+    const uint32_t P = pl->P, Tc = pl->T;
+    st->ldm = round_up32((uint64_t)Tc + 1, 64);
+    const uint64_t mbytes = (uint64_t)std::max<uint32_t>(P, 1) * st->ldm * 4;
+    if (mbytes >= (1ull << 32)) fail(BPX_ERR_UNSUPPORTED, "window-max tables above 4 GiB per level");
+    for (uint32_t l = 0; l < st->L; ++l) st->levels[l].M.alloc(mbytes);
+    st->ping.alloc(mbytes);
+    st->pong.alloc(mbytes);
+    const uint64_t nvox = (uint64_t)n[0] * n[1] * n[2];
+    st->list_a.alloc(4 * nvox);
+    st->list_b.alloc(4 * nvox);
+    const uint64_t max_level = std::max<uint64_t>(nvox / 8, 1);
+    st->U.alloc(4 * max_level);
+    st->V.alloc(4 * max_level);
+    st->flags.alloc(4 * max_level);
+    st->offs.alloc(4 * max_level);
+    st->sums.alloc(4 * ((max_level + 1023) / 1024 + 1));
+    st->total.alloc(sizeof(uint32_t));
+    st->stats.alloc(8 * sizeof(double));
+    st->partial.alloc(4 * sizeof(double) * 1024);
+    hip_check(hipHostMalloc(&st->host_total, sizeof(uint32_t) * 2, hipHostMallocDefault), "hipHostMalloc");
+    hip_check(hipHostMalloc(&st->host_stats, sizeof(double) * 8, hipHostMallocDefault), "hipHostMalloc");
+    pl->oct = std::move(st);
+    pl->path = kPathOctree;
+    return pl.release();
+}
+
+// Builds each level's window-max table from the raw rows by doubling widths.
+static void build_winmax(PlanImpl& pl, hipStream_t s) {
+    OctreeState& st = *pl.oct;
+    const uint32_t P = pl.P, Tc = pl.T;
+    uint32_t wmax = 1;
+    for (const auto& lv : st.levels) wmax = std::max(wmax, lv.W);
+    float* cur = static_cast<float*>(st.ping.ptr);
+    float* nxt = static_cast<float*>(st.pong.ptr);
+    // width 1: max(0, H[b])
+    hip_check(launch_winmax_double(static_cast<const float*>(pl.raw.ptr), Tc, cur, st.ldm, P, Tc, 0u, s),
+              "window-max kernel");
+    uint32_t w = 1;
+    for (;;) {
+        for (auto& lv : st.levels)
+            if (lv.W >= w && lv.W < 2u * w)
+                hip_check(launch_winmax_combine(cur, st.ldm, static_cast<float*>(lv.M.ptr), st.ldm, P, Tc,
+                                                lv.W - w, s),
+                          "window-max combine kernel");
+        if (2u * w > wmax) break;
+        hip_check(launch_winmax_double(cur, st.ldm, nxt, st.ldm, P, Tc, w, s), "window-max kernel");
+        std::swap(cur, nxt);
+        w *= 2u;
+    }
+}
+
+void execute_octree(PlanImpl& pl, hipStream_t s) {
+    OctreeState& st = *pl.oct;
+    const size_t nvox = (size_t)pl.g.n[0] * pl.g.n[1] * pl.g.n[2];
+    st.level_active.assign(st.L + 1, 0);
+    st.level_stats.assign(5 * st.L, 0.0);
+    hip_check(launch_fill(static_cast<float*>(pl.vol.ptr), nvox, 0.0f, s), "fill kernel");
+    if (pl.P == 0) return;
+    hip_check(hipEventRecord(pl.ev[0], s), "hipEventRecord");
+    hip_check(launch_fused_pad(static_cast<const float*>(pl.raw.ptr), pl.P, pl.T, pl.T, pl.H.ptr, pl.ldh,
+                               pl.format, 1.0f / pl.scale, s),
+              "row staging kernel");
+    hip_check(launch_stable(static_cast<const float*>(pl.pair8.ptr), pl.P, pl.T, pl.t0, pl.dT,
+                            static_cast<float*>(pl.H.ptr), pl.ldh * 2u, 2u, 1u, s),
+              "threshold table kernel");
+    build_winmax(pl, s);
+    hip_check(hipEventRecord(pl.ev[1], s), "hipEventRecord");
+    hip_check(hipMemsetAsync(st.stats.ptr, 0, 8 * sizeof(double), s), "hipMemsetAsync");
+    uint32_t* cur = static_cast<uint32_t*>(st.list_a.ptr);
+    uint32_t* nxt = static_cast<uint32_t*>(st.list_b.ptr);
+    const OctreeState::Level& l0 = st.levels[0];
+    uint32_t n = l0.n[0] * l0.n[1] * l0.n[2];
+    hip_check(launch_oct_all_blocks(l0.n[0], l0.n[1], l0.n[2], cur, s), "block list kernel");
+    for (uint32_t l = 0; l < st.L; ++l) {
+        const OctreeState::Level& lv = st.levels[l];
+        st.level_active[l] = n;
+        if (n == 0) break;
+        OctBoundArgs a{};
+        a.M = static_cast<const float*>(lv.M.ptr);
+        a.E = pl.H.ptr;
+        a.pairs = static_cast<const float4*>(pl.pair4.ptr);
+        a.blocks = cur;
+        a.nblocks = n;
+        a.U = static_cast<float*>(st.U.ptr);
+        a.V = static_cast<float*>(st.V.ptr);
+        a.P = pl.P;
+        a.Tc = pl.T;
+        a.ldm = st.ldm;
+        a.lde = pl.ldh;
+        a.m_bytes = (uint32_t)lv.M.bytes;
+        a.e_bytes = (uint32_t)pl.H.bytes;
+        a.g = pl.g;
+        a.B = lv.B;
+        a.cR = lv.cR;
+        a.inv2 = pl.inv2;
+        hip_check(launch_oct_bound(a, s), "octree bound kernel");
+        hip_check(launch_oct_reduce(a.U, a.V, n, static_cast<double*>(st.partial.ptr), 1024u,
+                                    static_cast<double*>(st.stats.ptr), (double)st.tau, s),
+                  "octree FP64 reduction");
+        const float* prob = nullptr;
+        if (st.proposer) prob = st.proposer(pl, l, cur, n, s);
+        hip_check(launch_oct_select(a.U, n, static_cast<const double*>(st.stats.ptr), prob, st.prob_lo,
+                                    static_cast<uint32_t*>(st.flags.ptr), static_cast<uint32_t*>(st.offs.ptr),
+                                    static_cast<uint32_t*>(st.sums.ptr), static_cast<uint32_t*>(st.total.ptr),
+                                    cur, nxt, s),
+                  "octree selection");
+        hip_check(hipMemcpyAsync(st.host_total, st.total.ptr, sizeof(uint32_t), hipMemcpyDeviceToHost, s),
+                  "hipMemcpyAsync");
+        hip_check(hipMemcpyAsync(st.host_stats, st.stats.ptr, 5 * sizeof(double), hipMemcpyDeviceToHost, s),
+                  "hipMemcpyAsync");
+        hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+        for (int k = 0; k < 5; ++k) st.level_stats[5 * l + k] = st.host_stats[k];
+        n = 8u * st.host_total[0];
+        std::swap(cur, nxt);
+    }
+    st.level_active[st.L] = n;
+    hip_check(hipEventRecord(pl.ev[2], s), "hipEventRecord");
+    OctFineArgs f{};
+    f.E = pl.H.ptr;
+    f.pairs = static_cast<const float4*>(pl.pair4.ptr);
+    f.voxels = cur;
+    f.n = n;
+    f.out = static_cast<float*>(pl.vol.ptr);
+    f.P = pl.P;
+    f.Tc = pl.T;
+    f.lde = pl.ldh;
+    f.e_bytes = (uint32_t)pl.H.bytes;
+    f.kahan_block = pl.kahan_block;
+    f.g = pl.g;
+    f.inv2 = pl.inv2;
+    hip_check(launch_oct_fine(f, s), "octree fine kernel");
+    hip_check(hipEventRecord(pl.ev[3], s), "hipEventRecord");
+}
+
+}  // namespace bpx

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <numeric>
@@ -421,6 +422,58 @@ bpx_status bpx_nlos_plan_create(const float* transient_data, const int64_t* data
     } catch (const std::bad_alloc&) {
         return set_error(BPX_ERR_OUT_OF_MEMORY, "host allocation failed");
     }
+}
+
+bpx_status bpx_nlos_prepare(const float* transient_data, const int64_t* data_shape, int data_ndim,
+                            const float* camera_grid_positions, const int64_t* camera_grid_shape,
+                            const float* laser_grid_positions, const int64_t* laser_grid_shape,
+                            const float camera_position[3], const float laser_position[3],
+                            float t0, float deltaT, int is_confocal,
+                            const float volume_position[3], const float volume_size[3],
+                            const uint32_t voxels_per_side[3], const float wall_normal[3],
+                            int assume_row_major, bpx_nlos_prepared* out) {
+    clear_error();
+    try {
+        if (!out) fail(BPX_ERR_INVALID_ARGUMENT, "NULL output");
+        std::memset(out, 0, sizeof(*out));
+        PipelineGeometry g = prepare(transient_data, data_shape, data_ndim, camera_grid_positions,
+                                     camera_grid_shape, laser_grid_positions, laser_grid_shape,
+                                     camera_position, laser_position, t0, deltaT, is_confocal,
+                                     volume_position, volume_size, voxels_per_side, assume_row_major);
+        uint32_t Tc = 0;
+        std::vector<float> chunk =
+            pipeline::transient_chunk(g.arr.data_p, g.rows, g.T, g.min_idx, g.max_idx, &Tc);
+        const float zeros3[3] = {0, 0, 0};
+        pipeline::basis(volume_position, volume_size, voxels_per_side, wall_normal ? wall_normal : zeros3,
+                        out->zero, out->inc);
+        out->chunk = static_cast<float*>(std::malloc(sizeof(float) * std::max<size_t>(chunk.size(), 1)));
+        out->pairs = static_cast<bpx_pair*>(std::malloc(sizeof(bpx_pair) * std::max<size_t>(g.pairs.size(), 1)));
+        if (!out->chunk || !out->pairs) {
+            bpx_nlos_prepared_free(out);
+            fail(BPX_ERR_OUT_OF_MEMORY, "host allocation failed");
+        }
+        std::memcpy(out->chunk, chunk.data(), sizeof(float) * chunk.size());
+        std::memcpy(out->pairs, g.pairs.data(), sizeof(bpx_pair) * g.pairs.size());
+        out->rows = g.rows;
+        out->Tc = Tc;
+        out->num_pairs = (uint32_t)g.pairs.size();
+        out->chunked_t0 = t0 + g.min_idx * deltaT;
+        out->min_index = g.min_idx;
+        out->max_index = g.max_idx;
+        return BPX_OK;
+    } catch (const Failure& e) {
+        return set_error(e.status, e.msg);
+    } catch (const std::bad_alloc&) {
+        return set_error(BPX_ERR_OUT_OF_MEMORY, "host allocation failed");
+    }
+}
+
+void bpx_nlos_prepared_free(bpx_nlos_prepared* p) {
+    if (!p) return;
+    std::free(p->chunk);
+    std::free(p->pairs);
+    p->chunk = nullptr;
+    p->pairs = nullptr;
 }
 
 bpx_status bpx_nlos_phasor(const float* transient_data, const int64_t* data_shape, int data_ndim,

@@ -305,6 +305,10 @@ PlanImpl* make_plan_single_tap(const CaptureIn& c, const uint32_t n[3], const fl
 }
 
 void PlanImpl::execute(hipStream_t s) {
+    if (path == kPathOctree) {
+        execute_octree(*this, s);
+        return;
+    }
     const size_t nvox = (size_t)g.n[0] * g.n[1] * g.n[2];
     if (path == kPathWindow) {
         if (P == 0 || T < 2) {

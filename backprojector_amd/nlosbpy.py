@@ -76,6 +76,36 @@ def backproject(transient_data, camera_grid_positions, laser_grid_positions, cam
     return out
 
 
+def prepare(transient_data, camera_grid_positions, laser_grid_positions, camera_position,
+            laser_position, t0, deltaT, is_confocal, volume_position, volume_size,
+            voxels_per_side, wall_normal=(0, 0, 0), assume_row_major=True):
+    """The reference pipeline's intermediates before the device call (backproject.hpp:779-847):
+    dict(chunk [rows][Tc] float32, pairs, zero, inc, chunked_t0, min_index, max_index)."""
+    import ctypes as C
+    from ._native import PAIR_DTYPE, NlosPrepared
+    H, cg, lg, cp, lp, vp, vs, N = _prep(transient_data, camera_grid_positions,
+                                         laser_grid_positions, camera_position, laser_position,
+                                         volume_position, volume_size, voxels_per_side)
+    wn = _vec3(wall_normal, "wall_normal")
+    dshape, cshape, lshape = _shape(H), _shape(cg), _shape(lg)
+    out = NlosPrepared()
+    check(lib().bpx_nlos_prepare(ptr(H), ptr(dshape), H.ndim, ptr(cg), ptr(cshape), ptr(lg),
+                                 ptr(lshape), ptr(cp), ptr(lp), float(t0), float(deltaT),
+                                 1 if is_confocal else 0, ptr(vp), ptr(vs), ptr(N), ptr(wn),
+                                 1 if assume_row_major else 0, C.byref(out)))
+    try:
+        rows, Tc = int(out.rows), int(out.Tc)
+        chunk = np.ctypeslib.as_array(out.chunk, shape=(rows * Tc,)).reshape(rows, Tc).copy()
+        praw = np.ctypeslib.as_array(C.cast(out.pairs, C.POINTER(C.c_uint8)),
+                                     shape=(int(out.num_pairs) * 32,)).copy()
+        pairs = praw.view(PAIR_DTYPE)
+        return dict(chunk=chunk, pairs=pairs, zero=np.array(out.zero[:], np.float32),
+                    inc=np.array(out.inc[:], np.float32), chunked_t0=np.float32(out.chunked_t0),
+                    min_index=np.float32(out.min_index), max_index=np.float32(out.max_index))
+    finally:
+        lib().bpx_nlos_prepared_free(C.byref(out))
+
+
 def phasor(transient_data, camera_grid_positions, laser_grid_positions, camera_position,
            laser_position, t0, deltaT, is_confocal, volume_position, volume_size,
            voxels_per_side, wavelength, assume_row_major=False, use_cpu=False):

@@ -42,7 +42,38 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="target CPU time of the bounded cpu_baseline sample (0 = skip)")
     ap.add_argument("--check-voxels", type=int, default=1024)
+    ap.add_argument("--octree", type=int, default=256,
+                    help="fine volume side of the octree-vs-plain comparison (0 = skip)")
+    ap.add_argument("--octree-tau", type=float, default=0.25)
     return ap.parse_args()
+
+
+OCCUPIED_FRACTION = 0.25   # occupied voxels: plain value >= 0.25 * max (error is measured there)
+
+
+def octree_comparison(bp, cap, side, tau, reps=3):
+    """Plain dense reconstruction vs octree refinement at side^3 on the same capture."""
+    N = (side,) * 3
+    prep = bp.nlosbpy.prepare(*cap.nlos_args(N))
+    args = (prep["chunk"], prep["pairs"], N, prep["zero"], prep["inc"], float(prep["chunked_t0"]),
+            cap.deltaT)
+    with bp.Plan(*args) as plain:
+        tp = plain.benchmark(1, reps)
+        vp = plain.volume()
+    with bp.Plan.octree(*args, levels=4, threshold=tau) as oct_:
+        to = oct_.benchmark(1, reps)
+        vo = oct_.volume()
+        st = oct_.octree_stats()
+    occ = vp >= OCCUPIED_FRACTION * vp.max()
+    err = float(np.linalg.norm((vo - vp)[occ]) / max(np.linalg.norm(vp[occ]), 1e-300))
+    upd_plain = len(prep["pairs"]) * side ** 3
+    return {"side": side, "levels": 4, "tau": tau, "plain_ms": tp["per_exec_ms"],
+            "octree_ms": to["per_exec_ms"], "speedup": tp["per_exec_ms"] / to["per_exec_ms"],
+            "rel_l2_occupied": err, "occupied_voxels": int(occ.sum()),
+            "occupied_definition": f"plain value >= {OCCUPIED_FRACTION} x max",
+            "fine_voxels_computed": st["active"][-1], "active_per_level": st["active"],
+            "work_fraction": (st["fine_updates"] + st["bound_updates"]) / upd_plain,
+            "fp64_running_max": st["running_max"], "learned": False}
 
 
 def log(*a):
@@ -126,6 +157,12 @@ def main():
                          f"{dt:.1f} s on {nth} threads"}
         log(f"cpu baseline: {cpu}")
 
+    plan.close()
+    octree = None
+    if args.octree > 0:
+        octree = octree_comparison(bp, cap, args.octree, args.octree_tau)
+        log(f"octree: {octree}")
+
     # ---- HBM traffic per launch of the gather kernel, from the committed rocprofv3 --pmc
     # summary (FETCH_SIZE / WRITE_SIZE scaled by the copy-kernel calibration)
     traffic = None
@@ -162,10 +199,10 @@ def main():
                      "bytes_per_update": sample_bytes},
         "cpu_baseline": cpu,
         "check": check,
+        "octree": octree,
         "baseline_ref": "Quadro P5000 5993 ms (README.md:106) -> 5.73e9 updates/s",
     }
     print(json.dumps(out), flush=True)
-    plan.close()
 
 
 if __name__ == "__main__":

@@ -29,6 +29,12 @@ void call_cuda_backprojection(const float* transient_chunk, uint32_t transient_s
                               const float* volume_zero_pos, const float* voxel_inc, float t0,
                               float deltaT);
 
+void call_cuda_octree_backprojection(const float* transient_chunk, uint32_t transient_size,
+                                     uint32_t T, const std::vector<CameraLaserPair> scanned_pairs,
+                                     float* voxel_volume, const uint32_t* voxels_per_side,
+                                     const float* volume_zero_pos, const float* voxel_inc,
+                                     float t0, float deltaT);
+
 void call_cuda_complex_backprojection(const std::complex<float>* transient_chunk,
                                       uint32_t transient_size, uint32_t T,
                                       const std::vector<CameraLaserPair> scanned_pairs,

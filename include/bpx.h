@@ -152,6 +152,69 @@ bpx_status bpx_nlos_phasor(const float* transient_data, const int64_t* data_shap
                            const uint32_t voxels_per_side[3], float wavelength,
                            int assume_row_major, int use_cpu, float* complex_volume_out);
 
+/* Intermediates of the reference pipeline up to the device call (backproject.hpp:779-847):
+ * the transient chunk [rows][Tc], the pair table, the GPU-path basis and chunked t0.  Arrays
+ * are allocated by the library; release them with bpx_nlos_prepared_free. */
+typedef struct bpx_nlos_prepared {
+    float* chunk;
+    uint64_t rows;
+    uint32_t Tc;
+    bpx_pair* pairs;
+    uint32_t num_pairs;
+    float zero[3];
+    float inc[9];
+    float chunked_t0;
+    float min_index, max_index;
+} bpx_nlos_prepared;
+bpx_status bpx_nlos_prepare(const float* transient_data, const int64_t* data_shape, int data_ndim,
+                            const float* camera_grid_positions, const int64_t* camera_grid_shape,
+                            const float* laser_grid_positions, const int64_t* laser_grid_shape,
+                            const float camera_position[3], const float laser_position[3],
+                            float t0, float deltaT, int is_confocal,
+                            const float volume_position[3], const float volume_size[3],
+                            const uint32_t voxels_per_side[3], const float wall_normal[3],
+                            int assume_row_major, bpx_nlos_prepared* out);
+void bpx_nlos_prepared_free(bpx_nlos_prepared* p);
+
+/* ------------------------------------------------------------------ octree refinement
+ * Coarse-to-fine reconstruction of the single-tap GPU path for confocal captures (the
+ * reference's octree, backproject_cuda.cu:183-367, 503-669, is non-functional and prints
+ * "GPU OCTREE BP NOT WORKING YET"; this is a new design).  The fine grid N (each side
+ * divisible by 2^levels) is reached from N / 2^levels by `levels` refinements.  Per level,
+ * every active block gets a sound upper bound U of its fine voxels' values (window maximum
+ * of each pair's samples over the block's bin window, summed over pairs) and the exact value
+ * of one representative fine voxel; the running maximum and the level's L2 norm are reduced
+ * in FP64.  A block is refined iff U >= threshold * running max (and, with a learned
+ * proposer attached, iff its proposal probability exceeds proposal_threshold).  Finest-level
+ * voxels are computed exactly (bit-identical taps to bpx_backprojection); voxels never
+ * reached are 0.  With samples >= 0 every voxel whose value is >= threshold * max is
+ * computed exactly. */
+typedef struct bpx_octree_options {
+    uint32_t levels;          /* refinements (default 4: 16^3 -> 256^3)                  */
+    float threshold;          /* tau (default 0.25)                                      */
+    float proposal_threshold; /* learned proposals: refine iff probability > this (0.5) */
+    uint32_t flags;           /* reserved, 0                                             */
+} bpx_octree_options;
+
+typedef struct bpx_octree_stats {
+    uint32_t levels;
+    uint32_t active[9];       /* blocks evaluated at levels 0..levels-1, fine voxels last */
+    double running_max[8];    /* FP64 running max of exact representative values        */
+    double bound_max[8];      /* FP64 max of the level's upper bounds                   */
+    double bound_norm[8];     /* FP64 L2 norm of the level's upper bounds               */
+    double threshold[8];      /* threshold applied at each level                        */
+    uint64_t fine_updates;    /* voxel-pair updates of the exact finest level           */
+    uint64_t bound_updates;   /* block-pair bound evaluations of all coarse levels       */
+} bpx_octree_stats;
+
+/* Same arguments as call_cuda_octree_backprojection (backproject_cuda.hpp:30-39). */
+bpx_status bpx_octree_backprojection(const float* transient_chunk, uint64_t transient_size,
+                                     uint32_t T, const bpx_pair* scanned_pairs, uint32_t num_pairs,
+                                     float* voxel_volume, const uint32_t voxels_per_side[3],
+                                     const float volume_zero_pos[3], const float voxel_inc[9],
+                                     float t0, float deltaT, const bpx_octree_options* options,
+                                     bpx_octree_stats* stats);
+
 /* ------------------------------------------------------------------ device-resident plans
  * A plan owns the device copy of one capture (samples staged once into the padded layout
  * and storage format) and its output volume; bpx_plan_execute runs the whole per-capture
@@ -175,6 +238,14 @@ bpx_status bpx_nlos_plan_create(const float* transient_data, const int64_t* data
                                 const uint32_t voxels_per_side[3], const float wall_normal[3],
                                 int assume_row_major, const bpx_options* options,
                                 bpx_plan* plan_out);
+/* Octree plan (see bpx_octree_backprojection); bpx_plan_execute runs the refinement. */
+bpx_status bpx_octree_plan_create(const float* transient_chunk, uint64_t transient_size, uint32_t T,
+                                  const bpx_pair* scanned_pairs, uint32_t num_pairs,
+                                  const uint32_t voxels_per_side[3], const float volume_zero_pos[3],
+                                  const float voxel_inc[9], float t0, float deltaT,
+                                  const bpx_octree_options* options, bpx_plan* plan_out);
+/* Statistics of the plan's last octree execution. */
+bpx_status bpx_octree_plan_stats(bpx_plan plan, bpx_octree_stats* stats);
 bpx_status bpx_plan_execute(bpx_plan plan, void* hip_stream);
 bpx_status bpx_plan_synchronize(bpx_plan plan);
 /* Copies the device volume to host (float or interleaved complex). */
