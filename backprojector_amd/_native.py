@@ -81,6 +81,7 @@ class OctreeStats(ctypes.Structure):
 
 
 FLAG_REFERENCE_PAIR_TRUNCATION = 0x1
+OCTREE_BOUNDS_ONLY = 0x1
 
 _lib = None
 
@@ -93,6 +94,9 @@ EXPORTED = [
     "bpx_plan_info", "bpx_plan_benchmark", "bpx_plan_destroy", "bpx_render_transient",
     "bpx_copy_benchmark", "bpx_octree_backprojection", "bpx_octree_plan_create",
     "bpx_octree_plan_stats", "bpx_nlos_prepare", "bpx_nlos_prepared_free",
+    "bpx_net_param_count", "bpx_net_create", "bpx_net_destroy", "bpx_net_get_params",
+    "bpx_net_set_params", "bpx_net_forward", "bpx_net_gradients", "bpx_net_train_octree",
+    "bpx_octree_plan_attach_net",
 ]
 
 
@@ -149,6 +153,19 @@ def lib():
                                    ctypes.POINTER(NlosPrepared)]
     L.bpx_nlos_prepared_free.argtypes = [ctypes.POINTER(NlosPrepared)]
     L.bpx_nlos_prepared_free.restype = None
+    L.bpx_net_param_count.restype = u32
+    L.bpx_net_param_count.argtypes = []
+    L.bpx_net_create.argtypes = [u64, ctypes.POINTER(c_p)]
+    L.bpx_net_destroy.argtypes = [c_p]
+    L.bpx_net_destroy.restype = None
+    L.bpx_net_get_params.argtypes = [c_p, c_p]
+    L.bpx_net_set_params.argtypes = [c_p, c_p]
+    L.bpx_net_forward.argtypes = [c_p, c_p, c_p, c_p]
+    L.bpx_net_gradients.argtypes = [c_p, c_p, c_p, c_p, c_p, f32,
+                                    ctypes.POINTER(ctypes.c_double), c_p]
+    L.bpx_net_train_octree.argtypes = [c_p, c_p, c_p, f32, f32, f32,
+                                       ctypes.POINTER(ctypes.c_double)]
+    L.bpx_octree_plan_attach_net.argtypes = [c_p, c_p]
     _lib = L
     return L
 

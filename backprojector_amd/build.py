@@ -29,6 +29,8 @@ SOURCES = [
     "bpx_compat.cpp",
     "bpx_octree.hip",
     "bpx_octree_plan.cpp",
+    "bpx_net.hip",
+    "bpx_net.cpp",
 ]
 
 COMMON = [
@@ -47,7 +49,8 @@ def hipcc() -> str:
 
 
 def _compile(src: str) -> str:
-    obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
+    stem, ext = os.path.splitext(src)
+    obj = os.path.join(BUILD, f"{stem}_{ext[1:]}.o")
     lang = ["-x", "hip"]
     cmd = [hipcc(), *lang, *COMMON, "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)

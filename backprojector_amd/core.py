@@ -173,8 +173,9 @@ class Plan:
 
     @classmethod
     def octree(cls, transient_chunk, pairs, voxels_per_side, volume_zero_pos, voxel_inc, t0,
-               deltaT, levels=4, threshold=0.25, proposal_threshold=0.5):
-        """Device-resident octree plan (see octree_backprojection)."""
+               deltaT, levels=4, threshold=0.25, proposal_threshold=0.5, bounds_only=False):
+        """Device-resident octree plan (see octree_backprojection).  bounds_only evaluates every
+        block of every coarse level and skips the finest level (training features)."""
         H, n = _samples(transient_chunk, False)
         pr = as_pairs(pairs)
         P = len(pr)
@@ -182,7 +183,8 @@ class Plan:
         self = cls.__new__(cls)
         self.N = tuple(int(x) for x in _u32(voxels_per_side).reshape(3))
         self.complex = False
-        o = nat.OctreeOptions(int(levels), float(threshold), float(proposal_threshold), 0)
+        o = nat.OctreeOptions(int(levels), float(threshold), float(proposal_threshold),
+                              nat.OCTREE_BOUNDS_ONLY if bounds_only else 0)
         h = ctypes.c_void_p()
         check(lib().bpx_octree_plan_create(ptr(H), n, T, ptr(pr), P, ptr(_u32(self.N)),
                                            ptr(_f32(volume_zero_pos)),
@@ -190,6 +192,11 @@ class Plan:
                                            float(deltaT), ctypes.byref(o), ctypes.byref(h)))
         self._h = h
         return self
+
+    def attach_net(self, net):
+        """Attaches a learned proposal Net to an octree plan (None detaches)."""
+        check(lib().bpx_octree_plan_attach_net(self._h, net._h if net is not None else None))
+        self._net = net
 
     def octree_stats(self):
         st = nat.OctreeStats()
@@ -239,3 +246,70 @@ class Plan:
 
     def __exit__(self, *exc):
         self.close()
+
+
+class Net:
+    """Learned octree proposal net (4 x 3x3x3 conv, 16 channels, BF16 MFMA, FP32 accumulation).
+    No reference counterpart: it narrows the refinement of octree plans it is attached to."""
+
+    def __init__(self, seed=0, params=None):
+        h = ctypes.c_void_p()
+        check(lib().bpx_net_create(int(seed), ctypes.byref(h)))
+        self._h = h
+        if params is not None:
+            self.set_params(params)
+
+    @staticmethod
+    def param_count():
+        return int(lib().bpx_net_param_count())
+
+    def params(self):
+        out = np.zeros(self.param_count(), np.float32)
+        check(lib().bpx_net_get_params(self._h, ptr(out)))
+        return out
+
+    def set_params(self, params):
+        p = _f32(params).reshape(-1)
+        if p.size != self.param_count():
+            raise ValueError(f"expected {self.param_count()} parameters, got {p.size}")
+        check(lib().bpx_net_set_params(self._h, ptr(p)))
+
+    def forward(self, x):
+        """x: [n0][n1][n2][16] float32 (channels >= 4 zero); returns logits [n0][n1][n2][16]."""
+        x = _f32(x)
+        dims = _u32(x.shape[:3])
+        out = np.zeros(x.shape[:3] + (16,), np.float32)
+        check(lib().bpx_net_forward(self._h, ptr(x), ptr(dims), ptr(out)))
+        return out
+
+    def gradients(self, x, labels, mask=None, pos_weight=1.0):
+        """Weighted BCE loss over [..][8] labels and its gradient w.r.t. every parameter."""
+        x = _f32(x)
+        dims = _u32(x.shape[:3])
+        lab = np.ascontiguousarray(labels, np.uint8)
+        m = None if mask is None else np.ascontiguousarray(mask, np.uint8)
+        loss = ctypes.c_double()
+        g = np.zeros(self.param_count(), np.float32)
+        check(lib().bpx_net_gradients(self._h, ptr(x), ptr(dims), ptr(lab), ptr(m),
+                                      float(pos_weight), ctypes.byref(loss), ptr(g)))
+        return loss.value, g
+
+    def train_octree(self, octree_plan, dense_plan, occupied_fraction=0.25, pos_weight=4.0,
+                     lr=1e-3):
+        """One Adam step on one scene (octree_plan built with bounds_only=True)."""
+        loss = ctypes.c_double()
+        check(lib().bpx_net_train_octree(self._h, octree_plan._h, dense_plan._h,
+                                         float(occupied_fraction), float(pos_weight), float(lr),
+                                         ctypes.byref(loss)))
+        return loss.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().bpx_net_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -42,24 +42,29 @@ struct DeviceBuffer {
 struct PlanImpl;
 
 // Octree refinement state (bpx_octree.hip kernels, bpx_octree_plan.cpp orchestration).
+struct NetImpl;
+
 struct OctreeState {
     struct Level {
         uint32_t B = 1, n[3] = {0, 0, 0}, W = 0;
         double R = 0;
         float cR = 0;
-        DeviceBuffer M;  // window-max table [P][ldm]
+        DeviceBuffer M;       // window-max table [P][ldm]
+        DeviceBuffer list;    // evaluated blocks (packed coordinates)
+        DeviceBuffer U, V;    // bound and representative exact value per evaluated block
+        DeviceBuffer stats;   // the level's 5 FP64 statistics (see oct_finish_kernel)
+        uint32_t count = 0;   // blocks evaluated in the last execution
     };
     uint32_t L = 4, ldm = 0;
     float tau = 0.25f, prob_lo = 0.5f;
+    bool bounds_only = false;
     std::vector<Level> levels;
-    DeviceBuffer ping, pong, list_a, list_b, U, V, flags, offs, sums, total, stats, partial;
+    DeviceBuffer ping, pong, fine_list, flags, offs, sums, total, stats, partial;
     uint32_t* host_total = nullptr;
     double* host_stats = nullptr;
     std::vector<uint32_t> level_active;   // blocks evaluated per level (+ fine voxels last)
     std::vector<double> level_stats;      // 5 FP64 values per level (see oct_finish_kernel)
-    // Learned proposals: returns a device probability per active block, or nullptr.
-    std::function<const float*(PlanImpl&, uint32_t level, const uint32_t* blocks, uint32_t n,
-                               hipStream_t)> proposer;
+    NetImpl* net = nullptr;               // learned proposals (not owned), or nullptr
     ~OctreeState();
 };
 
@@ -88,6 +93,34 @@ struct PlanImpl {
 };
 
 void execute_octree(PlanImpl& pl, hipStream_t s);
+
+// Proposal net (bpx_net.hip kernels, bpx_net.cpp orchestration).  Four 3x3x3 conv layers of
+// 16 channels (inputs zero-padded to 16, 8 output logits = children), ReLU between layers.
+struct NetImpl {
+    static constexpr int kLayers = 4;
+    static constexpr uint32_t kW = 27u * 16u * 16u;          // weights per layer (padded)
+    static constexpr uint32_t kParams = kLayers * (kW + 16u); // W[4][27][16][16] then b[4][16]
+    static constexpr uint32_t kIn = 4, kOut = 8;              // real input / output channels
+    DeviceBuffer params, grads, adam_m, adam_v;   // fp32 [kParams]
+    DeviceBuffer wb, wt;                          // bf16 weights and flipped/transposed copies
+    DeviceBuffer act[kLayers + 1];                // bf16 activations [V][16]: input, Y1..Y3, (unused)
+    DeviceBuffer logits, dlog, dx, dz, labels, loss, vmax, pooled_a, pooled_b;
+    uint64_t capacity = 0;                        // voxels the activation buffers hold
+    uint64_t step = 0;
+    hipStream_t stream = nullptr;
+    ~NetImpl();
+    void ensure(uint64_t voxels);
+    void refresh_bf16(hipStream_t s);
+    // forward over a dense grid whose input act[0] is filled; logits fp32 [V][16]
+    void forward(uint32_t n0, uint32_t n1, uint32_t n2, hipStream_t s);
+    // backward from dlog (bf16 [V][16]); accumulates into grads
+    void backward(uint32_t n0, uint32_t n1, uint32_t n2, hipStream_t s);
+    void adam(float lr, hipStream_t s);
+};
+
+// Proposal hook of the octree: child flags of level l from the net (U >= threshold && logit >
+// logit(prob_lo)); returns false when no net is attached.
+bool net_child_flags(PlanImpl& pl, uint32_t l, hipStream_t s);
 
 // Describes a capture given to a plan (host buffers).
 struct CaptureIn {

@@ -132,12 +132,36 @@ hipError_t launch_winmax_combine(const float* t2, uint32_t pitch2, float* out, u
 hipError_t launch_oct_bound(const OctBoundArgs& a, hipStream_t s);
 hipError_t launch_oct_reduce(const float* U, const float* V, uint32_t n, double* partial,
                              uint32_t max_parts, double* stats, double tau, hipStream_t s);
-hipError_t launch_oct_select(const float* U, uint32_t n, const double* stats, const float* prob,
-                             float lo, uint32_t* flags, uint32_t* offs, uint32_t* sums, uint32_t* total,
-                             const uint32_t* blocks, uint32_t* children, hipStream_t s);
 hipError_t launch_oct_fine(const OctFineArgs& a, hipStream_t s);
 hipError_t launch_fill(float* p, uint64_t n, float v, hipStream_t s);
 hipError_t launch_oct_all_blocks(uint32_t nx, uint32_t ny, uint32_t nz, uint32_t* out, hipStream_t s);
+hipError_t launch_oct_scan_children(const uint32_t* flags, uint32_t m, uint32_t* offs, uint32_t* sums,
+                                    uint32_t* total, const uint32_t* blocks, uint32_t* children,
+                                    hipStream_t s);
+
+// ------------------------------------------------------------------ proposal net kernels
+hipError_t launch_conv3_fwd(const uint16_t* X, const uint16_t* W, const float* bias, uint16_t* yb,
+                            float* yf, uint32_t n0, uint32_t n1, uint32_t n2, int act,
+                            uint32_t cout_valid, hipStream_t s);
+hipError_t launch_conv3_dw(const uint16_t* dZ, const uint16_t* X, float* dW, float* db, uint32_t n0,
+                           uint32_t n1, uint32_t n2, hipStream_t s);
+hipError_t launch_flip_transpose(const uint16_t* W, uint16_t* Wt, hipStream_t s);
+hipError_t launch_relu_back(const float* dX, const uint16_t* Y, uint16_t* dZ, uint64_t n, hipStream_t s);
+hipError_t launch_bce(const float* logits, const uint8_t* labels, const uint8_t* mask, uint32_t V,
+                      uint32_t nout, float pos_weight, float scale, uint16_t* dlogits, double* loss,
+                      hipStream_t s);
+hipError_t launch_adam(float* p, const float* g, float* m, float* v, uint16_t* pb, uint32_t n, uint32_t nb,
+                       float lr, float b1, float b2, float eps, float c1, float c2, hipStream_t s);
+hipError_t launch_to_bf16(const float* src, uint16_t* dst, uint64_t n, hipStream_t s);
+hipError_t launch_oct_features(const uint32_t* blocks, const float* U, const float* V, uint32_t n,
+                               const double* stats, uint32_t n1, uint32_t n2, uint16_t* X, hipStream_t s);
+hipError_t launch_maxpool2(const float* in, float* out, uint32_t m0, uint32_t m1, uint32_t m2, hipStream_t s);
+hipError_t launch_child_labels(const float* pooled, uint32_t n0, uint32_t n1, uint32_t n2, const double* vmax,
+                               float frac, uint8_t* labels, hipStream_t s);
+hipError_t launch_max_f64(const float* x, uint64_t n, double* out, hipStream_t s);
+hipError_t launch_oct_child_flags(const uint32_t* blocks, const float* U, uint32_t n, const double* stats,
+                                  const float* logits, uint32_t n1, uint32_t n2, float logit_thr,
+                                  uint32_t* flags, hipStream_t s);
 // All launchers enqueue on `stream` and return the hipError_t of the launch.
 hipError_t launch_stable(const float* pair8, uint32_t P, uint32_t Tc, float t0, float dT,
                          float* S, uint32_t pitch, uint32_t stride, uint32_t offset,

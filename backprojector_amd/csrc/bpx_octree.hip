@@ -202,20 +202,6 @@ hipError_t launch_oct_reduce(const float* U, const float* V, uint32_t n, double*
 }
 
 // ------------------------------------------------------------------ selection
-// flags[i] = refine block i: U >= threshold (fixed rule) or, with a proposal map, the learned
-// decision (prob >= hi) or low confidence (lo < prob < hi) falls back to refinement.
-__global__ __launch_bounds__(256) void oct_flag_kernel(const float* __restrict__ U, uint32_t n,
-                                                       const double* __restrict__ stats,
-                                                       const float* __restrict__ prob, float lo,
-                                                       uint32_t* __restrict__ flags) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    // the sound bound always applies; a learned proposal can only prune further
-    uint32_t f = (double)U[i] >= stats[4] ? 1u : 0u;
-    if (prob && !(prob[i] > lo)) f = 0u;
-    flags[i] = f;
-}
-
 // Exclusive scan of flags in chunks of 1024 (one work-group each), chunk totals in sums.
 __global__ __launch_bounds__(1024) void scan_chunk_kernel(const uint32_t* __restrict__ flags, uint32_t n,
                                                           uint32_t* __restrict__ offs, uint32_t* __restrict__ sums) {
@@ -259,33 +245,32 @@ __global__ __launch_bounds__(1024) void scan_sums_kernel(uint32_t* __restrict__ 
     if (threadIdx.x == 0) *total = carry;
 }
 
-// Children of flagged blocks, in parent order (8 per parent, z fastest).
-__global__ __launch_bounds__(256) void oct_children_kernel(const uint32_t* __restrict__ blocks,
-                                                           const uint32_t* __restrict__ flags,
-                                                           const uint32_t* __restrict__ offs,
-                                                           const uint32_t* __restrict__ sums, uint32_t n,
-                                                           uint32_t* __restrict__ children) {
+// Per-child variant: flags has 8 entries per parent (child c of parent i at 8i + c); the
+// selected children are written in parent order.
+__global__ __launch_bounds__(256) void oct_children8_kernel(const uint32_t* __restrict__ blocks,
+                                                            const uint32_t* __restrict__ flags,
+                                                            const uint32_t* __restrict__ offs,
+                                                            const uint32_t* __restrict__ sums, uint32_t m,
+                                                            uint32_t* __restrict__ children) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || !flags[i]) return;
+    if (i >= m || !flags[i]) return;
     const uint32_t o = sums[i / 1024u] + offs[i];
     uint32_t x, y, z;
-    unpack_block(blocks[i], x, y, z);
-    for (uint32_t c = 0; c < 8u; ++c) {
-        const uint32_t cx = 2u * x + ((c >> 2) & 1u), cy = 2u * y + ((c >> 1) & 1u), cz = 2u * z + (c & 1u);
-        children[8u * o + c] = cx | (cy << 10) | (cz << 20);
-    }
+    unpack_block(blocks[i / 8u], x, y, z);
+    const uint32_t c = i % 8u;
+    const uint32_t cx = 2u * x + ((c >> 2) & 1u), cy = 2u * y + ((c >> 1) & 1u), cz = 2u * z + (c & 1u);
+    children[o] = cx | (cy << 10) | (cz << 20);
 }
 
-hipError_t launch_oct_select(const float* U, uint32_t n, const double* stats, const float* prob,
-                             float lo, uint32_t* flags, uint32_t* offs, uint32_t* sums, uint32_t* total,
-                             const uint32_t* blocks, uint32_t* children, hipStream_t s) {
-    if (n == 0) return hipMemsetAsync(total, 0, sizeof(uint32_t), s);
-    const uint32_t chunks = (n + 1023u) / 1024u;
-    hipLaunchKernelGGL(oct_flag_kernel, dim3((n + 255u) / 256u), dim3(256), 0, s, U, n, stats, prob, lo, flags);
-    hipLaunchKernelGGL(scan_chunk_kernel, dim3(chunks), dim3(1024), 0, s, flags, n, offs, sums);
+hipError_t launch_oct_scan_children(const uint32_t* flags, uint32_t m, uint32_t* offs, uint32_t* sums,
+                                    uint32_t* total, const uint32_t* blocks, uint32_t* children,
+                                    hipStream_t s) {
+    if (m == 0) return hipMemsetAsync(total, 0, sizeof(uint32_t), s);
+    const uint32_t chunks = (m + 1023u) / 1024u;
+    hipLaunchKernelGGL(scan_chunk_kernel, dim3(chunks), dim3(1024), 0, s, flags, m, offs, sums);
     hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(1024), 0, s, sums, chunks, total);
-    hipLaunchKernelGGL(oct_children_kernel, dim3((n + 255u) / 256u), dim3(256), 0, s, blocks, flags, offs, sums,
-                       n, children);
+    hipLaunchKernelGGL(oct_children8_kernel, dim3((m + 255u) / 256u), dim3(256), 0, s, blocks, flags, offs, sums,
+                       m, children);
     return hipGetLastError();
 }
 

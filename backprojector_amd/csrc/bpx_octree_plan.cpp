@@ -1,5 +1,6 @@
 // Host side of the octree plan: level geometry, window-max tables, per-level bound ->
-// FP64 reduction -> selection -> children, then the exact finest level.
+// FP64 reduction -> child selection (fixed threshold or learned proposals), then the exact
+// finest level.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -36,6 +37,8 @@ PlanImpl* make_plan_octree(const CaptureIn& c, const uint32_t n[3], const float 
     if (o.levels == 0 || o.levels > 8) fail(BPX_ERR_INVALID_ARGUMENT, "octree levels must be 1..8");
     if (!(o.threshold >= 0.0f) || !std::isfinite(o.threshold))
         fail(BPX_ERR_INVALID_ARGUMENT, "octree threshold must be finite and >= 0");
+    if (!(o.proposal_threshold > 0.0f && o.proposal_threshold < 1.0f))
+        fail(BPX_ERR_INVALID_ARGUMENT, "proposal threshold must lie in (0, 1)");
     for (int k = 0; k < 3; ++k) {
         if (n[k] == 0 || n[k] % (1u << o.levels) != 0)
             fail(BPX_ERR_INVALID_ARGUMENT, "voxels_per_side must be divisible by 2^levels");
@@ -49,8 +52,13 @@ PlanImpl* make_plan_octree(const CaptureIn& c, const uint32_t n[3], const float 
     std::unique_ptr<PlanImpl> pl(make_plan_single_tap(c, n, zero, inc, &po));
     auto st = std::make_unique<OctreeState>();
     st->L = o.levels;
-    st->tau = o.threshold;
+    st->bounds_only = (o.flags & BPX_OCTREE_BOUNDS_ONLY) != 0;
+    st->tau = st->bounds_only ? 0.0f : o.threshold;
     st->prob_lo = o.proposal_threshold;
+    const uint32_t P = pl->P, Tc = pl->T;
+    st->ldm = round_up32((uint64_t)Tc + 1, 64);
+    const uint64_t mbytes = (uint64_t)std::max<uint32_t>(P, 1) * st->ldm * 4;
+    if (mbytes >= (1ull << 32)) fail(BPX_ERR_UNSUPPORTED, "window-max tables above 4 GiB per level");
     for (uint32_t l = 0; l < st->L; ++l) {
         OctreeState::Level lv;
         lv.B = 1u << (st->L - l);
@@ -58,24 +66,21 @@ PlanImpl* make_plan_octree(const CaptureIn& c, const uint32_t n[3], const float 
         lv.R = block_radius(pl->g, lv.B);
         lv.W = (uint32_t)std::ceil(4.0 * lv.R / (double)c.dT) + 8u;
         lv.cR = (float)(-3.0 - 2.0 * lv.R / (double)c.dT);
+        const uint64_t cells = (uint64_t)lv.n[0] * lv.n[1] * lv.n[2];
+        lv.M.alloc(mbytes);
+        lv.list.alloc(4 * cells);
+        lv.U.alloc(4 * cells);
+        lv.V.alloc(4 * cells);
+        lv.stats.alloc(8 * sizeof(double));
         st->levels.push_back(std::move(lv));
     }
-    const uint32_t P = pl->P, Tc = pl->T;
-    st->ldm = round_up32((uint64_t)Tc + 1, 64);
-    const uint64_t mbytes = (uint64_t)std::max<uint32_t>(P, 1) * st->ldm * 4;
-    if (mbytes >= (1ull << 32)) fail(BPX_ERR_UNSUPPORTED, "window-max tables above 4 GiB per level");
-    for (uint32_t l = 0; l < st->L; ++l) st->levels[l].M.alloc(mbytes);
     st->ping.alloc(mbytes);
     st->pong.alloc(mbytes);
     const uint64_t nvox = (uint64_t)n[0] * n[1] * n[2];
-    st->list_a.alloc(4 * nvox);
-    st->list_b.alloc(4 * nvox);
-    const uint64_t max_level = std::max<uint64_t>(nvox / 8, 1);
-    st->U.alloc(4 * max_level);
-    st->V.alloc(4 * max_level);
-    st->flags.alloc(4 * max_level);
-    st->offs.alloc(4 * max_level);
-    st->sums.alloc(4 * ((max_level + 1023) / 1024 + 1));
+    st->fine_list.alloc(4 * nvox);
+    st->flags.alloc(4 * nvox);
+    st->offs.alloc(4 * nvox);
+    st->sums.alloc(4 * ((nvox + 1023) / 1024 + 1));
     st->total.alloc(sizeof(uint32_t));
     st->stats.alloc(8 * sizeof(double));
     st->partial.alloc(4 * sizeof(double) * 1024);
@@ -116,6 +121,7 @@ void execute_octree(PlanImpl& pl, hipStream_t s) {
     const size_t nvox = (size_t)pl.g.n[0] * pl.g.n[1] * pl.g.n[2];
     st.level_active.assign(st.L + 1, 0);
     st.level_stats.assign(5 * st.L, 0.0);
+    for (auto& lv : st.levels) lv.count = 0;
     hip_check(launch_fill(static_cast<float*>(pl.vol.ptr), nvox, 0.0f, s), "fill kernel");
     if (pl.P == 0) return;
     hip_check(hipEventRecord(pl.ev[0], s), "hipEventRecord");
@@ -128,23 +134,24 @@ void execute_octree(PlanImpl& pl, hipStream_t s) {
     build_winmax(pl, s);
     hip_check(hipEventRecord(pl.ev[1], s), "hipEventRecord");
     hip_check(hipMemsetAsync(st.stats.ptr, 0, 8 * sizeof(double), s), "hipMemsetAsync");
-    uint32_t* cur = static_cast<uint32_t*>(st.list_a.ptr);
-    uint32_t* nxt = static_cast<uint32_t*>(st.list_b.ptr);
     const OctreeState::Level& l0 = st.levels[0];
     uint32_t n = l0.n[0] * l0.n[1] * l0.n[2];
-    hip_check(launch_oct_all_blocks(l0.n[0], l0.n[1], l0.n[2], cur, s), "block list kernel");
+    hip_check(launch_oct_all_blocks(l0.n[0], l0.n[1], l0.n[2], static_cast<uint32_t*>(l0.list.ptr), s),
+              "block list kernel");
     for (uint32_t l = 0; l < st.L; ++l) {
-        const OctreeState::Level& lv = st.levels[l];
+        OctreeState::Level& lv = st.levels[l];
+        lv.count = n;
         st.level_active[l] = n;
         if (n == 0) break;
+        const uint32_t* cur = static_cast<const uint32_t*>(lv.list.ptr);
         OctBoundArgs a{};
         a.M = static_cast<const float*>(lv.M.ptr);
         a.E = pl.H.ptr;
         a.pairs = static_cast<const float4*>(pl.pair4.ptr);
         a.blocks = cur;
         a.nblocks = n;
-        a.U = static_cast<float*>(st.U.ptr);
-        a.V = static_cast<float*>(st.V.ptr);
+        a.U = static_cast<float*>(lv.U.ptr);
+        a.V = static_cast<float*>(lv.V.ptr);
         a.P = pl.P;
         a.Tc = pl.T;
         a.ldm = st.ldm;
@@ -159,38 +166,46 @@ void execute_octree(PlanImpl& pl, hipStream_t s) {
         hip_check(launch_oct_reduce(a.U, a.V, n, static_cast<double*>(st.partial.ptr), 1024u,
                                     static_cast<double*>(st.stats.ptr), (double)st.tau, s),
                   "octree FP64 reduction");
-        const float* prob = nullptr;
-        if (st.proposer) prob = st.proposer(pl, l, cur, n, s);
-        hip_check(launch_oct_select(a.U, n, static_cast<const double*>(st.stats.ptr), prob, st.prob_lo,
-                                    static_cast<uint32_t*>(st.flags.ptr), static_cast<uint32_t*>(st.offs.ptr),
-                                    static_cast<uint32_t*>(st.sums.ptr), static_cast<uint32_t*>(st.total.ptr),
-                                    cur, nxt, s),
-                  "octree selection");
+        hip_check(hipMemcpyAsync(lv.stats.ptr, st.stats.ptr, 5 * sizeof(double), hipMemcpyDeviceToDevice, s),
+                  "hipMemcpyAsync");
+        // child flags: fixed rule (bound >= tau * running max), narrowed by learned proposals
+        if (!net_child_flags(pl, l, s))
+            hip_check(launch_oct_child_flags(cur, a.U, n, static_cast<const double*>(st.stats.ptr), nullptr,
+                                             lv.n[1], lv.n[2], 0.0f, static_cast<uint32_t*>(st.flags.ptr), s),
+                      "octree child flags");
+        uint32_t* next = l + 1 < st.L ? static_cast<uint32_t*>(st.levels[l + 1].list.ptr)
+                                      : static_cast<uint32_t*>(st.fine_list.ptr);
+        hip_check(launch_oct_scan_children(static_cast<const uint32_t*>(st.flags.ptr), 8u * n,
+                                           static_cast<uint32_t*>(st.offs.ptr),
+                                           static_cast<uint32_t*>(st.sums.ptr),
+                                           static_cast<uint32_t*>(st.total.ptr), cur, next, s),
+                  "octree child scan");
         hip_check(hipMemcpyAsync(st.host_total, st.total.ptr, sizeof(uint32_t), hipMemcpyDeviceToHost, s),
                   "hipMemcpyAsync");
         hip_check(hipMemcpyAsync(st.host_stats, st.stats.ptr, 5 * sizeof(double), hipMemcpyDeviceToHost, s),
                   "hipMemcpyAsync");
         hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
         for (int k = 0; k < 5; ++k) st.level_stats[5 * l + k] = st.host_stats[k];
-        n = 8u * st.host_total[0];
-        std::swap(cur, nxt);
+        n = st.host_total[0];
     }
-    st.level_active[st.L] = n;
+    st.level_active[st.L] = st.bounds_only ? 0u : n;
     hip_check(hipEventRecord(pl.ev[2], s), "hipEventRecord");
-    OctFineArgs f{};
-    f.E = pl.H.ptr;
-    f.pairs = static_cast<const float4*>(pl.pair4.ptr);
-    f.voxels = cur;
-    f.n = n;
-    f.out = static_cast<float*>(pl.vol.ptr);
-    f.P = pl.P;
-    f.Tc = pl.T;
-    f.lde = pl.ldh;
-    f.e_bytes = (uint32_t)pl.H.bytes;
-    f.kahan_block = pl.kahan_block;
-    f.g = pl.g;
-    f.inv2 = pl.inv2;
-    hip_check(launch_oct_fine(f, s), "octree fine kernel");
+    if (!st.bounds_only) {
+        OctFineArgs f{};
+        f.E = pl.H.ptr;
+        f.pairs = static_cast<const float4*>(pl.pair4.ptr);
+        f.voxels = static_cast<const uint32_t*>(st.fine_list.ptr);
+        f.n = n;
+        f.out = static_cast<float*>(pl.vol.ptr);
+        f.P = pl.P;
+        f.Tc = pl.T;
+        f.lde = pl.ldh;
+        f.e_bytes = (uint32_t)pl.H.bytes;
+        f.kahan_block = pl.kahan_block;
+        f.g = pl.g;
+        f.inv2 = pl.inv2;
+        hip_check(launch_oct_fine(f, s), "octree fine kernel");
+    }
     hip_check(hipEventRecord(pl.ev[3], s), "hipEventRecord");
 }
 

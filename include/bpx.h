@@ -193,8 +193,11 @@ typedef struct bpx_octree_options {
     uint32_t levels;          /* refinements (default 4: 16^3 -> 256^3)                  */
     float threshold;          /* tau (default 0.25)                                      */
     float proposal_threshold; /* learned proposals: refine iff probability > this (0.5) */
-    uint32_t flags;           /* reserved, 0                                             */
+    uint32_t flags;           /* BPX_OCTREE_*                                            */
 } bpx_octree_options;
+
+/* Evaluate every block of every coarse level and skip the finest level (training data). */
+#define BPX_OCTREE_BOUNDS_ONLY 0x1u
 
 typedef struct bpx_octree_stats {
     uint32_t levels;
@@ -272,6 +275,35 @@ void bpx_plan_destroy(bpx_plan plan);
  * reference and the calibration of the FETCH_SIZE / WRITE_SIZE counters (no reference
  * counterpart). */
 bpx_status bpx_copy_benchmark(uint64_t bytes, int32_t iterations, double* mean_ms);
+
+/* ------------------------------------------------------------------ learned octree proposals
+ * A small 3D conv net (4 layers of 3x3x3 convolutions, 16 channels, BF16 weights and
+ * activations on MFMA with FP32 accumulation, ReLU) maps a coarse octree level to one logit
+ * per child (8 outputs): "this child contains an occupied voxel".  Inputs per block:
+ * U / max U, V / running max, evaluated flag, U / rms U (see bpx_octree_options).  Attached to
+ * an octree plan it narrows the refinement: a child is refined iff its parent's bound passes the
+ * fixed threshold AND its probability exceeds proposal_threshold.  No reference counterpart.
+ * Parameters: W[4][27][16 out][16 in] then b[4][16], fp32 (bpx_net_param_count() floats). */
+typedef struct bpx_net_s* bpx_net;
+uint32_t bpx_net_param_count(void);
+bpx_status bpx_net_create(uint64_t seed, bpx_net* out);
+void bpx_net_destroy(bpx_net net);
+bpx_status bpx_net_get_params(bpx_net net, float* params);
+bpx_status bpx_net_set_params(bpx_net net, const float* params);
+/* input: fp32 [n0][n1][n2][16] (channels >= 4 ignored: they must be 0); logits out: fp32
+ * [n0][n1][n2][16] (channels 0..7 meaningful). */
+bpx_status bpx_net_forward(bpx_net net, const float* input, const uint32_t dims[3], float* logits);
+/* Weighted BCE loss (sum over voxels with mask != 0, or all when mask is NULL, x 8 outputs)
+ * and its gradient w.r.t. every parameter (grads: bpx_net_param_count() floats). */
+bpx_status bpx_net_gradients(bpx_net net, const float* input, const uint32_t dims[3],
+                             const uint8_t* labels /* [V][8] */, const uint8_t* mask /* [V] */,
+                             float pos_weight, double* loss, float* grads);
+/* One Adam step on one scene: octree_plan executed with BPX_OCTREE_BOUNDS_ONLY (features of every
+ * level), dense_plan executed on the same grid (labels: child max >= occupied_fraction x max). */
+bpx_status bpx_net_train_octree(bpx_net net, bpx_plan octree_plan, bpx_plan dense_plan,
+                                float occupied_fraction, float pos_weight, float lr, double* loss);
+/* Attach (or, with NULL, detach) a net to an octree plan; the plan does not own it. */
+bpx_status bpx_octree_plan_attach_net(bpx_plan plan, bpx_net net);
 
 /* ------------------------------------------------------------------ synthetic captures
  * Renders a transient capture of hidden Lambertian geometry (no reference counterpart).
