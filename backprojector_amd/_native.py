@@ -179,7 +179,10 @@ def check(status: int) -> None:
 
 
 def ptr(a: np.ndarray):
-    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+    """Pointer argument for a numpy array.  data_as() keeps a reference to the array inside the
+    returned ctypes object, so temporaries such as ptr(_f32(x)) stay alive for the whole call
+    (the library runs without the GIL, so a freed temporary could be reused meanwhile)."""
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
 
 
 def device_count() -> int:

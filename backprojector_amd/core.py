@@ -174,8 +174,8 @@ class Plan:
     @classmethod
     def octree(cls, transient_chunk, pairs, voxels_per_side, volume_zero_pos, voxel_inc, t0,
                deltaT, levels=4, threshold=0.25, proposal_threshold=0.5, bounds_only=False):
-        """Device-resident octree plan (see octree_backprojection).  bounds_only evaluates every
-        block of every coarse level and skips the finest level (training features)."""
+        """Device-resident octree plan (see octree_backprojection).  bounds_only runs the coarse
+        levels only and skips the exact finest level (training data for Net.train_octree)."""
         H, n = _samples(transient_chunk, False)
         pr = as_pairs(pairs)
         P = len(pr)
@@ -275,7 +275,7 @@ class Net:
         check(lib().bpx_net_set_params(self._h, ptr(p)))
 
     def forward(self, x):
-        """x: [n0][n1][n2][16] float32 (channels >= 4 zero); returns logits [n0][n1][n2][16]."""
+        """x: [n0][n1][n2][16] float32 (channels >= 5 zero); returns logits [n0][n1][n2][16]."""
         x = _f32(x)
         dims = _u32(x.shape[:3])
         out = np.zeros(x.shape[:3] + (16,), np.float32)
@@ -296,7 +296,8 @@ class Net:
 
     def train_octree(self, octree_plan, dense_plan, occupied_fraction=0.25, pos_weight=4.0,
                      lr=1e-3):
-        """One Adam step on one scene (octree_plan built with bounds_only=True)."""
+        """One Adam step on one scene: octree_plan (typically bounds_only=True) and dense_plan
+        have both been executed on the same grid."""
         loss = ctypes.c_double()
         check(lib().bpx_net_train_octree(self._h, octree_plan._h, dense_plan._h,
                                          float(occupied_fraction), float(pos_weight), float(lr),

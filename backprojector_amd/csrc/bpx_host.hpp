@@ -60,6 +60,8 @@ struct OctreeState {
     bool bounds_only = false;
     std::vector<Level> levels;
     DeviceBuffer ping, pong, fine_list, flags, offs, sums, total, stats, partial;
+    DeviceBuffer bound_part_u, bound_part_v;  // pair-segment partial bounds (kBoundPartCapacity floats)
+    static constexpr uint64_t kBoundPartCapacity = 1ull << 20;
     uint32_t* host_total = nullptr;
     double* host_stats = nullptr;
     std::vector<uint32_t> level_active;   // blocks evaluated per level (+ fine voxels last)
@@ -100,11 +102,11 @@ struct NetImpl {
     static constexpr int kLayers = 4;
     static constexpr uint32_t kW = 27u * 16u * 16u;          // weights per layer (padded)
     static constexpr uint32_t kParams = kLayers * (kW + 16u); // W[4][27][16][16] then b[4][16]
-    static constexpr uint32_t kIn = 4, kOut = 8;              // real input / output channels
+    static constexpr uint32_t kIn = 5, kOut = 8;              // real input / output channels
     DeviceBuffer params, grads, adam_m, adam_v;   // fp32 [kParams]
     DeviceBuffer wb, wt;                          // bf16 weights and flipped/transposed copies
     DeviceBuffer act[kLayers + 1];                // bf16 activations [V][16]: input, Y1..Y3, (unused)
-    DeviceBuffer logits, dlog, dx, dz, labels, loss, vmax, pooled_a, pooled_b;
+    DeviceBuffer logits, dlog, dx, dz, labels, mask, loss, vmax, pooled_a, pooled_b;
     uint64_t capacity = 0;                        // voxels the activation buffers hold
     uint64_t step = 0;
     hipStream_t stream = nullptr;

@@ -112,6 +112,9 @@ struct OctBoundArgs {
     uint32_t B;             // block edge in fine voxels
     float cR;               // window-start offset: -3 - 2R/deltaT
     float inv2;
+    uint32_t chunk;         // pairs per segment (blockIdx.y); == P when not split
+    float* partU;           // [segments][nblocks] partial sums when split, else nullptr
+    float* partV;
 };
 struct OctFineArgs {
     const void* E;
@@ -129,7 +132,9 @@ hipError_t launch_winmax_double(const float* in, uint32_t in_pitch, float* out, 
                                 uint32_t rows, uint32_t Tc, uint32_t half, hipStream_t s);
 hipError_t launch_winmax_combine(const float* t2, uint32_t pitch2, float* out, uint32_t out_pitch,
                                  uint32_t rows, uint32_t Tc, uint32_t shift, hipStream_t s);
-hipError_t launch_oct_bound(const OctBoundArgs& a, hipStream_t s);
+// Pair segments for a level of nblocks blocks (partials must fit part_capacity floats).
+uint32_t oct_bound_segments(uint32_t nblocks, uint32_t P, uint64_t part_capacity);
+hipError_t launch_oct_bound(const OctBoundArgs& a, uint32_t segments, hipStream_t s);
 hipError_t launch_oct_reduce(const float* U, const float* V, uint32_t n, double* partial,
                              uint32_t max_parts, double* stats, double tau, hipStream_t s);
 hipError_t launch_oct_fine(const OctFineArgs& a, hipStream_t s);
@@ -154,7 +159,8 @@ hipError_t launch_adam(float* p, const float* g, float* m, float* v, uint16_t* p
                        float lr, float b1, float b2, float eps, float c1, float c2, hipStream_t s);
 hipError_t launch_to_bf16(const float* src, uint16_t* dst, uint64_t n, hipStream_t s);
 hipError_t launch_oct_features(const uint32_t* blocks, const float* U, const float* V, uint32_t n,
-                               const double* stats, uint32_t n1, uint32_t n2, uint16_t* X, hipStream_t s);
+                               const double* stats, uint32_t n1, uint32_t n2, float level_code, uint16_t* X,
+                               uint8_t* mask, hipStream_t s);
 hipError_t launch_maxpool2(const float* in, float* out, uint32_t m0, uint32_t m1, uint32_t m2, hipStream_t s);
 hipError_t launch_child_labels(const float* pooled, uint32_t n0, uint32_t n1, uint32_t n2, const double* vmax,
                                float frac, uint8_t* labels, hipStream_t s);

@@ -23,6 +23,7 @@ void NetImpl::ensure(uint64_t voxels) {
     dx.alloc(voxels * 16u * 4u);
     dz.alloc(voxels * 16u * 2u);
     labels.alloc(voxels * 8u);
+    mask.alloc(voxels);
     capacity = voxels;
 }
 
@@ -114,24 +115,28 @@ static NetImpl* make_net(uint64_t seed) {
     return n.release();
 }
 
-// Features of level l (dense grid) into the net's input buffer.
-static void level_features(NetImpl& net, const OctreeState::Level& lv, hipStream_t s) {
+// Features of level l (dense grid) into the net's input buffer; with_mask also marks the
+// evaluated cells in net.mask.
+static void level_features(NetImpl& net, const OctreeState::Level& lv, bool with_mask, hipStream_t s) {
     const uint64_t V = (uint64_t)lv.n[0] * lv.n[1] * lv.n[2];
     net.ensure(V);
     hip_check(hipMemsetAsync(net.act[0].ptr, 0, V * 32u, s), "hipMemsetAsync");
+    if (with_mask) hip_check(hipMemsetAsync(net.mask.ptr, 0, V, s), "hipMemsetAsync");
+    const float level_code = std::log2((float)lv.B) / 8.0f;
     hip_check(launch_oct_features(static_cast<const uint32_t*>(lv.list.ptr), static_cast<const float*>(lv.U.ptr),
                                   static_cast<const float*>(lv.V.ptr), lv.count,
-                                  static_cast<const double*>(lv.stats.ptr), lv.n[1], lv.n[2],
-                                  static_cast<uint16_t*>(net.act[0].ptr), s),
+                                  static_cast<const double*>(lv.stats.ptr), lv.n[1], lv.n[2], level_code,
+                                  static_cast<uint16_t*>(net.act[0].ptr),
+                                  with_mask ? static_cast<uint8_t*>(net.mask.ptr) : nullptr, s),
               "octree features");
 }
 
 bool net_child_flags(PlanImpl& pl, uint32_t l, hipStream_t s) {
     OctreeState& st = *pl.oct;
-    if (!st.net || st.bounds_only) return false;
+    if (!st.net) return false;
     NetImpl& net = *st.net;
     const OctreeState::Level& lv = st.levels[l];
-    level_features(net, lv, s);
+    level_features(net, lv, false, s);
     net.forward(lv.n[0], lv.n[1], lv.n[2], s);
     const float p = st.prob_lo;
     const float logit_thr = std::log(p / (1.0f - p));
@@ -277,9 +282,7 @@ bpx_status bpx_net_train_octree(bpx_net net, bpx_plan octree_plan, bpx_plan dens
         if (std::memcmp(op.g.n, dp.g.n, sizeof(op.g.n)) != 0)
             fail(BPX_ERR_INVALID_ARGUMENT, "octree and dense plans have different grids");
         OctreeState& st = *op.oct;
-        for (const auto& lv : st.levels)
-            if (lv.count != lv.n[0] * lv.n[1] * lv.n[2])
-                fail(BPX_ERR_INVALID_ARGUMENT, "train on an octree executed with BPX_OCTREE_BOUNDS_ONLY");
+        if (st.levels[0].count == 0) fail(BPX_ERR_INVALID_ARGUMENT, "the octree plan has not been executed");
         NetImpl& n = *net->impl;
         const hipStream_t s = n.stream;
         hip_check(hipStreamSynchronize(op.stream), "hipStreamSynchronize");
@@ -306,17 +309,19 @@ bpx_status bpx_net_train_octree(bpx_net net, bpx_plan octree_plan, bpx_plan dens
         hip_check(hipMemsetAsync(n.loss.ptr, 0, sizeof(double), s), "hipMemsetAsync");
         for (uint32_t l = 0; l < st.L; ++l) {
             const auto& lv = st.levels[l];
+            if (lv.count == 0) break;
             const uint64_t V = (uint64_t)lv.n[0] * lv.n[1] * lv.n[2];
-            level_features(n, lv, s);
+            level_features(n, lv, true, s);
             const float* children = l + 1 < st.L ? static_cast<const float*>(n.pooled_a.ptr) + off[l + 1]
                                                  : static_cast<const float*>(dp.vol.ptr);
             hip_check(launch_child_labels(children, lv.n[0], lv.n[1], lv.n[2], static_cast<const double*>(n.vmax.ptr),
                                           occupied_fraction, static_cast<uint8_t*>(n.labels.ptr), s),
                       "child labels");
             n.forward(lv.n[0], lv.n[1], lv.n[2], s);
-            const float scale = 1.0f / (float)(V * NetImpl::kOut * st.L);
+            // every level weighs equally: mean over its evaluated blocks x 8 children
+            const float scale = 1.0f / (float)((uint64_t)lv.count * NetImpl::kOut * st.L);
             hip_check(launch_bce(static_cast<const float*>(n.logits.ptr), static_cast<const uint8_t*>(n.labels.ptr),
-                                 nullptr, (uint32_t)V, NetImpl::kOut, pos_weight, scale,
+                                 static_cast<const uint8_t*>(n.mask.ptr), (uint32_t)V, NetImpl::kOut, pos_weight, scale,
                                  static_cast<uint16_t*>(n.dlog.ptr), static_cast<double*>(n.loss.ptr), s),
                       "bce");
             n.backward(lv.n[0], lv.n[1], lv.n[2], s);

@@ -243,30 +243,36 @@ hipError_t launch_to_bf16(const float* src, uint16_t* dst, uint64_t n, hipStream
 }
 
 // ------------------------------------------------------------------ octree features / labels
-// Dense level grid of input features (bf16 [V][16]) from the level's active block list:
-//   c0 = U / max U, c1 = V / running max, c2 = 1 (evaluated), c3 = U / rms U; others 0.
+// Dense level grid of input features (bf16 [V][16]) from the level's evaluated block list
+// (grid cells never evaluated stay 0):
+//   c0 = U / running max, c1 = V / running max, c2 = 1 (evaluated), c3 = U / max U,
+//   c4 = level_code (log2 of the block edge / 8); others 0.  mask[v] = 1 where evaluated.
 __global__ void oct_features_kernel(const uint32_t* __restrict__ blocks, const float* __restrict__ U,
                                     const float* __restrict__ Vv, uint32_t n, const double* __restrict__ stats,
-                                    uint32_t n1, uint32_t n2, uint16_t* __restrict__ X) {
+                                    uint32_t n1, uint32_t n2, float level_code, uint16_t* __restrict__ X,
+                                    uint8_t* __restrict__ mask) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t c = blocks[i];
     const uint32_t x = c & 1023u, y = (c >> 10) & 1023u, z = c >> 20;
-    const size_t o = (((size_t)x * n1 + y) * n2 + z) * 16u;
-    const double umax = stats[1] > 0.0 ? stats[1] : 1.0;
+    const size_t v = ((size_t)x * n1 + y) * n2 + z;
+    const size_t o = v * 16u;
     const double mrun = stats[0] > 0.0 ? stats[0] : 1.0;
-    const double urms = stats[2] > 0.0 ? sqrt(stats[2] / (double)n) : 1.0;
-    X[o + 0] = to_bf16((float)((double)U[i] / umax));
+    const double umax = stats[1] > 0.0 ? stats[1] : 1.0;
+    X[o + 0] = to_bf16((float)((double)U[i] / mrun));
     X[o + 1] = to_bf16((float)((double)Vv[i] / mrun));
     X[o + 2] = to_bf16(1.0f);
-    X[o + 3] = to_bf16((float)((double)U[i] / urms));
+    X[o + 3] = to_bf16((float)((double)U[i] / umax));
+    X[o + 4] = to_bf16(level_code);
+    if (mask) mask[v] = 1u;
 }
 
 hipError_t launch_oct_features(const uint32_t* blocks, const float* U, const float* V, uint32_t n,
-                               const double* stats, uint32_t n1, uint32_t n2, uint16_t* X, hipStream_t s) {
+                               const double* stats, uint32_t n1, uint32_t n2, float level_code, uint16_t* X,
+                               uint8_t* mask, hipStream_t s) {
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(oct_features_kernel, dim3((n + 255u) / 256u), dim3(256), 0, s, blocks, U, V, n, stats, n1,
-                       n2, X);
+                       n2, level_code, X, mask);
     return hipGetLastError();
 }
 

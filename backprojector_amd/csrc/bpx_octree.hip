@@ -12,6 +12,8 @@
 // read.  Running max and L2 norm of each level are reduced in FP64; blocks with
 // U >= tau * max_so_far are refined.  The finest level is computed exactly (same fused
 // sample+threshold layout and the same per-update arithmetic as the dense kernel).
+#include <algorithm>
+
 #include "bpx_device.hpp"
 #include "bpx_internal.hpp"
 
@@ -75,7 +77,9 @@ __device__ __forceinline__ void unpack_block(uint32_t c, uint32_t& x, uint32_t& 
     z = c >> 20;
 }
 
-// One lane per block; pairs in the wave-uniform loop (pair data in SGPRs).
+// One lane per block; pairs in the wave-uniform loop (pair data in SGPRs).  Coarse levels
+// have few blocks, so the pair range is split into segments (blockIdx.y) whose partial sums
+// are added in segment order by oct_bound_combine_kernel (deterministic).
 __global__ __launch_bounds__(256) void oct_bound_kernel(OctBoundArgs a) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t ii = min(i, a.nblocks - 1u);
@@ -100,7 +104,8 @@ __global__ __launch_bounds__(256) void oct_bound_kernel(OctBoundArgs a) {
     const auto er = make_rsrc(a.E, a.e_bytes);
     const float fTc = (float)a.Tc;
     float U = 0.0f, Vs = 0.0f;
-    for (uint32_t p = 0; p < a.P; ++p) {
+    const uint32_t p0 = blockIdx.y * a.chunk, p1 = min(a.P, p0 + a.chunk);
+    for (uint32_t p = p0; p < p1; ++p) {
         const float4 pr = a.pairs[p];
         // window start for the bounding sphere
         const float sm = sq3(pr.x - m[0], pr.y - m[1], pr.z - m[2]);
@@ -115,14 +120,55 @@ __global__ __launch_bounds__(256) void oct_bound_kernel(OctBoundArgs a) {
         Vs = Vs + FusedF32::pick(er, j, p * a.lde * 8u, s);
     }
     if (i < a.nblocks) {
-        a.U[i] = U;
-        a.V[i] = Vs;
+        if (a.partU) {
+            a.partU[(size_t)blockIdx.y * a.nblocks + i] = U;
+            a.partV[(size_t)blockIdx.y * a.nblocks + i] = Vs;
+        } else {
+            a.U[i] = U;
+            a.V[i] = Vs;
+        }
     }
 }
 
-hipError_t launch_oct_bound(const OctBoundArgs& a, hipStream_t s) {
-    if (a.nblocks == 0) return hipSuccess;
-    hipLaunchKernelGGL(oct_bound_kernel, dim3((a.nblocks + 255u) / 256u), dim3(256), 0, s, a);
+__global__ __launch_bounds__(256) void oct_bound_combine_kernel(const float* __restrict__ partU,
+                                                                const float* __restrict__ partV, uint32_t n,
+                                                                uint32_t segments, float* __restrict__ U,
+                                                                float* __restrict__ V) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float u = 0.0f, v = 0.0f;
+    for (uint32_t s = 0; s < segments; ++s) {
+        u = u + partU[(size_t)s * n + i];
+        v = v + partV[(size_t)s * n + i];
+    }
+    U[i] = u;
+    V[i] = v;
+}
+
+uint32_t oct_bound_segments(uint32_t nblocks, uint32_t P, uint64_t part_capacity) {
+    // aim for >= 2^19 lanes in flight, segments of >= 64 pairs, partials within capacity
+    if (nblocks == 0 || P == 0) return 1u;
+    uint64_t s = ((1ull << 19) + nblocks - 1) / nblocks;
+    s = std::min<uint64_t>(s, std::max<uint32_t>(P / 64u, 1u));
+    s = std::min<uint64_t>(s, part_capacity / nblocks);
+    return (uint32_t)std::max<uint64_t>(s, 1u);
+}
+
+hipError_t launch_oct_bound(const OctBoundArgs& a_in, uint32_t segments, hipStream_t s) {
+    if (a_in.nblocks == 0) return hipSuccess;
+    OctBoundArgs a = a_in;
+    if (segments <= 1u || !a.partU || !a.partV) {
+        a.chunk = a.P;
+        a.partU = a.partV = nullptr;
+        segments = 1u;
+    } else {
+        a.chunk = (a.P + segments - 1u) / segments;
+        segments = (a.P + a.chunk - 1u) / a.chunk;
+    }
+    hipLaunchKernelGGL(oct_bound_kernel, dim3((a.nblocks + 255u) / 256u, segments), dim3(256), 0, s, a);
+    if (segments > 1u)
+        hipLaunchKernelGGL(oct_bound_combine_kernel, dim3((a.nblocks + 255u) / 256u), dim3(256), 0, s, a.partU,
+                           a.partV, a.nblocks, segments, a.U, a.V);
     return hipGetLastError();
 }
 

@@ -53,7 +53,7 @@ PlanImpl* make_plan_octree(const CaptureIn& c, const uint32_t n[3], const float 
     auto st = std::make_unique<OctreeState>();
     st->L = o.levels;
     st->bounds_only = (o.flags & BPX_OCTREE_BOUNDS_ONLY) != 0;
-    st->tau = st->bounds_only ? 0.0f : o.threshold;
+    st->tau = o.threshold;
     st->prob_lo = o.proposal_threshold;
     const uint32_t P = pl->P, Tc = pl->T;
     st->ldm = round_up32((uint64_t)Tc + 1, 64);
@@ -84,6 +84,8 @@ PlanImpl* make_plan_octree(const CaptureIn& c, const uint32_t n[3], const float 
     st->total.alloc(sizeof(uint32_t));
     st->stats.alloc(8 * sizeof(double));
     st->partial.alloc(4 * sizeof(double) * 1024);
+    st->bound_part_u.alloc(4 * OctreeState::kBoundPartCapacity);
+    st->bound_part_v.alloc(4 * OctreeState::kBoundPartCapacity);
     hip_check(hipHostMalloc(&st->host_total, sizeof(uint32_t) * 2, hipHostMallocDefault), "hipHostMalloc");
     hip_check(hipHostMalloc(&st->host_stats, sizeof(double) * 8, hipHostMallocDefault), "hipHostMalloc");
     pl->oct = std::move(st);
@@ -119,6 +121,7 @@ static void build_winmax(PlanImpl& pl, hipStream_t s) {
 void execute_octree(PlanImpl& pl, hipStream_t s) {
     OctreeState& st = *pl.oct;
     const size_t nvox = (size_t)pl.g.n[0] * pl.g.n[1] * pl.g.n[2];
+    hip_check(hipGetLastError(), "HIP error pending before the octree execution");
     st.level_active.assign(st.L + 1, 0);
     st.level_stats.assign(5 * st.L, 0.0);
     for (auto& lv : st.levels) lv.count = 0;
@@ -136,8 +139,11 @@ void execute_octree(PlanImpl& pl, hipStream_t s) {
     hip_check(hipMemsetAsync(st.stats.ptr, 0, 8 * sizeof(double), s), "hipMemsetAsync");
     const OctreeState::Level& l0 = st.levels[0];
     uint32_t n = l0.n[0] * l0.n[1] * l0.n[2];
+    if (n == 0) fail(BPX_ERR_INVALID_ARGUMENT, "empty coarsest octree level");
+    const std::string what = "block list kernel (" + std::to_string(l0.n[0]) + "x" + std::to_string(l0.n[1]) +
+                             "x" + std::to_string(l0.n[2]) + ")";
     hip_check(launch_oct_all_blocks(l0.n[0], l0.n[1], l0.n[2], static_cast<uint32_t*>(l0.list.ptr), s),
-              "block list kernel");
+              what.c_str());
     for (uint32_t l = 0; l < st.L; ++l) {
         OctreeState::Level& lv = st.levels[l];
         lv.count = n;
@@ -162,7 +168,10 @@ void execute_octree(PlanImpl& pl, hipStream_t s) {
         a.B = lv.B;
         a.cR = lv.cR;
         a.inv2 = pl.inv2;
-        hip_check(launch_oct_bound(a, s), "octree bound kernel");
+        a.partU = static_cast<float*>(st.bound_part_u.ptr);
+        a.partV = static_cast<float*>(st.bound_part_v.ptr);
+        hip_check(launch_oct_bound(a, oct_bound_segments(n, pl.P, OctreeState::kBoundPartCapacity), s),
+                  "octree bound kernel");
         hip_check(launch_oct_reduce(a.U, a.V, n, static_cast<double*>(st.partial.ptr), 1024u,
                                     static_cast<double*>(st.stats.ptr), (double)st.tau, s),
                   "octree FP64 reduction");

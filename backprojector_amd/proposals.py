@@ -1,0 +1,229 @@
+"""Learned octree proposals: training on synthetic captures, validation, and the shipped weights.
+
+No reference counterpart (the reference's octree, backproject_cuda.cu:183-367, prunes with a
+fixed rule and does not run).  The net (libbpx, bpx_net_*) sees each coarse level of a
+bounds-only octree execution -- per block: U / max U, V / running max, evaluated flag,
+U / rms U -- and is trained with weighted binary cross-entropy to predict, for each of the 8
+children, whether the child holds an "occupied" voxel of the dense reconstruction (value >=
+OCCUPIED_FRACTION x max).  At inference a child is refined iff its parent passes the sound
+bound rule (U >= tau x running max) AND its probability exceeds the proposal threshold.
+
+Everything runs on the GPU through the library's own kernels: dense and octree plans,
+features, labels (FP64 max, max-pool pyramid), forward, backward, Adam.
+
+    python -m backprojector_amd.proposals train --scenes 2000 --out weights/octree_proposal
+    python -m backprojector_amd.proposals validate --weights weights/octree_proposal
+"""
+from __future__ import annotations
+
+import argparse
+import concurrent.futures as cf
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import core, nlosbpy, scenes
+
+OCCUPIED_FRACTION = 0.25
+WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "octree_proposal")
+TRAIN_SEED_BASE = 50000         # training scenes 50000 ... 50000 + n - 1
+VALIDATION_SEED_BASE = 60000    # held-out test captures 60000 ...
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def training_scene(rng):
+    """Random hidden geometry: 0-12 points, 0-3 patches (0.1-0.4 m), 0-2 spheres."""
+    parts = []
+    npts = int(rng.integers(0, 13))
+    if npts:
+        parts.append(scenes.points(rng, npts))
+    for _ in range(int(rng.integers(0, 4))):
+        parts.append(scenes.patch(rng, size=float(rng.uniform(0.1, 0.4)), spacing=0.004))
+    for _ in range(int(rng.integers(0, 3))):
+        parts.append(scenes.sphere(rng, spacing=0.006))
+    if not parts:
+        parts.append(scenes.points(rng, 1))
+    return np.concatenate(parts)
+
+
+def plan_args(cap, voxels):
+    """(chunk, pairs, N, zero, inc, t0, deltaT) through the reference pipeline's preparation."""
+    N = (voxels,) * 3
+    prep = nlosbpy.prepare(*cap.nlos_args(N))
+    return (prep["chunk"], prep["pairs"], N, prep["zero"], prep["inc"],
+            float(prep["chunked_t0"]), cap.deltaT)
+
+
+def training_capture(index, side, voxels, T=4096):
+    seed = TRAIN_SEED_BASE + index
+    rng = np.random.default_rng(seed)
+    cap = scenes.render(training_scene(rng), side, T, seed=seed)
+    return plan_args(cap, voxels)
+
+
+def train(scenes_n=2000, steps_per_scene=3, side=64, voxels=256, levels=4, lr=2e-3,
+          pos_weight=8.0, seed=0, log_every=50, init=None, tau=0.25, on_policy=0.5,
+          policy_threshold=0.1):
+    """Trains a net on `scenes_n` distinct synthetic scenes (Adam, cosine learning rate).
+
+    Each scene's octree runs its coarse levels with the bound rule (threshold tau), so the net
+    sees the same evaluated sets and features as at inference; for the last `on_policy`
+    fraction of the scenes the net being trained is attached (proposal threshold
+    policy_threshold) and the training data follow its own selections.  Returns (net, history)."""
+    net = core.Net(seed, params=init)
+    hist = []
+    total = scenes_n * steps_per_scene
+    step = 0
+    t_start = time.perf_counter()
+    window = []
+    with cf.ThreadPoolExecutor(1) as pool:     # renders the next capture while the GPU trains
+        fut = pool.submit(training_capture, 0, side, voxels)
+        for i in range(scenes_n):
+            args = fut.result()
+            if i + 1 < scenes_n:
+                fut = pool.submit(training_capture, i + 1, side, voxels)
+            attach = i >= scenes_n * (1.0 - on_policy)
+            with core.Plan.octree(*args, levels=levels, threshold=tau, bounds_only=True,
+                                  proposal_threshold=policy_threshold) as op, \
+                    core.Plan(*args) as dp:
+                if attach:
+                    op.attach_net(net)
+                try:
+                    op.execute()
+                    dp.execute()
+                except Exception as e:
+                    raise RuntimeError(f"scene {i}: chunk {args[0].shape}, {len(args[1])} pairs, "
+                                       f"N {args[2]}, t0 {args[5]}: {e}") from e
+                for _ in range(steps_per_scene):
+                    lr_t = lr * (0.1 + 0.9 * 0.5 * (1 + math.cos(math.pi * step / total)))
+                    loss = net.train_octree(op, dp, OCCUPIED_FRACTION, pos_weight, lr_t)
+                    step += 1
+                    window.append(loss)
+            if (i + 1) % log_every == 0 or i + 1 == scenes_n:
+                m = float(np.mean(window))
+                hist.append({"scenes": i + 1, "loss": m})
+                log(f"scene {i + 1}/{scenes_n}: mean loss {m:.5f} "
+                    f"({time.perf_counter() - t_start:.0f} s)")
+                window = []
+    return net, hist
+
+
+def rel_l2_occupied(vol, dense):
+    occ = dense >= OCCUPIED_FRACTION * dense.max()
+    return float(np.linalg.norm((vol - dense)[occ]) / max(np.linalg.norm(dense[occ]), 1e-300))
+
+
+def compare(cap, voxels, net, p_values, tau=0.25, levels=4, reps=3):
+    """Dense vs bound-rule octree vs learned octree (each proposal threshold) on one capture."""
+    args = plan_args(cap, voxels)
+    with core.Plan(*args) as dp:
+        td = dp.benchmark(1, reps)["per_exec_ms"]
+        dense = dp.volume()
+    rows = []
+    for p in [None] + list(p_values):
+        with core.Plan.octree(*args, levels=levels, threshold=tau,
+                              proposal_threshold=p if p is not None else 0.5) as op:
+            if p is not None:
+                op.attach_net(net)
+            t = op.benchmark(1, reps)["per_exec_ms"]
+            vol = op.volume()
+            st = op.octree_stats()
+        rows.append({"proposal_threshold": p, "ms": t, "speedup": td / t,
+                     "rel_l2_occupied": rel_l2_occupied(vol, dense),
+                     "fine_voxels": st["active"][-1], "active": st["active"]})
+    return {"dense_ms": td, "occupied_voxels": int((dense >= OCCUPIED_FRACTION * dense.max()).sum()),
+            "rows": rows}
+
+
+def validate(net, n=10, voxels=256, p_values=(0.05, 0.1, 0.2, 0.3, 0.5), side=128, tau=0.25):
+    """Held-out metric-like captures (8 points + 2 patches, seeds 2000+)."""
+    per = []
+    for k in range(n):
+        cap = scenes.metric_capture(side=side, seed=VALIDATION_SEED_BASE + k)
+        r = compare(cap, voxels, net, p_values, tau)
+        per.append(r)
+        log(f"validation scene {k}: dense {r['dense_ms']:.1f} ms; " + "; ".join(
+            f"p={row['proposal_threshold']}: x{row['speedup']:.2f} err {row['rel_l2_occupied']:.4f} "
+            f"active {row['active']}" for row in r["rows"]))
+    summary = []
+    for j, p in enumerate([None] + list(p_values)):
+        sp = [r["rows"][j]["speedup"] for r in per]
+        er = [r["rows"][j]["rel_l2_occupied"] for r in per]
+        summary.append({"proposal_threshold": p, "mean_speedup": float(np.mean(sp)),
+                        "min_speedup": float(np.min(sp)), "mean_err": float(np.mean(er)),
+                        "max_err": float(np.max(er))})
+    return summary, per
+
+
+def pick_threshold(summary, max_err=0.015):
+    """Largest-speedup proposal threshold whose worst validation error stays <= max_err."""
+    ok = [s for s in summary if s["proposal_threshold"] is not None and s["max_err"] <= max_err]
+    if not ok:
+        return None
+    return max(ok, key=lambda s: s["mean_speedup"])["proposal_threshold"]
+
+
+def save(net, path, meta):
+    os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+    np.save(path + ".npy", net.params())
+    with open(path + ".json", "w") as f:
+        json.dump(meta, f, indent=1)
+
+
+def load(path=WEIGHTS):
+    """(Net with the saved parameters, metadata dict)."""
+    params = np.load(path + ".npy")
+    meta = json.load(open(path + ".json")) if os.path.exists(path + ".json") else {}
+    return core.Net(0, params=params), meta
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    t = sub.add_parser("train")
+    t.add_argument("--scenes", type=int, default=2000)
+    t.add_argument("--steps-per-scene", type=int, default=3)
+    t.add_argument("--side", type=int, default=64)
+    t.add_argument("--voxels", type=int, default=256)
+    t.add_argument("--lr", type=float, default=2e-3)
+    t.add_argument("--pos-weight", type=float, default=8.0)
+    t.add_argument("--seed", type=int, default=0)
+    t.add_argument("--validate", type=int, default=10)
+    t.add_argument("--log-every", type=int, default=50)
+    t.add_argument("--out", default=WEIGHTS)
+    v = sub.add_parser("validate")
+    v.add_argument("--weights", default=WEIGHTS)
+    v.add_argument("--n", type=int, default=10)
+    v.add_argument("--voxels", type=int, default=256)
+    a = ap.parse_args(argv)
+    if a.cmd == "train":
+        t0 = time.perf_counter()
+        net, hist = train(a.scenes, a.steps_per_scene, a.side, a.voxels, 4, a.lr, a.pos_weight,
+                          a.seed, a.log_every)
+        meta = {"scenes": a.scenes, "steps_per_scene": a.steps_per_scene,
+                "capture": f"confocal {a.side}x{a.side}, T=4096, 1 mm bins",
+                "voxels": a.voxels, "levels": 4, "lr": a.lr, "pos_weight": a.pos_weight,
+                "occupied_fraction": OCCUPIED_FRACTION, "train_seconds": time.perf_counter() - t0,
+                "history": hist}
+        save(net, a.out, meta)
+        if a.validate:
+            summary, _ = validate(net, a.validate, a.voxels)
+            meta["validation"] = summary
+            meta["proposal_threshold"] = pick_threshold(summary)
+            save(net, a.out, meta)
+        print(json.dumps({k: meta[k] for k in meta if k != "history"}), flush=True)
+    else:
+        net, meta = load(a.weights)
+        summary, _ = validate(net, a.n, a.voxels)
+        print(json.dumps({"validation": summary, "pick": pick_threshold(summary)}), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -196,7 +196,8 @@ typedef struct bpx_octree_options {
     uint32_t flags;           /* BPX_OCTREE_*                                            */
 } bpx_octree_options;
 
-/* Evaluate every block of every coarse level and skip the finest level (training data). */
+/* Run the coarse levels only and skip the exact finest level (training data for the learned
+ * proposals; with threshold 0 every block of every coarse level is evaluated). */
 #define BPX_OCTREE_BOUNDS_ONLY 0x1u
 
 typedef struct bpx_octree_stats {
@@ -280,7 +281,8 @@ bpx_status bpx_copy_benchmark(uint64_t bytes, int32_t iterations, double* mean_m
  * A small 3D conv net (4 layers of 3x3x3 convolutions, 16 channels, BF16 weights and
  * activations on MFMA with FP32 accumulation, ReLU) maps a coarse octree level to one logit
  * per child (8 outputs): "this child contains an occupied voxel".  Inputs per block:
- * U / max U, V / running max, evaluated flag, U / rms U (see bpx_octree_options).  Attached to
+ * U / running max, V / running max, evaluated flag, U / max U, log2(block edge) / 8 (see
+ * bpx_octree_options); cells never evaluated are 0.  Attached to
  * an octree plan it narrows the refinement: a child is refined iff its parent's bound passes the
  * fixed threshold AND its probability exceeds proposal_threshold.  No reference counterpart.
  * Parameters: W[4][27][16 out][16 in] then b[4][16], fp32 (bpx_net_param_count() floats). */
@@ -290,7 +292,7 @@ bpx_status bpx_net_create(uint64_t seed, bpx_net* out);
 void bpx_net_destroy(bpx_net net);
 bpx_status bpx_net_get_params(bpx_net net, float* params);
 bpx_status bpx_net_set_params(bpx_net net, const float* params);
-/* input: fp32 [n0][n1][n2][16] (channels >= 4 ignored: they must be 0); logits out: fp32
+/* input: fp32 [n0][n1][n2][16] (channels >= 5 must be 0); logits out: fp32
  * [n0][n1][n2][16] (channels 0..7 meaningful). */
 bpx_status bpx_net_forward(bpx_net net, const float* input, const uint32_t dims[3], float* logits);
 /* Weighted BCE loss (sum over voxels with mask != 0, or all when mask is NULL, x 8 outputs)
@@ -298,8 +300,10 @@ bpx_status bpx_net_forward(bpx_net net, const float* input, const uint32_t dims[
 bpx_status bpx_net_gradients(bpx_net net, const float* input, const uint32_t dims[3],
                              const uint8_t* labels /* [V][8] */, const uint8_t* mask /* [V] */,
                              float pos_weight, double* loss, float* grads);
-/* One Adam step on one scene: octree_plan executed with BPX_OCTREE_BOUNDS_ONLY (features of every
- * level), dense_plan executed on the same grid (labels: child max >= occupied_fraction x max). */
+/* One Adam step on one scene.  octree_plan has been executed (typically with
+ * BPX_OCTREE_BOUNDS_ONLY, with or without a net attached); the loss covers the blocks that
+ * execution evaluated.  dense_plan has been executed on the same grid; a child's label is
+ * (max of its voxels >= occupied_fraction x max of the volume). */
 bpx_status bpx_net_train_octree(bpx_net net, bpx_plan octree_plan, bpx_plan dense_plan,
                                 float occupied_fraction, float pos_weight, float lr, double* loss);
 /* Attach (or, with NULL, detach) a net to an octree plan; the plan does not own it. */

@@ -15,7 +15,7 @@ from __future__ import annotations
 
 import numpy as np
 
-LAYERS, KW, NIN, NOUT = 4, 27 * 16 * 16, 4, 8
+LAYERS, KW, NIN, NOUT = 4, 27 * 16 * 16, 5, 8
 NPARAMS = LAYERS * (KW + 16)
 
 

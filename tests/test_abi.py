@@ -129,3 +129,20 @@ def test_renderer_poisson_is_seeded_and_unbiased(bpx):
     m = mean > 50
     z = (a[m] - mean[m]) / np.sqrt(mean[m])
     assert abs(z.mean()) < 0.5 and 0.6 < z.std() < 1.4
+
+
+def test_pointer_arguments_keep_temporaries_alive():
+    """ptr(_f32(x)) passes a temporary: the ctypes argument must own a reference to it, since
+    the library runs without the GIL and a freed buffer could be reused during the call."""
+    import gc
+    import weakref
+    from backprojector_amd import _native as nat
+    a = np.arange(8, dtype=np.uint32)
+    r = weakref.ref(a)
+    p = nat.ptr(a)
+    del a
+    gc.collect()
+    assert r() is not None
+    del p
+    gc.collect()
+    assert r() is None

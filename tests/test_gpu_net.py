@@ -112,7 +112,7 @@ def test_gradient_descends_loss(gpu):
     assert l1 < l0, (l0, l1)
 
 
-def small_scene_plans(gpu, seed, N=64, side=16, levels=3):
+def small_scene_plans(gpu, seed, N=64, side=16, levels=3, threshold=0.0):
     rng = np.random.default_rng(seed)
     hidden = np.concatenate([scenes.points(rng, 3), scenes.patch(rng, size=0.2, spacing=0.005)])
     cap = scenes.render(hidden, side, 4096, seed=seed)
@@ -122,7 +122,8 @@ def small_scene_plans(gpu, seed, N=64, side=16, levels=3):
     zero = np.array([-0.5, -0.5, 0.3], f32)
     inc = np.diag([f32(1) / f32(N - 1)] * 3).astype(f32).ravel()
     args = (H, pr, Nv, zero, inc, 0.0, 0.001)
-    return args, gpu.Plan.octree(*args, levels=levels, bounds_only=True), gpu.Plan(*args)
+    return (args, gpu.Plan.octree(*args, levels=levels, threshold=threshold, bounds_only=True),
+            gpu.Plan(*args))
 
 
 def test_train_octree_lowers_loss(gpu):
@@ -143,6 +144,22 @@ def test_train_octree_lowers_loss(gpu):
     first, last = np.mean(losses[:6]), np.mean(losses[-6:])
     print(f"training loss {first:.4f} -> {last:.4f}")
     assert last < 0.8 * first
+
+
+def test_train_on_policy_partial_levels(gpu):
+    """Training from an octree that evaluated only part of each level (bound rule + the net
+    itself attached): the loss covers the evaluated blocks and stays finite."""
+    net = gpu.Net(4)
+    _, op, dp = small_scene_plans(gpu, 5, threshold=0.25)
+    op.attach_net(net)
+    dp.execute()
+    losses = []
+    for _ in range(8):
+        op.execute()
+        st = op.octree_stats()
+        assert st["active"][0] == 8 ** 3 and st["active"][1] <= 16 ** 3 and st["active"][3] == 0
+        losses.append(net.train_octree(op, dp, occupied_fraction=0.25, pos_weight=4.0, lr=3e-3))
+    assert all(np.isfinite(losses)) and all(x > 0 for x in losses)
 
 
 def bias_only_net(gpu, value):
