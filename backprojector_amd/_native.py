@@ -63,7 +63,22 @@ class NlosPrepared(ctypes.Structure):
 
 class OctreeOptions(ctypes.Structure):
     _fields_ = [("levels", ctypes.c_uint32), ("threshold", ctypes.c_float),
-                ("proposal_threshold", ctypes.c_float), ("flags", ctypes.c_uint32)]
+                ("proposal_threshold", ctypes.c_float), ("flags", ctypes.c_uint32),
+                ("level_proposal_threshold", ctypes.c_float * 8)]
+
+
+def make_octree_options(levels=4, threshold=0.25, proposal_threshold=0.5, bounds_only=False):
+    """proposal_threshold: one probability, or one per coarse level."""
+    o = OctreeOptions(int(levels), float(threshold), 0.5, OCTREE_BOUNDS_ONLY if bounds_only else 0)
+    if np.ndim(proposal_threshold) == 0:
+        o.proposal_threshold = float(proposal_threshold)
+    else:
+        per = [float(x) for x in proposal_threshold]
+        if len(per) != int(levels):
+            raise ValueError(f"expected {levels} level proposal thresholds, got {len(per)}")
+        for i, x in enumerate(per):
+            o.level_proposal_threshold[i] = x
+    return o
 
 
 class OctreeStats(ctypes.Structure):
@@ -96,7 +111,7 @@ EXPORTED = [
     "bpx_octree_plan_stats", "bpx_nlos_prepare", "bpx_nlos_prepared_free",
     "bpx_net_param_count", "bpx_net_create", "bpx_net_destroy", "bpx_net_get_params",
     "bpx_net_set_params", "bpx_net_forward", "bpx_net_gradients", "bpx_net_train_octree",
-    "bpx_octree_plan_attach_net",
+    "bpx_octree_plan_attach_net", "bpx_octree_plan_set_options",
 ]
 
 
@@ -148,6 +163,7 @@ def lib():
     L.bpx_octree_plan_create.argtypes = [c_p, u64, u32, c_p, u32, c_p, c_p, c_p, f32, f32, c_p,
                                          ctypes.POINTER(c_p)]
     L.bpx_octree_plan_stats.argtypes = [c_p, c_p]
+    L.bpx_octree_plan_set_options.argtypes = [c_p, c_p]
     L.bpx_nlos_prepare.argtypes = [c_p, c_p, ctypes.c_int, c_p, c_p, c_p, c_p, c_p, c_p, f32, f32,
                                    ctypes.c_int, c_p, c_p, c_p, c_p, ctypes.c_int,
                                    ctypes.POINTER(NlosPrepared)]

@@ -114,7 +114,7 @@ def octree_backprojection(transient_chunk, pairs, voxels_per_side, volume_zero_p
     T = (n // P) if P else 0
     N = _u32(voxels_per_side).reshape(3)
     out = np.zeros(tuple(int(x) for x in N), np.float32)
-    o = nat.OctreeOptions(int(levels), float(threshold), float(proposal_threshold), 0)
+    o = nat.make_octree_options(levels, threshold, proposal_threshold)
     st = nat.OctreeStats()
     check(lib().bpx_octree_backprojection(ptr(H), n, T, ptr(pr), P, ptr(out), ptr(N),
                                           ptr(_f32(volume_zero_pos)),
@@ -183,8 +183,8 @@ class Plan:
         self = cls.__new__(cls)
         self.N = tuple(int(x) for x in _u32(voxels_per_side).reshape(3))
         self.complex = False
-        o = nat.OctreeOptions(int(levels), float(threshold), float(proposal_threshold),
-                              nat.OCTREE_BOUNDS_ONLY if bounds_only else 0)
+        o = nat.make_octree_options(levels, threshold, proposal_threshold, bounds_only)
+        self.levels = int(levels)
         h = ctypes.c_void_p()
         check(lib().bpx_octree_plan_create(ptr(H), n, T, ptr(pr), P, ptr(_u32(self.N)),
                                            ptr(_f32(volume_zero_pos)),
@@ -192,6 +192,11 @@ class Plan:
                                            float(deltaT), ctypes.byref(o), ctypes.byref(h)))
         self._h = h
         return self
+
+    def set_octree_options(self, threshold=0.25, proposal_threshold=0.5, bounds_only=False):
+        """Replaces an octree plan's thresholds (proposal_threshold: one value or one per level)."""
+        o = nat.make_octree_options(self.levels, threshold, proposal_threshold, bounds_only)
+        check(lib().bpx_octree_plan_set_options(self._h, ctypes.byref(o)))
 
     def attach_net(self, net):
         """Attaches a learned proposal Net to an octree plan (None detaches)."""

@@ -238,6 +238,15 @@ bpx_status bpx_octree_plan_stats(bpx_plan plan, bpx_octree_stats* stats) {
     });
 }
 
+bpx_status bpx_octree_plan_set_options(bpx_plan plan, const bpx_octree_options* options) {
+    return guarded([&] {
+        if (!plan || !options) fail(BPX_ERR_INVALID_ARGUMENT, "NULL argument");
+        if (!plan->impl->oct) fail(BPX_ERR_INVALID_ARGUMENT, "not an octree plan");
+        hip_check(hipStreamSynchronize(plan->impl->stream), "hipStreamSynchronize");
+        octree_set_options(*plan->impl->oct, *options);
+    });
+}
+
 bpx_status bpx_plan_execute(bpx_plan plan, void* hip_stream) {
     return guarded([&] {
         if (!plan) fail(BPX_ERR_INVALID_ARGUMENT, "NULL plan");

@@ -51,7 +51,7 @@ void call_cuda_octree_backprojection(const float* transient_chunk, uint32_t tran
                                  voxels_per_side, volume_zero_pos, voxel_inc, t0, deltaT);
         return;
     }
-    const bpx_octree_options o{levels, 0.25f, 0.5f, 0u};
+    const bpx_octree_options o{levels, 0.25f, 0.5f, 0u, {}};
     throw_if(bpx_octree_backprojection(transient_chunk, transient_size, T,
                                        reinterpret_cast<const bpx_pair*>(scanned_pairs.data()),
                                        (uint32_t)scanned_pairs.size(), voxel_volume, voxels_per_side,

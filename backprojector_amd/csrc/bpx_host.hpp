@@ -56,7 +56,8 @@ struct OctreeState {
         uint32_t count = 0;   // blocks evaluated in the last execution
     };
     uint32_t L = 4, ldm = 0;
-    float tau = 0.25f, prob_lo = 0.5f;
+    float tau = 0.25f;
+    float prob[8] = {0.5f, 0.5f, 0.5f, 0.5f, 0.5f, 0.5f, 0.5f, 0.5f};  // per-level proposal thresholds
     bool bounds_only = false;
     std::vector<Level> levels;
     DeviceBuffer ping, pong, fine_list, flags, offs, sums, total, stats, partial;
@@ -141,6 +142,8 @@ PlanImpl* make_plan_window(const CaptureIn& c, const uint32_t n[3], const float 
                            const float vsize[3]);
 PlanImpl* make_plan_octree(const CaptureIn& c, const uint32_t n[3], const float zero[3],
                            const float inc[9], const bpx_octree_options* o);
+// Validates options against the plan's level count and installs thresholds and flags.
+void octree_set_options(OctreeState& st, const bpx_octree_options& o);
 
 }  // namespace bpx
 

@@ -128,6 +128,16 @@ struct OctFineArgs {
 };
 
 // ------------------------------------------------------------------ kernel launchers
+// All levels' window-max tables in one pass (one work-group per row, row held in LDS); returns
+// hipErrorInvalidValue when a row does not fit (callers then use the doubling kernels).
+struct WinmaxLevels {
+    float* out[8];
+    uint32_t W[8];
+    uint32_t count;
+};
+bool winmax_fused_fits(uint32_t Tc);
+hipError_t launch_winmax_fused(const float* in, uint32_t in_pitch, uint32_t rows, uint32_t Tc, uint32_t out_pitch,
+                               const WinmaxLevels& lv, hipStream_t s);
 hipError_t launch_winmax_double(const float* in, uint32_t in_pitch, float* out, uint32_t out_pitch,
                                 uint32_t rows, uint32_t Tc, uint32_t half, hipStream_t s);
 hipError_t launch_winmax_combine(const float* t2, uint32_t pitch2, float* out, uint32_t out_pitch,

@@ -138,7 +138,7 @@ bool net_child_flags(PlanImpl& pl, uint32_t l, hipStream_t s) {
     const OctreeState::Level& lv = st.levels[l];
     level_features(net, lv, false, s);
     net.forward(lv.n[0], lv.n[1], lv.n[2], s);
-    const float p = st.prob_lo;
+    const float p = st.prob[l];
     const float logit_thr = std::log(p / (1.0f - p));
     hip_check(launch_oct_child_flags(static_cast<const uint32_t*>(lv.list.ptr), static_cast<const float*>(lv.U.ptr),
                                      lv.count, static_cast<const double*>(st.stats.ptr),

@@ -54,6 +54,61 @@ __global__ __launch_bounds__(256) void winmax_combine_kernel(const float* __rest
     out[(size_t)p * out_pitch + b] = m;
 }
 
+// One work-group per row: the width-1 table max(0, H) goes to LDS, widths double in LDS
+// (ping-pong), and each level's table is written out when its width is reached, with the same
+// arithmetic as winmax_double_kernel / winmax_combine_kernel.
+constexpr uint32_t kWinmaxLdsBytes = 64u * 1024u;
+
+bool winmax_fused_fits(uint32_t Tc) { return 2ull * 4ull * ((uint64_t)Tc + 1u) <= kWinmaxLdsBytes; }
+
+__global__ __launch_bounds__(1024) void winmax_fused_kernel(const float* __restrict__ in, uint32_t in_pitch,
+                                                            uint32_t Tc, uint32_t out_pitch, WinmaxLevels lv) {
+    extern __shared__ float lds[];
+    float* a = lds;
+    float* b = lds + (Tc + 1u);
+    const uint32_t p = blockIdx.x;
+    const float* r = in + (size_t)p * in_pitch;
+    const uint32_t n = Tc + 1u;
+    for (uint32_t t = threadIdx.x; t < n; t += blockDim.x) a[t] = t < Tc ? fmaxf(0.0f, r[t]) : 0.0f;
+    __syncthreads();
+    uint32_t wmax = 1u;
+    for (uint32_t k = 0; k < lv.count; ++k) wmax = max(wmax, lv.W[k]);
+    for (uint32_t w = 1u;; w *= 2u) {
+        for (uint32_t k = 0; k < lv.count; ++k) {
+            if (lv.W[k] < w || lv.W[k] >= 2u * w) continue;
+            const uint32_t shift = lv.W[k] - w;
+            float* o = lv.out[k] + (size_t)p * out_pitch;
+            for (uint32_t t = threadIdx.x; t < n; t += blockDim.x) {
+                float m = a[t];
+                if (t + shift < Tc) m = fmaxf(m, a[t + shift]);
+                o[t] = m;
+            }
+        }
+        if (2u * w > wmax) break;
+        for (uint32_t t = threadIdx.x; t < n; t += blockDim.x) {
+            float m = 0.0f;
+            if (t < Tc) {
+                m = fmaxf(m, a[t]);
+                if (t + w < Tc) m = fmaxf(m, a[t + w]);
+            }
+            b[t] = m;
+        }
+        __syncthreads();
+        float* tmp = a;
+        a = b;
+        b = tmp;
+    }
+}
+
+hipError_t launch_winmax_fused(const float* in, uint32_t in_pitch, uint32_t rows, uint32_t Tc, uint32_t out_pitch,
+                               const WinmaxLevels& lv, hipStream_t s) {
+    if (!winmax_fused_fits(Tc) || lv.count == 0 || lv.count > 8) return hipErrorInvalidValue;
+    if (rows == 0) return hipSuccess;
+    const uint32_t lds = 2u * 4u * (Tc + 1u);
+    hipLaunchKernelGGL(winmax_fused_kernel, dim3(rows), dim3(1024), lds, s, in, in_pitch, Tc, out_pitch, lv);
+    return hipGetLastError();
+}
+
 hipError_t launch_winmax_double(const float* in, uint32_t in_pitch, float* out, uint32_t out_pitch,
                                 uint32_t rows, uint32_t Tc, uint32_t half, hipStream_t s) {
     const uint64_t n = (uint64_t)rows * (Tc + 1u);
@@ -77,55 +132,73 @@ __device__ __forceinline__ void unpack_block(uint32_t c, uint32_t& x, uint32_t& 
     z = c >> 20;
 }
 
-// One lane per block; pairs in the wave-uniform loop (pair data in SGPRs).  Coarse levels
-// have few blocks, so the pair range is split into segments (blockIdx.y) whose partial sums
-// are added in segment order by oct_bound_combine_kernel (deterministic).
+// K blocks per lane (consecutive list entries: siblings, so their bins share cache lines);
+// pairs in the wave-uniform loop (pair data in SGPRs).  Coarse levels have few blocks, so the
+// pair range is split into segments (blockIdx.y) whose partial sums are added in segment
+// order by oct_bound_combine_kernel (deterministic).
+template <int K>
 __global__ __launch_bounds__(256) void oct_bound_kernel(OctBoundArgs a) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t ii = min(i, a.nblocks - 1u);
-    uint32_t bx, by, bz;
-    unpack_block(a.blocks[ii], bx, by, bz);
-    // block centre (double precision, then float) and representative fine voxel
+    const uint32_t base = (blockIdx.x * blockDim.x + threadIdx.x) * K;
     const double h = 0.5 * (double)(a.B - 1u);
-    float m[3], v[3];
-    for (int k = 0; k < 3; ++k)
-        m[k] = (float)((double)a.g.zero[k] + a.g.inc[k] * ((double)bx * a.B + h) +
-                       a.g.inc[3 + k] * ((double)by * a.B + h) + a.g.inc[6 + k] * ((double)bz * a.B + h));
     const uint32_t rep = a.B / 2u;
-    {
+    float mx[K], my[K], mz[K], vx[K], vy[K], vz[K];
+#pragma unroll
+    for (int r = 0; r < K; ++r) {
+        uint32_t bx, by, bz;
+        unpack_block(a.blocks[min(base + r, a.nblocks - 1u)], bx, by, bz);
+        // block centre (double precision, then float) and representative fine voxel
+        float m[3], v[3];
+        for (int k = 0; k < 3; ++k)
+            m[k] = (float)((double)a.g.zero[k] + a.g.inc[k] * ((double)bx * a.B + h) +
+                           a.g.inc[3 + k] * ((double)by * a.B + h) + a.g.inc[6 + k] * ((double)bz * a.B + h));
         const float fx = (float)(bx * a.B + rep), fy = (float)(by * a.B + rep), fz = (float)(bz * a.B + rep);
         for (int k = 0; k < 3; ++k) {
             float t = a.g.zero[k] + a.g.inc[k] * fx;
             t = t + a.g.inc[3 + k] * fy;
             v[k] = t + a.g.inc[6 + k] * fz;
         }
+        mx[r] = m[0];
+        my[r] = m[1];
+        mz[r] = m[2];
+        vx[r] = v[0];
+        vy[r] = v[1];
+        vz[r] = v[2];
     }
     const auto mr = make_rsrc(a.M, a.m_bytes);
     const auto er = make_rsrc(a.E, a.e_bytes);
     const float fTc = (float)a.Tc;
-    float U = 0.0f, Vs = 0.0f;
+    float U[K], Vs[K];
+#pragma unroll
+    for (int r = 0; r < K; ++r) U[r] = Vs[r] = 0.0f;
     const uint32_t p0 = blockIdx.y * a.chunk, p1 = min(a.P, p0 + a.chunk);
     for (uint32_t p = p0; p < p1; ++p) {
         const float4 pr = a.pairs[p];
-        // window start for the bounding sphere
-        const float sm = sq3(pr.x - m[0], pr.y - m[1], pr.z - m[2]);
-        const float dm = __builtin_amdgcn_sqrtf(sm);
-        const float qlo = __builtin_fmaf(dm, a.inv2, pr.w + a.cR);
-        const uint32_t blo = (uint32_t)(int)__builtin_amdgcn_fmed3f(qlo, 0.0f, fTc);
-        U = U + buf_f32(mr, blo * 4u, p * a.ldm * 4u);
-        // exact tap of the representative fine voxel
-        const float s = sq3(pr.x - v[0], pr.y - v[1], pr.z - v[2]);
-        const float q1 = __builtin_fmaf(__builtin_amdgcn_sqrtf(s), a.inv2, pr.w);
-        const uint32_t j = (uint32_t)(int)__builtin_amdgcn_fmed3f(q1, 0.0f, fTc);
-        Vs = Vs + FusedF32::pick(er, j, p * a.lde * 8u, s);
+        const uint32_t moff = p * a.ldm * 4u, eoff = p * a.lde * 8u;
+        const float c1R = pr.w + a.cR;
+#pragma unroll
+        for (int r = 0; r < K; ++r) {
+            // window start for the bounding sphere
+            const float sm = sq3(pr.x - mx[r], pr.y - my[r], pr.z - mz[r]);
+            const float qlo = __builtin_fmaf(__builtin_amdgcn_sqrtf(sm), a.inv2, c1R);
+            const uint32_t blo = (uint32_t)(int)__builtin_amdgcn_fmed3f(qlo, 0.0f, fTc);
+            U[r] = U[r] + buf_f32(mr, blo * 4u, moff);
+            // exact tap of the representative fine voxel
+            const float s = sq3(pr.x - vx[r], pr.y - vy[r], pr.z - vz[r]);
+            const float q1 = __builtin_fmaf(__builtin_amdgcn_sqrtf(s), a.inv2, pr.w);
+            const uint32_t j = (uint32_t)(int)__builtin_amdgcn_fmed3f(q1, 0.0f, fTc);
+            Vs[r] = Vs[r] + FusedF32::pick(er, j, eoff, s);
+        }
     }
-    if (i < a.nblocks) {
+#pragma unroll
+    for (int r = 0; r < K; ++r) {
+        const uint32_t i = base + r;
+        if (i >= a.nblocks) continue;
         if (a.partU) {
-            a.partU[(size_t)blockIdx.y * a.nblocks + i] = U;
-            a.partV[(size_t)blockIdx.y * a.nblocks + i] = Vs;
+            a.partU[(size_t)blockIdx.y * a.nblocks + i] = U[r];
+            a.partV[(size_t)blockIdx.y * a.nblocks + i] = Vs[r];
         } else {
-            a.U[i] = U;
-            a.V[i] = Vs;
+            a.U[i] = U[r];
+            a.V[i] = Vs[r];
         }
     }
 }
@@ -146,9 +219,9 @@ __global__ __launch_bounds__(256) void oct_bound_combine_kernel(const float* __r
 }
 
 uint32_t oct_bound_segments(uint32_t nblocks, uint32_t P, uint64_t part_capacity) {
-    // aim for >= 2^19 lanes in flight, segments of >= 64 pairs, partials within capacity
+    // aim for >= 2^18 lanes in flight, segments of >= 64 pairs, partials within capacity
     if (nblocks == 0 || P == 0) return 1u;
-    uint64_t s = ((1ull << 19) + nblocks - 1) / nblocks;
+    uint64_t s = ((1ull << 18) + nblocks - 1) / nblocks;
     s = std::min<uint64_t>(s, std::max<uint32_t>(P / 64u, 1u));
     s = std::min<uint64_t>(s, part_capacity / nblocks);
     return (uint32_t)std::max<uint64_t>(s, 1u);
@@ -165,7 +238,12 @@ hipError_t launch_oct_bound(const OctBoundArgs& a_in, uint32_t segments, hipStre
         a.chunk = (a.P + segments - 1u) / segments;
         segments = (a.P + a.chunk - 1u) / a.chunk;
     }
-    hipLaunchKernelGGL(oct_bound_kernel, dim3((a.nblocks + 255u) / 256u, segments), dim3(256), 0, s, a);
+    if (segments > 1u) {
+        hipLaunchKernelGGL(oct_bound_kernel<1>, dim3((a.nblocks + 255u) / 256u, segments), dim3(256), 0, s, a);
+    } else {
+        const uint32_t lanes = (a.nblocks + 1u) / 2u;
+        hipLaunchKernelGGL(oct_bound_kernel<2>, dim3((lanes + 255u) / 256u, 1u), dim3(256), 0, s, a);
+    }
     if (segments > 1u)
         hipLaunchKernelGGL(oct_bound_combine_kernel, dim3((a.nblocks + 255u) / 256u), dim3(256), 0, s, a.partU,
                            a.partV, a.nblocks, segments, a.U, a.V);

@@ -30,15 +30,30 @@ static double block_radius(const Grid& g, uint32_t B) {
     return 0.5 * (double)(B - 1u) * best;
 }
 
-PlanImpl* make_plan_octree(const CaptureIn& c, const uint32_t n[3], const float zero[3],
-                           const float inc[9], const bpx_octree_options* oo) {
-    bpx_octree_options o{4u, 0.25f, 0.5f, 0u};
-    if (oo) o = *oo;
-    if (o.levels == 0 || o.levels > 8) fail(BPX_ERR_INVALID_ARGUMENT, "octree levels must be 1..8");
+void octree_set_options(OctreeState& st, const bpx_octree_options& o) {
+    if (o.levels != st.L) fail(BPX_ERR_INVALID_ARGUMENT, "octree levels do not match the plan");
     if (!(o.threshold >= 0.0f) || !std::isfinite(o.threshold))
         fail(BPX_ERR_INVALID_ARGUMENT, "octree threshold must be finite and >= 0");
-    if (!(o.proposal_threshold > 0.0f && o.proposal_threshold < 1.0f))
-        fail(BPX_ERR_INVALID_ARGUMENT, "proposal threshold must lie in (0, 1)");
+    auto valid_p = [](float p) { return p > 0.0f && p < 1.0f; };
+    if (!valid_p(o.proposal_threshold)) fail(BPX_ERR_INVALID_ARGUMENT, "proposal threshold must lie in (0, 1)");
+    float prob[8];
+    for (uint32_t l = 0; l < 8; ++l) {
+        const float p = o.level_proposal_threshold[l];
+        if (p != 0.0f && !valid_p(p))
+            fail(BPX_ERR_INVALID_ARGUMENT, "level proposal thresholds must be 0 or lie in (0, 1)");
+        prob[l] = p != 0.0f ? p : o.proposal_threshold;
+    }
+    if (o.flags & ~BPX_OCTREE_BOUNDS_ONLY) fail(BPX_ERR_INVALID_ARGUMENT, "unknown octree flags");
+    st.tau = o.threshold;
+    std::memcpy(st.prob, prob, sizeof(prob));
+    st.bounds_only = (o.flags & BPX_OCTREE_BOUNDS_ONLY) != 0;
+}
+
+PlanImpl* make_plan_octree(const CaptureIn& c, const uint32_t n[3], const float zero[3],
+                           const float inc[9], const bpx_octree_options* oo) {
+    bpx_octree_options o{4u, 0.25f, 0.5f, 0u, {}};
+    if (oo) o = *oo;
+    if (o.levels == 0 || o.levels > 8) fail(BPX_ERR_INVALID_ARGUMENT, "octree levels must be 1..8");
     for (int k = 0; k < 3; ++k) {
         if (n[k] == 0 || n[k] % (1u << o.levels) != 0)
             fail(BPX_ERR_INVALID_ARGUMENT, "voxels_per_side must be divisible by 2^levels");
@@ -52,9 +67,7 @@ PlanImpl* make_plan_octree(const CaptureIn& c, const uint32_t n[3], const float 
     std::unique_ptr<PlanImpl> pl(make_plan_single_tap(c, n, zero, inc, &po));
     auto st = std::make_unique<OctreeState>();
     st->L = o.levels;
-    st->bounds_only = (o.flags & BPX_OCTREE_BOUNDS_ONLY) != 0;
-    st->tau = o.threshold;
-    st->prob_lo = o.proposal_threshold;
+    octree_set_options(*st, o);
     const uint32_t P = pl->P, Tc = pl->T;
     st->ldm = round_up32((uint64_t)Tc + 1, 64);
     const uint64_t mbytes = (uint64_t)std::max<uint32_t>(P, 1) * st->ldm * 4;
@@ -74,8 +87,10 @@ PlanImpl* make_plan_octree(const CaptureIn& c, const uint32_t n[3], const float 
         lv.stats.alloc(8 * sizeof(double));
         st->levels.push_back(std::move(lv));
     }
-    st->ping.alloc(mbytes);
-    st->pong.alloc(mbytes);
+    if (!winmax_fused_fits(Tc) || st->L > 8) {  // doubling passes through global memory
+        st->ping.alloc(mbytes);
+        st->pong.alloc(mbytes);
+    }
     const uint64_t nvox = (uint64_t)n[0] * n[1] * n[2];
     st->fine_list.alloc(4 * nvox);
     st->flags.alloc(4 * nvox);
@@ -93,10 +108,22 @@ PlanImpl* make_plan_octree(const CaptureIn& c, const uint32_t n[3], const float 
     return pl.release();
 }
 
-// Builds each level's window-max table from the raw rows by doubling widths.
+// Builds each level's window-max table from the raw rows by doubling widths (in LDS, one pass,
+// when a row fits; else through the ping/pong buffers).
 static void build_winmax(PlanImpl& pl, hipStream_t s) {
     OctreeState& st = *pl.oct;
     const uint32_t P = pl.P, Tc = pl.T;
+    if (!st.ping.ptr) {
+        WinmaxLevels wl{};
+        for (uint32_t l = 0; l < st.L; ++l) {
+            wl.out[wl.count] = static_cast<float*>(st.levels[l].M.ptr);
+            wl.W[wl.count] = st.levels[l].W;
+            ++wl.count;
+        }
+        hip_check(launch_winmax_fused(static_cast<const float*>(pl.raw.ptr), Tc, P, Tc, st.ldm, wl, s),
+                  "window-max kernel (LDS)");
+        return;
+    }
     uint32_t wmax = 1;
     for (const auto& lv : st.levels) wmax = std::max(wmax, lv.W);
     float* cur = static_cast<float*>(st.ping.ptr);
@@ -128,6 +155,8 @@ void execute_octree(PlanImpl& pl, hipStream_t s) {
     hip_check(launch_fill(static_cast<float*>(pl.vol.ptr), nvox, 0.0f, s), "fill kernel");
     if (pl.P == 0) return;
     hip_check(hipEventRecord(pl.ev[0], s), "hipEventRecord");
+    void* E = pl.H.ptr;
+    const uint32_t e_bytes = (uint32_t)pl.H.bytes;
     hip_check(launch_fused_pad(static_cast<const float*>(pl.raw.ptr), pl.P, pl.T, pl.T, pl.H.ptr, pl.ldh,
                                pl.format, 1.0f / pl.scale, s),
               "row staging kernel");
@@ -152,7 +181,7 @@ void execute_octree(PlanImpl& pl, hipStream_t s) {
         const uint32_t* cur = static_cast<const uint32_t*>(lv.list.ptr);
         OctBoundArgs a{};
         a.M = static_cast<const float*>(lv.M.ptr);
-        a.E = pl.H.ptr;
+        a.E = E;
         a.pairs = static_cast<const float4*>(pl.pair4.ptr);
         a.blocks = cur;
         a.nblocks = n;
@@ -163,7 +192,7 @@ void execute_octree(PlanImpl& pl, hipStream_t s) {
         a.ldm = st.ldm;
         a.lde = pl.ldh;
         a.m_bytes = (uint32_t)lv.M.bytes;
-        a.e_bytes = (uint32_t)pl.H.bytes;
+        a.e_bytes = e_bytes;
         a.g = pl.g;
         a.B = lv.B;
         a.cR = lv.cR;
@@ -201,7 +230,7 @@ void execute_octree(PlanImpl& pl, hipStream_t s) {
     hip_check(hipEventRecord(pl.ev[2], s), "hipEventRecord");
     if (!st.bounds_only) {
         OctFineArgs f{};
-        f.E = pl.H.ptr;
+        f.E = E;
         f.pairs = static_cast<const float4*>(pl.pair4.ptr);
         f.voxels = static_cast<const uint32_t*>(st.fine_list.ptr);
         f.n = n;
@@ -209,7 +238,7 @@ void execute_octree(PlanImpl& pl, hipStream_t s) {
         f.P = pl.P;
         f.Tc = pl.T;
         f.lde = pl.ldh;
-        f.e_bytes = (uint32_t)pl.H.bytes;
+        f.e_bytes = e_bytes;
         f.kahan_block = pl.kahan_block;
         f.g = pl.g;
         f.inv2 = pl.inv2;

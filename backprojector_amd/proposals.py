@@ -31,7 +31,14 @@ from . import core, nlosbpy, scenes
 OCCUPIED_FRACTION = 0.25
 WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "octree_proposal")
 TRAIN_SEED_BASE = 50000         # training scenes 50000 ... 50000 + n - 1
-VALIDATION_SEED_BASE = 60000    # held-out test captures 60000 ...
+VALIDATION_SEED_BASE = 55000    # captures that choose the proposal thresholds
+TEST_SEED_BASE = 60000          # held-out test captures 60000 ...
+
+# Proposal-threshold configurations tried on the validation captures: one probability for
+# every level, or one per level (coarse levels cautious: a veto there drops a large block).
+CANDIDATES = [0.05, 0.1, 0.2, 0.3, 0.5,
+              [0.05, 0.1, 0.2, 0.4], [0.05, 0.1, 0.3, 0.5], [0.1, 0.2, 0.3, 0.5],
+              [0.05, 0.1, 0.2, 0.6], [0.1, 0.2, 0.4, 0.6], [0.05, 0.2, 0.4, 0.7]]
 
 
 def log(*a):
@@ -70,13 +77,17 @@ def training_capture(index, side, voxels, T=4096):
 
 def train(scenes_n=2000, steps_per_scene=3, side=64, voxels=256, levels=4, lr=2e-3,
           pos_weight=8.0, seed=0, log_every=50, init=None, tau=0.25, on_policy=0.5,
-          policy_threshold=0.1):
+          policy_threshold=0.1, label_fraction=0.2):
     """Trains a net on `scenes_n` distinct synthetic scenes (Adam, cosine learning rate).
 
     Each scene's octree runs its coarse levels with the bound rule (threshold tau), so the net
     sees the same evaluated sets and features as at inference; for the last `on_policy`
     fraction of the scenes the net being trained is attached (proposal threshold
-    policy_threshold) and the training data follow its own selections.  Returns (net, history)."""
+    policy_threshold) and the training data follow its own selections.  Children are labelled
+    occupied at label_fraction x max, below the evaluation's OCCUPIED_FRACTION, so that voxels
+    near the occupancy boundary lie inside the positive class (a safety margin: one missed
+    occupied voxel costs far more error than one extra computed voxel costs time).
+    Returns (net, history)."""
     net = core.Net(seed, params=init)
     hist = []
     total = scenes_n * steps_per_scene
@@ -103,7 +114,7 @@ def train(scenes_n=2000, steps_per_scene=3, side=64, voxels=256, levels=4, lr=2e
                                        f"N {args[2]}, t0 {args[5]}: {e}") from e
                 for _ in range(steps_per_scene):
                     lr_t = lr * (0.1 + 0.9 * 0.5 * (1 + math.cos(math.pi * step / total)))
-                    loss = net.train_octree(op, dp, OCCUPIED_FRACTION, pos_weight, lr_t)
+                    loss = net.train_octree(op, dp, label_fraction, pos_weight, lr_t)
                     step += 1
                     window.append(loss)
             if (i + 1) % log_every == 0 or i + 1 == scenes_n:
@@ -120,50 +131,58 @@ def rel_l2_occupied(vol, dense):
     return float(np.linalg.norm((vol - dense)[occ]) / max(np.linalg.norm(dense[occ]), 1e-300))
 
 
-def compare(cap, voxels, net, p_values, tau=0.25, levels=4, reps=3):
-    """Dense vs bound-rule octree vs learned octree (each proposal threshold) on one capture."""
+def compare(cap, voxels, net, configs, tau=0.25, levels=4, reps=3):
+    """Dense vs bound-rule octree vs learned octree (each proposal configuration) on one
+    capture; one octree plan is reused with its thresholds replaced."""
     args = plan_args(cap, voxels)
     with core.Plan(*args) as dp:
         td = dp.benchmark(1, reps)["per_exec_ms"]
         dense = dp.volume()
     rows = []
-    for p in [None] + list(p_values):
-        with core.Plan.octree(*args, levels=levels, threshold=tau,
-                              proposal_threshold=p if p is not None else 0.5) as op:
-            if p is not None:
+    with core.Plan.octree(*args, levels=levels, threshold=tau) as op:
+        for cfg in [None] + list(configs):
+            if cfg is None:
+                op.attach_net(None)
+            else:
                 op.attach_net(net)
+                op.set_octree_options(tau, cfg)
             t = op.benchmark(1, reps)["per_exec_ms"]
             vol = op.volume()
             st = op.octree_stats()
-        rows.append({"proposal_threshold": p, "ms": t, "speedup": td / t,
-                     "rel_l2_occupied": rel_l2_occupied(vol, dense),
-                     "fine_voxels": st["active"][-1], "active": st["active"]})
+            rows.append({"proposal_threshold": cfg, "ms": t, "speedup": td / t,
+                         "rel_l2_occupied": rel_l2_occupied(vol, dense),
+                         "fine_voxels": st["active"][-1], "active": st["active"]})
     return {"dense_ms": td, "occupied_voxels": int((dense >= OCCUPIED_FRACTION * dense.max()).sum()),
             "rows": rows}
 
 
-def validate(net, n=10, voxels=256, p_values=(0.05, 0.1, 0.2, 0.3, 0.5), side=128, tau=0.25):
-    """Held-out metric-like captures (8 points + 2 patches, seeds 2000+)."""
+def evaluate(net, seeds, voxels=256, configs=CANDIDATES, side=128, tau=0.25, label="validation"):
+    """C5-like captures (128^2 confocal, T=4096, 8 points + 2 patches) for each seed."""
     per = []
-    for k in range(n):
-        cap = scenes.metric_capture(side=side, seed=VALIDATION_SEED_BASE + k)
-        r = compare(cap, voxels, net, p_values, tau)
+    for k, seed in enumerate(seeds):
+        cap = scenes.metric_capture(side=side, seed=seed)
+        r = compare(cap, voxels, net, configs, tau)
         per.append(r)
-        log(f"validation scene {k}: dense {r['dense_ms']:.1f} ms; " + "; ".join(
-            f"p={row['proposal_threshold']}: x{row['speedup']:.2f} err {row['rel_l2_occupied']:.4f} "
-            f"active {row['active']}" for row in r["rows"]))
+        log(f"{label} capture {seed}: dense {r['dense_ms']:.1f} ms; " + "; ".join(
+            f"p={row['proposal_threshold']}: x{row['speedup']:.2f} err {row['rel_l2_occupied']:.4f}"
+            for row in r["rows"]))
     summary = []
-    for j, p in enumerate([None] + list(p_values)):
+    for j, cfg in enumerate([None] + list(configs)):
         sp = [r["rows"][j]["speedup"] for r in per]
         er = [r["rows"][j]["rel_l2_occupied"] for r in per]
-        summary.append({"proposal_threshold": p, "mean_speedup": float(np.mean(sp)),
+        summary.append({"proposal_threshold": cfg, "mean_speedup": float(np.mean(sp)),
                         "min_speedup": float(np.min(sp)), "mean_err": float(np.mean(er)),
                         "max_err": float(np.max(er))})
     return summary, per
 
 
+def validate(net, n=10, voxels=256, configs=CANDIDATES, side=128, tau=0.25):
+    return evaluate(net, range(VALIDATION_SEED_BASE, VALIDATION_SEED_BASE + n), voxels, configs,
+                    side, tau, "validation")
+
+
 def pick_threshold(summary, max_err=0.015):
-    """Largest-speedup proposal threshold whose worst validation error stays <= max_err."""
+    """Fastest proposal configuration whose worst validation error stays <= max_err."""
     ok = [s for s in summary if s["proposal_threshold"] is not None and s["max_err"] <= max_err]
     if not ok:
         return None
@@ -184,6 +203,21 @@ def load(path=WEIGHTS):
     return core.Net(0, params=params), meta
 
 
+def select_and_test(net, meta, n_val=10, n_test=20, voxels=256):
+    """Chooses the proposal configuration on the validation captures, then reports it (and
+    the bound rule alone) on the held-out test captures."""
+    summary, _ = validate(net, n_val, voxels)
+    meta["validation"] = summary
+    pick = pick_threshold(summary)
+    meta["proposal_threshold"] = pick
+    if pick is not None and n_test:
+        test, _ = evaluate(net, range(TEST_SEED_BASE, TEST_SEED_BASE + n_test), voxels, [pick],
+                           label="test")
+        meta["test"] = {"captures": n_test, "seeds": f"{TEST_SEED_BASE}..{TEST_SEED_BASE + n_test - 1}",
+                        "bound_rule": test[0], "learned": test[1]}
+    return meta
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -196,33 +230,39 @@ def main(argv=None):
     t.add_argument("--pos-weight", type=float, default=8.0)
     t.add_argument("--seed", type=int, default=0)
     t.add_argument("--validate", type=int, default=10)
+    t.add_argument("--test", type=int, default=20)
     t.add_argument("--log-every", type=int, default=50)
+    t.add_argument("--label-fraction", type=float, default=0.2)
     t.add_argument("--out", default=WEIGHTS)
     v = sub.add_parser("validate")
     v.add_argument("--weights", default=WEIGHTS)
     v.add_argument("--n", type=int, default=10)
+    v.add_argument("--test", type=int, default=20)
     v.add_argument("--voxels", type=int, default=256)
+    v.add_argument("--out", default=None, help="write weights + updated metadata here")
     a = ap.parse_args(argv)
     if a.cmd == "train":
         t0 = time.perf_counter()
         net, hist = train(a.scenes, a.steps_per_scene, a.side, a.voxels, 4, a.lr, a.pos_weight,
-                          a.seed, a.log_every)
-        meta = {"scenes": a.scenes, "steps_per_scene": a.steps_per_scene,
+                          a.seed, a.log_every, label_fraction=a.label_fraction)
+        meta = {"scenes": a.scenes, "seeds": f"{TRAIN_SEED_BASE}..{TRAIN_SEED_BASE + a.scenes - 1}",
+                "steps_per_scene": a.steps_per_scene,
                 "capture": f"confocal {a.side}x{a.side}, T=4096, 1 mm bins",
                 "voxels": a.voxels, "levels": 4, "lr": a.lr, "pos_weight": a.pos_weight,
-                "occupied_fraction": OCCUPIED_FRACTION, "train_seconds": time.perf_counter() - t0,
+                "occupied_fraction": OCCUPIED_FRACTION, "label_fraction": a.label_fraction,
+                "train_seconds": time.perf_counter() - t0,
                 "history": hist}
         save(net, a.out, meta)
         if a.validate:
-            summary, _ = validate(net, a.validate, a.voxels)
-            meta["validation"] = summary
-            meta["proposal_threshold"] = pick_threshold(summary)
+            select_and_test(net, meta, a.validate, a.test, a.voxels)
             save(net, a.out, meta)
         print(json.dumps({k: meta[k] for k in meta if k != "history"}), flush=True)
     else:
         net, meta = load(a.weights)
-        summary, _ = validate(net, a.n, a.voxels)
-        print(json.dumps({"validation": summary, "pick": pick_threshold(summary)}), flush=True)
+        select_and_test(net, meta, a.n, a.test, a.voxels)
+        if a.out:
+            save(net, a.out, meta)
+        print(json.dumps({k: meta[k] for k in meta if k != "history"}), flush=True)
 
 
 if __name__ == "__main__":

@@ -194,6 +194,7 @@ typedef struct bpx_octree_options {
     float threshold;          /* tau (default 0.25)                                      */
     float proposal_threshold; /* learned proposals: refine iff probability > this (0.5) */
     uint32_t flags;           /* BPX_OCTREE_*                                            */
+    float level_proposal_threshold[8]; /* per coarse level (0 = use proposal_threshold)  */
 } bpx_octree_options;
 
 /* Run the coarse levels only and skip the exact finest level (training data for the learned
@@ -250,6 +251,8 @@ bpx_status bpx_octree_plan_create(const float* transient_chunk, uint64_t transie
                                   const bpx_octree_options* options, bpx_plan* plan_out);
 /* Statistics of the plan's last octree execution. */
 bpx_status bpx_octree_plan_stats(bpx_plan plan, bpx_octree_stats* stats);
+/* Replaces the thresholds and flags of an octree plan (options->levels must match the plan). */
+bpx_status bpx_octree_plan_set_options(bpx_plan plan, const bpx_octree_options* options);
 bpx_status bpx_plan_execute(bpx_plan plan, void* hip_stream);
 bpx_status bpx_plan_synchronize(bpx_plan plan);
 /* Copies the device volume to host (float or interleaved complex). */

@@ -56,6 +56,21 @@ def test_octree_threshold_zero_is_dense(gpu, oracle):
     assert st["active"][3] == np.prod(N)
 
 
+def test_octree_large_level_two_blocks_per_lane(gpu):
+    """A level of 64^3 = 262144 blocks runs the unsegmented two-blocks-per-lane bound kernel;
+    threshold 0 must still reproduce the dense volume bit for bit (LDS window-max build too)."""
+    H, pr, N, zero, inc = confocal_setup(side=16, N=128, T=4096)
+    dense = gpu.backprojection(H, pr, N, zero, inc, 0.0, 0.001)
+    vol, st = gpu.octree_backprojection(H, pr, N, zero, inc, 0.0, 0.001, levels=2, threshold=0.0)
+    assert st["active"] == [32 ** 3, 64 ** 3, 128 ** 3]
+    assert np.array_equal(vol, dense)
+    vol2, st2 = gpu.octree_backprojection(H, pr, N, zero, inc, 0.0, 0.001, levels=2,
+                                          threshold=0.25)
+    computed = vol2 != 0
+    assert np.array_equal(vol2[computed], dense[computed])
+    assert np.all(computed[dense >= 0.25 * dense.max()])
+
+
 def test_octree_plan_reuse_and_rejects(gpu):
     H, pr, N, zero, inc = confocal_setup(side=16, N=32)
     with gpu.Plan.octree(H, pr, N, zero, inc, 0.0, 0.001, levels=2, threshold=0.3) as plan:
