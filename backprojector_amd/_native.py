@@ -95,6 +95,28 @@ class OctreeStats(ctypes.Structure):
                     fine_updates=int(self.fine_updates), bound_updates=int(self.bound_updates))
 
 
+class CaptureDesc(ctypes.Structure):
+    _fields_ = [("transient_chunk", ctypes.c_void_p), ("transient_size", ctypes.c_uint64),
+                ("T", ctypes.c_uint32), ("scanned_pairs", ctypes.c_void_p),
+                ("num_pairs", ctypes.c_uint32), ("voxels_per_side", ctypes.c_uint32 * 3),
+                ("volume_zero_pos", ctypes.c_float * 3), ("voxel_inc", ctypes.c_float * 9),
+                ("t0", ctypes.c_float), ("deltaT", ctypes.c_float)]
+
+
+class BatchConfig(ctypes.Structure):
+    _fields_ = [("lane_voxels", ctypes.c_uint32), ("groups", ctypes.c_uint32),
+                ("order", ctypes.c_uint32)]
+
+    def to_dict(self):
+        return dict(lane_voxels=int(self.lane_voxels), groups=int(self.groups),
+                    order=int(self.order))
+
+
+class BatchTiming(ctypes.Structure):
+    _fields_ = [("per_exec_ms", ctypes.c_double), ("updates_per_s", ctypes.c_double),
+                ("updates", ctypes.c_uint64), ("iterations", ctypes.c_int32)]
+
+
 FLAG_REFERENCE_PAIR_TRUNCATION = 0x1
 OCTREE_BOUNDS_ONLY = 0x1
 
@@ -112,6 +134,9 @@ EXPORTED = [
     "bpx_net_param_count", "bpx_net_create", "bpx_net_destroy", "bpx_net_get_params",
     "bpx_net_set_params", "bpx_net_forward", "bpx_net_gradients", "bpx_net_train_octree",
     "bpx_octree_plan_attach_net", "bpx_octree_plan_set_options",
+    "bpx_batch_create", "bpx_batch_size", "bpx_batch_tune", "bpx_batch_set_config",
+    "bpx_batch_get_config", "bpx_batch_execute", "bpx_batch_synchronize", "bpx_batch_read_volume",
+    "bpx_batch_latencies", "bpx_batch_benchmark", "bpx_batch_destroy",
 ]
 
 
@@ -182,6 +207,19 @@ def lib():
     L.bpx_net_train_octree.argtypes = [c_p, c_p, c_p, f32, f32, f32,
                                        ctypes.POINTER(ctypes.c_double)]
     L.bpx_octree_plan_attach_net.argtypes = [c_p, c_p]
+    L.bpx_batch_create.argtypes = [ctypes.POINTER(CaptureDesc), u32, c_p, ctypes.POINTER(c_p)]
+    L.bpx_batch_size.argtypes = [c_p]
+    L.bpx_batch_size.restype = u32
+    L.bpx_batch_tune.argtypes = [c_p, i32, ctypes.POINTER(BatchConfig)]
+    L.bpx_batch_set_config.argtypes = [c_p, ctypes.POINTER(BatchConfig)]
+    L.bpx_batch_get_config.argtypes = [c_p, ctypes.POINTER(BatchConfig)]
+    L.bpx_batch_execute.argtypes = [c_p, c_p]
+    L.bpx_batch_synchronize.argtypes = [c_p]
+    L.bpx_batch_read_volume.argtypes = [c_p, u32, c_p]
+    L.bpx_batch_latencies.argtypes = [c_p, c_p]
+    L.bpx_batch_benchmark.argtypes = [c_p, i32, i32, ctypes.POINTER(BatchTiming)]
+    L.bpx_batch_destroy.argtypes = [c_p]
+    L.bpx_batch_destroy.restype = None
     _lib = L
     return L
 

@@ -319,3 +319,97 @@ class Net:
             self.close()
         except Exception:
             pass
+
+
+class Batch:
+    """Many confocal fp32 captures reconstructed by one persistent work-queue launch replayed
+    from a hipGraph (no reference counterpart; each volume equals backprojection() of that
+    capture alone).  captures: iterable of (transient_chunk, pairs, voxels_per_side,
+    volume_zero_pos, voxel_inc, t0, deltaT)."""
+
+    def __init__(self, captures, kahan_block=0, reference_pair_truncation=False):
+        self._keep = []
+        descs = []
+        self.shapes = []
+        for (chunk, pairs, N, zero, inc, t0, dT) in captures:
+            H, n = _samples(chunk, False)
+            pr = as_pairs(pairs)
+            P = len(pr)
+            Nv = _u32(N).reshape(3)
+            d = nat.CaptureDesc()
+            d.transient_chunk = H.ctypes.data
+            d.transient_size = n
+            d.T = (n // P) if P else 1     # an empty capture still needs T > 0
+            d.scanned_pairs = pr.ctypes.data
+            d.num_pairs = P
+            d.voxels_per_side[:] = [int(x) for x in Nv]
+            d.volume_zero_pos[:] = [float(x) for x in _f32(zero).reshape(3)]
+            d.voxel_inc[:] = [float(x) for x in _f32(inc).reshape(9)]
+            d.t0 = float(t0)
+            d.deltaT = float(dT)
+            self._keep += [H, pr]
+            descs.append(d)
+            self.shapes.append(tuple(int(x) for x in Nv))
+        arr = (nat.CaptureDesc * max(len(descs), 1))(*descs)
+        flags = nat.FLAG_REFERENCE_PAIR_TRUNCATION if reference_pair_truncation else 0
+        opt = nat.make_options("f32", kahan_block, flags, -1)
+        h = ctypes.c_void_p()
+        check(lib().bpx_batch_create(arr, len(descs), ctypes.byref(opt), ctypes.byref(h)))
+        self._h = h
+        self._keep = None   # the library staged every capture on the device
+
+    def __len__(self):
+        return int(lib().bpx_batch_size(self._h))
+
+    def tune(self, reps=2):
+        c = nat.BatchConfig()
+        check(lib().bpx_batch_tune(self._h, int(reps), ctypes.byref(c)))
+        return c.to_dict()
+
+    def config(self):
+        c = nat.BatchConfig()
+        check(lib().bpx_batch_get_config(self._h, ctypes.byref(c)))
+        return c.to_dict()
+
+    def set_config(self, lane_voxels=2, groups=0, order=0):
+        c = nat.BatchConfig(int(lane_voxels), int(groups), int(order))
+        check(lib().bpx_batch_set_config(self._h, ctypes.byref(c)))
+
+    def execute(self, stream=None):
+        check(lib().bpx_batch_execute(self._h, stream))
+
+    def synchronize(self):
+        check(lib().bpx_batch_synchronize(self._h))
+
+    def volume(self, i):
+        out = np.zeros(self.shapes[i], np.float32)
+        check(lib().bpx_batch_read_volume(self._h, int(i), ptr(out)))
+        return out
+
+    def latencies(self):
+        out = np.zeros(len(self.shapes), np.float64)
+        check(lib().bpx_batch_latencies(self._h, ptr(out)))
+        return out
+
+    def benchmark(self, warmup=1, iterations=5):
+        t = nat.BatchTiming()
+        check(lib().bpx_batch_benchmark(self._h, int(warmup), int(iterations), ctypes.byref(t)))
+        return dict(per_exec_ms=t.per_exec_ms, updates_per_s=t.updates_per_s,
+                    updates=int(t.updates), iterations=t.iterations)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().bpx_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

@@ -1,0 +1,330 @@
+// Batched uneven confocal captures: one device-resident plan per capture (rows and threshold
+// tables staged once), all gathers in ONE persistent work-queue launch, and the whole
+// per-batch sequence (staging of every capture, queue reset, persistent gather) captured in a
+// hipGraph that bpx_batch_execute replays.  A project-written autotuner times a few launch
+// configurations (voxels per lane, resident groups, queue order) on the batch itself.
+// No reference counterpart: the reference reconstructs one capture per call
+// (call_cuda_backprojection, backproject_cuda.cu:370-500).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <vector>
+
+#include "bpx_host.hpp"
+
+namespace bpx {
+
+struct BatchImpl {
+    std::vector<std::unique_ptr<PlanImpl>> plans;
+    std::vector<uint64_t> updates;   // voxel-pair updates per capture
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[2] = {};
+    DeviceBuffer caps, flags, items, next, remaining, tiles;
+    DeviceBuffer stamps;   // uint64 [1 + captures]: batch start, then each capture's completion
+    uint32_t n_items = 0;
+    bpx_batch_config cfg{2u, 0u, 0u};
+    uint32_t groups = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    double wall_rate_khz = 0.0;
+    ~BatchImpl() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    void build(const bpx_batch_config& c);
+};
+
+// Row staging + threshold table of a fused fp32 confocal plan (the per-capture part of
+// PlanImpl::execute before its gather kernel).
+static void stage_plan(PlanImpl& pl, hipStream_t s) {
+    if (pl.P == 0) return;
+    hip_check(launch_fused_pad(static_cast<const float*>(pl.raw.ptr), pl.P, pl.T, pl.T, pl.H.ptr, pl.ldh, pl.format,
+                               1.0f / pl.scale, s),
+              "row staging kernel");
+    hip_check(launch_stable(static_cast<const float*>(pl.pair8.ptr), pl.P, pl.T, pl.t0, pl.dT,
+                            static_cast<float*>(pl.H.ptr), pl.ldh * 2u, 2u, 1u, s),
+              "threshold table kernel");
+}
+
+void BatchImpl::build(const bpx_batch_config& c) {
+    if (c.lane_voxels != 2u && c.lane_voxels != 4u) fail(BPX_ERR_INVALID_ARGUMENT, "lane_voxels must be 2 or 4");
+    if (c.order > 1u) fail(BPX_ERR_INVALID_ARGUMENT, "order must be 0 or 1");
+    const int K = (int)c.lane_voxels;
+    const uint32_t n = (uint32_t)plans.size();
+    // per-capture arguments and tiles for this K
+    std::vector<ConfocalArgs> args(n);
+    std::vector<uint32_t> fl(n), nt(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        const PlanImpl& pl = *plans[i];
+        ConfocalArgs& a = args[i];
+        a.H = pl.H.ptr;
+        a.pairs = static_cast<const float4*>(pl.pair4.ptr);
+        a.out = pl.vol.ptr;
+        a.P = pl.P;
+        a.Tc = pl.T;
+        a.ldh = pl.ldh;
+        a.h_bytes = (uint32_t)pl.H.bytes;
+        a.g = pl.g;
+        a.t = make_tiles(pl.g.n, K);
+        a.inv2 = pl.inv2;
+        a.scale = pl.scale;
+        a.lz = pl.lz;
+        a.kahan_block = pl.kahan_block;
+        fl[i] = (pl.sep ? 1u : 0u) | (pl.sep && pl.flat ? 2u : 0u);
+        nt[i] = pl.P ? 4u * a.t.count : 0u;  // one item per wave quarter of every tile
+    }
+    // queue: (capture, tile * 4 + quarter), the quarters of a tile adjacent; order 0 = captures
+    // with most pairs first, order 1 = capture order
+    std::vector<uint32_t> order(n);
+    std::iota(order.begin(), order.end(), 0u);
+    if (c.order == 0)
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
+            return (uint64_t)plans[x]->P * c.lane_voxels > (uint64_t)plans[y]->P * c.lane_voxels;
+        });
+    std::vector<uint32_t> it;
+    for (uint32_t i : order)
+        for (uint32_t t = 0; t < nt[i]; ++t) {
+            it.push_back(i);
+            it.push_back(t);
+        }
+    n_items = (uint32_t)(it.size() / 2);
+    caps.alloc(sizeof(ConfocalArgs) * std::max<uint32_t>(n, 1));
+    flags.alloc(4u * std::max<uint32_t>(n, 1));
+    tiles.alloc(4u * std::max<uint32_t>(n, 1));
+    items.alloc(8u * std::max<uint32_t>(n_items, 1));
+    hip_check(hipMemcpy(caps.ptr, args.data(), sizeof(ConfocalArgs) * n, hipMemcpyHostToDevice), "hipMemcpy");
+    hip_check(hipMemcpy(flags.ptr, fl.data(), 4u * n, hipMemcpyHostToDevice), "hipMemcpy");
+    hip_check(hipMemcpy(tiles.ptr, nt.data(), 4u * n, hipMemcpyHostToDevice), "hipMemcpy");
+    if (n_items) hip_check(hipMemcpy(items.ptr, it.data(), 8u * n_items, hipMemcpyHostToDevice), "hipMemcpy");
+    const int resident = batch_resident_groups(K);
+    if (resident <= 0) fail(BPX_ERR_DEVICE, "cannot query the occupancy of the batch kernel");
+    groups = c.groups ? std::min<uint32_t>(c.groups, (uint32_t)resident) : (uint32_t)resident;
+    groups = std::max<uint32_t>(1u, std::min<uint32_t>(groups, std::max<uint32_t>(n_items, 1u)));
+    cfg = c;
+    cfg.groups = groups;
+    // the per-batch sequence as one graph (no memset nodes: the queue reset is a kernel)
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    try {
+        for (auto& pl : plans) {
+            if (pl->P == 0) {
+                const size_t nvox = (size_t)pl->g.n[0] * pl->g.n[1] * pl->g.n[2];
+                hip_check(launch_fill(static_cast<float*>(pl->vol.ptr), nvox, 0.0f, stream), "fill kernel");
+            } else {
+                stage_plan(*pl, stream);
+            }
+        }
+        hip_check(launch_batch_reset(static_cast<uint32_t*>(next.ptr), static_cast<uint32_t*>(remaining.ptr),
+                                     static_cast<const uint32_t*>(tiles.ptr), n, static_cast<uint64_t*>(stamps.ptr),
+                                     stream),
+                  "batch reset kernel");
+        if (n_items) {
+            BatchArgs b{};
+            b.caps = static_cast<const ConfocalArgs*>(caps.ptr);
+            b.flags = static_cast<const uint32_t*>(flags.ptr);
+            b.items = static_cast<const uint2*>(items.ptr);
+            b.n_items = n_items;
+            b.next = static_cast<uint32_t*>(next.ptr);
+            b.remaining = static_cast<uint32_t*>(remaining.ptr);
+            b.done = static_cast<uint64_t*>(stamps.ptr) + 1;
+            hip_check(launch_batch_confocal(b, K, groups, stream), "batch gather kernel");
+        }
+    } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    hip_check(hipStreamEndCapture(stream, &graph), "hipStreamEndCapture");
+    hip_check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+}
+
+}  // namespace bpx
+
+using namespace bpx;
+
+struct bpx_batch_s {
+    std::unique_ptr<BatchImpl> impl;
+};
+
+namespace {
+template <class F>
+bpx_status batch_guard(F&& f) {
+    clear_error();
+    try {
+        f();
+        return BPX_OK;
+    } catch (const Failure& e) {
+        return set_error(e.status, e.msg);
+    } catch (const std::bad_alloc&) {
+        return set_error(BPX_ERR_OUT_OF_MEMORY, "host allocation failed");
+    }
+}
+
+double run_timed(BatchImpl& b, int32_t iters) {
+    hip_check(hipEventRecord(b.ev[0], b.stream), "hipEventRecord");
+    for (int32_t i = 0; i < iters; ++i) hip_check(hipGraphLaunch(b.exec, b.stream), "hipGraphLaunch");
+    hip_check(hipEventRecord(b.ev[1], b.stream), "hipEventRecord");
+    hip_check(hipEventSynchronize(b.ev[1]), "hipEventSynchronize");
+    float ms = 0.0f;
+    hip_check(hipEventElapsedTime(&ms, b.ev[0], b.ev[1]), "hipEventElapsedTime");
+    return (double)ms / std::max(iters, 1);
+}
+}  // namespace
+
+extern "C" {
+
+bpx_status bpx_batch_create(const bpx_capture_desc* captures, uint32_t count, const bpx_options* options,
+                            bpx_batch* out) {
+    return batch_guard([&] {
+        if (!out) fail(BPX_ERR_INVALID_ARGUMENT, "NULL output");
+        *out = nullptr;
+        if (count && !captures) fail(BPX_ERR_INVALID_ARGUMENT, "NULL captures");
+        if (!device_ready()) fail(BPX_ERR_NO_DEVICE, "no gfx950 device available");
+        bpx_options o{BPX_STORAGE_F32, 32u, 0u, -1};
+        if (options) o = *options;
+        if (o.storage != BPX_STORAGE_F32) fail(BPX_ERR_UNSUPPORTED, "batches take fp32 storage");
+        o.variant = 5;  // fused rows: the layout the batch kernel reads
+        auto b = std::make_unique<BatchImpl>();
+        hip_check(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate");
+        for (auto& e : b->ev) hip_check(hipEventCreate(&e), "hipEventCreate");
+        int dev = 0, rate = 0;
+        hip_check(hipGetDevice(&dev), "hipGetDevice");
+        hip_check(hipDeviceGetAttribute(&rate, hipDeviceAttributeWallClockRate, dev), "hipDeviceGetAttribute");
+        b->wall_rate_khz = rate > 0 ? (double)rate : 1.0e5;
+        for (uint32_t i = 0; i < count; ++i) {
+            const bpx_capture_desc& d = captures[i];
+            CaptureIn c{d.transient_chunk, d.transient_size, d.T, d.scanned_pairs, d.num_pairs, d.t0, d.deltaT, false};
+            std::unique_ptr<PlanImpl> pl(make_plan_single_tap(c, d.voxels_per_side, d.volume_zero_pos, d.voxel_inc, &o));
+            if (pl->P && (pl->path != kPathConfocal || !pl->fused || pl->format != kF32))
+                fail(BPX_ERR_UNSUPPORTED, "capture " + std::to_string(i) + " is not a confocal capture");
+            b->updates.push_back((uint64_t)pl->P * pl->g.n[0] * pl->g.n[1] * pl->g.n[2]);
+            b->plans.push_back(std::move(pl));
+        }
+        const uint32_t n = std::max<uint32_t>(count, 1);
+        b->next.alloc(4);
+        b->remaining.alloc(4u * n);
+        b->stamps.alloc(8u * (n + 1u));
+        b->build(bpx_batch_config{2u, 0u, 0u});
+        auto h = std::make_unique<bpx_batch_s>();
+        h->impl = std::move(b);
+        *out = h.release();
+    });
+}
+
+void bpx_batch_destroy(bpx_batch batch) {
+    if (!batch) return;
+    (void)hipStreamSynchronize(batch->impl->stream);
+    delete batch;
+}
+
+bpx_status bpx_batch_set_config(bpx_batch batch, const bpx_batch_config* config) {
+    return batch_guard([&] {
+        if (!batch || !config) fail(BPX_ERR_INVALID_ARGUMENT, "NULL argument");
+        hip_check(hipStreamSynchronize(batch->impl->stream), "hipStreamSynchronize");
+        batch->impl->build(*config);
+    });
+}
+
+bpx_status bpx_batch_get_config(bpx_batch batch, bpx_batch_config* config) {
+    return batch_guard([&] {
+        if (!batch || !config) fail(BPX_ERR_INVALID_ARGUMENT, "NULL argument");
+        *config = batch->impl->cfg;
+    });
+}
+
+bpx_status bpx_batch_tune(bpx_batch batch, int32_t reps, bpx_batch_config* chosen) {
+    return batch_guard([&] {
+        if (!batch) fail(BPX_ERR_INVALID_ARGUMENT, "NULL batch");
+        BatchImpl& b = *batch->impl;
+        hip_check(hipStreamSynchronize(b.stream), "hipStreamSynchronize");
+        reps = std::max(reps, 1);
+        bpx_batch_config best{2u, 0u, 0u};
+        double best_ms = INFINITY;
+        for (uint32_t k : {2u, 4u}) {
+            const int resident = batch_resident_groups((int)k);
+            for (uint32_t frac : {1u, 2u})
+                for (uint32_t order : {0u, 1u}) {
+                    const bpx_batch_config c{k, resident > 0 ? std::max(1u, (uint32_t)resident / frac) : 0u, order};
+                    b.build(c);
+                    (void)run_timed(b, 1);  // warm-up
+                    const double ms = run_timed(b, reps);
+                    if (ms < best_ms) {
+                        best_ms = ms;
+                        best = b.cfg;
+                    }
+                }
+        }
+        b.build(best);
+        if (chosen) *chosen = b.cfg;
+    });
+}
+
+bpx_status bpx_batch_execute(bpx_batch batch, void* hip_stream) {
+    return batch_guard([&] {
+        if (!batch) fail(BPX_ERR_INVALID_ARGUMENT, "NULL batch");
+        BatchImpl& b = *batch->impl;
+        hipStream_t s = hip_stream ? (hipStream_t)hip_stream : b.stream;
+        hip_check(hipGraphLaunch(b.exec, s), "hipGraphLaunch");
+    });
+}
+
+bpx_status bpx_batch_synchronize(bpx_batch batch) {
+    return batch_guard([&] {
+        if (!batch) fail(BPX_ERR_INVALID_ARGUMENT, "NULL batch");
+        hip_check(hipStreamSynchronize(batch->impl->stream), "hipStreamSynchronize");
+    });
+}
+
+bpx_status bpx_batch_read_volume(bpx_batch batch, uint32_t index, float* voxel_volume) {
+    return batch_guard([&] {
+        if (!batch || !voxel_volume) fail(BPX_ERR_INVALID_ARGUMENT, "NULL argument");
+        BatchImpl& b = *batch->impl;
+        if (index >= b.plans.size()) fail(BPX_ERR_INVALID_ARGUMENT, "capture index out of range");
+        hip_check(hipStreamSynchronize(b.stream), "hipStreamSynchronize");
+        const PlanImpl& pl = *b.plans[index];
+        const size_t nvox = (size_t)pl.g.n[0] * pl.g.n[1] * pl.g.n[2];
+        hip_check(hipMemcpy(voxel_volume, pl.vol.ptr, 4u * nvox, hipMemcpyDeviceToHost), "hipMemcpy");
+    });
+}
+
+bpx_status bpx_batch_latencies(bpx_batch batch, double* ms) {
+    return batch_guard([&] {
+        if (!batch || !ms) fail(BPX_ERR_INVALID_ARGUMENT, "NULL argument");
+        BatchImpl& b = *batch->impl;
+        hip_check(hipStreamSynchronize(b.stream), "hipStreamSynchronize");
+        const size_t n = b.plans.size();
+        std::vector<uint64_t> st(n + 1);
+        hip_check(hipMemcpy(st.data(), b.stamps.ptr, 8u * (n + 1), hipMemcpyDeviceToHost), "hipMemcpy");
+        for (size_t i = 0; i < n; ++i)
+            ms[i] = b.plans[i]->P ? (double)(int64_t)(st[1 + i] - st[0]) / b.wall_rate_khz : 0.0;
+    });
+}
+
+bpx_status bpx_batch_benchmark(bpx_batch batch, int32_t warmup, int32_t iterations, bpx_batch_timing* out) {
+    return batch_guard([&] {
+        if (!batch || !out) fail(BPX_ERR_INVALID_ARGUMENT, "NULL argument");
+        BatchImpl& b = *batch->impl;
+        hip_check(hipStreamSynchronize(b.stream), "hipStreamSynchronize");
+        if (warmup > 0) (void)run_timed(b, warmup);
+        const int32_t it = std::max(iterations, 1);
+        out->per_exec_ms = run_timed(b, it);
+        out->iterations = it;
+        uint64_t total = 0;
+        for (uint64_t u : b.updates) total += u;
+        out->updates = total;
+        out->updates_per_s = out->per_exec_ms > 0.0 ? (double)total / (out->per_exec_ms * 1e-3) : 0.0;
+    });
+}
+
+uint32_t bpx_batch_size(bpx_batch batch) { return batch ? (uint32_t)batch->impl->plans.size() : 0u; }
+
+}  // extern "C"

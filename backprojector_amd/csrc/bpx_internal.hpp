@@ -52,6 +52,16 @@ struct ConfocalArgs {
 };
 
 // Factorised exhaustive-grid kernel parameters (threshold table in path-length space).
+// Batched confocal fp32 captures (persistent work queue, see batch_confocal_kernel).
+struct BatchArgs {
+    const ConfocalArgs* caps;   // per capture; tiles for the batch's voxels per lane
+    const uint32_t* flags;      // per capture: bit 0 SEP, bit 1 FLAT
+    const uint2* items;         // (capture, tile * 4 + wave quarter) in queue order
+    uint32_t n_items;
+    uint32_t* next;             // queue head (0 at launch)
+    uint32_t* remaining;        // per capture: items not finished
+    uint64_t* done;             // per capture: wall clock when its last item finished (0 at launch)
+};
 struct GridArgs {
     const void* H;          // padded rows [L*C][ldh], pair p = l*C + c
     const float4* lasers;   // (x, y, z, laser_wall) per laser point
@@ -201,6 +211,10 @@ hipError_t launch_confocal(const ConfocalArgs& a, int format, int variant, bool 
                            hipStream_t stream);
 hipError_t launch_grid(const GridArgs& a, int format, int variant, hipStream_t stream);
 hipError_t launch_pairs(const PairsArgs& a, int format, int variant, hipStream_t stream);
+hipError_t launch_batch_reset(uint32_t* next, uint32_t* remaining, const uint32_t* tiles, uint32_t n,
+                              uint64_t* start, hipStream_t s);
+hipError_t launch_batch_confocal(const BatchArgs& b, int K, uint32_t groups, hipStream_t s);
+int batch_resident_groups(int K);  // work-groups resident at once (0 if unknown)
 hipError_t launch_window(const WindowArgs& a, int complex_samples, hipStream_t stream);
 hipError_t launch_abs_max(const float* src, uint64_t n, float* result_dev, hipStream_t stream);
 hipError_t launch_copy(void* dst, const void* src, uint64_t bytes, hipStream_t stream);

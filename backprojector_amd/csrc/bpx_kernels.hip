@@ -349,16 +349,19 @@ Tiles make_tiles(const uint32_t n[3], int K) {
 struct LaneVoxels {
     uint32_t ix, iy, iz0;
 };
-__device__ __forceinline__ LaneVoxels lane_voxels(const Tiles& t, int K) {
-    const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
+// Wave w (0..3) of a tile covers the 4 x 4 x 4K quarter (w & 1, w >> 1) of its 8 x 8 columns.
+__device__ __forceinline__ LaneVoxels lane_voxels_of(const Tiles& t, int K, uint32_t tile, uint32_t w) {
     const uint32_t tz = tile % t.tz, txy = tile / t.tz;
     const uint32_t ty = txy % t.ty, tx = txy / t.ty;
-    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    const uint32_t lane = threadIdx.x & 63u;
     LaneVoxels lv;
     lv.ix = tx * 8u + (w & 1u) * 4u + (lane & 3u);
     lv.iy = ty * 8u + (w >> 1) * 4u + ((lane >> 2) & 3u);
     lv.iz0 = tz * (4u * (uint32_t)K) + (lane >> 4);
     return lv;
+}
+__device__ __forceinline__ LaneVoxels lane_voxels(const Tiles& t, int K) {
+    return lane_voxels_of(t, K, xcd_remap(blockIdx.x, gridDim.x), threadIdx.x >> 6);
 }
 
 // A.1 voxel position, backproject_cuda.cu:144-148.
@@ -421,10 +424,12 @@ struct FusedSource {
     __device__ __forceinline__ acc_t fetch(uint32_t j, float s) const { return Fz::pick(er, j, soff, s); }
 };
 
+// Wave quarter w of one 8 x 8 x 4K tile of a confocal capture (the body of the gather
+// kernels; no work-group barrier, so a wave can run it alone).
 template <class Src, int K, bool SEP, bool FLAT>
-__global__ __launch_bounds__(256) void confocal_kernel(ConfocalArgs a) {
+__device__ __forceinline__ void confocal_tile(const ConfocalArgs& a, uint32_t tile, uint32_t w) {
     using acc_t = typename Src::acc_t;
-    const LaneVoxels lv = lane_voxels(a.t, K);
+    const LaneVoxels lv = lane_voxels_of(a.t, K, tile, w);
     const uint32_t cx = min(lv.ix, a.g.n[0] - 1u), cy = min(lv.iy, a.g.n[1] - 1u);
     float vx[SEP ? 1 : K], vy[SEP ? 1 : K], vz[K], dz2[K];
 #pragma unroll
@@ -500,6 +505,90 @@ __global__ __launch_bounds__(256) void confocal_kernel(ConfocalArgs a) {
             }
         }
     }
+}
+
+template <class Src, int K, bool SEP, bool FLAT>
+__global__ __launch_bounds__(256) void confocal_kernel(ConfocalArgs a) {
+    confocal_tile<Src, K, SEP, FLAT>(a, xcd_remap(blockIdx.x, gridDim.x), threadIdx.x >> 6);
+}
+
+// ------------------------------------------------------------------- batched captures
+// Persistent work queue over (capture, tile quarter) items of many confocal fp32 captures:
+// every WAVE takes the next item, runs the same tile body as the single-capture kernel (each
+// volume is bit-identical to its own plan's), and the wave that finishes a capture's last
+// quarter stamps the wall clock.  Waves leave when the queue is empty.
+// The queue head and the completion counters are updated by every lane with an increment that
+// is 1 on lane 0 and 0 elsewhere, never under a lane-divergent branch: a branch on "lane 0"
+// around the head update inside this loop lets the compiler split the wave into lane 0 and the
+// rest (the rest then re-read a stale head and never leave).  There is no work-group barrier.
+template <int K>
+__global__ __launch_bounds__(256) void batch_confocal_kernel(BatchArgs b) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t one = lane == 0u ? 1u : 0u;
+    // a wave can take at most every item once: the bound makes termination unconditional
+    for (uint32_t taken = 0; taken <= b.n_items; ++taken) {
+        // all lanes active: the first active lane is lane 0, whose old value is the item
+        const uint32_t it = __builtin_amdgcn_readfirstlane(atomicAdd(b.next, one));
+        if (it >= b.n_items) return;
+        const uint2 item = b.items[it];
+        const uint32_t c = __builtin_amdgcn_readfirstlane(item.x);
+        const uint32_t q = __builtin_amdgcn_readfirstlane(item.y);  // tile * 4 + quarter
+        const ConfocalArgs a = b.caps[c];
+        const uint32_t f = b.flags[c];
+        if (f == 3u) confocal_tile<FusedSource<FusedF32>, K, true, true>(a, q >> 2, q & 3u);
+        else if (f == 1u) confocal_tile<FusedSource<FusedF32>, K, true, false>(a, q >> 2, q & 3u);
+        else confocal_tile<FusedSource<FusedF32>, K, false, false>(a, q >> 2, q & 3u);
+        __threadfence();
+        const uint32_t left = atomicSub(&b.remaining[c], one);
+        const unsigned long long stamp = (one && left == 1u) ? (unsigned long long)wall_clock64() : 0ull;
+        atomicMax(reinterpret_cast<unsigned long long*>(&b.done[c]), stamp);
+    }
+}
+
+// Resets the queue (next = 0, remaining = items per capture, completion stamps 0) and stamps
+// the start time: stamps[0] = start, stamps[1 + i] = completion of capture i.
+__global__ void batch_reset_kernel(uint32_t* next, uint32_t* remaining, const uint32_t* tiles, uint32_t n,
+                                   uint64_t* start) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        remaining[i] = tiles[i];
+        start[1 + i] = 0ull;  // completion stamps (done) follow the start stamp
+    }
+    if (i == 0) {
+        *next = 0u;
+        *start = wall_clock64();
+    }
+}
+
+hipError_t launch_batch_reset(uint32_t* next, uint32_t* remaining, const uint32_t* tiles, uint32_t n,
+                              uint64_t* start, hipStream_t s) {
+    hipLaunchKernelGGL(batch_reset_kernel, dim3((n + 255u) / 256u + (n == 0u ? 1u : 0u)), dim3(256), 0, s, next,
+                       remaining, tiles, n, start);
+    return hipGetLastError();
+}
+
+int batch_resident_groups(int K) {
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    const hipError_t e = K == 4 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_confocal_kernel<4>, 256, 0)
+                                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_confocal_kernel<2>, 256, 0);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return cus * per_cu;
+}
+
+hipError_t launch_batch_confocal(const BatchArgs& b, int K, uint32_t groups, hipStream_t s) {
+    if (groups == 0u) return hipErrorInvalidValue;
+    if (K == 4) hipLaunchKernelGGL(batch_confocal_kernel<4>, dim3(groups), dim3(256), 0, s, b);
+    else if (K == 2) hipLaunchKernelGGL(batch_confocal_kernel<2>, dim3(groups), dim3(256), 0, s, b);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
 }
 
 // ------------------------------------------------------------------- exhaustive grid

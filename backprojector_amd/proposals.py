@@ -181,9 +181,11 @@ def validate(net, n=10, voxels=256, configs=CANDIDATES, side=128, tau=0.25):
                     side, tau, "validation")
 
 
-def pick_threshold(summary, max_err=0.015):
-    """Fastest proposal configuration whose worst validation error stays <= max_err."""
-    ok = [s for s in summary if s["proposal_threshold"] is not None and s["max_err"] <= max_err]
+def pick_threshold(summary, max_err=0.02, mean_err=0.01):
+    """Fastest proposal configuration whose worst validation error stays <= max_err and whose
+    mean error stays <= mean_err."""
+    ok = [s for s in summary if s["proposal_threshold"] is not None and s["max_err"] <= max_err
+          and s["mean_err"] <= mean_err]
     if not ok:
         return None
     return max(ok, key=lambda s: s["mean_speedup"])["proposal_threshold"]
@@ -233,6 +235,9 @@ def main(argv=None):
     t.add_argument("--test", type=int, default=20)
     t.add_argument("--log-every", type=int, default=50)
     t.add_argument("--label-fraction", type=float, default=0.2)
+    t.add_argument("--on-policy", type=float, default=0.5,
+                   help="fraction of the scenes (the last ones) trained with the net attached")
+    t.add_argument("--init", default=None, help="start from these weights (another epoch)")
     t.add_argument("--out", default=WEIGHTS)
     v = sub.add_parser("validate")
     v.add_argument("--weights", default=WEIGHTS)
@@ -243,13 +248,16 @@ def main(argv=None):
     a = ap.parse_args(argv)
     if a.cmd == "train":
         t0 = time.perf_counter()
+        init = np.load(a.init + ".npy") if a.init else None
         net, hist = train(a.scenes, a.steps_per_scene, a.side, a.voxels, 4, a.lr, a.pos_weight,
-                          a.seed, a.log_every, label_fraction=a.label_fraction)
+                          a.seed, a.log_every, init=init, on_policy=a.on_policy,
+                          label_fraction=a.label_fraction)
         meta = {"scenes": a.scenes, "seeds": f"{TRAIN_SEED_BASE}..{TRAIN_SEED_BASE + a.scenes - 1}",
                 "steps_per_scene": a.steps_per_scene,
                 "capture": f"confocal {a.side}x{a.side}, T=4096, 1 mm bins",
                 "voxels": a.voxels, "levels": 4, "lr": a.lr, "pos_weight": a.pos_weight,
                 "occupied_fraction": OCCUPIED_FRACTION, "label_fraction": a.label_fraction,
+                "on_policy": a.on_policy, "init": os.path.basename(a.init) if a.init else None,
                 "train_seconds": time.perf_counter() - t0,
                 "history": hist}
         save(net, a.out, meta)

@@ -141,3 +141,20 @@ def metric_capture(T=4096, side=128, seed=1000, patch_spacing=0.002) -> Capture:
     hidden = np.concatenate([points(rng, 8), patch(rng, spacing=patch_spacing),
                              patch(rng, spacing=patch_spacing)])
     return render(hidden, side, T, 0.0, 0.001, True, seed=seed)
+
+
+def batch_captures(count=64, seed=1004, sides=(16, 32, 64, 96, 128), Ts=(1024, 2048, 4096),
+                   voxels=(16, 32, 64, 96)):
+    """Uneven confocal captures for batched reconstruction: capture i draws its wall side, bin
+    count T (bin width 4.096 m / T) and voxels per side from SeedSequence(seed).spawn(count)[i];
+    hidden scene 2-8 points and one 0.3 m patch.  Returns [(Capture, voxels_per_side)]."""
+    out = []
+    for ss in np.random.SeedSequence(seed).spawn(count):
+        rng = np.random.default_rng(ss)
+        side = int(rng.choice(sides))
+        T = int(rng.choice(Ts))
+        N = int(rng.choice(voxels))
+        hidden = np.concatenate([points(rng, int(rng.integers(2, 9))), patch(rng, spacing=0.004)])
+        cap = render(hidden, side, T, 0.0, 4.096 / T, True, seed=int(rng.integers(1 << 31)))
+        out.append((cap, N))
+    return out

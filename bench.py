@@ -45,35 +45,78 @@ def parse():
     ap.add_argument("--octree", type=int, default=256,
                     help="fine volume side of the octree-vs-plain comparison (0 = skip)")
     ap.add_argument("--octree-tau", type=float, default=0.25)
+    ap.add_argument("--octree-proposal", type=float, default=None,
+                    help="proposal threshold of the learned octree (default: the shipped choice)")
+    ap.add_argument("--batch", type=int, default=64,
+                    help="uneven captures of the batched comparison (0 = skip)")
     return ap.parse_args()
 
 
-OCCUPIED_FRACTION = 0.25   # occupied voxels: plain value >= 0.25 * max (error is measured there)
-
-
-def octree_comparison(bp, cap, side, tau, reps=3):
-    """Plain dense reconstruction vs octree refinement at side^3 on the same capture."""
+def octree_comparison(bp, cap, side, tau, proposal=None, reps=3):
+    """Plain dense reconstruction vs octree refinement at side^3 on the same capture: the
+    sound bound rule alone, and with the shipped learned proposal net attached."""
+    from backprojector_amd import proposals
+    occ_frac = proposals.OCCUPIED_FRACTION
     N = (side,) * 3
-    prep = bp.nlosbpy.prepare(*cap.nlos_args(N))
-    args = (prep["chunk"], prep["pairs"], N, prep["zero"], prep["inc"], float(prep["chunked_t0"]),
-            cap.deltaT)
+    args = proposals.plan_args(cap, side)
     with bp.Plan(*args) as plain:
         tp = plain.benchmark(1, reps)
         vp = plain.volume()
+    occ = vp >= occ_frac * vp.max()
+    upd_plain = len(args[1]) * side ** 3
+    net, meta = proposals.load()
+    if proposal is None:
+        proposal = meta.get("proposal_threshold") or 0.2
+    rows = {}
     with bp.Plan.octree(*args, levels=4, threshold=tau) as oct_:
-        to = oct_.benchmark(1, reps)
-        vo = oct_.volume()
-        st = oct_.octree_stats()
-    occ = vp >= OCCUPIED_FRACTION * vp.max()
-    err = float(np.linalg.norm((vo - vp)[occ]) / max(np.linalg.norm(vp[occ]), 1e-300))
-    upd_plain = len(prep["pairs"]) * side ** 3
+        for name, p in (("bound_rule", None), ("learned", proposal)):
+            if p is not None:
+                oct_.attach_net(net)
+                oct_.set_octree_options(tau, p)
+            t = oct_.benchmark(1, reps)
+            vo = oct_.volume()
+            st = oct_.octree_stats()
+            err = float(np.linalg.norm((vo - vp)[occ]) / max(np.linalg.norm(vp[occ]), 1e-300))
+            rows[name] = {"octree_ms": t["per_exec_ms"], "speedup": tp["per_exec_ms"] / t["per_exec_ms"],
+                          "rel_l2_occupied": err, "fine_voxels_computed": st["active"][-1],
+                          "active_per_level": st["active"],
+                          "work_fraction": (st["fine_updates"] + st["bound_updates"]) / upd_plain,
+                          "fp64_running_max": st["running_max"]}
+            if p is not None:
+                rows[name]["proposal_threshold"] = p
     return {"side": side, "levels": 4, "tau": tau, "plain_ms": tp["per_exec_ms"],
-            "octree_ms": to["per_exec_ms"], "speedup": tp["per_exec_ms"] / to["per_exec_ms"],
-            "rel_l2_occupied": err, "occupied_voxels": int(occ.sum()),
-            "occupied_definition": f"plain value >= {OCCUPIED_FRACTION} x max",
-            "fine_voxels_computed": st["active"][-1], "active_per_level": st["active"],
-            "work_fraction": (st["fine_updates"] + st["bound_updates"]) / upd_plain,
-            "fp64_running_max": st["running_max"], "learned": False}
+            "occupied_voxels": int(occ.sum()),
+            "occupied_definition": f"plain value >= {occ_frac} x max",
+            "net": {"trained_scenes": meta.get("scenes"), "capture": meta.get("capture"),
+                    "test": meta.get("test")},
+            **rows}
+
+
+def batch_comparison(bp, count, seed=1004, serial=True):
+    """Uneven confocal captures: one persistent work-queue launch (autotuned, hipGraph replay)
+    against one plan execution per capture."""
+    from backprojector_amd import proposals, scenes
+    t_r = time.perf_counter()
+    caps = scenes.batch_captures(count, seed)
+    args = [proposals.plan_args(c, N) for c, N in caps]
+    log(f"batch: rendered and prepared {count} captures in {time.perf_counter() - t_r:.1f}s")
+    with bp.Batch(args) as b:
+        cfg = b.tune(reps=2)
+        t = b.benchmark(1, 5)
+        b.execute()
+        lat = b.latencies()
+    out = {"captures": count, "seeds": f"SeedSequence({seed}).spawn({count})",
+           "updates": t["updates"], "batch_ms": t["per_exec_ms"], "updates_per_s": t["updates_per_s"],
+           "latency_ms_p50": float(np.percentile(lat, 50)), "latency_ms_p99": float(np.percentile(lat, 99)),
+           "config": cfg}
+    if serial:
+        ms = 0.0
+        for a in args:
+            with bp.Plan(*a) as pl:
+                ms += pl.benchmark(1, 2)["per_exec_ms"]
+        out["one_launch_per_capture_ms"] = ms
+        out["speedup_vs_one_launch_per_capture"] = ms / t["per_exec_ms"]
+    return out
 
 
 def log(*a):
@@ -160,8 +203,12 @@ def main():
     plan.close()
     octree = None
     if args.octree > 0:
-        octree = octree_comparison(bp, cap, args.octree, args.octree_tau)
+        octree = octree_comparison(bp, cap, args.octree, args.octree_tau, args.octree_proposal)
         log(f"octree: {octree}")
+    batch = None
+    if args.batch > 0:
+        batch = batch_comparison(bp, args.batch)
+        log(f"batch: {batch}")
 
     # ---- HBM traffic per launch of the gather kernel, from the committed rocprofv3 --pmc
     # summary (FETCH_SIZE / WRITE_SIZE scaled by the copy-kernel calibration)
@@ -200,6 +247,7 @@ def main():
         "cpu_baseline": cpu,
         "check": check,
         "octree": octree,
+        "batch": batch,
         "baseline_ref": "Quadro P5000 5993 ms (README.md:106) -> 5.73e9 updates/s",
     }
     print(json.dumps(out), flush=True)

@@ -280,6 +280,54 @@ void bpx_plan_destroy(bpx_plan plan);
  * counterpart). */
 bpx_status bpx_copy_benchmark(uint64_t bytes, int32_t iterations, double* mean_ms);
 
+/* ------------------------------------------------------------------ batched uneven captures
+ * Many confocal fp32 captures of different sizes reconstructed by ONE persistent work-queue
+ * launch over (capture, tile) items; the per-batch sequence (staging of every capture, queue
+ * reset, gather) is a hipGraph replayed by bpx_batch_execute.  Each volume is bit-identical to
+ * bpx_backprojection of that capture alone.  No reference counterpart: the reference runs one
+ * capture per call_cuda_backprojection (backproject_cuda.cu:370-500); a batch is that call
+ * applied to each capture. */
+typedef struct bpx_capture_desc {
+    const float* transient_chunk;   /* as bpx_backprojection */
+    uint64_t transient_size;
+    uint32_t T;
+    const bpx_pair* scanned_pairs;
+    uint32_t num_pairs;
+    uint32_t voxels_per_side[3];
+    float volume_zero_pos[3];
+    float voxel_inc[9];
+    float t0, deltaT;
+} bpx_capture_desc;
+typedef struct bpx_batch_config {
+    uint32_t lane_voxels;  /* voxels per lane: 2 or 4                                   */
+    uint32_t groups;       /* persistent work-groups (0 = all that are resident at once) */
+    uint32_t order;        /* queue: 0 = captures with most pairs first, 1 = given order */
+} bpx_batch_config;
+typedef struct bpx_batch_timing {
+    double per_exec_ms;    /* mean device time of one batch execution (HIP events)       */
+    double updates_per_s;  /* voxel-pair updates of all captures / per_exec_ms            */
+    uint64_t updates;
+    int32_t iterations;
+} bpx_batch_timing;
+typedef struct bpx_batch_s* bpx_batch;
+/* options: storage must be BPX_STORAGE_F32 (kahan_block and flags as bpx_backprojection_ex). */
+bpx_status bpx_batch_create(const bpx_capture_desc* captures, uint32_t count, const bpx_options* options,
+                            bpx_batch* out);
+uint32_t bpx_batch_size(bpx_batch batch);
+/* Autotuner: times every configuration of {2, 4} voxels per lane x {all, half} resident groups
+ * x both queue orders on this batch (reps timed executions each) and keeps the fastest. */
+bpx_status bpx_batch_tune(bpx_batch batch, int32_t reps, bpx_batch_config* chosen);
+bpx_status bpx_batch_set_config(bpx_batch batch, const bpx_batch_config* config);
+bpx_status bpx_batch_get_config(bpx_batch batch, bpx_batch_config* config);
+bpx_status bpx_batch_execute(bpx_batch batch, void* hip_stream);
+bpx_status bpx_batch_synchronize(bpx_batch batch);
+bpx_status bpx_batch_read_volume(bpx_batch batch, uint32_t index, float* voxel_volume);
+/* Completion time of each capture in the last execution, ms after the batch's queue reset
+ * (device wall clock stamped when a capture's last tile finishes). */
+bpx_status bpx_batch_latencies(bpx_batch batch, double* ms);
+bpx_status bpx_batch_benchmark(bpx_batch batch, int32_t warmup, int32_t iterations, bpx_batch_timing* out);
+void bpx_batch_destroy(bpx_batch batch);
+
 /* ------------------------------------------------------------------ learned octree proposals
  * A small 3D conv net (4 layers of 3x3x3 convolutions, 16 channels, BF16 weights and
  * activations on MFMA with FP32 accumulation, ReLU) maps a coarse octree level to one logit

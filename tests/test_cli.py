@@ -1,0 +1,137 @@
+"""The backprojector command line (main.cpp:29-187) on synthetic HDF5 NLOS datasets: dataset
+parsing, bounce selection, data-order handling and resolution rescale on the CPU (--dry-run);
+end-to-end reconstructions on the GPU against the Python binding on the same arrays."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from tests import h5lite
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(ROOT, "backprojector_amd", "bin", "backprojector")
+
+pytestmark = pytest.mark.skipif(h5lite.path() is None or not os.path.exists(CLI),
+                                reason="HDF5 C library or the built CLI is missing")
+f32 = np.float32
+
+
+def wall(side):
+    g = np.linspace(-1, 1, side, dtype=np.float64)
+    X, Y = np.meshgrid(g, g, indexing="ij")
+    return np.stack([X, Y, np.zeros_like(X)], -1).astype(f32)
+
+
+def confocal_arrays(side=8, T=512, bounces=3, channels=2, seed=0):
+    rng = np.random.default_rng(seed)
+    # bounce b of every sample holds (b + 1) * base, channel c adds 1000 c: selection is visible
+    base = rng.integers(0, 5, (side, side, 1, T, 1)).astype(f32)
+    data = np.concatenate([base * (b + 1) for b in range(bounces)], axis=2)
+    data = np.concatenate([data + 1000 * c for c in range(channels)], axis=4)
+    return data, wall(side)
+
+
+def run(args, check=True):
+    r = subprocess.run([CLI] + args, capture_output=True, text=True, timeout=120)
+    if check and r.returncode != 0:
+        raise AssertionError(r.stderr + r.stdout)
+    return r
+
+
+def dry(path, *extra):
+    return json.loads(run([path, "--dry-run", *extra]).stdout)
+
+
+def test_dry_run_row_major_confocal_bounces(tmp_path):
+    data, grid = confocal_arrays()
+    p = str(tmp_path / "scene.hdf5")
+    h5lite.nlos_dataset(p, data, grid, grid, [0, -1, 0.5], [0, -1, 0.5], 0.1, 0.01, True,
+                        [0, 0, 0.8], [1.0, 0.5, 1.0])
+    d = dry(p)
+    assert d["confocal"] and d["data_order"] == "row-major" and d["bounces"] == 1
+    assert d["transient_shape"] == [8, 8, 512]
+    # default third_bounce = 1: bounce 3 is index 1 of the bounce axis, channel 0
+    assert d["transient_sum"] == pytest.approx(float(data[:, :, 1, :, 0].sum()))
+    # equal resolutions scale with the volume's aspect, truncated (main.cpp:128-131)
+    assert d["voxels"] == [32, 16, 32]
+    assert d["volume_position"] == [0, 0, pytest.approx(0.8)] and d["t0"] == pytest.approx(0.1)
+    assert d["output"] == str(tmp_path / "scene_recon.hdf5") and d["mode"] == "gpu"
+    # -b 4 sums bounces 3 and 4 (indices 1 and 2)
+    d4 = dry(p, "-b", "4", "-r", "16,8,4", "-c", "0.1,0.2,0.9", "-s2,2,2", "--phasor", "-w", "0.04")
+    assert d4["bounces"] == 2
+    assert d4["transient_sum"] == pytest.approx(float(data[:, :, 1:3, :, 0].sum()))
+    assert d4["voxels"] == [16, 8, 4] and d4["mode"] == "phasor" and d4["wavelength"] == pytest.approx(0.04)
+    assert d4["volume_position"] == [pytest.approx(0.1), pytest.approx(0.2), pytest.approx(0.9)]
+    assert d4["volume_size"] == [2, 2, 2]
+
+
+def test_dry_run_third_bounce_attribute(tmp_path):
+    data, grid = confocal_arrays()
+    p = str(tmp_path / "tb.hdf5")
+    h5lite.nlos_dataset(p, data, grid, grid, [0, -1, 0.5], [0, -1, 0.5], 0.0, 0.01, True,
+                        [0, 0, 0.8], [1, 1, 1], third_bounce=0)
+    d = dry(p)   # third_bounce = 0: bounce 3 is index 0
+    assert d["transient_sum"] == pytest.approx(float(data[:, :, 0, :, 0].sum()))
+
+
+def test_dry_run_column_major_exhaustive(tmp_path):
+    rng = np.random.default_rng(1)
+    data = rng.integers(0, 7, (3, 4, 5, 2, 2, 64, 1)).astype(f32)   # [lx][ly][cx][cy][b][T][c]
+    lg, cg = rng.normal(size=(3, 4, 3)).astype(f32), rng.normal(size=(5, 2, 3)).astype(f32)
+    p = str(tmp_path / "ex.hdf5")
+    h5lite.nlos_dataset(p, data, cg, lg, [0, -1, 0.5], [0.1, -1, 0.5], 0.0, 0.01, False,
+                        [0, 0, 0.8], [1, 1, 1], order="column-major")
+    d = dry(p)
+    assert not d["confocal"] and d["data_order"] == "column-major"
+    assert d["transient_shape"] == [3, 4, 5, 2, 64]
+    assert d["camera_grid_shape"] == [5, 2, 3] and d["laser_grid_shape"] == [3, 4, 3]
+    assert d["transient_sum"] == pytest.approx(float(data[..., 1, :, 0].sum()))
+
+
+def test_errors(tmp_path):
+    data, grid = confocal_arrays()
+    p = str(tmp_path / "bad.hdf5")
+    h5lite.write(p, {"data": data}, {"data order": "row-major"})
+    r = run([p, "--dry-run"], check=False)
+    assert r.returncode == 1 and "missing" in r.stderr
+    r = run([str(tmp_path / "nope.hdf5")], check=False)
+    assert r.returncode == 1 and "cannot open" in r.stderr
+    r = run([p, "--frobnicate"], check=False)
+    assert r.returncode == 1 and "unknown flag" in r.stderr
+    p2 = str(tmp_path / "ok.hdf5")
+    h5lite.nlos_dataset(p2, data, grid, grid, [0, -1, 0.5], [0, -1, 0.5], 0.0, 0.01, True, [0, 0, 0.8], [1, 1, 1])
+    r = run([p2, "-b", "9", "--dry-run"], check=False)
+    assert r.returncode == 1 and "bounce" in r.stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["gpu", "cpu", "phasor", "octree"])
+def test_cli_end_to_end(tmp_path, gpu, mode):
+    from backprojector_amd import nlosbpy, scenes
+    cap = scenes.render(scenes.config_scene(1, np.random.default_rng(4)), 16, 4096, seed=4)
+    data = cap.transient[:, :, None, :, None]        # one bounce (the third), one channel
+    p = str(tmp_path / "cap.hdf5")
+    h5lite.nlos_dataset(p, data, cap.camera_grid, cap.laser_grid, cap.camera_position,
+                        cap.laser_position, cap.t0, cap.deltaT, True, cap.volume_position,
+                        cap.volume_size, third_bounce=0)
+    out = str(tmp_path / f"vol_{mode}.hdf5")
+    flags = {"gpu": [], "cpu": ["--cpu"], "phasor": ["--phasor", "-w", "0.04"], "octree": ["--octree"]}[mode]
+    r = run([p, "-r", "32,32,32", "-o", out] + flags)
+    assert f"Backprojected to {out}" in r.stdout
+    vol, props = h5lite.read_volume(out)
+    assert vol.shape == (32, 32, 32)
+    assert props["chunk"] == (32, 32, 32) and props["deflate"] == 4 and np.isnan(props["fill"])
+    args = cap.nlos_args((32, 32, 32))
+    if mode == "phasor":
+        ref = np.abs(nlosbpy.phasor(*args, 0.04, assume_row_major=True))
+        assert np.array_equal(vol, ref.astype(f32))
+    elif mode == "octree":
+        dense = nlosbpy.backproject(*args, (0, 0, 0), assume_row_major=True)
+        computed = vol != 0
+        assert np.array_equal(vol[computed], dense[computed])
+        assert np.all(computed[dense >= 0.25 * dense.max()])
+    else:
+        ref = nlosbpy.backproject(*args, (0, 0, 0), assume_row_major=True, use_cpu=(mode == "cpu"))
+        assert np.array_equal(vol, ref)

@@ -1,0 +1,107 @@
+"""Batched uneven captures (one persistent work-queue launch replayed from a hipGraph) against
+each capture reconstructed alone and against the oracle."""
+import numpy as np
+import pytest
+
+from backprojector_amd import scenes
+from backprojector_amd._native import BpxError
+
+pytestmark = pytest.mark.gpu
+f32 = np.float32
+
+
+def capture(rng, side, T, N, coded=False):
+    hidden = np.concatenate([scenes.points(rng, 3), scenes.patch(rng, size=0.2, spacing=0.01)])
+    cap = scenes.render(hidden, side, T, 0.0, 4.096 / T, True, seed=int(rng.integers(1 << 30)))
+    pr = scenes.wall_pairs(cap.camera_grid, cap.camera_grid, True)
+    H = cap.transient.reshape(len(pr), -1).astype(f32)
+    if coded:  # integer sample codes: every sum is exact, so the oracle must match exactly
+        b = np.arange(T)[None, :]
+        p = np.arange(len(pr))[:, None]
+        H = (((7 * b + 13 * p) % 61) + 1).astype(f32)
+    Nv = (N, N + 3, N - 2)            # uneven, not multiples of the tile edges
+    zero = np.array([-0.5, -0.5, 0.3], f32)
+    inc = np.diag([f32(1) / f32(n - 1) for n in Nv]).astype(f32).ravel()
+    return (H, pr, Nv, zero, inc, 0.0, 4.096 / T)
+
+
+def uneven(seed=0):
+    rng = np.random.default_rng(seed)
+    spec = [(8, 1024, 12), (16, 2048, 20), (12, 512, 9), (24, 1024, 16), (16, 4096, 24), (8, 512, 30)]
+    return [capture(rng, *s) for s in spec]
+
+
+def test_batch_equals_single_captures(gpu):
+    caps = uneven()
+    alone = [gpu.backprojection(*c) for c in caps]
+    with gpu.Batch(caps) as b:
+        assert len(b) == len(caps)
+        for rep in range(3):   # graph replays with eager work and synchronisation between them
+            b.execute()
+            b.synchronize()
+            for i, ref in enumerate(alone):
+                assert np.array_equal(b.volume(i), ref), (rep, i)
+            gpu.backprojection(*caps[0])
+        lat = b.latencies()
+        assert np.all(lat > 0) and np.all(np.isfinite(lat))
+
+
+@pytest.mark.parametrize("cfg", [dict(lane_voxels=4, groups=7, order=1),
+                                 dict(lane_voxels=2, groups=1, order=0),
+                                 dict(lane_voxels=4, groups=0, order=0)])
+def test_batch_configs_identical(gpu, cfg):
+    caps = uneven(1)
+    alone = [gpu.backprojection(*c) for c in caps]
+    with gpu.Batch(caps) as b:
+        b.set_config(**cfg)
+        got = b.config()
+        assert got["lane_voxels"] == cfg["lane_voxels"] and got["order"] == cfg["order"]
+        b.execute()
+        for i, ref in enumerate(alone):
+            assert np.array_equal(b.volume(i), ref), (cfg, i)
+
+
+def test_batch_tune_and_benchmark(gpu):
+    caps = uneven(2)
+    alone = [gpu.backprojection(*c) for c in caps]
+    with gpu.Batch(caps) as b:
+        cfg = b.tune(reps=1)
+        assert cfg["lane_voxels"] in (2, 4) and cfg["groups"] >= 1
+        t = b.benchmark(1, 2)
+        P = [len(c[1]) for c in caps]
+        assert t["updates"] == sum(p * int(np.prod(c[2])) for p, c in zip(P, caps))
+        assert t["per_exec_ms"] > 0 and t["updates_per_s"] > 0
+        for i, ref in enumerate(alone):
+            assert np.array_equal(b.volume(i), ref)
+
+
+def test_batch_coded_matches_oracle(gpu, oracle):
+    rng = np.random.default_rng(5)
+    caps = [capture(rng, 8, 512, 10, coded=True), capture(rng, 6, 256, 7, coded=True)]
+    with gpu.Batch(caps) as b:
+        b.execute()
+        for i, c in enumerate(caps):
+            ref = oracle.a1_volume_f64acc(*c)
+            assert np.array_equal(b.volume(i), ref.astype(f32))
+
+
+def test_batch_edge_cases(gpu):
+    caps = uneven(3)[:2]
+    H, pr, N, zero, inc, t0, dT = caps[0]
+    empty = (np.zeros(0, f32), pr[:0], N, zero, inc, t0, dT)
+    with gpu.Batch([empty, caps[1]]) as b:
+        b.execute()
+        assert not b.volume(0).any()
+        assert np.array_equal(b.volume(1), gpu.backprojection(*caps[1]))
+    with gpu.Batch([]) as b:
+        assert len(b) == 0
+        b.execute()
+        b.synchronize()
+    # a non-confocal capture is rejected
+    ex = pr[:4].copy()
+    ex["laser_point"] = ex["camera_point"][::-1]
+    with pytest.raises(BpxError):
+        gpu.Batch([(H[:4], ex, N, zero, inc, t0, dT)])
+    with pytest.raises(BpxError):
+        with gpu.Batch(caps) as b:
+            b.set_config(lane_voxels=3)
