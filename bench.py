@@ -87,7 +87,7 @@ def octree_comparison(bp, cap, side, tau, proposal=None, reps=3):
     return {"side": side, "levels": 4, "tau": tau, "plain_ms": tp["per_exec_ms"],
             "occupied_voxels": int(occ.sum()),
             "occupied_definition": f"plain value >= {occ_frac} x max",
-            "net": {"trained_scenes": meta.get("scenes"), "capture": meta.get("capture"),
+            "net": {"trained_scenes": meta.get("scenes"), "description": meta.get("description"),
                     "test": meta.get("test")},
             **rows}
 
