@@ -521,6 +521,19 @@ __global__ __launch_bounds__(256) void confocal_kernel(ConfocalArgs a) {
 // is 1 on lane 0 and 0 elsewhere, never under a lane-divergent branch: a branch on "lane 0"
 // around the head update inside this loop lets the compiler split the wave into lane 0 and the
 // rest (the rest then re-read a stale head and never leave).  There is no work-group barrier.
+// A struct every lane reads from the same address, moved word by word into scalar registers:
+// the buffer descriptors built from it are then wave-uniform (no per-pair waterfall loops).
+template <class T>
+__device__ __forceinline__ T wave_uniform_load(const T* p) {
+    static_assert(sizeof(T) % 4 == 0, "word-sized struct");
+    T out;
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(p);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(&out);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / 4); ++i) dst[i] = __builtin_amdgcn_readfirstlane(src[i]);
+    return out;
+}
+
 template <int K>
 __global__ __launch_bounds__(256) void batch_confocal_kernel(BatchArgs b) {
     const uint32_t lane = threadIdx.x & 63u;
@@ -533,8 +546,8 @@ __global__ __launch_bounds__(256) void batch_confocal_kernel(BatchArgs b) {
         const uint2 item = b.items[it];
         const uint32_t c = __builtin_amdgcn_readfirstlane(item.x);
         const uint32_t q = __builtin_amdgcn_readfirstlane(item.y);  // tile * 4 + quarter
-        const ConfocalArgs a = b.caps[c];
-        const uint32_t f = b.flags[c];
+        const ConfocalArgs a = wave_uniform_load(b.caps + c);
+        const uint32_t f = __builtin_amdgcn_readfirstlane(b.flags[c]);
         if (f == 3u) confocal_tile<FusedSource<FusedF32>, K, true, true>(a, q >> 2, q & 3u);
         else if (f == 1u) confocal_tile<FusedSource<FusedF32>, K, true, false>(a, q >> 2, q & 3u);
         else confocal_tile<FusedSource<FusedF32>, K, false, false>(a, q >> 2, q & 3u);

@@ -1,0 +1,43 @@
+"""What the trained proposal net gains over the octree without it, on 128^2 confocal captures
+(T = 4096) reconstructed at 256^3: the shipped weights at their selected proposal threshold
+against the sound bound rule alone and the dense reconstruction."""
+import numpy as np
+import pytest
+
+from backprojector_amd import proposals, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("seed", [1000, 60009])   # the benchmark capture, a held-out test capture
+def test_learned_proposals_gain(gpu, seed):
+    cap = scenes.metric_capture(seed=seed)
+    args = proposals.plan_args(cap, 256)
+    net, meta = proposals.load()
+    p = meta["proposal_threshold"]
+    with gpu.Plan(*args) as dp:
+        t_dense = dp.benchmark(1, 2)["per_exec_ms"]
+        dense = dp.volume()
+    res = {}
+    with gpu.Plan.octree(*args, levels=4, threshold=0.25) as op:
+        for name in ("bound_rule", "learned"):
+            if name == "learned":
+                op.attach_net(net)
+                op.set_octree_options(0.25, p)
+            t = op.benchmark(1, 2)["per_exec_ms"]
+            vol = op.volume()
+            st = op.octree_stats()
+            work = (st["fine_updates"] + st["bound_updates"]) / (len(args[1]) * 256 ** 3)
+            res[name] = dict(ms=t, work=work, err=proposals.rel_l2_occupied(vol, dense), vol=vol)
+    b, l = res["bound_rule"], res["learned"]
+    print(f"seed {seed}: dense {t_dense:.1f} ms; bound rule {b['ms']:.1f} ms work {b['work']:.3f}; "
+          f"learned {l['ms']:.1f} ms work {l['work']:.3f} err {l['err']:.4f}")
+    # every computed voxel is exact (bit-identical to the dense reconstruction)
+    for r in (b, l):
+        c = r["vol"] != 0
+        assert np.array_equal(r["vol"][c], dense[c])
+    assert b["err"] == 0.0                     # the bound rule is sound
+    assert l["err"] <= 0.02                    # the net's vetoes stay within 2% over occupied voxels
+    assert l["work"] <= b["work"] / 3          # voxel-pair work cut by more than 3x ...
+    assert l["ms"] < b["ms"] / 2               # ... and more than 2x faster than without the net
+    assert t_dense / l["ms"] >= 3.0
