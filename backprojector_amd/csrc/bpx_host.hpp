@@ -107,7 +107,8 @@ struct NetImpl {
     DeviceBuffer params, grads, adam_m, adam_v;   // fp32 [kParams]
     DeviceBuffer wb, wt;                          // bf16 weights and flipped/transposed copies
     DeviceBuffer act[kLayers + 1];                // bf16 activations [V][16]: input, Y1..Y3, (unused)
-    DeviceBuffer logits, dlog, dx, dz, labels, mask, loss, vmax, pooled_a, pooled_b;
+    DeviceBuffer logits, dlog, dx, dz, labels, mask, loss, vmax, vmax_part, pooled_a, pooled_b;
+    static constexpr uint32_t kMaxParts = 2048;   // work-groups of the FP64 max
     uint64_t capacity = 0;                        // voxels the activation buffers hold
     uint64_t step = 0;
     hipStream_t stream = nullptr;

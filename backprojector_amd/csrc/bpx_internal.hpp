@@ -184,7 +184,8 @@ hipError_t launch_oct_features(const uint32_t* blocks, const float* U, const flo
 hipError_t launch_maxpool2(const float* in, float* out, uint32_t m0, uint32_t m1, uint32_t m2, hipStream_t s);
 hipError_t launch_child_labels(const float* pooled, uint32_t n0, uint32_t n1, uint32_t n2, const double* vmax,
                                float frac, uint8_t* labels, hipStream_t s);
-hipError_t launch_max_f64(const float* x, uint64_t n, double* out, hipStream_t s);
+hipError_t launch_max_f64(const float* x, uint64_t n, double* partial, uint32_t max_parts, double* out,
+                          hipStream_t s);
 hipError_t launch_oct_child_flags(const uint32_t* blocks, const float* U, uint32_t n, const double* stats,
                                   const float* logits, uint32_t n1, uint32_t n2, float logit_thr,
                                   uint32_t* flags, hipStream_t s);

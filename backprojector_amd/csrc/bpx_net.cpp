@@ -96,6 +96,7 @@ static NetImpl* make_net(uint64_t seed) {
     n->wt.alloc(2ull * NetImpl::kLayers * NetImpl::kW);
     n->loss.alloc(sizeof(double));
     n->vmax.alloc(sizeof(double));
+    n->vmax_part.alloc(sizeof(double) * NetImpl::kMaxParts);
     // He initialisation over the real fan-in; padded input (layer 0) and output (layer 3)
     // channels start at zero; biases zero.
     std::vector<float> p(NetImpl::kParams, 0.0f);
@@ -296,7 +297,8 @@ bpx_status bpx_net_train_octree(bpx_net net, bpx_plan octree_plan, bpx_plan dens
             total += (uint64_t)st.levels[l].n[0] * st.levels[l].n[1] * st.levels[l].n[2];
         }
         if (n.pooled_a.bytes < 4 * std::max<uint64_t>(total, 1)) n.pooled_a.alloc(4 * std::max<uint64_t>(total, 1));
-        hip_check(launch_max_f64(static_cast<const float*>(dp.vol.ptr), nvox, static_cast<double*>(n.vmax.ptr), s),
+        hip_check(launch_max_f64(static_cast<const float*>(dp.vol.ptr), nvox, static_cast<double*>(n.vmax_part.ptr),
+                                 NetImpl::kMaxParts, static_cast<double*>(n.vmax.ptr), s),
                   "FP64 max");
         const float* finer = static_cast<const float*>(dp.vol.ptr);
         for (int l = (int)st.L - 1; l >= 1; --l) {

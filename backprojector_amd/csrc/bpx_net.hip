@@ -12,6 +12,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "bpx_internal.hpp"
 
 namespace bpx {
@@ -87,6 +89,11 @@ __global__ __launch_bounds__(256) void conv3_dw_kernel(const uint16_t* __restric
                                                        const uint16_t* __restrict__ X,
                                                        float* __restrict__ dW, float* __restrict__ db,
                                                        uint32_t n0, uint32_t n1, uint32_t n2) {
+    __shared__ float sdw[27 * 256];  // the work-group's partial dW (one global atomic per element)
+    __shared__ float sdb[16];
+    for (uint32_t i = threadIdx.x; i < 27u * 256u; i += blockDim.x) sdw[i] = 0.0f;
+    if (threadIdx.x < 16u) sdb[threadIdx.x] = 0.0f;
+    __syncthreads();
     const uint32_t V = n0 * n1 * n2;
     const uint32_t nchunks = (V + 15u) / 16u;
     const uint32_t wid = blockIdx.x * 4u + (threadIdx.x >> 6), nw = gridDim.x * 4u;
@@ -126,15 +133,18 @@ __global__ __launch_bounds__(256) void conv3_dw_kernel(const uint16_t* __restric
             acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
         }
     }
-    // D lane layout: rows (cout) 4q + i, column (cin) r
+    // D lane layout: rows (cout) 4q + i, column (cin) r; waves of the group add in LDS
 #pragma unroll
     for (int t = 0; t < 27; ++t)
 #pragma unroll
         for (int i = 0; i < 4; ++i)
-            atomicAdd(&dW[((size_t)t * 16u + 4u * q + (uint32_t)i) * 16u + r], acc[t][i]);
+            atomicAdd(&sdw[((uint32_t)t * 16u + 4u * q + (uint32_t)i) * 16u + r], acc[t][i]);
     // bias: lane (q, r) summed dZ[.][r] over its voxels
     for (int off = 16; off < 64; off <<= 1) bsum += __shfl_xor(bsum, off, 64);
-    if (q == 0 && db) atomicAdd(&db[r], bsum);
+    if (q == 0) atomicAdd(&sdb[r], bsum);
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < 27u * 256u; i += blockDim.x) atomicAdd(&dW[i], sdw[i]);
+    if (threadIdx.x < 16u && db) atomicAdd(&db[threadIdx.x], sdb[threadIdx.x]);
 }
 
 hipError_t launch_conv3_dw(const uint16_t* dZ, const uint16_t* X, float* dW, float* db, uint32_t n0,
@@ -176,7 +186,7 @@ hipError_t launch_relu_back(const float* dX, const uint16_t* Y, uint16_t* dZ, ui
 
 // Weighted binary cross-entropy on logits [V][16] (channels < nout) against labels [V][nout]
 // (0/1 bytes), only where mask[v] != 0.  dlogits in bf16 (zero elsewhere), loss sum in double.
-__global__ void bce_kernel(const float* __restrict__ logits, const uint8_t* __restrict__ labels,
+__global__ __launch_bounds__(256) void bce_kernel(const float* __restrict__ logits, const uint8_t* __restrict__ labels,
                            const uint8_t* __restrict__ mask, uint32_t V, uint32_t nout, float pos_weight,
                            float scale, uint16_t* __restrict__ dlogits, double* __restrict__ loss) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -197,7 +207,14 @@ __global__ void bce_kernel(const float* __restrict__ logits, const uint8_t* __re
         dlogits[i] = to_bf16(g);
     }
     for (int off = 32; off > 0; off >>= 1) l += __shfl_xor(l, off, 64);
-    if ((threadIdx.x & 63u) == 0u && l != 0.0) atomicAdd(loss, l);
+    // one atomic per work-group (a double atomic per wave contends on one address)
+    __shared__ double wsum[4];
+    if ((threadIdx.x & 63u) == 0u) wsum[threadIdx.x >> 6] = l;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double t = (wsum[0] + wsum[1]) + (wsum[2] + wsum[3]);
+        if (t != 0.0) atomicAdd(loss, t);
+    }
 }
 
 hipError_t launch_bce(const float* logits, const uint8_t* labels, const uint8_t* mask, uint32_t V,
@@ -318,12 +335,28 @@ hipError_t launch_child_labels(const float* pooled, uint32_t n0, uint32_t n1, ui
     return hipGetLastError();
 }
 
-// FP64 max of a float array (single work-group, grid-stride): out[0] = max.
+// FP64 max of a float array in two passes: every work-group writes the max of its grid-stride
+// share to partial[blockIdx], then one work-group reduces the partials into out[0].
 __global__ __launch_bounds__(256) void max_f64_kernel(const float* __restrict__ x, uint64_t n,
                                                       double* __restrict__ out) {
     __shared__ double sh[256];
     double m = -INFINITY;
-    for (uint64_t i = threadIdx.x; i < n; i += blockDim.x) m = fmax(m, (double)x[i]);
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        m = fmax(m, (double)x[i]);
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    for (uint32_t o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = sh[0];
+}
+
+__global__ __launch_bounds__(256) void max_f64_finish_kernel(const double* __restrict__ partial, uint32_t n,
+                                                             double* __restrict__ out) {
+    __shared__ double sh[256];
+    double m = -INFINITY;
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) m = fmax(m, partial[i]);
     sh[threadIdx.x] = m;
     __syncthreads();
     for (uint32_t o = 128; o > 0; o >>= 1) {
@@ -333,8 +366,12 @@ __global__ __launch_bounds__(256) void max_f64_kernel(const float* __restrict__ 
     if (threadIdx.x == 0) out[0] = sh[0];
 }
 
-hipError_t launch_max_f64(const float* x, uint64_t n, double* out, hipStream_t s) {
-    hipLaunchKernelGGL(max_f64_kernel, dim3(1), dim3(256), 0, s, x, n, out);
+hipError_t launch_max_f64(const float* x, uint64_t n, double* partial, uint32_t max_parts, double* out,
+                          hipStream_t s) {
+    uint32_t parts = (uint32_t)std::min<uint64_t>((n + 255u) / 256u, max_parts);
+    if (parts == 0u) parts = 1u;
+    hipLaunchKernelGGL(max_f64_kernel, dim3(parts), dim3(256), 0, s, x, n, partial);
+    hipLaunchKernelGGL(max_f64_finish_kernel, dim3(1), dim3(256), 0, s, partial, parts, out);
     return hipGetLastError();
 }
 
