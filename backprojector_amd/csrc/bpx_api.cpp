@@ -69,7 +69,7 @@ static bpx_status guarded(F&& f) {
     }
 }
 
-static void run_plan_to_host(PlanImpl* raw, float* out) {
+void run_plan_to_host(PlanImpl* raw, float* out) {
     std::unique_ptr<PlanImpl> pl(raw);
     pl->execute(pl->stream);
     const size_t nvox = (size_t)pl->g.n[0] * pl->g.n[1] * pl->g.n[2];

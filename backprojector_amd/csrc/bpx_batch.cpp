@@ -1,7 +1,7 @@
-// Batched uneven confocal captures: one device-resident plan per capture (rows and threshold
-// tables staged once), all gathers in ONE persistent work-queue launch, and the whole
-// per-batch sequence (staging of every capture, queue reset, persistent gather) captured in a
-// hipGraph that bpx_batch_execute replays.  A project-written autotuner times a few launch
+// Batched uneven confocal captures: one device-resident plan per capture, the staging of every
+// capture (rows and threshold tables) in one pair of launches, all gathers in ONE persistent
+// work-queue launch, and the whole per-batch sequence (staging, queue reset, persistent
+// gather) captured in a hipGraph that bpx_batch_execute replays.  A project-written autotuner times a few launch
 // configurations (voxels per lane, resident groups, queue order) on the batch itself.
 // No reference counterpart: the reference reconstructs one capture per call
 // (call_cuda_backprojection, backproject_cuda.cu:370-500).
@@ -23,6 +23,8 @@ struct BatchImpl {
     hipEvent_t ev[2] = {};
     DeviceBuffer caps, flags, items, next, remaining, tiles;
     DeviceBuffer stamps;   // uint64 [1 + captures]: batch start, then each capture's completion
+    DeviceBuffer stage_jobs;                      // StageJob per capture with pairs
+    uint32_t n_stage = 0, stage_zb = 0, stage_sb = 0;
     uint32_t n_items = 0;
     bpx_batch_config cfg{2u, 0u, 0u};
     uint32_t groups = 0;
@@ -38,18 +40,6 @@ struct BatchImpl {
     }
     void build(const bpx_batch_config& c);
 };
-
-// Row staging + threshold table of a fused fp32 confocal plan (the per-capture part of
-// PlanImpl::execute before its gather kernel).
-static void stage_plan(PlanImpl& pl, hipStream_t s) {
-    if (pl.P == 0) return;
-    hip_check(launch_fused_pad(static_cast<const float*>(pl.raw.ptr), pl.P, pl.T, pl.T, pl.H.ptr, pl.ldh, pl.format,
-                               1.0f / pl.scale, s),
-              "row staging kernel");
-    hip_check(launch_stable(static_cast<const float*>(pl.pair8.ptr), pl.P, pl.T, pl.t0, pl.dT,
-                            static_cast<float*>(pl.H.ptr), pl.ldh * 2u, 2u, 1u, s),
-              "threshold table kernel");
-}
 
 void BatchImpl::build(const bpx_batch_config& c) {
     if (c.lane_voxels != 2u && c.lane_voxels != 4u) fail(BPX_ERR_INVALID_ARGUMENT, "lane_voxels must be 2 or 4");
@@ -93,6 +83,26 @@ void BatchImpl::build(const bpx_batch_config& c) {
             it.push_back(t);
         }
     n_items = (uint32_t)(it.size() / 2);
+    // staging of every capture in one pair of launches: each job's blocks follow the previous job's
+    std::vector<StageJob> jobs;
+    uint64_t zb = 0, sb = 0;
+    for (auto& pl : plans) {
+        if (pl->P == 0) continue;
+        StageJob j = stage_job(*pl);
+        j.z_first = (uint32_t)zb;
+        j.s_first = (uint32_t)sb;
+        zb += stage_z_blocks(j);
+        sb += stage_row_blocks(j);
+        if (sb >= (1ull << 31)) fail(BPX_ERR_UNSUPPORTED, "batch staging above 2^39 row elements");
+        jobs.push_back(j);
+    }
+    n_stage = (uint32_t)jobs.size();
+    stage_zb = (uint32_t)zb;
+    stage_sb = (uint32_t)sb;
+    stage_jobs.alloc(sizeof(StageJob) * std::max<size_t>(jobs.size(), 1));
+    if (n_stage)
+        hip_check(hipMemcpy(stage_jobs.ptr, jobs.data(), sizeof(StageJob) * n_stage, hipMemcpyHostToDevice),
+                  "hipMemcpy");
     caps.alloc(sizeof(ConfocalArgs) * std::max<uint32_t>(n, 1));
     flags.alloc(4u * std::max<uint32_t>(n, 1));
     tiles.alloc(4u * std::max<uint32_t>(n, 1));
@@ -115,13 +125,12 @@ void BatchImpl::build(const bpx_batch_config& c) {
     hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
     try {
         for (auto& pl : plans) {
-            if (pl->P == 0) {
-                const size_t nvox = (size_t)pl->g.n[0] * pl->g.n[1] * pl->g.n[2];
-                hip_check(launch_fill(static_cast<float*>(pl->vol.ptr), nvox, 0.0f, stream), "fill kernel");
-            } else {
-                stage_plan(*pl, stream);
-            }
+            if (pl->P != 0) continue;
+            const size_t nvox = (size_t)pl->g.n[0] * pl->g.n[1] * pl->g.n[2];
+            hip_check(launch_fill(static_cast<float*>(pl->vol.ptr), nvox, 0.0f, stream), "fill kernel");
         }
+        hip_check(launch_stage(static_cast<const StageJob*>(stage_jobs.ptr), n_stage, stage_zb, stage_sb, stream),
+                  "batch staging kernels");
         hip_check(launch_batch_reset(static_cast<uint32_t*>(next.ptr), static_cast<uint32_t*>(remaining.ptr),
                                      static_cast<const uint32_t*>(tiles.ptr), n, static_cast<uint64_t*>(stamps.ptr),
                                      stream),

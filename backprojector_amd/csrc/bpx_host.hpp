@@ -89,6 +89,8 @@ struct PlanImpl {
     float vpos[3]{}, vsize_in[3]{}, wzero[3]{}, wvs[3]{}, half_width = 0, c1_lo = 0, c1_hi = 0;
     bool all_confocal = false;
     DeviceBuffer raw, H, S, Z, Z2, pair8, pair4, lasers, spads, vol;
+    DeviceBuffer stage;                  // one StageJob (fused confocal layouts)
+    uint32_t stage_zb = 0, stage_sb = 0; // its blocks in the two staging passes
     hipEvent_t ev[8] = {};
     std::unique_ptr<OctreeState> oct;
     ~PlanImpl();
@@ -96,6 +98,11 @@ struct PlanImpl {
 };
 
 void execute_octree(PlanImpl& pl, hipStream_t s);
+
+// The staging job of a fused confocal plan (block offsets 0); launch_stage with the plan's
+// device copy stages the plan's rows.
+StageJob stage_job(const PlanImpl& pl);
+void stage_rows(PlanImpl& pl, hipStream_t s);
 
 // Proposal net (bpx_net.hip kernels, bpx_net.cpp orchestration).  Four 3x3x3 conv layers of
 // 16 channels (inputs zero-padded to 16, 8 output logits = children), ReLU between layers.
@@ -135,7 +142,16 @@ struct CaptureIn {
     uint32_t P;
     float t0, dT;
     bool cplx;
+    // Device-side cropping (real samples only): when chunk is NULL and src is set, row p of the
+    // chunk is bins [from, from + ncopy) of source row p (pitch src_T) placed at
+    // [cfrom, cfrom + ncopy), zero elsewhere -- get_transient_chunk (backproject.hpp:689-740)
+    // done by one copy of the source rows and a crop kernel.
+    const float* src = nullptr;
+    uint32_t src_T = 0, from = 0, cfrom = 0, ncopy = 0;
 };
+
+// Executes a plan, copies its volume to host memory and destroys the plan.
+void run_plan_to_host(PlanImpl* plan, float* out);
 
 PlanImpl* make_plan_single_tap(const CaptureIn& c, const uint32_t n[3], const float zero[3],
                                const float inc[9], const bpx_options* opt);
@@ -184,7 +200,13 @@ void time_limits(const float laser_position[3], const float* lgrid, uint32_t Lx,
 // backproject.hpp:820-839
 void basis(const float volume_position[3], const float volume_size[3], const uint32_t n[3],
            const float wall_normal[3], float zero[3], float inc[9]);
-// backproject.hpp:689-740 (row-major); returns the chunk [rows][Tc]
+// backproject.hpp:689-740 (row-major): the chunk's length Tc and the window of every row that
+// is copied (source bins [from, from + ncopy) to chunk bins [cfrom, cfrom + ncopy)).
+struct ChunkWindow {
+    uint32_t Tc, from, cfrom, ncopy;
+};
+ChunkWindow chunk_window(uint32_t T, float min_idx, float max_idx);
+// The same chunk materialised on the host, [rows][Tc].
 std::vector<float> transient_chunk(const float* data, uint64_t rows, uint32_t T, float min_idx,
                                    float max_idx, uint32_t* Tc);
 // backproject.hpp:430-451 taps (reference parametrisation when sigma <= 0)

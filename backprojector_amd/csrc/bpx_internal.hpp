@@ -190,11 +190,35 @@ hipError_t launch_oct_child_flags(const uint32_t* blocks, const float* U, uint32
                                   const float* logits, uint32_t n1, uint32_t n2, float logit_thr,
                                   uint32_t* flags, hipStream_t s);
 // All launchers enqueue on `stream` and return the hipError_t of the launch.
-hipError_t launch_stable(const float* pair8, uint32_t P, uint32_t Tc, float t0, float dT,
+// Confocal threshold table S[p][j] (j = 0..Tc) from the pair-independent path-length table
+// Z[j] = smallest total path whose single-tap bin is >= j (launch_ztable kind 0, Tc + 1 entries).
+hipError_t launch_stable(const float* pair8, uint32_t P, uint32_t Tc, const float* Z,
                          float* S, uint32_t pitch, uint32_t stride, uint32_t offset,
                          hipStream_t stream);
-hipError_t launch_fused_pad(const float* src, uint32_t rows, uint32_t Tc, uint32_t src_pitch,
-                            void* dst, uint32_t lde, int format, float inv_scale, hipStream_t stream);
+
+// One confocal capture's staging into a fused sample+threshold layout: Z[0..Tc] first, then
+// every row element (scaled samples and the pair's threshold slot) written once.  A launch
+// serves any number of jobs; each job owns a contiguous range of 256-thread blocks in each of
+// the two passes (z_first / s_first, ascending in job order).
+struct StageJob {
+    const float* src;     // raw samples [rows][Tc] (interleaved re, im for complex formats)
+    void* dst;            // fused rows [rows][lde] elements
+    const float* pair8;   // pair table, 8 floats per pair
+    float* Z;             // [Tc + 1] path-length thresholds (written by the first pass)
+    uint32_t rows, Tc, lde;
+    int32_t format;
+    float inv_scale, t0, dT;
+    uint32_t z_first, s_first;
+    uint32_t pad_;
+};
+// raw[r][c] = src[map[r]][from + c - cfrom] for c in [cfrom, cfrom + ncopy), else 0
+// (r < rows, c < Tc; src rows have pitch src_T).
+hipError_t launch_crop_rows(const float* src, uint32_t src_T, const uint32_t* map, uint32_t rows, uint32_t Tc,
+                            uint32_t from, uint32_t cfrom, uint32_t ncopy, float* raw, hipStream_t stream);
+uint32_t stage_z_blocks(const StageJob& j);
+uint32_t stage_row_blocks(const StageJob& j);
+hipError_t launch_stage(const StageJob* jobs, uint32_t n_jobs, uint32_t z_blocks, uint32_t row_blocks,
+                        hipStream_t stream);
 bool variant_fused(int variant);
 int fused_elem_bytes(int format);
 int fused_s_offset(int format);

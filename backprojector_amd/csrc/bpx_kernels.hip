@@ -10,41 +10,47 @@
 // picks the two candidate bins {m, m+1}, and one comparison against the table decides.  The
 // host guarantees |approximate - exact quotient| < 0.5 (bpx_api.cpp, tap_error_bound), so the
 // selected bin is bit-identical to the fp32 formula for every voxel and pair.
+#include <algorithm>
+
 #include "bpx_device.hpp"
 #include "bpx_internal.hpp"
 
 namespace bpx {
 
 // =================================================================== threshold tables
-// Confocal pair: input s = squared wall-point-to-voxel distance; d1 = d2 = sqrt(s).
+// Confocal pair: input s = squared wall-point-to-voxel distance; d1 = d2 = sqrt(s).  The bin
+// round((total - t0) / deltaT) is non-decreasing in total, so "bin >= j" is "total >= Z[j]"
+// with Z[j] the smallest such total (ztable_kernel, kind 0): the search over s needs no
+// division.
 struct StablePred {
-    float lw, cw, t0, dT, jf;
+    float lw, cw, z;
     __device__ bool operator()(uint32_t key) const {
-        const float s = fkey_inv(key);
-        const float d = exact_sqrt(s);
+        const float d = exact_sqrt(fkey_inv(key));
         float total = lw + d;
         total = total + d;
         total = total + cw;
-        return tap_round(total, t0, dT) >= jf;
+        return total >= z;
     }
 };
+
+__device__ __forceinline__ float stable_threshold(float lw, float cw, float z) {
+    const StablePred pred{lw, cw, z};
+    const float d = 0.5f * ((z - lw) - cw);
+    const float guess = d > 0.0f ? d * d : 0.0f;
+    return fkey_inv(min_true_key(pred, fkey(guess), fkey(0.0f), fkey(__builtin_inff())));
+}
 
 // S[p][j] is written at float index p*pitch + j*stride + offset (stride 1, offset 0 for a
 // plain table; the element stride/slot of a fused layout otherwise).
 __global__ __launch_bounds__(256) void stable_kernel(const float* __restrict__ pair8, uint32_t P,
-                                                     uint32_t Tc, float t0, float dT,
+                                                     uint32_t Tc, const float* __restrict__ Z,
                                                      float* __restrict__ S, uint32_t pitch,
                                                      uint32_t stride, uint32_t offset) {
     const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t n = Tc + 1u;
     if (gid >= (uint64_t)P * n) return;
     const uint32_t p = (uint32_t)(gid / n), j = (uint32_t)(gid % n);
-    const float cw = pair8[8u * p + 6u], lw = pair8[8u * p + 7u];
-    const StablePred pred{lw, cw, t0, dT, (float)j};
-    const float d = 0.5f * (t0 + ((float)j - 0.5f) * dT - lw - cw);
-    const float guess = d > 0.0f ? d * d : 0.0f;
-    const uint32_t k = min_true_key(pred, fkey(guess), fkey(0.0f), fkey(__builtin_inff()));
-    S[(size_t)p * pitch + (size_t)j * stride + offset] = fkey_inv(k);
+    S[(size_t)p * pitch + (size_t)j * stride + offset] = stable_threshold(pair8[8u * p + 7u], pair8[8u * p + 6u], Z[j]);
 }
 
 // Path-length space: kind 0 = single tap (backproject_cuda.cu:155), kind 1 / 2 = start / end
@@ -74,13 +80,13 @@ __global__ __launch_bounds__(256) void ztable_kernel(int kind, float t0, float d
     Z[j] = fkey_inv(k);
 }
 
-hipError_t launch_stable(const float* pair8, uint32_t P, uint32_t Tc, float t0, float dT,
+hipError_t launch_stable(const float* pair8, uint32_t P, uint32_t Tc, const float* Z,
                          float* S, uint32_t pitch, uint32_t stride, uint32_t offset,
                          hipStream_t stream) {
     const uint64_t n = (uint64_t)P * (Tc + 1u);
     const uint32_t blocks = (uint32_t)((n + 255u) / 256u);
-    hipLaunchKernelGGL(stable_kernel, dim3(blocks), dim3(256), 0, stream, pair8, P, Tc, t0, dT, S,
-                       pitch, stride, offset);
+    hipLaunchKernelGGL(stable_kernel, dim3(blocks), dim3(256), 0, stream, pair8, P, Tc, Z, S, pitch,
+                       stride, offset);
     return hipGetLastError();
 }
 
@@ -140,25 +146,21 @@ __global__ __launch_bounds__(256) void pad_rows_kernel(const float* __restrict__
     store_sample(dst, gid, format, re, im);
 }
 
-// Sample slots of the fused layouts (see FusedF32 etc.); threshold slots are zeroed here and
-// written afterwards by stable_kernel.
+// Sample slots of the fused layouts (see FusedF32 etc.): padded row index i holds raw sample
+// i - 1; index 0 and indices past Tc hold 0.
 __device__ __forceinline__ float padded_sample(const float* row, uint32_t i, uint32_t Tc, float inv_scale,
                                                int comp, int cplx) {
     if (i < 1u || i > Tc) return 0.0f;
     return cplx ? row[2 * (i - 1u) + comp] * inv_scale : row[i - 1u] * inv_scale;
 }
 
-__global__ __launch_bounds__(256) void fused_pad_kernel(const float* __restrict__ src, uint32_t rows,
-                                                        uint32_t Tc, uint32_t src_pitch, void* dst,
-                                                        uint32_t lde, int format, float inv_scale) {
-    const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= (uint64_t)rows * lde) return;
-    const uint32_t p = (uint32_t)(gid / lde), j = (uint32_t)(gid % lde);
-    const int cplx = format_complex(format) ? 1 : 0;
-    const float* row = src + (size_t)p * src_pitch * (cplx ? 2u : 1u);
+// Element e of a fused row: the sample slot(s) of padded index j and the threshold slot S.
+__device__ __forceinline__ void store_fused(void* dst, uint64_t e, int format, const float* row, uint32_t j,
+                                            uint32_t Tc, float inv_scale, float S) {
+    const uint32_t sb = __float_as_uint(S);
     switch (format) {
         case kF32:
-            static_cast<float2*>(dst)[gid] = make_float2(padded_sample(row, j, Tc, inv_scale, 0, 0), 0.0f);
+            static_cast<float2*>(dst)[e] = make_float2(padded_sample(row, j, Tc, inv_scale, 0, 0), S);
             break;
         case kF16:
         case kBF16: {
@@ -166,12 +168,12 @@ __global__ __launch_bounds__(256) void fused_pad_kernel(const float* __restrict_
             const float b = padded_sample(row, j + 1u, Tc, inv_scale, 0, 0);
             const uint32_t ha = format == kF16 ? f32_to_f16(a) : f32_to_bf16_rne(a);
             const uint32_t hb = format == kF16 ? f32_to_f16(b) : f32_to_bf16_rne(b);
-            static_cast<uint2*>(dst)[gid] = make_uint2(0u, ha | (hb << 16));
+            static_cast<uint2*>(dst)[e] = make_uint2(sb, ha | (hb << 16));
             break;
         }
         case kC64:
-            static_cast<float4*>(dst)[gid] = make_float4(padded_sample(row, j, Tc, inv_scale, 0, 1),
-                                                         padded_sample(row, j, Tc, inv_scale, 1, 1), 0.0f, 0.0f);
+            static_cast<float4*>(dst)[e] = make_float4(padded_sample(row, j, Tc, inv_scale, 0, 1),
+                                                       padded_sample(row, j, Tc, inv_scale, 1, 1), S, 0.0f);
             break;
         default: {
             const bool bf = format == kCBF16;
@@ -180,17 +182,91 @@ __global__ __launch_bounds__(256) void fused_pad_kernel(const float* __restrict_
                                 (q(padded_sample(row, j, Tc, inv_scale, 1, 1)) << 16);
             const uint32_t c1 = q(padded_sample(row, j + 1u, Tc, inv_scale, 0, 1)) |
                                 (q(padded_sample(row, j + 1u, Tc, inv_scale, 1, 1)) << 16);
-            static_cast<uint4*>(dst)[gid] = make_uint4(0u, c0, c1, 0u);
+            static_cast<uint4*>(dst)[e] = make_uint4(sb, c0, c1, 0u);
             break;
         }
     }
 }
 
-hipError_t launch_fused_pad(const float* src, uint32_t rows, uint32_t Tc, uint32_t src_pitch,
-                            void* dst, uint32_t lde, int format, float inv_scale, hipStream_t stream) {
-    const uint64_t n = (uint64_t)rows * lde;
-    hipLaunchKernelGGL(fused_pad_kernel, dim3((uint32_t)((n + 255u) / 256u)), dim3(256), 0, stream,
-                       src, rows, Tc, src_pitch, dst, lde, format, inv_scale);
+// The job owning block `blk` of a pass: the last job whose first block is <= blk (jobs with no
+// blocks share their first block with the next job and are skipped).  Uniform per block.
+template <bool ZPASS>
+__device__ __forceinline__ uint32_t stage_job_of(const StageJob* jobs, uint32_t n, uint32_t blk) {
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        const uint32_t f = ZPASS ? jobs[mid].z_first : jobs[mid].s_first;
+        if (f <= blk) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(256) void stage_z_kernel(const StageJob* __restrict__ jobs, uint32_t n_jobs,
+                                                      uint32_t blocks) {
+    for (uint32_t blk = blockIdx.x; blk < blocks; blk += gridDim.x) {
+        const StageJob& jb = jobs[stage_job_of<true>(jobs, n_jobs, blk)];
+        const uint32_t j = (blk - jb.z_first) * 256u + threadIdx.x;
+        if (j > jb.Tc) continue;
+        const ZPred pred{0, jb.t0, jb.dT, 0.0f, (float)j};
+        const float g = jb.t0 + ((float)j - 0.5f) * jb.dT;
+        jb.Z[j] = fkey_inv(min_true_key(pred, fkey(g), fkey(-__builtin_inff()), fkey(__builtin_inff())));
+    }
+}
+
+__global__ __launch_bounds__(256) void stage_rows_kernel(const StageJob* __restrict__ jobs, uint32_t n_jobs,
+                                                         uint32_t blocks) {
+    for (uint32_t blk = blockIdx.x; blk < blocks; blk += gridDim.x) {
+        const StageJob& jb = jobs[stage_job_of<false>(jobs, n_jobs, blk)];
+        const uint64_t e = (uint64_t)(blk - jb.s_first) * 256u + threadIdx.x;
+        if (e >= (uint64_t)jb.rows * jb.lde) continue;
+        const uint32_t p = (uint32_t)(e / jb.lde), j = (uint32_t)(e % jb.lde);
+        const int cplx = format_complex(jb.format) ? 1 : 0;
+        const float* row = jb.src + (size_t)p * jb.Tc * (cplx ? 2u : 1u);
+        float S = 0.0f;
+        if (j <= jb.Tc) S = stable_threshold(jb.pair8[8u * p + 7u], jb.pair8[8u * p + 6u], jb.Z[j]);
+        store_fused(jb.dst, e, jb.format, row, j, jb.Tc, jb.inv_scale, S);
+    }
+}
+
+__global__ __launch_bounds__(256) void crop_rows_kernel(const float* __restrict__ src, uint32_t src_T,
+                                                        const uint32_t* __restrict__ map, uint32_t rows,
+                                                        uint32_t Tc, uint32_t from, uint32_t cfrom,
+                                                        uint32_t ncopy, float* __restrict__ raw) {
+    const uint64_t n = (uint64_t)rows * Tc, stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; gid < n; gid += stride) {
+        const uint32_t r = (uint32_t)(gid / Tc), c = (uint32_t)(gid % Tc);
+        float v = 0.0f;
+        if (c >= cfrom && c - cfrom < ncopy) v = src[(size_t)map[r] * src_T + from + (c - cfrom)];
+        raw[gid] = v;
+    }
+}
+
+hipError_t launch_crop_rows(const float* src, uint32_t src_T, const uint32_t* map, uint32_t rows, uint32_t Tc,
+                            uint32_t from, uint32_t cfrom, uint32_t ncopy, float* raw, hipStream_t stream) {
+    const uint64_t n = (uint64_t)rows * Tc;
+    if (n == 0) return hipSuccess;
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>((n + 255u) / 256u, 1u << 20);
+    hipLaunchKernelGGL(crop_rows_kernel, dim3(blocks), dim3(256), 0, stream, src, src_T, map, rows, Tc, from, cfrom,
+                       ncopy, raw);
+    return hipGetLastError();
+}
+
+uint32_t stage_z_blocks(const StageJob& j) { return (j.Tc + 1u + 255u) / 256u; }
+uint32_t stage_row_blocks(const StageJob& j) {
+    return (uint32_t)(((uint64_t)j.rows * j.lde + 255u) / 256u);
+}
+
+hipError_t launch_stage(const StageJob* jobs, uint32_t n_jobs, uint32_t z_blocks, uint32_t row_blocks,
+                        hipStream_t stream) {
+    if (n_jobs == 0) return hipSuccess;
+    constexpr uint32_t kMaxGrid = 1u << 20;
+    if (z_blocks)
+        hipLaunchKernelGGL(stage_z_kernel, dim3(std::min(z_blocks, kMaxGrid)), dim3(256), 0, stream, jobs, n_jobs,
+                           z_blocks);
+    if (row_blocks)
+        hipLaunchKernelGGL(stage_rows_kernel, dim3(std::min(row_blocks, kMaxGrid)), dim3(256), 0, stream, jobs,
+                           n_jobs, row_blocks);
     return hipGetLastError();
 }
 

@@ -157,12 +157,7 @@ void execute_octree(PlanImpl& pl, hipStream_t s) {
     hip_check(hipEventRecord(pl.ev[0], s), "hipEventRecord");
     void* E = pl.H.ptr;
     const uint32_t e_bytes = (uint32_t)pl.H.bytes;
-    hip_check(launch_fused_pad(static_cast<const float*>(pl.raw.ptr), pl.P, pl.T, pl.T, pl.H.ptr, pl.ldh,
-                               pl.format, 1.0f / pl.scale, s),
-              "row staging kernel");
-    hip_check(launch_stable(static_cast<const float*>(pl.pair8.ptr), pl.P, pl.T, pl.t0, pl.dT,
-                            static_cast<float*>(pl.H.ptr), pl.ldh * 2u, 2u, 1u, s),
-              "threshold table kernel");
+    stage_rows(pl, s);
     build_winmax(pl, s);
     hip_check(hipEventRecord(pl.ev[1], s), "hipEventRecord");
     hip_check(hipMemsetAsync(st.stats.ptr, 0, 8 * sizeof(double), s), "hipMemsetAsync");

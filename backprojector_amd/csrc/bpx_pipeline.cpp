@@ -208,23 +208,33 @@ void basis(const float volume_position[3], const float volume_size[3], const uin
         for (int c = 0; c < 3; ++c) inc[3 * r + c] = (ax[r][c] * volume_size[c]) / (float)(n[c] - 1u);
 }
 
-std::vector<float> transient_chunk(const float* data, uint64_t rows, uint32_t T, float min_idx,
-                                   float max_idx, uint32_t* Tc_out) {
+ChunkWindow chunk_window(uint32_t T, float min_idx, float max_idx) {
     const float fmin = std::floor(min_idx), fmax = std::ceil(max_idx);
     if (!(fmax - fmin >= 1.0f) || fmax - fmin > 4194304.0f)
         fail(BPX_ERR_INVALID_ARGUMENT, "time limits give an empty or oversized chunk");
-    const uint64_t Tc = (uint64_t)(fmax - fmin);
+    ChunkWindow w{};
+    w.Tc = (uint32_t)(fmax - fmin);
     const uint64_t from = fmin > 0.0f ? (uint64_t)fmin : 0;
     uint64_t to = (uint64_t)(fmax > 0.0f ? fmax : 0.0f);
     if (to > T) to = T;
     const uint64_t cfrom = fmin > 0.0f ? 0 : (uint64_t)std::fabs(fmin);
-    std::vector<float> out(rows * Tc, 0.0f);
-    if (to > from) {
-        const uint64_t n = to - from;
-        for (uint64_t r = 0; r < rows; ++r)
-            if (cfrom + n <= Tc) std::memcpy(&out[r * Tc + cfrom], data + r * T + from, 4 * n);
+    // backproject.hpp:701-706; a row window that would overrun the chunk is not copied
+    if (to > from && cfrom + (to - from) <= w.Tc) {
+        w.from = (uint32_t)from;
+        w.cfrom = (uint32_t)cfrom;
+        w.ncopy = (uint32_t)(to - from);
     }
-    *Tc_out = (uint32_t)Tc;
+    return w;
+}
+
+std::vector<float> transient_chunk(const float* data, uint64_t rows, uint32_t T, float min_idx,
+                                   float max_idx, uint32_t* Tc_out) {
+    const ChunkWindow w = chunk_window(T, min_idx, max_idx);
+    std::vector<float> out(rows * w.Tc, 0.0f);
+    if (w.ncopy)
+        for (uint64_t r = 0; r < rows; ++r)
+            std::memcpy(&out[r * w.Tc + w.cfrom], data + r * T + w.from, 4 * (size_t)w.ncopy);
+    *Tc_out = w.Tc;
     return out;
 }
 
@@ -336,6 +346,24 @@ PipelineGeometry prepare(const float* transient_data, const int64_t* data_shape,
     return g;
 }
 
+// The capture as the device plans take it: the (row-major) source rows with the chunk window
+// of get_transient_chunk, cropped on the device (no host chunk); t0 of the chunk is
+// chunked_t0 = t0 + min_T_index * deltaT, unfloored (backproject.hpp:847).
+CaptureIn device_chunk(const PipelineGeometry& g, float t0, float deltaT) {
+    const pipeline::ChunkWindow w = pipeline::chunk_window(g.T, g.min_idx, g.max_idx);
+    if (g.rows != g.pairs.size())
+        fail(BPX_ERR_INVALID_ARGUMENT, "transient_size (" + std::to_string(g.rows * w.Tc) + ") != num_pairs * T (" +
+                                           std::to_string((uint64_t)g.pairs.size() * w.Tc) + ")");
+    CaptureIn c{nullptr, g.rows * w.Tc, w.Tc, g.pairs.data(), (uint32_t)g.pairs.size(),
+                t0 + g.min_idx * deltaT, deltaT, false};
+    c.src = g.arr.data_p;
+    c.src_T = g.T;
+    c.from = w.from;
+    c.cfrom = w.cfrom;
+    c.ncopy = w.ncopy;
+    return c;
+}
+
 }  // namespace
 
 extern "C" {
@@ -355,14 +383,11 @@ bpx_status bpx_nlos_backproject(const float* transient_data, const int64_t* data
                                      camera_grid_shape, laser_grid_positions, laser_grid_shape,
                                      camera_position, laser_position, t0, deltaT, is_confocal,
                                      volume_position, volume_size, voxels_per_side, assume_row_major);
-        uint32_t Tc = 0;
-        std::vector<float> chunk =
-            pipeline::transient_chunk(g.arr.data_p, g.rows, g.T, g.min_idx, g.max_idx, &Tc);
-        const float chunked_t0 = t0 + g.min_idx * deltaT;  // :847, unfloored
+        const CaptureIn c = device_chunk(g, t0, deltaT);
         if (use_cpu) {
-            return bpx_window_backprojection(chunk.data(), chunk.size(), Tc, g.pairs.data(),
-                                             (uint32_t)g.pairs.size(), volume_out, voxels_per_side,
-                                             volume_position, volume_size, chunked_t0, deltaT, 0);
+            if (!volume_position || !volume_size) fail(BPX_ERR_INVALID_ARGUMENT, "NULL volume argument");
+            run_plan_to_host(make_plan_window(c, voxels_per_side, volume_position, volume_size), volume_out);
+            return BPX_OK;
         }
         for (int k = 0; k < 3; ++k)
             if (voxels_per_side[k] < 2)
@@ -372,9 +397,8 @@ bpx_status bpx_nlos_backproject(const float* transient_data, const int64_t* data
         float zero[3], inc[9];
         pipeline::basis(volume_position, volume_size, voxels_per_side, wall_normal ? wall_normal : zeros3,
                         zero, inc);
-        return bpx_backprojection(chunk.data(), chunk.size(), Tc, g.pairs.data(),
-                                  (uint32_t)g.pairs.size(), volume_out, voxels_per_side, zero, inc,
-                                  chunked_t0, deltaT);
+        run_plan_to_host(make_plan_single_tap(c, voxels_per_side, zero, inc, nullptr), volume_out);
+        return BPX_OK;
     } catch (const Failure& e) {
         return set_error(e.status, e.msg);
     } catch (const std::bad_alloc&) {
@@ -399,10 +423,7 @@ bpx_status bpx_nlos_plan_create(const float* transient_data, const int64_t* data
                                      camera_grid_shape, laser_grid_positions, laser_grid_shape,
                                      camera_position, laser_position, t0, deltaT, is_confocal,
                                      volume_position, volume_size, voxels_per_side, assume_row_major);
-        uint32_t Tc = 0;
-        std::vector<float> chunk =
-            pipeline::transient_chunk(g.arr.data_p, g.rows, g.T, g.min_idx, g.max_idx, &Tc);
-        const float chunked_t0 = t0 + g.min_idx * deltaT;
+        const CaptureIn c = device_chunk(g, t0, deltaT);
         for (int k = 0; k < 3; ++k)
             if (voxels_per_side[k] < 2)
                 fail(BPX_ERR_INVALID_ARGUMENT,
@@ -411,8 +432,6 @@ bpx_status bpx_nlos_plan_create(const float* transient_data, const int64_t* data
         float zero[3], inc[9];
         pipeline::basis(volume_position, volume_size, voxels_per_side, wall_normal ? wall_normal : zeros3,
                         zero, inc);
-        CaptureIn c{chunk.data(), chunk.size(), Tc, g.pairs.data(), (uint32_t)g.pairs.size(),
-                    chunked_t0, deltaT, false};
         auto p = std::make_unique<bpx_plan_s>();
         p->impl.reset(make_plan_single_tap(c, voxels_per_side, zero, inc, options));
         *plan_out = p.release();

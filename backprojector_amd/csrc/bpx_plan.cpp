@@ -83,9 +83,23 @@ static double max_total_path(const std::vector<bpx_pair>& pr, const double corne
 }
 
 // Power-of-two scale s so that max|x| / s lies in [2^14, 2^15) (exact rescaling).
-static float choose_scale(const float* x, uint64_t n) {
+// Largest |sample| of the chunk (host chunk, or the copied windows of the source rows).
+static float chunk_max_abs(const CaptureIn& c) {
     float m = 0.0f;
-    for (uint64_t i = 0; i < n; ++i) m = std::max(m, std::fabs(x[i]));
+    if (c.chunk) {
+        const uint64_t n = c.size * (c.cplx ? 2 : 1);
+        for (uint64_t i = 0; i < n; ++i) m = std::max(m, std::fabs(c.chunk[i]));
+    } else if (c.src) {
+        for (uint32_t p = 0; p < c.P; ++p) {
+            const float* r = c.src + (size_t)p * c.src_T + c.from;
+            for (uint32_t i = 0; i < c.ncopy; ++i) m = std::max(m, std::fabs(r[i]));
+        }
+    }
+    return m;
+}
+
+static float choose_scale(const CaptureIn& c) {
+    const float m = chunk_max_abs(c);
     if (!(m > 0.0f) || !std::isfinite(m)) return 1.0f;
     int e;
     std::frexp(m, &e);  // m in [2^(e-1), 2^e)
@@ -93,7 +107,12 @@ static float choose_scale(const float* x, uint64_t n) {
 }
 
 static void validate_common(const CaptureIn& c, const uint32_t n[3]) {
-    if (!c.chunk && c.size) fail(BPX_ERR_INVALID_ARGUMENT, "transient_chunk is NULL");
+    if (!c.chunk && !c.src && c.size) fail(BPX_ERR_INVALID_ARGUMENT, "transient_chunk is NULL");
+    if (!c.chunk && c.src) {
+        if (c.cplx) fail(BPX_ERR_INVALID_ARGUMENT, "device cropping takes real samples");
+        if ((uint64_t)c.from + c.ncopy > c.src_T || (uint64_t)c.cfrom + c.ncopy > c.T)
+            fail(BPX_ERR_INVALID_ARGUMENT, "crop window outside the source or chunk rows");
+    }
     if (!c.pairs && c.P) fail(BPX_ERR_INVALID_ARGUMENT, "scanned_pairs is NULL");
     if (c.T == 0) fail(BPX_ERR_INVALID_ARGUMENT, "T must be > 0");
     if ((uint64_t)c.P * c.T != c.size)
@@ -130,6 +149,28 @@ static void stage_raw(PlanImpl& pl, const CaptureIn& c, const std::vector<uint32
     const size_t comp = c.cplx ? 2 : 1;
     const size_t row = (size_t)c.T * comp;
     pl.raw.alloc(sizeof(float) * row * std::max<size_t>(idx.size(), 1));
+    if (!c.chunk) {
+        if (idx.empty()) return;
+        if (idx.size() == c.P && c.from == 0 && c.cfrom == 0 && c.ncopy == c.T && c.src_T == c.T) {
+            hip_check(hipMemcpy(pl.raw.ptr, c.src, sizeof(float) * row * c.P, hipMemcpyHostToDevice),
+                      "hipMemcpy(transient)");
+            return;
+        }
+        // the source rows in one copy, then the crop (and compaction of dropped pairs) on the device
+        DeviceBuffer full, map;
+        full.alloc(sizeof(float) * (size_t)c.P * c.src_T);
+        map.alloc(sizeof(uint32_t) * idx.size());
+        hip_check(hipMemcpy(full.ptr, c.src, sizeof(float) * (size_t)c.P * c.src_T, hipMemcpyHostToDevice),
+                  "hipMemcpy(transient)");
+        hip_check(hipMemcpy(map.ptr, idx.data(), sizeof(uint32_t) * idx.size(), hipMemcpyHostToDevice),
+                  "hipMemcpy(row map)");
+        hip_check(launch_crop_rows(static_cast<const float*>(full.ptr), c.src_T,
+                                   static_cast<const uint32_t*>(map.ptr), (uint32_t)idx.size(), c.T, c.from,
+                                   c.cfrom, c.ncopy, static_cast<float*>(pl.raw.ptr), pl.stream),
+                  "crop kernel");
+        hip_check(hipStreamSynchronize(pl.stream), "hipStreamSynchronize");
+        return;
+    }
     if (idx.size() == c.P) {
         hip_check(hipMemcpy(pl.raw.ptr, c.chunk, sizeof(float) * row * c.P, hipMemcpyHostToDevice),
                   "hipMemcpy(transient)");
@@ -232,7 +273,7 @@ PlanImpl* make_plan_single_tap(const CaptureIn& c, const uint32_t n[3], const fl
     if (pl->P == 0) return pl.release();
 
     // samples: raw rows resident on the device; scale for 16-bit storage
-    pl->scale = (pl->format == kF32 || pl->format == kC64) ? 1.0f : choose_scale(c.chunk, c.size * (c.cplx ? 2 : 1));
+    pl->scale = (pl->format == kF32 || pl->format == kC64) ? 1.0f : choose_scale(c);
     stage_raw(*pl, c, idx);
     // padded rows (split) or the fused sample+threshold layout, [P][ldh] elements
     pl->ldh = round_up((uint64_t)c.T + 2, pl->fused ? 16 : 64);
@@ -248,11 +289,19 @@ PlanImpl* make_plan_single_tap(const CaptureIn& c, const uint32_t n[3], const fl
               "hipMemcpy(pairs)");
 
     if (pl->path == kPathConfocal) {
+        pl->Z.alloc(sizeof(float) * ((size_t)c.T + 1));
         if (!pl->fused) {
             pl->lds = round_up((uint64_t)c.T + 1, 64);
             const uint64_t sbytes = (uint64_t)pl->P * pl->lds * 4;
             if (sbytes >= (1ull << 32)) fail(BPX_ERR_UNSUPPORTED, "threshold table above 4 GiB");
             pl->S.alloc(sbytes);
+        } else {
+            const StageJob job = stage_job(*pl);
+            pl->stage_zb = stage_z_blocks(job);
+            pl->stage_sb = stage_row_blocks(job);
+            pl->stage.alloc(sizeof(StageJob));
+            hip_check(hipMemcpy(pl->stage.ptr, &job, sizeof(StageJob), hipMemcpyHostToDevice),
+                      "hipMemcpy(staging job)");
         }
         pl->inv2 = (float)(2.0 / (double)c.dT);
         std::vector<float> p4((size_t)pl->P * 4);
@@ -304,6 +353,27 @@ PlanImpl* make_plan_single_tap(const CaptureIn& c, const uint32_t n[3], const fl
     return pl.release();
 }
 
+StageJob stage_job(const PlanImpl& pl) {
+    StageJob j{};
+    j.src = static_cast<const float*>(pl.raw.ptr);
+    j.dst = pl.H.ptr;
+    j.pair8 = static_cast<const float*>(pl.pair8.ptr);
+    j.Z = static_cast<float*>(pl.Z.ptr);
+    j.rows = pl.P;
+    j.Tc = pl.T;
+    j.lde = pl.ldh;
+    j.format = pl.format;
+    j.inv_scale = 1.0f / pl.scale;
+    j.t0 = pl.t0;
+    j.dT = pl.dT;
+    return j;
+}
+
+void stage_rows(PlanImpl& pl, hipStream_t s) {
+    hip_check(launch_stage(static_cast<const StageJob*>(pl.stage.ptr), 1u, pl.stage_zb, pl.stage_sb, s),
+              "row staging kernels");
+}
+
 void PlanImpl::execute(hipStream_t s) {
     if (path == kPathOctree) {
         execute_octree(*this, s);
@@ -347,25 +417,21 @@ void PlanImpl::execute(hipStream_t s) {
         hip_check(hipMemsetAsync(vol.ptr, 0, nvox * (cplx ? 8 : 4), s), "hipMemsetAsync");
         return;
     }
+    // events: ev[0] | split-layout row padding | ev[1] | tables (fused: all staging) | ev[2] |
+    // gather | ev[3]
     hip_check(hipEventRecord(ev[0], s), "hipEventRecord");
-    if (fused)
-        hip_check(launch_fused_pad(static_cast<const float*>(raw.ptr), P, T, T, H.ptr, ldh, format,
-                                   1.0f / scale, s),
-                  "row staging kernel");
-    else
+    if (!fused)
         hip_check(launch_pad_rows(static_cast<const float*>(raw.ptr), P, T, T, H.ptr, ldh, format,
                                   1.0f / scale, s),
                   "row staging kernel");
     hip_check(hipEventRecord(ev[1], s), "hipEventRecord");
     if (path == kPathConfocal) {
         if (fused) {
-            const uint32_t stride = (uint32_t)fused_elem_bytes(format) / 4u;
-            hip_check(launch_stable(static_cast<const float*>(pair8.ptr), P, T, t0, dT,
-                                    static_cast<float*>(H.ptr), ldh * stride, stride,
-                                    (uint32_t)fused_s_offset(format), s),
-                      "threshold table kernel");
+            stage_rows(*this, s);
         } else {
-            hip_check(launch_stable(static_cast<const float*>(pair8.ptr), P, T, t0, dT,
+            hip_check(launch_ztable(0, t0, dT, 0.0f, T + 1, static_cast<float*>(Z.ptr), s),
+                      "path-length table kernel");
+            hip_check(launch_stable(static_cast<const float*>(pair8.ptr), P, T, static_cast<const float*>(Z.ptr),
                                     static_cast<float*>(S.ptr), lds, 1u, 0u, s),
                       "threshold table kernel");
         }

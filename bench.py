@@ -47,6 +47,8 @@ def parse():
     ap.add_argument("--octree-tau", type=float, default=0.25)
     ap.add_argument("--octree-proposal", type=float, default=None,
                     help="proposal threshold of the learned octree (default: the shipped choice)")
+    ap.add_argument("--one-shot", type=int, default=3,
+                    help="timed one-shot nlosbpy.backproject calls (0 = skip)")
     ap.add_argument("--batch", type=int, default=64,
                     help="uneven captures of the batched comparison (0 = skip)")
     return ap.parse_args()
@@ -179,6 +181,27 @@ def main():
              "rel_l2_vs_fp64_oracle": rel(got, fp64)}
     log(f"check: {check}")
 
+    # ---- one-shot calls (host arrays in, host volume out; PCIe and plan set-up included)
+    one_shot = None
+    if args.one_shot > 0:
+        nargs = cap.nlos_args(N)
+        bp.nlosbpy.backproject(*nargs, (0, 0, 0), assume_row_major=True)
+        t1 = time.perf_counter()
+        for _ in range(args.one_shot):
+            bp.nlosbpy.backproject(*nargs, (0, 0, 0), assume_row_major=True)
+        dev_ms = (time.perf_counter() - t1) * 1e3 / args.one_shot
+        t1 = time.perf_counter()
+        for _ in range(args.one_shot):
+            pre = bp.nlosbpy.prepare(*nargs, (0, 0, 0), assume_row_major=True)
+            bp.backprojection(pre["chunk"], pre["pairs"], N, pre["zero"], pre["inc"], pre["chunked_t0"],
+                              cap.deltaT)
+        host_ms = (time.perf_counter() - t1) * 1e3 / args.one_shot
+        one_shot = {"calls": args.one_shot, "nlosbpy_backproject_ms": dev_ms,
+                    "updates_per_s_pcie_inclusive": updates / (dev_ms * 1e-3),
+                    "host_chunk_then_backprojection_ms": host_ms,
+                    "source_bytes": int(cap.transient.nbytes)}
+        log(f"one-shot: {one_shot}")
+
     # ---- CPU baseline: the reference CPU path (oracle restatement, OpenMP), bounded sample
     cpu = None
     if args.cpu_seconds > 0:
@@ -246,6 +269,7 @@ def main():
                      "bytes_per_update": sample_bytes},
         "cpu_baseline": cpu,
         "check": check,
+        "one_shot": one_shot,
         "octree": octree,
         "batch": batch,
         "baseline_ref": "Quadro P5000 5993 ms (README.md:106) -> 5.73e9 updates/s",

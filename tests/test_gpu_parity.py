@@ -249,6 +249,38 @@ def test_nlosbpy_backproject_matches_oracle_pipeline(gpu, oracle):
     assert rel_l2(vol, exact) <= 1e-6 and rel_l2(vol, ref) <= 1e-5
 
 
+@pytest.mark.parametrize("t0,T,nan_pair,confocal", [
+    (2.0, 2048, False, True),     # window starts inside the rows and runs past the last bin
+    (4.0, 2048, True, True),      # window starts before bin 0 (zero head), one NaN pair dropped
+    (1.0, 8192, False, True),     # window strictly inside the rows
+    (2.0, 2048, True, False),     # exhaustive pairs, dropped pairs compacted on the device
+])
+def test_nlosbpy_device_crop_equals_host_chunk(gpu, t0, T, nan_pair, confocal):
+    """The one-shot calls crop the source rows on the device (get_transient_chunk,
+    backproject.hpp:689-740); the result must equal the plan built from the host chunk that
+    the same pipeline returns (nlosbpy.prepare), GPU and CPU-path semantics alike."""
+    cap = scenes.render(scenes.config_scene(1), 12, T, t0=t0, confocal=confocal,
+                        laser_side=None if confocal else 4, seed=11)
+    N = (16, 16, 16)
+    args = list(cap.nlos_args(N))
+    if nan_pair:
+        args[1] = args[1].copy()
+        args[1][3, 5, 0] = np.nan
+    pre = gpu.nlosbpy.prepare(*args, np.zeros(3), assume_row_major=True)
+    lo, hi = np.floor(pre["min_index"]), np.ceil(pre["max_index"])
+    assert (lo < 0) == (t0 == 4.0) and (hi > T) == (T == 2048)
+    ok = np.isfinite(pre["pairs"]["camera_point"]).all(1)
+    assert ok.all() != nan_pair
+    vol = gpu.nlosbpy.backproject(*args, np.zeros(3), assume_row_major=True)
+    ref = gpu.backprojection(pre["chunk"], pre["pairs"], N, pre["zero"], pre["inc"], pre["chunked_t0"],
+                             cap.deltaT)
+    assert np.array_equal(vol, ref) and np.abs(ref).max() > 0
+    vc = gpu.nlosbpy.backproject(*args, np.zeros(3), assume_row_major=True, use_cpu=True)
+    rc = gpu.window_backprojection(pre["chunk"], pre["pairs"], N, cap.volume_position, cap.volume_size,
+                                   pre["chunked_t0"], cap.deltaT)
+    assert np.array_equal(vc, rc) and np.abs(rc).max() > 0
+
+
 def test_nlosbpy_column_major_equals_row_major(gpu):
     cap = scenes.render(scenes.config_scene(1), 16, 2048, t0=2.0, seed=3)
     N = (12, 12, 12)
