@@ -219,6 +219,13 @@ class Plan:
         check(lib().bpx_plan_read_volume(self._h, ptr(out)))
         return out
 
+    def thresholds(self):
+        """The confocal threshold table of the last execution, [pairs_used][Tc + 1] float32."""
+        inf = self.info()
+        out = np.zeros((inf["pairs_used"], inf["Tc"] + 1), np.float32)
+        check(lib().bpx_plan_read_thresholds(self._h, ptr(out), out.size))
+        return out
+
     def info(self):
         path, k = ctypes.c_int32(), ctypes.c_int32()
         used, tc, sc = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_float()

@@ -7,6 +7,8 @@
 #include <mutex>
 #include <string>
 
+#include <vector>
+
 #include "bpx_host.hpp"
 
 namespace bpx {
@@ -269,6 +271,27 @@ bpx_status bpx_plan_read_volume(bpx_plan plan, float* voxel_volume) {
         hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
         hip_check(hipMemcpy(voxel_volume, pl.vol.ptr, nvox * (pl.cplx ? 8 : 4), hipMemcpyDeviceToHost),
                   "hipMemcpy(volume)");
+    });
+}
+
+bpx_status bpx_plan_read_thresholds(bpx_plan plan, float* out, uint64_t capacity) {
+    return guarded([&] {
+        if (!plan || !out) fail(BPX_ERR_INVALID_ARGUMENT, "NULL argument");
+        const PlanImpl& pl = *plan->impl;
+        if (pl.path != kPathConfocal && pl.path != kPathOctree)
+            fail(BPX_ERR_UNSUPPORTED, "threshold tables belong to confocal plans");
+        const uint64_t n = (uint64_t)pl.P * (pl.T + 1u);
+        if (capacity < n) fail(BPX_ERR_INVALID_ARGUMENT, "output holds fewer than pairs_used * (Tc + 1) floats");
+        if (n == 0) return;
+        hip_check(hipStreamSynchronize(pl.stream), "hipStreamSynchronize");
+        const DeviceBuffer& src = pl.fused ? pl.H : pl.S;
+        const size_t stride = pl.fused ? (size_t)fused_elem_bytes(pl.format) / 4 : 1;
+        const size_t pitch = pl.fused ? (size_t)pl.ldh * stride : pl.lds;
+        const size_t off = pl.fused ? (size_t)fused_s_offset(pl.format) : 0;
+        std::vector<float> host(src.bytes / 4);
+        hip_check(hipMemcpy(host.data(), src.ptr, host.size() * 4, hipMemcpyDeviceToHost), "hipMemcpy(thresholds)");
+        for (uint64_t p = 0; p < pl.P; ++p)
+            for (uint64_t j = 0; j <= pl.T; ++j) out[p * (pl.T + 1u) + j] = host[p * pitch + j * stride + off];
     });
 }
 

@@ -148,6 +148,9 @@ struct CaptureIn {
     // done by one copy of the source rows and a crop kernel.
     const float* src = nullptr;
     uint32_t src_T = 0, from = 0, cfrom = 0, ncopy = 0;
+    // The chunk already in device memory ([P][T] in the layout of `chunk`); the plan takes the
+    // buffer over (leaving it empty) when every pair is used, else gathers the used rows.
+    DeviceBuffer* device_rows = nullptr;
 };
 
 // Executes a plan, copies its volume to host memory and destroys the plan.

@@ -212,7 +212,7 @@ struct StageJob {
     uint32_t pad_;
 };
 // raw[r][c] = src[map[r]][from + c - cfrom] for c in [cfrom, cfrom + ncopy), else 0
-// (r < rows, c < Tc; src rows have pitch src_T).
+// (r < rows, c < Tc; src rows have pitch src_T; map NULL = identity).
 hipError_t launch_crop_rows(const float* src, uint32_t src_T, const uint32_t* map, uint32_t rows, uint32_t Tc,
                             uint32_t from, uint32_t cfrom, uint32_t ncopy, float* raw, hipStream_t stream);
 uint32_t stage_z_blocks(const StageJob& j);
@@ -230,8 +230,8 @@ hipError_t launch_pad_rows(const float* src, uint32_t rows, uint32_t Tc, uint32_
 hipError_t launch_prefix(const float* src, uint32_t rows, uint32_t T, int complex_samples,
                          void* dst, uint32_t ldp, hipStream_t stream);
 hipError_t launch_phasor_fir(const float* src, uint32_t rows, uint32_t Tc, const float* taps_c,
-                             const float* taps_s, int pulse, void* dst, uint32_t ldh, int format,
-                             float inv_scale, hipStream_t stream);
+                             const float* taps_s, int pulse, void* dst, uint32_t ldh, uint32_t lead,
+                             int format, float inv_scale, hipStream_t stream);
 hipError_t launch_confocal(const ConfocalArgs& a, int format, int variant, bool sep, bool flat,
                            hipStream_t stream);
 hipError_t launch_grid(const GridArgs& a, int format, int variant, hipStream_t stream);

@@ -20,24 +20,31 @@ namespace bpx {
 // =================================================================== threshold tables
 // Confocal pair: input s = squared wall-point-to-voxel distance; d1 = d2 = sqrt(s).  The bin
 // round((total - t0) / deltaT) is non-decreasing in total, so "bin >= j" is "total >= Z[j]"
-// with Z[j] the smallest such total (ztable_kernel, kind 0): the search over s needs no
-// division.
-struct StablePred {
+// with Z[j] the smallest such total (ztable_kernel, kind 0): the search needs no division.
+// total = ((lw + d) + d) + cw is non-decreasing in d and d = sqrt_rn(s) in s, so the smallest
+// s with total >= Z[j] is the smallest s with sqrt_rn(s) >= d*, d* the smallest d with
+// total(d) >= Z[j]: two short searches, the first over additions only, the second over one
+// correctly rounded square root.
+struct ChainPred {
     float lw, cw, z;
     __device__ bool operator()(uint32_t key) const {
-        const float d = exact_sqrt(fkey_inv(key));
+        const float d = fkey_inv(key);
         float total = lw + d;
         total = total + d;
         total = total + cw;
         return total >= z;
     }
 };
+struct SqrtPred {
+    float d;
+    __device__ bool operator()(uint32_t key) const { return exact_sqrt(fkey_inv(key)) >= d; }
+};
 
 __device__ __forceinline__ float stable_threshold(float lw, float cw, float z) {
-    const StablePred pred{lw, cw, z};
-    const float d = 0.5f * ((z - lw) - cw);
-    const float guess = d > 0.0f ? d * d : 0.0f;
-    return fkey_inv(min_true_key(pred, fkey(guess), fkey(0.0f), fkey(__builtin_inff())));
+    const float g = 0.5f * ((z - lw) - cw);
+    const float dstar = fkey_inv(min_true_key(ChainPred{lw, cw, z}, fkey(g > 0.0f ? g : 0.0f), fkey(0.0f),
+                                              fkey(__builtin_inff())));
+    return fkey_inv(min_true_key(SqrtPred{dstar}, fkey(dstar * dstar), fkey(0.0f), fkey(__builtin_inff())));
 }
 
 // S[p][j] is written at float index p*pitch + j*stride + offset (stride 1, offset 0 for a
@@ -237,7 +244,7 @@ __global__ __launch_bounds__(256) void crop_rows_kernel(const float* __restrict_
     for (uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; gid < n; gid += stride) {
         const uint32_t r = (uint32_t)(gid / Tc), c = (uint32_t)(gid % Tc);
         float v = 0.0f;
-        if (c >= cfrom && c - cfrom < ncopy) v = src[(size_t)map[r] * src_T + from + (c - cfrom)];
+        if (c >= cfrom && c - cfrom < ncopy) v = src[(size_t)(map ? map[r] : r) * src_T + from + (c - cfrom)];
         raw[gid] = v;
     }
 }
@@ -326,11 +333,12 @@ hipError_t launch_prefix(const float* src, uint32_t rows, uint32_t T, int comple
 
 // Phasor-field filter (backproject.hpp:459-476): out[t] = sum_{j<pulse} in[t+j-pulse/2] *
 // (c_j + i s_j) over 0 < idx < Tc, products rounded, summed in j order; written straight
-// into a padded complex destination row (fused staging/quantisation).
+// into a complex destination row of ldh elements, out[t] at element lead + t (lead 1: the
+// padded rows of the gather kernels; lead 0: plain [rows][Tc] complex rows), zero elsewhere.
 __global__ __launch_bounds__(256) void phasor_fir_kernel(const float* __restrict__ src, uint32_t rows,
                                                          uint32_t Tc, const float* __restrict__ tc,
                                                          const float* __restrict__ ts, int pulse,
-                                                         void* dst, uint32_t ldh, int format,
+                                                         void* dst, uint32_t ldh, uint32_t lead, int format,
                                                          float inv_scale) {
     extern __shared__ float taps[];
     for (int j = threadIdx.x; j < pulse; j += blockDim.x) {
@@ -342,8 +350,8 @@ __global__ __launch_bounds__(256) void phasor_fir_kernel(const float* __restrict
     if (gid >= (uint64_t)rows * ldh) return;
     const uint32_t p = (uint32_t)(gid / ldh), i = (uint32_t)(gid % ldh);
     float re = 0.0f, im = 0.0f;
-    if (i >= 1u && i <= Tc) {
-        const int64_t t = (int64_t)i - 1;
+    if (i >= lead && i - lead < Tc) {
+        const int64_t t = (int64_t)(i - lead);
         const float* row = src + (size_t)p * Tc;
         const int half = pulse / 2;
         for (int j = 0; j < pulse; ++j) {
@@ -361,12 +369,14 @@ __global__ __launch_bounds__(256) void phasor_fir_kernel(const float* __restrict
 }
 
 hipError_t launch_phasor_fir(const float* src, uint32_t rows, uint32_t Tc, const float* taps_c,
-                             const float* taps_s, int pulse, void* dst, uint32_t ldh, int format,
-                             float inv_scale, hipStream_t stream) {
+                             const float* taps_s, int pulse, void* dst, uint32_t ldh, uint32_t lead,
+                             int format, float inv_scale, hipStream_t stream) {
     const uint64_t n = (uint64_t)rows * ldh;
+    if (n == 0) return hipSuccess;
+    if ((n + 255u) / 256u > 0x7fffffffull || (uint64_t)lead + Tc > ldh) return hipErrorInvalidValue;
     hipLaunchKernelGGL(phasor_fir_kernel, dim3((uint32_t)((n + 255u) / 256u)), dim3(256),
                        sizeof(float) * 2 * pulse, stream, src, rows, Tc, taps_c, taps_s, pulse, dst,
-                       ldh, format, inv_scale);
+                       ldh, lead, format, inv_scale);
     return hipGetLastError();
 }
 

@@ -275,33 +275,55 @@ int phasor_taps(float wavelength, float dT, uint32_t times, float sigma, std::ve
 
 }  // namespace pipeline
 
+static void upload_taps(const std::vector<float>& c, const std::vector<float>& s, DeviceBuffer& dc,
+                        DeviceBuffer& ds) {
+    if (c.empty() || c.size() > 4096 || c.size() != s.size())
+        fail(BPX_ERR_UNSUPPORTED, "phasor pulses of 1..4096 taps are supported");
+    dc.alloc(sizeof(float) * c.size());
+    ds.alloc(sizeof(float) * s.size());
+    hip_check(hipMemcpy(dc.ptr, c.data(), sizeof(float) * c.size(), hipMemcpyHostToDevice), "hipMemcpy");
+    hip_check(hipMemcpy(ds.ptr, s.data(), sizeof(float) * s.size(), hipMemcpyHostToDevice), "hipMemcpy");
+}
+
 // Filters host rows on the device (phasor_fir_kernel) into host complex output [rows][T].
 void phasor_filter_host(const float* in, uint64_t rows, uint32_t T, const std::vector<float>& c,
                         const std::vector<float>& s, float* out) {
     if (rows == 0 || T == 0) return;
-    if (c.empty() || c.size() > 4096 || c.size() != s.size())
-        fail(BPX_ERR_UNSUPPORTED, "phasor pulses of 1..4096 taps are supported");
     DeviceBuffer din, dout, dc, ds;
+    upload_taps(c, s, dc, ds);
+    if (rows * T >= (1ull << 38)) fail(BPX_ERR_UNSUPPORTED, "phasor input too large");
     din.alloc(sizeof(float) * rows * T);
     dout.alloc(sizeof(float) * 2 * rows * T);
-    dc.alloc(sizeof(float) * c.size());
-    ds.alloc(sizeof(float) * s.size());
     hip_check(hipMemcpy(din.ptr, in, sizeof(float) * rows * T, hipMemcpyHostToDevice), "hipMemcpy");
-    hip_check(hipMemcpy(dc.ptr, c.data(), sizeof(float) * c.size(), hipMemcpyHostToDevice), "hipMemcpy");
-    hip_check(hipMemcpy(ds.ptr, s.data(), sizeof(float) * s.size(), hipMemcpyHostToDevice), "hipMemcpy");
-    // write unpadded rows: destination pitch T with the FIR output at [1, T] shifted by one,
-    // so stage into a padded buffer then compact on the host copy
-    DeviceBuffer dpad;
-    const uint32_t ldh = T + 2;
-    dpad.alloc(sizeof(float) * 2 * rows * ldh);
-    if (rows * ldh >= (1ull << 32)) fail(BPX_ERR_UNSUPPORTED, "phasor input too large");
     hip_check(launch_phasor_fir(static_cast<const float*>(din.ptr), (uint32_t)rows, T,
                                 static_cast<const float*>(dc.ptr), static_cast<const float*>(ds.ptr),
-                                (int)c.size(), dpad.ptr, ldh, kC64, 1.0f, nullptr),
+                                (int)c.size(), dout.ptr, T, 0u, kC64, 1.0f, nullptr),
               "phasor FIR kernel");
-    hip_check(hipMemcpy2D(out, sizeof(float) * 2 * T, static_cast<const char*>(dpad.ptr) + 8,
-                          sizeof(float) * 2 * ldh, sizeof(float) * 2 * T, rows, hipMemcpyDeviceToHost),
-              "hipMemcpy2D(phasor)");
+    hip_check(hipMemcpy(out, dout.ptr, sizeof(float) * 2 * rows * T, hipMemcpyDeviceToHost), "hipMemcpy(phasor)");
+}
+
+// The phasor-filtered chunk on the device: the source rows in one copy, the chunk window cropped
+// (get_transient_chunk), then the FIR into [rows][Tc] complex rows -- no host chunk and no
+// host round trip of the complex samples.
+DeviceBuffer phasor_rows_device(const float* src, uint64_t rows, uint32_t src_T, const pipeline::ChunkWindow& w,
+                                const std::vector<float>& c, const std::vector<float>& s) {
+    DeviceBuffer full, chunk, out, dc, ds;
+    upload_taps(c, s, dc, ds);
+    if (rows >= (1ull << 32) || rows * w.Tc >= (1ull << 38)) fail(BPX_ERR_UNSUPPORTED, "phasor input too large");
+    out.alloc(sizeof(float) * 2 * std::max<uint64_t>(rows * w.Tc, 1));
+    if (rows == 0) return out;
+    full.alloc(sizeof(float) * rows * src_T);
+    chunk.alloc(sizeof(float) * rows * w.Tc);
+    hip_check(hipMemcpy(full.ptr, src, sizeof(float) * rows * src_T, hipMemcpyHostToDevice), "hipMemcpy(transient)");
+    hip_check(launch_crop_rows(static_cast<const float*>(full.ptr), src_T, nullptr, (uint32_t)rows, w.Tc, w.from,
+                               w.cfrom, w.ncopy, static_cast<float*>(chunk.ptr), nullptr),
+              "crop kernel");
+    hip_check(launch_phasor_fir(static_cast<const float*>(chunk.ptr), (uint32_t)rows, w.Tc,
+                                static_cast<const float*>(dc.ptr), static_cast<const float*>(ds.ptr),
+                                (int)c.size(), out.ptr, w.Tc, 0u, kC64, 1.0f, nullptr),
+              "phasor FIR kernel");
+    hip_check(hipStreamSynchronize(nullptr), "hipStreamSynchronize");
+    return out;
 }
 
 }  // namespace bpx
@@ -512,20 +534,25 @@ bpx_status bpx_nlos_phasor(const float* transient_data, const int64_t* data_shap
                                      volume_position, volume_size, voxels_per_side, assume_row_major);
         // :962-969 -- the float limits are truncated to int
         const float imin = (float)(int)g.min_idx, imax = (float)(int)g.max_idx;
-        uint32_t Tc = 0;
-        std::vector<float> chunk = pipeline::transient_chunk(g.arr.data_p, g.rows, g.T, imin, imax, &Tc);
+        const pipeline::ChunkWindow w = pipeline::chunk_window(g.T, imin, imax);
         std::vector<float> cs, sn;
         if (pipeline::phasor_taps(wavelength, deltaT, 4, 0.0f, cs, sn) <= 0)
             fail(BPX_ERR_INVALID_ARGUMENT, "the phasor pulse has no taps for this wavelength");
         if (!device_ready()) fail(BPX_ERR_NO_DEVICE, "no gfx950 device available");
-        std::vector<float> cchunk(2 * chunk.size());
-        phasor_filter_host(chunk.data(), g.rows, Tc, cs, sn, cchunk.data());
+        if (g.rows != g.pairs.size())
+            fail(BPX_ERR_INVALID_ARGUMENT, "transient_size (" + std::to_string(g.rows * w.Tc) +
+                                               ") != num_pairs * T (" +
+                                               std::to_string((uint64_t)g.pairs.size() * w.Tc) + ")");
+        DeviceBuffer rows = phasor_rows_device(g.arr.data_p, g.rows, g.T, w, cs, sn);
         const float chunked_t0 = t0 + imin * deltaT;  // :1011
+        CaptureIn c{nullptr, g.rows * w.Tc, w.Tc, g.pairs.data(), (uint32_t)g.pairs.size(), chunked_t0, deltaT,
+                    true};
+        c.device_rows = &rows;
         if (use_cpu) {
-            return bpx_window_backprojection(cchunk.data(), chunk.size(), Tc, g.pairs.data(),
-                                             (uint32_t)g.pairs.size(), complex_volume_out,
-                                             voxels_per_side, volume_position, volume_size,
-                                             chunked_t0, deltaT, 1);
+            if (!volume_position || !volume_size) fail(BPX_ERR_INVALID_ARGUMENT, "NULL volume argument");
+            run_plan_to_host(make_plan_window(c, voxels_per_side, volume_position, volume_size),
+                             complex_volume_out);
+            return BPX_OK;
         }
         for (int k = 0; k < 3; ++k)
             if (voxels_per_side[k] < 2)
@@ -534,9 +561,9 @@ bpx_status bpx_nlos_phasor(const float* transient_data, const int64_t* data_shap
         const float zeros3[3] = {0, 0, 0};
         float zero[3], inc[9];
         pipeline::basis(volume_position, volume_size, voxels_per_side, zeros3, zero, inc);
-        return bpx_complex_backprojection(cchunk.data(), chunk.size(), Tc, g.pairs.data(),
-                                          (uint32_t)g.pairs.size(), complex_volume_out,
-                                          voxels_per_side, zero, inc, chunked_t0, deltaT);
+        run_plan_to_host(make_plan_single_tap(c, voxels_per_side, zero, inc, nullptr),
+                         complex_volume_out);
+        return BPX_OK;
     } catch (const Failure& e) {
         return set_error(e.status, e.msg);
     } catch (const std::bad_alloc&) {

@@ -89,6 +89,15 @@ static float chunk_max_abs(const CaptureIn& c) {
     if (c.chunk) {
         const uint64_t n = c.size * (c.cplx ? 2 : 1);
         for (uint64_t i = 0; i < n; ++i) m = std::max(m, std::fabs(c.chunk[i]));
+    } else if (c.device_rows) {
+        const uint64_t n = c.size * (c.cplx ? 2 : 1);
+        DeviceBuffer r;
+        r.alloc(sizeof(float));
+        hip_check(hipMemset(r.ptr, 0, sizeof(float)), "hipMemset");
+        hip_check(launch_abs_max(static_cast<const float*>(c.device_rows->ptr), n, static_cast<float*>(r.ptr),
+                                 nullptr),
+                  "abs max kernel");
+        hip_check(hipMemcpy(&m, r.ptr, sizeof(float), hipMemcpyDeviceToHost), "hipMemcpy");
     } else if (c.src) {
         for (uint32_t p = 0; p < c.P; ++p) {
             const float* r = c.src + (size_t)p * c.src_T + c.from;
@@ -107,7 +116,9 @@ static float choose_scale(const CaptureIn& c) {
 }
 
 static void validate_common(const CaptureIn& c, const uint32_t n[3]) {
-    if (!c.chunk && !c.src && c.size) fail(BPX_ERR_INVALID_ARGUMENT, "transient_chunk is NULL");
+    if (!c.chunk && !c.src && !c.device_rows && c.size) fail(BPX_ERR_INVALID_ARGUMENT, "transient_chunk is NULL");
+    if (c.device_rows && c.device_rows->bytes < sizeof(float) * c.size * (c.cplx ? 2 : 1))
+        fail(BPX_ERR_INVALID_ARGUMENT, "device rows smaller than the chunk");
     if (!c.chunk && c.src) {
         if (c.cplx) fail(BPX_ERR_INVALID_ARGUMENT, "device cropping takes real samples");
         if ((uint64_t)c.from + c.ncopy > c.src_T || (uint64_t)c.cfrom + c.ncopy > c.T)
@@ -148,6 +159,26 @@ static std::vector<uint32_t> used_pairs(const CaptureIn& c, uint32_t flags) {
 static void stage_raw(PlanImpl& pl, const CaptureIn& c, const std::vector<uint32_t>& idx) {
     const size_t comp = c.cplx ? 2 : 1;
     const size_t row = (size_t)c.T * comp;
+    if (!c.chunk && c.device_rows) {
+        if (idx.size() == c.P && c.P) {
+            pl.raw.release();
+            std::swap(pl.raw.ptr, c.device_rows->ptr);
+            std::swap(pl.raw.bytes, c.device_rows->bytes);
+            return;
+        }
+        pl.raw.alloc(sizeof(float) * row * std::max<size_t>(idx.size(), 1));
+        if (idx.empty()) return;
+        DeviceBuffer map;
+        map.alloc(sizeof(uint32_t) * idx.size());
+        hip_check(hipMemcpy(map.ptr, idx.data(), sizeof(uint32_t) * idx.size(), hipMemcpyHostToDevice),
+                  "hipMemcpy(row map)");
+        hip_check(launch_crop_rows(static_cast<const float*>(c.device_rows->ptr), (uint32_t)row,
+                                   static_cast<const uint32_t*>(map.ptr), (uint32_t)idx.size(), (uint32_t)row, 0u, 0u,
+                                   (uint32_t)row, static_cast<float*>(pl.raw.ptr), pl.stream),
+                  "row gather kernel");
+        hip_check(hipStreamSynchronize(pl.stream), "hipStreamSynchronize");
+        return;
+    }
     pl.raw.alloc(sizeof(float) * row * std::max<size_t>(idx.size(), 1));
     if (!c.chunk) {
         if (idx.empty()) return;

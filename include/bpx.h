@@ -262,10 +262,17 @@ bpx_status bpx_plan_read_volume(bpx_plan plan, float* voxel_volume);
 bpx_status bpx_plan_info(bpx_plan plan, int32_t* path, int32_t* lane_voxels, uint32_t* pairs_used,
                          uint32_t* Tc, float* scale);
 
+/* Copies the threshold table of a confocal plan's last execution to host: out[p][j],
+ * p < pairs_used, j <= Tc, = the smallest squared wall-point-to-voxel distance whose single-tap
+ * bin round((((lw + d) + d) + cw - t0) / deltaT), d = sqrtf(s), is >= j (backproject_cuda.cu:
+ * 39-45, 155 for a confocal pair).  The gather kernels pick a voxel's bin with it; exposed so
+ * that the table can be checked on its own.  `capacity` = floats available at `out`. */
+bpx_status bpx_plan_read_thresholds(bpx_plan plan, float* out, uint64_t capacity);
+
 typedef struct bpx_timing {
     double total_ms;        /* host wall time of all timed executions                    */
     double per_exec_ms;     /* total_ms / iterations                                      */
-    double table_kernel_ms; /* mean device time of the threshold-table kernel per execution */
+    double table_kernel_ms; /* mean device time of the staging / threshold-table kernels    */
     double gather_kernel_ms;/* mean device time of the gather kernel per execution         */
     int32_t iterations;
 } bpx_timing;

@@ -68,16 +68,58 @@ static inline float a1_distance(const float *p1, const float *p2)
 
 /* backproject_cuda.cu:31-46 compute_distance then :155 (footprint 0, so the +-fp/2 term is
  * a subtraction of +0.0f, which changes no value).  Returns round((total-t0)/deltaT). */
+static inline float a1_bin_of_distances(float lw, float cw, float d1, float d2, float t0, float dT)
+{
+    float total = lw + d1;                                 /* :45 left to right          */
+    total = total + d2;
+    total = total + cw;
+    const float y = (total - t0) - 0.0f / 2.0f;            /* :155 voxel_footprint = 0   */
+    const float q = y / dT;
+    return roundf(q);
+}
+
 float or_a1_rounded_bin(const or_pair *pr, const float v[3], float t0, float dT)
 {
     const float d1 = a1_distance(pr->laser_point, v);      /* :39 laser point -> voxel  */
     const float d2 = a1_distance(v, pr->camera_point);     /* :43 voxel -> camera point */
-    float total = pr->laser_wall + d1;                     /* :45 left to right          */
-    total = total + d2;
-    total = total + pr->camera_wall;
-    const float y = (total - t0) - 0.0f / 2.0f;            /* :155 voxel_footprint = 0   */
-    const float q = y / dT;
-    return roundf(q);
+    return a1_bin_of_distances(pr->laser_wall, pr->camera_wall, d1, d2, t0, dT);
+}
+
+/* The same bin for a confocal pair given the squared distance s of its (single) wall point to
+ * the voxel: d1 == d2 == sqrtf(s), since (a*a + b*b) + c*c is the same for v - w and w - v. */
+float or_a1_confocal_bin(float lw, float cw, float s, float t0, float dT)
+{
+    const float d = sqrtf(s);
+    return a1_bin_of_distances(lw, cw, d, d, t0, dT);
+}
+
+/* Threshold table of a confocal pair list: out[p][j] (j < n) = the smallest float s >= +0 with
+ * or_a1_confocal_bin(lw_p, cw_p, s) >= j, found by plain bisection over the bit patterns of
+ * the non-negative floats (ordered like their values; the bin never decreases in s).  +inf
+ * satisfies every j (its bin is +inf). */
+void or_confocal_thresholds(const float *lw, const float *cw, uint32_t P, uint32_t n, float t0, float dT,
+                            float *out)
+{
+#pragma omp parallel for schedule(static)
+    for (int64_t pj = 0; pj < (int64_t)P * n; ++pj) {
+        const uint32_t p = (uint32_t)(pj / n), j = (uint32_t)(pj % n);
+        const float jf = (float)j;
+        uint32_t lo = 0u, hi = 0x7f800000u;   /* +0 .. +inf */
+        float s;
+        memcpy(&s, &lo, 4);
+        if (or_a1_confocal_bin(lw[p], cw[p], s, t0, dT) >= jf) {
+            out[pj] = s;
+            continue;
+        }
+        while (hi - lo > 1u) {           /* invariant: bin(lo) < j <= bin(hi) */
+            const uint32_t mid = lo + (hi - lo) / 2u;
+            memcpy(&s, &mid, 4);
+            if (or_a1_confocal_bin(lw[p], cw[p], s, t0, dT) >= jf) hi = mid;
+            else lo = mid;
+        }
+        memcpy(&s, &hi, 4);
+        out[pj] = s;
+    }
 }
 
 /* The contributing bin index, or -1 when the single tap falls outside [0, T-1]

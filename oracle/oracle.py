@@ -84,6 +84,23 @@ def a1_bins(pairs, voxel_xyz, pair_idx, zero, inc, t0, dT, T):
     return bins, rnd
 
 
+def a1_confocal_bin(lw, cw, s, t0, dT):
+    """or_a1_confocal_bin: rounded bin of a confocal pair at squared distance s."""
+    L = lib()
+    L.or_a1_confocal_bin.restype = ctypes.c_float
+    return float(L.or_a1_confocal_bin(ctypes.c_float(lw), ctypes.c_float(cw), ctypes.c_float(s),
+                                      ctypes.c_float(t0), ctypes.c_float(dT)))
+
+
+def confocal_thresholds(lw, cw, n, t0, dT):
+    """or_confocal_thresholds: [P][n] smallest squared distance whose bin is >= j."""
+    lw, cw = _f32(lw).reshape(-1), _f32(cw).reshape(-1)
+    out = np.zeros((len(lw), int(n)), np.float32)
+    lib().or_confocal_thresholds(_p(lw), _p(cw), ctypes.c_uint32(len(lw)), ctypes.c_uint32(n),
+                                 ctypes.c_float(t0), ctypes.c_float(dT), _p(out))
+    return out
+
+
 def a1_volume(H, pairs, N, zero, inc, t0, dT, block=0):
     """or_a1_volume_f32: reference GPU-path volume in the reference summation order."""
     H = _f32(H)

@@ -85,6 +85,28 @@ def test_confocal_bins_bit_exact_all_variants_and_non_separable_basis(gpu, oracl
         assert np.array_equal(vol2, ref2.astype(f32)), f"variant {v}"
 
 
+@pytest.mark.parametrize("variant,storage,cplx", [(-1, "f32", False), (0, "f32", False), (2, "f32", False),
+                                                  (5, "f16", False), (5, "bf16", False), (5, "f32", True)])
+def test_threshold_table_bit_exact(gpu, oracle, variant, storage, cplx):
+    """Every (pair, bin) threshold of the staged table equals the oracle's bisection over all
+    non-negative floats of the reference bin formula (fused and split layouts, all formats)."""
+    rng = np.random.default_rng(17)
+    pr = confocal_pairs(24, jitter=0.01, seed=3)
+    pr["camera_wall"] = rng.uniform(0.6, 2.2, len(pr)).astype(f32)
+    pr["laser_wall"] = rng.uniform(0.6, 2.2, len(pr)).astype(f32)
+    T = 1500
+    H = coded_samples(len(pr), T).astype(np.complex64) * (1 + 1j) if cplx else coded_samples(len(pr), T)
+    zero, inc = grid_basis((8, 8, 8))
+    for t0, dT in [(2.0, 0.001), (1.1, 0.0013), (0.0, 2.0 ** -9)]:
+        with gpu.Plan(H, pr, (8, 8, 8), zero, inc, t0, dT, storage=storage, variant=variant) as plan:
+            plan.execute()
+            got = plan.thresholds()
+        ref = oracle.confocal_thresholds(pr["laser_wall"], pr["camera_wall"], T + 1, t0, dT)
+        assert got.shape == ref.shape
+        bad = np.flatnonzero(got.view(np.uint32) != ref.view(np.uint32))
+        assert bad.size == 0, (t0, dT, bad[:5], got.ravel()[bad[:5]], ref.ravel()[bad[:5]])
+
+
 def test_round_half_away_ties_on_gpu(gpu, oracle):
     # total = 4 exactly (walls 1.25, voxel 0.75 above the wall point); t0 places q at k + 0.5
     pr = np.zeros(1, gpu.PAIR_DTYPE)
@@ -279,6 +301,40 @@ def test_nlosbpy_device_crop_equals_host_chunk(gpu, t0, T, nan_pair, confocal):
     rc = gpu.window_backprojection(pre["chunk"], pre["pairs"], N, cap.volume_position, cap.volume_size,
                                    pre["chunked_t0"], cap.deltaT)
     assert np.array_equal(vc, rc) and np.abs(rc).max() > 0
+
+
+@pytest.mark.parametrize("t0,nan_pair,use_cpu", [(2.0, False, False), (4.0, True, False), (4.0, True, True)])
+def test_nlosbpy_phasor_device_rows_equal_host_route(gpu, t0, nan_pair, use_cpu):
+    """nlosbpy.phasor crops and filters on the device and hands the complex rows to the plan;
+    the volume must equal the host route: chunk with the int-truncated limits
+    (backproject.hpp:962-969), phasor_pulse, then the complex backprojection."""
+    cap = scenes.render(scenes.config_scene(1), 12, 2048, t0=t0, seed=13)
+    N = (12, 12, 12)
+    args = list(cap.nlos_args(N))
+    if nan_pair:
+        args[1] = args[1].copy()
+        args[1][2, 7, 1] = np.nan
+    wl = 0.005
+    pre = gpu.nlosbpy.prepare(*args, np.zeros(3), assume_row_major=True)
+    imin, imax = f32(int(pre["min_index"])), f32(int(pre["max_index"]))
+    lo, hi = int(np.floor(imin)), int(np.ceil(imax))
+    H = cap.transient.reshape(-1, cap.transient.shape[-1])
+    T = H.shape[1]
+    src_from, src_to, c_from = max(lo, 0), min(max(hi, 0), T), (-lo if lo < 0 else 0)
+    chunk = np.zeros((H.shape[0], hi - lo), f32)
+    if src_to > src_from and c_from + src_to - src_from <= hi - lo:
+        chunk[:, c_from:c_from + src_to - src_from] = H[:, src_from:src_to]
+    cplx = gpu.phasor_pulse(chunk, wl, cap.deltaT)
+    ct0 = f32(cap.t0) + imin * f32(cap.deltaT)
+    if use_cpu:
+        ref = gpu.window_backprojection(cplx, pre["pairs"], N, cap.volume_position, cap.volume_size, ct0,
+                                        cap.deltaT)
+    else:
+        ref = gpu.backprojection(cplx, pre["pairs"], N, pre["zero"], pre["inc"], ct0, cap.deltaT)
+    vol = gpu.nlosbpy.phasor(*args, wl, assume_row_major=True, use_cpu=use_cpu)
+    assert (lo < 0) == (t0 == 4.0)
+    assert vol.dtype == np.complex64 and np.abs(ref).max() > 0
+    assert np.array_equal(vol, ref)
 
 
 def test_nlosbpy_column_major_equals_row_major(gpu):

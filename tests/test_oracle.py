@@ -67,6 +67,55 @@ def test_a1_round_half_away_from_zero(oracle):
     assert rnd[0] == -1.0 and bins[0] == -1
 
 
+def test_confocal_bin_of_squared_distance_matches_voxel_bin(oracle):
+    # a confocal pair's bin from the squared distance s equals the voxel formula's bin
+    rng = np.random.default_rng(3)
+    for _ in range(200):
+        w = rng.uniform(-1, 1, 3).astype(f32)
+        w[2] = 0
+        lw, cw = f32(rng.uniform(0.5, 2)), f32(rng.uniform(0.5, 2))
+        pr = _pair(oracle, w, w, lw, cw)
+        zero = rng.uniform(-0.5, 0.5, 3).astype(f32)
+        zero[2] = f32(rng.uniform(0.3, 1.3))
+        inc = (np.eye(3) * 0.01).astype(f32).ravel()
+        t0, dT = float(rng.uniform(0, 2)), float(rng.uniform(0.0007, 0.003))
+        _, rnd = oracle.a1_bins(pr, [[3, 5, 7]], [0], zero, inc, t0, dT, 1 << 20)
+        v = oracle.a1_voxel_position(zero, inc, 3, 5, 7)
+        a, b, c = w - v
+        s = (a * a + b * b) + c * c
+        assert oracle.a1_confocal_bin(lw, cw, s, t0, dT) == rnd[0]
+
+
+def test_confocal_thresholds_are_the_bin_switch_points(oracle):
+    # S[p][j] is the smallest s with bin >= j: bin(S) >= j and bin(previous float) < j
+    rng = np.random.default_rng(5)
+    lw = rng.uniform(0.8, 1.6, 6).astype(f32)
+    cw = rng.uniform(0.8, 1.6, 6).astype(f32)
+    t0, dT, n = 2.25, 0.0011, 1200
+    S = oracle.confocal_thresholds(lw, cw, n, t0, dT)
+    assert np.all(np.diff(S, axis=1) >= 0)
+    for p in range(len(lw)):
+        for j in range(0, n, 7):
+            s = S[p, j]
+            assert oracle.a1_confocal_bin(lw[p], cw[p], s, t0, dT) >= j
+            if s > 0:
+                below = np.nextafter(s, f32(0))
+                assert oracle.a1_confocal_bin(lw[p], cw[p], below, t0, dT) < j
+    # bins below the shortest path start at s = 0; the bin never decreases in s
+    first = oracle.a1_confocal_bin(lw[0], cw[0], 0.0, t0, dT)
+    assert first > 0 and np.all(S[0, :int(first) + 1] == 0)
+    ss = np.sort(rng.uniform(0, 4, 400).astype(f32))
+    b = [oracle.a1_confocal_bin(lw[1], cw[1], x, t0, dT) for x in ss]
+    assert np.all(np.diff(b) >= 0)
+    # a known answer: total = 2 + 2 sqrt(s) with dT = 2^-8 and t0 = 0 reaches bin j >= 513 at
+    # 2 sqrt(s) = (j - 0.5) / 256 - 2 exactly (ties round away, every step exact)
+    S2 = oracle.confocal_thresholds([1.0], [1.0], 600, 0.0, 2.0 ** -8)
+    d = ((560 - 0.5) / 256 - 2) / 2
+    assert oracle.a1_confocal_bin(1.0, 1.0, f32(d) * f32(d), 0.0, 2.0 ** -8) == 560
+    assert S2[0, 560] <= f32(d) * f32(d)
+    assert oracle.a1_confocal_bin(1.0, 1.0, np.nextafter(S2[0, 560], f32(0)), 0.0, 2.0 ** -8) == 559
+
+
 def test_a1_voxel_position_order(oracle):
     # v[k] = ((zero + inc[k] x) + inc[3+k] y) + inc[6+k] z: rows of inc are index steps
     inc = np.array([1, 10, 100, 2, 20, 200, 3, 30, 300], f32) * f32(0.5)
