@@ -205,10 +205,10 @@ def load(path=WEIGHTS):
     return core.Net(0, params=params), meta
 
 
-def select_and_test(net, meta, n_val=10, n_test=20, voxels=256):
+def select_and_test(net, meta, n_val=10, n_test=20, voxels=256, configs=CANDIDATES):
     """Chooses the proposal configuration on the validation captures, then reports it (and
     the bound rule alone) on the held-out test captures."""
-    summary, _ = validate(net, n_val, voxels)
+    summary, _ = validate(net, n_val, voxels, configs)
     meta["validation"] = summary
     pick = pick_threshold(summary)
     meta["proposal_threshold"] = pick
@@ -245,6 +245,8 @@ def main(argv=None):
     v.add_argument("--test", type=int, default=20)
     v.add_argument("--voxels", type=int, default=256)
     v.add_argument("--out", default=None, help="write weights + updated metadata here")
+    v.add_argument("--candidates", default=None,
+                   help="JSON list of proposal thresholds (a number, or one per level) to choose from")
     a = ap.parse_args(argv)
     if a.cmd == "train":
         t0 = time.perf_counter()
@@ -267,7 +269,8 @@ def main(argv=None):
         print(json.dumps({k: meta[k] for k in meta if k != "history"}), flush=True)
     else:
         net, meta = load(a.weights)
-        select_and_test(net, meta, a.n, a.test, a.voxels)
+        configs = json.loads(a.candidates) if a.candidates else CANDIDATES
+        select_and_test(net, meta, a.n, a.test, a.voxels, configs)
         if a.out:
             save(net, a.out, meta)
         print(json.dumps({k: meta[k] for k in meta if k != "history"}), flush=True)
