@@ -47,6 +47,8 @@ def parse():
     ap.add_argument("--octree-tau", type=float, default=0.25)
     ap.add_argument("--octree-proposal", type=float, default=None,
                     help="proposal threshold of the learned octree (default: the shipped choice)")
+    ap.add_argument("--configs", type=int, default=1,
+                    help="also time the exhaustive (C2) and complex / 16-bit (C3) workloads (0 = skip)")
     ap.add_argument("--one-shot", type=int, default=3,
                     help="timed one-shot nlosbpy.backproject calls (0 = skip)")
     ap.add_argument("--batch", type=int, default=64,
@@ -118,6 +120,45 @@ def batch_comparison(bp, count, seed=1004, serial=True):
                 ms += pl.benchmark(1, 2)["per_exec_ms"]
         out["one_launch_per_capture_ms"] = ms
         out["speedup_vs_one_launch_per_capture"] = ms / t["per_exec_ms"]
+    return out
+
+
+def _plan_line(pl, N, sample_bytes, workload, reps):
+    t = pl.benchmark(1, reps)
+    info = pl.info()
+    upd = info["pairs_used"] * N[0] * N[1] * N[2]
+    g = t["gather_kernel_ms"]
+    return {"workload": workload, "kernel_path": info["path"], "lane_voxels": info["lane_voxels"],
+            "updates": upd, "exec_ms": t["per_exec_ms"], "gather_ms": g, "staging_ms": t["table_kernel_ms"],
+            "updates_per_s": upd / (t["per_exec_ms"] * 1e-3), "gather_updates_per_s": upd / (g * 1e-3),
+            "bytes_per_update": sample_bytes,
+            "gather_frac_hbm": upd * sample_bytes / (g * 1e-3) / 1e9 / HBM_PEAK_GBS}
+
+
+def config_measurements(bp, cap, reps=5):
+    """The other workloads of the scope table, one plan each (device-resident, timed with HIP
+    events): C2 exhaustive pairs with exact bins; C3 complex (phasor-filtered) and 16-bit
+    storage at 128^3 from the headline capture."""
+    from backprojector_amd import scenes
+    out = {}
+    N2 = (64, 64, 64)
+    c2 = scenes.render(scenes.config_scene(2), 16, 4096, confocal=False, laser_side=16, seed=1002)
+    with bp.Plan.from_nlos(*c2.nlos_args(N2), wall_normal=(0, 0, 0), assume_row_major=True) as pl:
+        out["C2"] = _plan_line(pl, N2, 4, "exhaustive 16x16 lasers x 16x16 SPADs -> 64^3, T=4096, f32", reps)
+    del c2
+    N = (128, 128, 128)
+    for st, b in (("f16", 2), ("bf16", 2)):
+        with bp.Plan.from_nlos(*cap.nlos_args(N), wall_normal=(0, 0, 0), assume_row_major=True,
+                               storage=st) as pl:
+            out[f"C3_real_{st}"] = _plan_line(pl, N, b, f"confocal 128x128 -> 128^3, T=4096, {st} storage", reps)
+    pre = bp.nlosbpy.prepare(*cap.nlos_args(N), (0, 0, 0), assume_row_major=True)
+    cplx = bp.phasor_pulse(pre["chunk"], 0.004, cap.deltaT)
+    for st, b in (("f32", 8), ("f16", 4), ("bf16", 4)):
+        with bp.Plan(cplx, pre["pairs"], N, pre["zero"], pre["inc"], pre["chunked_t0"], cap.deltaT,
+                     storage=st) as pl:
+            name = {"f32": "complex64", "f16": "complex f16", "bf16": "complex bf16"}[st]
+            out[f"C3_{name.replace(' ', '_')}"] = _plan_line(
+                pl, N, b, f"confocal 128x128 -> 128^3, T=4096, phasor-filtered (4 mm), {name} storage", reps)
     return out
 
 
@@ -223,6 +264,12 @@ def main():
                          f"{dt:.1f} s on {nth} threads"}
         log(f"cpu baseline: {cpu}")
 
+    configs = None
+    if args.configs:
+        configs = config_measurements(bp, cap)
+        log("configs: " + "; ".join(f"{k} {v['gather_ms']:.2f} ms {v['gather_updates_per_s']:.3g}/s "
+                                    f"frac {v['gather_frac_hbm']:.2f}" for k, v in configs.items()))
+
     plan.close()
     octree = None
     if args.octree > 0:
@@ -270,6 +317,7 @@ def main():
         "cpu_baseline": cpu,
         "check": check,
         "one_shot": one_shot,
+        "configs": configs,
         "octree": octree,
         "batch": batch,
         "baseline_ref": "Quadro P5000 5993 ms (README.md:106) -> 5.73e9 updates/s",
