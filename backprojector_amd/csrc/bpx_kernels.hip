@@ -733,22 +733,35 @@ __global__ __launch_bounds__(256) void grid_kernel(GridArgs a) {
 #pragma unroll
             for (int r = 0; r < K; ++r)
                 d1[r] = exact_sqrt(sq3(lp.x - v[r][0], lp.y - v[r][1], lp.z - v[r][2]));
+            // all CS x K gathers of this laser are issued before any is added (a partial last
+            // chunk reads its last SPAD's row for the missing ones and drops those values)
+            acc_t val[CS][K];
 #pragma unroll
             for (int cs = 0; cs < CS; ++cs) {
-                if ((uint32_t)cs < ncs) {
-                    const uint32_t p = l * a.C + c0 + cs;
-                    const uint32_t soff_h = p * a.ldh * (uint32_t)Smp::kBytes;
+                const uint32_t p = l * a.C + min(c0 + cs, a.C - 1u);
+                const uint32_t soff_h = p * a.ldh * (uint32_t)Smp::kBytes;
 #pragma unroll
-                    for (int r = 0; r < K; ++r) {
-                        float total = lp.w + d1[r];
-                        total = total + d2[cs][r];
-                        total = total + cw[cs];
-                        const float q1 = __builtin_fmaf(total, a.inv, a.c1);
-                        const uint32_t j = (uint32_t)(int)__builtin_amdgcn_fmed3f(q1, 0.0f, fTc);
-                        const uint32_t hj = j + (total >= zs[j] ? 1u : 0u);
-                        acc_add(blk[r], Smp::load(hr, hj, soff_h));
-                    }
+                for (int r = 0; r < K; ++r) {
+                    float total = lp.w + d1[r];
+                    total = total + d2[cs][r];
+                    total = total + cw[cs];
+                    const float q1 = __builtin_fmaf(total, a.inv, a.c1);
+                    const uint32_t j = (uint32_t)(int)__builtin_amdgcn_fmed3f(q1, 0.0f, fTc);
+                    const uint32_t hj = j + (total >= zs[j] ? 1u : 0u);
+                    val[cs][r] = Smp::load(hr, hj, soff_h);
                 }
+            }
+            if (ncs == (uint32_t)CS) {
+#pragma unroll
+                for (int cs = 0; cs < CS; ++cs)
+#pragma unroll
+                    for (int r = 0; r < K; ++r) acc_add(blk[r], val[cs][r]);
+            } else {
+#pragma unroll
+                for (int cs = 0; cs < CS; ++cs)
+                    if ((uint32_t)cs < ncs)
+#pragma unroll
+                        for (int r = 0; r < K; ++r) acc_add(blk[r], val[cs][r]);
             }
             since += ncs;
             if (since >= a.kahan_block) {
