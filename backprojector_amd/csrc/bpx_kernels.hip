@@ -817,6 +817,8 @@ __global__ __launch_bounds__(256) void pairs_kernel(PairsArgs a) {
             const float lpx = pr[3], lpy = pr[4], lpz = pr[5];
             const float cw = pr[6], lw = pr[7];
             const uint32_t soff_h = p * a.ldh * (uint32_t)Smp::kBytes;
+            // the K gathers are issued before any is added (K loads in flight per lane)
+            acc_t val[K];
 #pragma unroll
             for (int r = 0; r < K; ++r) {
                 const float d1 = exact_sqrt(sq3(lpx - v[r][0], lpy - v[r][1], lpz - v[r][2]));
@@ -827,8 +829,10 @@ __global__ __launch_bounds__(256) void pairs_kernel(PairsArgs a) {
                 const float q1 = __builtin_fmaf(total, a.inv, a.c1);
                 const uint32_t j = (uint32_t)(int)__builtin_amdgcn_fmed3f(q1, 0.0f, fTc);
                 const uint32_t hj = j + (total >= zs[j] ? 1u : 0u);
-                acc_add(blk[r], Smp::load(hr, hj, soff_h));
+                val[r] = Smp::load(hr, hj, soff_h);
             }
+#pragma unroll
+            for (int r = 0; r < K; ++r) acc_add(blk[r], val[r]);
         }
 #pragma unroll
         for (int r = 0; r < K; ++r) {
