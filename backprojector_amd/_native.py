@@ -118,6 +118,7 @@ class BatchTiming(ctypes.Structure):
 
 
 FLAG_REFERENCE_PAIR_TRUNCATION = 0x1
+FLAG_CENTRED_WALL_BASIS = 0x2
 OCTREE_BOUNDS_ONLY = 0x1
 
 _lib = None

@@ -149,14 +149,18 @@ class Plan:
     def from_nlos(cls, transient_data, camera_grid_positions, laser_grid_positions,
                   camera_position, laser_position, t0, deltaT, is_confocal, volume_position,
                   volume_size, voxels_per_side, wall_normal=(0, 0, 0), assume_row_major=True,
-                  storage="f32", kahan_block=0, variant=-1):
-        """Plan of nlosbpy.backproject's GPU path (bp::backproject, backproject.hpp:762-891)."""
+                  storage="f32", kahan_block=0, variant=-1, centred_wall_basis=False):
+        """Plan of nlosbpy.backproject's GPU path (bp::backproject, backproject.hpp:762-891).
+        centred_wall_basis=True builds a non-zero wall normal's volume basis right-handed and
+        centred on volume_position (the reference's basis, backproject.hpp:825-839, points z
+        away from the volume for the normal (0, -1, 0) and is NaN for normals parallel to z)."""
         H = _f32(transient_data)
         cg, lg = _f32(camera_grid_positions), _f32(laser_grid_positions)
         N = _u32(voxels_per_side).reshape(3)
         sh = lambda a: np.ascontiguousarray(np.asarray(a.shape, np.int64))
         ds, cs, ls = sh(H), sh(cg), sh(lg)
-        opt = nat.make_options(storage, kahan_block, 0, variant)
+        opt = nat.make_options(storage, kahan_block, nat.FLAG_CENTRED_WALL_BASIS if centred_wall_basis else 0,
+                               variant)
         self = cls.__new__(cls)
         self.N = tuple(int(x) for x in N)
         self.complex = False

@@ -201,8 +201,9 @@ void time_limits(const float laser_position[3], const float* lgrid, uint32_t Lx,
                  const float volume_size[3], const float volume_position[3], float t0, float dT,
                  float* min_idx, float* max_idx);
 // backproject.hpp:820-839
+// centred: the wall-aligned basis of BPX_FLAG_CENTRED_WALL_BASIS instead of the reference's.
 void basis(const float volume_position[3], const float volume_size[3], const uint32_t n[3],
-           const float wall_normal[3], float zero[3], float inc[9]);
+           const float wall_normal[3], float zero[3], float inc[9], bool centred = false);
 // backproject.hpp:689-740 (row-major): the chunk's length Tc and the window of every row that
 // is copied (source bins [from, from + ncopy) to chunk bins [cfrom, cfrom + ncopy)).
 struct ChunkWindow {

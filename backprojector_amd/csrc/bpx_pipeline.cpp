@@ -187,8 +187,37 @@ void time_limits(const float laser_position[3], const float* lgrid, uint32_t Lx,
     *max_idx = mx / dT + 100.0f;
 }
 
+// Right-handed wall-aligned axes centred on the volume position (BPX_FLAG_CENTRED_WALL_BASIS):
+// y = -normal, x = unit(y x helper) with helper world z (world x when |y . z| > 0.9), z = x x y;
+// axis r advances volume_size[r] / (N[r] - 1) per voxel, and the volume's middle is at
+// volume_position.
+static void centred_basis(const float volume_position[3], const float volume_size[3], const uint32_t n[3],
+                          const float wall_normal[3], float zero[3], float inc[9]) {
+    const double nn = std::sqrt((double)wall_normal[0] * wall_normal[0] + (double)wall_normal[1] * wall_normal[1] +
+                                (double)wall_normal[2] * wall_normal[2]);
+    if (!(nn > 0.0) || !std::isfinite(nn)) fail(BPX_ERR_INVALID_ARGUMENT, "wall_normal must be finite");
+    const double y[3] = {-wall_normal[0] / nn, -wall_normal[1] / nn, -wall_normal[2] / nn};
+    const double h[3] = {std::fabs(y[2]) > 0.9 ? 1.0 : 0.0, 0.0, std::fabs(y[2]) > 0.9 ? 0.0 : 1.0};
+    double x[3] = {y[1] * h[2] - y[2] * h[1], y[2] * h[0] - y[0] * h[2], y[0] * h[1] - y[1] * h[0]};
+    const double xn = std::sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+    for (double& c : x) c /= xn;
+    const double z[3] = {x[1] * y[2] - x[2] * y[1], x[2] * y[0] - x[0] * y[2], x[0] * y[1] - x[1] * y[0]};
+    const double* ax[3] = {x, y, z};
+    for (int k = 0; k < 3; ++k) {
+        double half = 0.0;
+        for (int r = 0; r < 3; ++r) half += ax[r][k] * (double)volume_size[r];
+        zero[k] = (float)((double)volume_position[k] - 0.5 * half);
+    }
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) inc[3 * r + c] = (float)(ax[r][c] * (double)volume_size[r]) / (float)(n[r] - 1u);
+}
+
 void basis(const float volume_position[3], const float volume_size[3], const uint32_t n[3],
-           const float wall_normal[3], float zero[3], float inc[9]) {
+           const float wall_normal[3], float zero[3], float inc[9], bool centred) {
+    if (centred && (wall_normal[0] != 0.0f || wall_normal[1] != 0.0f || wall_normal[2] != 0.0f)) {
+        centred_basis(volume_position, volume_size, n, wall_normal, zero, inc);
+        return;
+    }
     float ax[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
     if (wall_normal[0] != 0.0f || wall_normal[1] != 0.0f || wall_normal[2] != 0.0f) {
         const float y[3] = {-wall_normal[0], -wall_normal[1], -wall_normal[2]};
@@ -453,7 +482,7 @@ bpx_status bpx_nlos_plan_create(const float* transient_data, const int64_t* data
         const float zeros3[3] = {0, 0, 0};
         float zero[3], inc[9];
         pipeline::basis(volume_position, volume_size, voxels_per_side, wall_normal ? wall_normal : zeros3,
-                        zero, inc);
+                        zero, inc, options && (options->flags & BPX_FLAG_CENTRED_WALL_BASIS));
         auto p = std::make_unique<bpx_plan_s>();
         p->impl.reset(make_plan_single_tap(c, voxels_per_side, zero, inc, options));
         *plan_out = p.release();

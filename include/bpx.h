@@ -77,6 +77,13 @@ typedef struct bpx_options {
 /* Drop the last P mod min(256, P) pairs exactly as the reference launcher does
  * (backproject_cuda.cu:150, 418); off by default (every pair is used). */
 #define BPX_FLAG_REFERENCE_PAIR_TRUNCATION 0x1u
+/* bpx_nlos_plan_create with a non-zero wall normal: build the volume basis right-handed and
+ * centred on volume_position (axis r spaced volume_size[r] / (N[r] - 1); y = -normal, x from
+ * y x world-z, or y x world-x when the normal is within ~25 degrees of z), instead of the
+ * reference's construction (backproject.hpp:825-839), which gives a negative z axis for the
+ * common normal (0, -1, 0) (README.md:116) and NaN axes for a normal parallel to z.  No effect
+ * when the normal is zero; the identity basis results for (0, -1, 0). */
+#define BPX_FLAG_CENTRED_WALL_BASIS 0x2u
 
 /* ------------------------------------------------------------------ diagnostics */
 const char* bpx_last_error(void);   /* thread-local; "" when the last call succeeded */

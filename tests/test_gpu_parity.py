@@ -256,7 +256,64 @@ def test_phasor_filter_bit_exact(gpu, oracle, wl):
     assert np.array_equal(got, want)
 
 
+def test_phasor_filter_explicit_sigma_vs_fp64(gpu):
+    """The explicit-sigma pulse (the 4 mm / sigma 12 mm phasor filter, which the reference's
+    sigma = 4 lambda / 6 cannot express): taps k = 1..2h+1, h = ceil(3 sigma / dT), Gaussian of
+    t = dT (k - pulse/2) times the carrier exp(2 pi i k dT / lambda), applied like the reference
+    filter (out[t] = sum_j in[t + j - pulse/2] tap_j over 0 < idx < T).  Checked against the same
+    sum in float64; no reference output exists for this form (parity unpinned by design)."""
+    rng = np.random.default_rng(9)
+    x = rng.poisson(4.0, (23, 1200)).astype(f32)
+    wl, dT, sigma = 0.004, 0.001, 0.012
+    got = gpu.phasor_pulse(x, wl, dT, sigma=sigma)
+    half = int(np.ceil(3 * sigma / dT))
+    pulse = 2 * half + 1
+    k = np.arange(1, pulse + 1, dtype=np.float64)
+    t = dT * (k - pulse / 2.0)
+    g = np.exp(-t * t / (2 * sigma * sigma))
+    taps = (np.cos(2 * np.pi * k * dT / wl) * g).astype(f32).astype(np.float64) + \
+        1j * (np.sin(2 * np.pi * k * dT / wl) * g).astype(f32).astype(np.float64)
+    T = x.shape[1]
+    want = np.zeros(x.shape, np.complex128)
+    for j in range(pulse):
+        idx = np.arange(T) + j - pulse // 2
+        ok = (idx > 0) & (idx < T)
+        want[:, ok] += x[:, idx[ok]].astype(np.float64) * taps[j]
+    assert rel_l2(got, want) <= 2e-7
+    assert abs(np.abs(got).max() - np.abs(want).max()) <= 1e-5 * np.abs(want).max()
+
+
 # ----------------------------------------------------------------- reference pipeline
+def test_config_c1_literal_and_shifted(gpu, oracle):
+    """The 32x32 confocal, T = 2048, t0 = 0, 32^3 workload as specified.  Its rendered capture is
+    empty: every path through the hidden points is longer than T * deltaT = 2.048 m.  Voxel bins
+    start at 2028 (the voxels nearest the wall), so with non-zero samples on the raw rows only a
+    few voxel-pair bins are in range and all others must be dropped: the GPU volume equals the
+    oracle's exactly.  With t0 = 2.0 m the scene is inside the window and the CPU-path semantics
+    match the oracle."""
+    cap = scenes.render(scenes.config_scene(1), 32, 2048, t0=0.0, seed=1001)
+    assert not cap.transient.any()
+    N = (32, 32, 32)
+    for use_cpu in (False, True):
+        v = gpu.nlosbpy.backproject(*cap.nlos_args(N), np.zeros(3), assume_row_major=True, use_cpu=use_cpu)
+        assert not v.any()
+    pre = gpu.nlosbpy.prepare(*cap.nlos_args(N), np.zeros(3), assume_row_major=True)
+    P = len(pre["pairs"])
+    coded = coded_samples(P, 2048)
+    v0 = gpu.backprojection(coded, pre["pairs"], N, pre["zero"], pre["inc"], 0.0, cap.deltaT)
+    e0 = oracle.a1_volume_f64acc(coded, pre["pairs"], N, pre["zero"], pre["inc"], 0.0, cap.deltaT)
+    assert np.array_equal(v0, e0.astype(f32))
+    assert 0 < np.count_nonzero(e0) < e0.size // 100
+    shifted = scenes.render(scenes.config_scene(1), 32, 2048, t0=2.0, seed=1001)
+    assert shifted.transient.any()
+    ref, _ = oracle.backproject_pipeline(shifted.transient, shifted.camera_grid, shifted.laser_grid,
+                                         shifted.camera_position, shifted.laser_position, shifted.t0,
+                                         shifted.deltaT, True, shifted.volume_position, shifted.volume_size, N,
+                                         use_cpu=True)
+    v = gpu.nlosbpy.backproject(*shifted.nlos_args(N), np.zeros(3), assume_row_major=True, use_cpu=True)
+    assert rel_l2(v, ref) <= 1e-5 and np.abs(ref).max() > 0
+
+
 def test_nlosbpy_backproject_matches_oracle_pipeline(gpu, oracle):
     cap = scenes.render(scenes.config_scene(1), 32, 2048, t0=2.0, seed=1001)
     N = (24, 24, 24)
@@ -335,6 +392,30 @@ def test_nlosbpy_phasor_device_rows_equal_host_route(gpu, t0, nan_pair, use_cpu)
     assert (lo < 0) == (t0 == 4.0)
     assert vol.dtype == np.complex64 and np.abs(ref).max() > 0
     assert np.array_equal(vol, ref)
+
+
+def test_centred_wall_basis(gpu):
+    """BPX_FLAG_CENTRED_WALL_BASIS: the normal (0, -1, 0) gives the identity basis (the same
+    volume as no normal); a normal along -z (NaN axes in the reference construction) and a
+    normal along +x give right-handed bases whose voxels are the identity grid's voxels
+    permuted -- the volumes are the identity volume permuted, bit for bit."""
+    cap = scenes.render(scenes.config_scene(1), 24, 4096, t0=0.0, seed=1)
+    N = (32, 32, 32)
+
+    def vol(normal, centred):
+        with gpu.Plan.from_nlos(*cap.nlos_args(N), wall_normal=normal, assume_row_major=True,
+                                centred_wall_basis=centred) as pl:
+            pl.execute()
+            return pl.volume()
+
+    ident = vol((0, 0, 0), False)
+    assert np.abs(ident).max() > 0
+    assert np.array_equal(vol((0, -1, 0), True), ident)
+    assert np.array_equal(vol((0, 0, 0), True), ident)
+    along_z = vol((0, 0, -1), True)              # axes (0,1,0), (0,0,1), (1,0,0)
+    assert np.all(np.isfinite(along_z)) and np.array_equal(along_z, np.transpose(ident, (1, 2, 0)))
+    along_x = vol((1, 0, 0), True)               # axes (0,1,0), (-1,0,0), (0,0,1)
+    assert np.array_equal(along_x, np.transpose(ident[::-1], (1, 0, 2)))
 
 
 def test_nlosbpy_column_major_equals_row_major(gpu):
