@@ -333,12 +333,13 @@ class Net:
 
 
 class Batch:
-    """Many confocal fp32 captures reconstructed by one persistent work-queue launch replayed
-    from a hipGraph (no reference counterpart; each volume equals backprojection() of that
-    capture alone).  captures: iterable of (transient_chunk, pairs, voxels_per_side,
-    volume_zero_pos, voxel_inc, t0, deltaT)."""
+    """Many confocal captures reconstructed by one persistent work-queue launch replayed from a
+    hipGraph (no reference counterpart; each volume equals backprojection() of that capture
+    alone with the same storage).  captures: iterable of (transient_chunk, pairs,
+    voxels_per_side, volume_zero_pos, voxel_inc, t0, deltaT); storage "f32", "f16" or "bf16"
+    for all of them."""
 
-    def __init__(self, captures, kahan_block=0, reference_pair_truncation=False):
+    def __init__(self, captures, kahan_block=0, reference_pair_truncation=False, storage="f32"):
         self._keep = []
         descs = []
         self.shapes = []
@@ -363,7 +364,7 @@ class Batch:
             self.shapes.append(tuple(int(x) for x in Nv))
         arr = (nat.CaptureDesc * max(len(descs), 1))(*descs)
         flags = nat.FLAG_REFERENCE_PAIR_TRUNCATION if reference_pair_truncation else 0
-        opt = nat.make_options("f32", kahan_block, flags, -1)
+        opt = nat.make_options(storage, kahan_block, flags, -1)
         h = ctypes.c_void_p()
         check(lib().bpx_batch_create(arr, len(descs), ctypes.byref(opt), ctypes.byref(h)))
         self._h = h

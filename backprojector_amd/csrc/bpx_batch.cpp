@@ -26,6 +26,7 @@ struct BatchImpl {
     DeviceBuffer stage_jobs;                      // StageJob per capture with pairs
     uint32_t n_stage = 0, stage_zb = 0, stage_sb = 0;
     uint32_t n_items = 0;
+    int format = kF32;     // storage of every capture (fused rows)
     bpx_batch_config cfg{2u, 0u, 0u};
     uint32_t groups = 0;
     hipGraph_t graph = nullptr;
@@ -111,7 +112,7 @@ void BatchImpl::build(const bpx_batch_config& c) {
     hip_check(hipMemcpy(flags.ptr, fl.data(), 4u * n, hipMemcpyHostToDevice), "hipMemcpy");
     hip_check(hipMemcpy(tiles.ptr, nt.data(), 4u * n, hipMemcpyHostToDevice), "hipMemcpy");
     if (n_items) hip_check(hipMemcpy(items.ptr, it.data(), 8u * n_items, hipMemcpyHostToDevice), "hipMemcpy");
-    const int resident = batch_resident_groups(K);
+    const int resident = batch_resident_groups(K, format);
     if (resident <= 0) fail(BPX_ERR_DEVICE, "cannot query the occupancy of the batch kernel");
     groups = c.groups ? std::min<uint32_t>(c.groups, (uint32_t)resident) : (uint32_t)resident;
     groups = std::max<uint32_t>(1u, std::min<uint32_t>(groups, std::max<uint32_t>(n_items, 1u)));
@@ -144,7 +145,7 @@ void BatchImpl::build(const bpx_batch_config& c) {
             b.next = static_cast<uint32_t*>(next.ptr);
             b.remaining = static_cast<uint32_t*>(remaining.ptr);
             b.done = static_cast<uint64_t*>(stamps.ptr) + 1;
-            hip_check(launch_batch_confocal(b, K, groups, stream), "batch gather kernel");
+            hip_check(launch_batch_confocal(b, K, format, groups, stream), "batch gather kernel");
         }
     } catch (...) {
         hipGraph_t g = nullptr;
@@ -200,9 +201,10 @@ bpx_status bpx_batch_create(const bpx_capture_desc* captures, uint32_t count, co
         if (!device_ready()) fail(BPX_ERR_NO_DEVICE, "no gfx950 device available");
         bpx_options o{BPX_STORAGE_F32, 32u, 0u, -1};
         if (options) o = *options;
-        if (o.storage != BPX_STORAGE_F32) fail(BPX_ERR_UNSUPPORTED, "batches take fp32 storage");
+        if (o.storage > BPX_STORAGE_BF16) fail(BPX_ERR_INVALID_ARGUMENT, "unknown storage format");
         o.variant = 5;  // fused rows: the layout the batch kernel reads
         auto b = std::make_unique<BatchImpl>();
+        b->format = o.storage == BPX_STORAGE_F16 ? kF16 : o.storage == BPX_STORAGE_BF16 ? kBF16 : kF32;
         hip_check(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate");
         for (auto& e : b->ev) hip_check(hipEventCreate(&e), "hipEventCreate");
         int dev = 0, rate = 0;
@@ -213,7 +215,7 @@ bpx_status bpx_batch_create(const bpx_capture_desc* captures, uint32_t count, co
             const bpx_capture_desc& d = captures[i];
             CaptureIn c{d.transient_chunk, d.transient_size, d.T, d.scanned_pairs, d.num_pairs, d.t0, d.deltaT, false};
             std::unique_ptr<PlanImpl> pl(make_plan_single_tap(c, d.voxels_per_side, d.volume_zero_pos, d.voxel_inc, &o));
-            if (pl->P && (pl->path != kPathConfocal || !pl->fused || pl->format != kF32))
+            if (pl->P && (pl->path != kPathConfocal || !pl->fused || pl->format != b->format))
                 fail(BPX_ERR_UNSUPPORTED, "capture " + std::to_string(i) + " is not a confocal capture");
             b->updates.push_back((uint64_t)pl->P * pl->g.n[0] * pl->g.n[1] * pl->g.n[2]);
             b->plans.push_back(std::move(pl));
@@ -259,7 +261,7 @@ bpx_status bpx_batch_tune(bpx_batch batch, int32_t reps, bpx_batch_config* chose
         bpx_batch_config best{2u, 0u, 0u};
         double best_ms = INFINITY;
         for (uint32_t k : {2u, 4u}) {
-            const int resident = batch_resident_groups((int)k);
+            const int resident = batch_resident_groups((int)k, b.format);
             for (uint32_t frac : {1u, 2u})
                 for (uint32_t order : {0u, 1u}) {
                     const bpx_batch_config c{k, resident > 0 ? std::max(1u, (uint32_t)resident / frac) : 0u, order};

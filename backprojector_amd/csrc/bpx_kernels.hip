@@ -620,7 +620,7 @@ __device__ __forceinline__ T wave_uniform_load(const T* p) {
     return out;
 }
 
-template <int K>
+template <int K, class Src>
 __global__ __launch_bounds__(256) void batch_confocal_kernel(BatchArgs b) {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t one = lane == 0u ? 1u : 0u;
@@ -634,9 +634,9 @@ __global__ __launch_bounds__(256) void batch_confocal_kernel(BatchArgs b) {
         const uint32_t q = __builtin_amdgcn_readfirstlane(item.y);  // tile * 4 + quarter
         const ConfocalArgs a = wave_uniform_load(b.caps + c);
         const uint32_t f = __builtin_amdgcn_readfirstlane(b.flags[c]);
-        if (f == 3u) confocal_tile<FusedSource<FusedF32>, K, true, true>(a, q >> 2, q & 3u);
-        else if (f == 1u) confocal_tile<FusedSource<FusedF32>, K, true, false>(a, q >> 2, q & 3u);
-        else confocal_tile<FusedSource<FusedF32>, K, false, false>(a, q >> 2, q & 3u);
+        if (f == 3u) confocal_tile<Src, K, true, true>(a, q >> 2, q & 3u);
+        else if (f == 1u) confocal_tile<Src, K, true, false>(a, q >> 2, q & 3u);
+        else confocal_tile<Src, K, false, false>(a, q >> 2, q & 3u);
         __threadfence();
         const uint32_t left = atomicSub(&b.remaining[c], one);
         const unsigned long long stamp = (one && left == 1u) ? (unsigned long long)wall_clock64() : 0ull;
@@ -666,27 +666,47 @@ hipError_t launch_batch_reset(uint32_t* next, uint32_t* remaining, const uint32_
     return hipGetLastError();
 }
 
-int batch_resident_groups(int K) {
+// The batch kernel of a storage format (every capture of a batch shares it) and K.
+template <int K>
+static const void* batch_kernel_of(int format) {
+    switch (format) {
+        case kF32: return reinterpret_cast<const void*>(batch_confocal_kernel<K, FusedSource<FusedF32>>);
+        case kF16: return reinterpret_cast<const void*>(batch_confocal_kernel<K, FusedSource<FusedH16<false>>>);
+        case kBF16: return reinterpret_cast<const void*>(batch_confocal_kernel<K, FusedSource<FusedH16<true>>>);
+        default: return nullptr;
+    }
+}
+
+static const void* batch_kernel(int K, int format) {
+    return K == 4 ? batch_kernel_of<4>(format) : K == 2 ? batch_kernel_of<2>(format) : nullptr;
+}
+
+int batch_resident_groups(int K, int format) {
+    const void* fn = batch_kernel(K, format);
     int dev = 0, cus = 0, per_cu = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
+    if (!fn || hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
         (void)hipGetLastError();
         return 0;
     }
-    const hipError_t e = K == 4 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_confocal_kernel<4>, 256, 0)
-                                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_confocal_kernel<2>, 256, 0);
-    if (e != hipSuccess) {
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, 0) != hipSuccess) {
         (void)hipGetLastError();
         return 0;
     }
     return cus * per_cu;
 }
 
-hipError_t launch_batch_confocal(const BatchArgs& b, int K, uint32_t groups, hipStream_t s) {
+hipError_t launch_batch_confocal(const BatchArgs& b, int K, int format, uint32_t groups, hipStream_t s) {
     if (groups == 0u) return hipErrorInvalidValue;
-    if (K == 4) hipLaunchKernelGGL(batch_confocal_kernel<4>, dim3(groups), dim3(256), 0, s, b);
-    else if (K == 2) hipLaunchKernelGGL(batch_confocal_kernel<2>, dim3(groups), dim3(256), 0, s, b);
+#define BPX_BATCH_LAUNCH(KK, SRC) hipLaunchKernelGGL((batch_confocal_kernel<KK, SRC>), dim3(groups), dim3(256), 0, s, b)
+    if (K == 4 && format == kF32) BPX_BATCH_LAUNCH(4, FusedSource<FusedF32>);
+    else if (K == 2 && format == kF32) BPX_BATCH_LAUNCH(2, FusedSource<FusedF32>);
+    else if (K == 4 && format == kF16) BPX_BATCH_LAUNCH(4, FusedSource<FusedH16<false>>);
+    else if (K == 2 && format == kF16) BPX_BATCH_LAUNCH(2, FusedSource<FusedH16<false>>);
+    else if (K == 4 && format == kBF16) BPX_BATCH_LAUNCH(4, FusedSource<FusedH16<true>>);
+    else if (K == 2 && format == kBF16) BPX_BATCH_LAUNCH(2, FusedSource<FusedH16<true>>);
     else return hipErrorInvalidValue;
+#undef BPX_BATCH_LAUNCH
     return hipGetLastError();
 }
 

@@ -113,6 +113,11 @@ def batch_comparison(bp, count, seed=1004, serial=True):
            "updates": t["updates"], "batch_ms": t["per_exec_ms"], "updates_per_s": t["updates_per_s"],
            "latency_ms_p50": float(np.percentile(lat, 50)), "latency_ms_p99": float(np.percentile(lat, 99)),
            "config": cfg}
+    for st in ("bf16",):   # 16-bit rows (per-capture scale): half the row memory, same queue
+        with bp.Batch(args, storage=st) as b:
+            c16 = b.tune(reps=2)
+            t16 = b.benchmark(1, 5)
+        out[st] = {"batch_ms": t16["per_exec_ms"], "updates_per_s": t16["updates_per_s"], "config": c16}
     if serial:
         ms = 0.0
         for a in args:

@@ -324,7 +324,8 @@ typedef struct bpx_batch_timing {
     int32_t iterations;
 } bpx_batch_timing;
 typedef struct bpx_batch_s* bpx_batch;
-/* options: storage must be BPX_STORAGE_F32 (kahan_block and flags as bpx_backprojection_ex). */
+/* options: storage F32, F16 or BF16 for every capture (16-bit rows with a per-capture
+ * power-of-two scale, as bpx_backprojection_ex); kahan_block and flags as there. */
 bpx_status bpx_batch_create(const bpx_capture_desc* captures, uint32_t count, const bpx_options* options,
                             bpx_batch* out);
 uint32_t bpx_batch_size(bpx_batch batch);

@@ -46,6 +46,23 @@ def test_batch_equals_single_captures(gpu):
         assert np.all(lat > 0) and np.all(np.isfinite(lat))
 
 
+@pytest.mark.parametrize("storage", ["f16", "bf16"])
+def test_batch_16bit_storage_equals_single_captures(gpu, storage):
+    """fp16 / bf16 rows with a per-capture scale: each batched volume equals the capture
+    reconstructed alone with the same storage, for both lane shapes."""
+    caps = uneven(2)
+    alone = [gpu.backprojection(*c, storage=storage) for c in caps]
+    with gpu.Batch(caps, storage=storage) as b:
+        for k in (2, 4):
+            b.set_config(lane_voxels=k, groups=0, order=0)
+            b.execute()
+            b.synchronize()
+            for i, ref in enumerate(alone):
+                assert np.array_equal(b.volume(i), ref), (k, i)
+    f32_alone = gpu.backprojection(*caps[4])   # integer counts: exact in fp16, rounded in bf16
+    assert np.linalg.norm(alone[4] - f32_alone) <= 2e-3 * np.linalg.norm(f32_alone)
+
+
 @pytest.mark.parametrize("cfg", [dict(lane_voxels=4, groups=7, order=1),
                                  dict(lane_voxels=2, groups=1, order=0),
                                  dict(lane_voxels=4, groups=0, order=0)])
