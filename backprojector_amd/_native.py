@@ -64,12 +64,14 @@ class NlosPrepared(ctypes.Structure):
 class OctreeOptions(ctypes.Structure):
     _fields_ = [("levels", ctypes.c_uint32), ("threshold", ctypes.c_float),
                 ("proposal_threshold", ctypes.c_float), ("flags", ctypes.c_uint32),
-                ("level_proposal_threshold", ctypes.c_float * 8)]
+                ("level_proposal_threshold", ctypes.c_float * 8), ("keep_bound_ratio", ctypes.c_float)]
 
 
-def make_octree_options(levels=4, threshold=0.25, proposal_threshold=0.5, bounds_only=False):
+def make_octree_options(levels=4, threshold=0.25, proposal_threshold=0.5, bounds_only=False,
+                        keep_bound_ratio=0.0):
     """proposal_threshold: one probability, or one per coarse level."""
     o = OctreeOptions(int(levels), float(threshold), 0.5, OCTREE_BOUNDS_ONLY if bounds_only else 0)
+    o.keep_bound_ratio = float(keep_bound_ratio)
     if np.ndim(proposal_threshold) == 0:
         o.proposal_threshold = float(proposal_threshold)
     else:

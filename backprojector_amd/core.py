@@ -197,9 +197,12 @@ class Plan:
         self._h = h
         return self
 
-    def set_octree_options(self, threshold=0.25, proposal_threshold=0.5, bounds_only=False):
-        """Replaces an octree plan's thresholds (proposal_threshold: one value or one per level)."""
-        o = nat.make_octree_options(self.levels, threshold, proposal_threshold, bounds_only)
+    def set_octree_options(self, threshold=0.25, proposal_threshold=0.5, bounds_only=False,
+                           keep_bound_ratio=0.0):
+        """Replaces an octree plan's thresholds (proposal_threshold: one value or one per level;
+        keep_bound_ratio > 0: children of blocks whose bound reaches that fraction of the running
+        max are refined whatever an attached net says)."""
+        o = nat.make_octree_options(self.levels, threshold, proposal_threshold, bounds_only, keep_bound_ratio)
         check(lib().bpx_octree_plan_set_options(self._h, ctypes.byref(o)))
 
     def attach_net(self, net):

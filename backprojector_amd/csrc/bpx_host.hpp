@@ -58,6 +58,7 @@ struct OctreeState {
     uint32_t L = 4, ldm = 0;
     float tau = 0.25f;
     float prob[8] = {0.5f, 0.5f, 0.5f, 0.5f, 0.5f, 0.5f, 0.5f, 0.5f};  // per-level proposal thresholds
+    float keep = 0.0f;    // U >= keep x running max overrides the net's veto (0: never)
     bool bounds_only = false;
     std::vector<Level> levels;
     DeviceBuffer ping, pong, fine_list, flags, offs, sums, total, stats, partial;

@@ -187,7 +187,7 @@ hipError_t launch_child_labels(const float* pooled, uint32_t n0, uint32_t n1, ui
 hipError_t launch_max_f64(const float* x, uint64_t n, double* partial, uint32_t max_parts, double* out,
                           hipStream_t s);
 hipError_t launch_oct_child_flags(const uint32_t* blocks, const float* U, uint32_t n, const double* stats,
-                                  const float* logits, uint32_t n1, uint32_t n2, float logit_thr,
+                                  const float* logits, uint32_t n1, uint32_t n2, float logit_thr, float keep,
                                   uint32_t* flags, hipStream_t s);
 // All launchers enqueue on `stream` and return the hipError_t of the launch.
 // Confocal threshold table S[p][j] (j = 0..Tc) from the pair-independent path-length table

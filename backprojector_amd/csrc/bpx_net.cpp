@@ -144,7 +144,7 @@ bool net_child_flags(PlanImpl& pl, uint32_t l, hipStream_t s) {
     hip_check(launch_oct_child_flags(static_cast<const uint32_t*>(lv.list.ptr), static_cast<const float*>(lv.U.ptr),
                                      lv.count, static_cast<const double*>(st.stats.ptr),
                                      static_cast<const float*>(net.logits.ptr), lv.n[1], lv.n[2], logit_thr,
-                                     static_cast<uint32_t*>(st.flags.ptr), s),
+                                     st.keep, static_cast<uint32_t*>(st.flags.ptr), s),
               "learned child flags");
     return true;
 }

@@ -377,10 +377,12 @@ hipError_t launch_max_f64(const float* x, uint64_t n, double* partial, uint32_t 
 
 // Learned child flags of a level: flag[8i + c] = (U[i] >= threshold) && sigmoid(logit) > p,
 // logits read from the dense level grid at the block's position.
+// Children of block b: refined iff U >= tau x running max (stats[4]) and, with a net, its logit
+// exceeds the threshold or U >= keep x running max (stats[0]; keep 0 never overrides).
 __global__ void oct_child_flags_kernel(const uint32_t* __restrict__ blocks, const float* __restrict__ U,
                                        uint32_t n, const double* __restrict__ stats,
                                        const float* __restrict__ logits, uint32_t n1, uint32_t n2,
-                                       float logit_thr, uint32_t* __restrict__ flags) {
+                                       float logit_thr, float keep, uint32_t* __restrict__ flags) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n * 8u) return;
     const uint32_t b = i / 8u, c = i % 8u;
@@ -389,18 +391,19 @@ __global__ void oct_child_flags_kernel(const uint32_t* __restrict__ blocks, cons
         const uint32_t cc = blocks[b];
         const uint32_t x = cc & 1023u, y = (cc >> 10) & 1023u, z = cc >> 20;
         const float lg = logits[(((size_t)x * n1 + y) * n2 + z) * 16u + c];
-        if (!(lg > logit_thr)) f = 0u;
+        const bool keep_bound = keep > 0.0f && (double)U[b] >= (double)keep * stats[0];
+        if (!(lg > logit_thr) && !keep_bound) f = 0u;
     }
     flags[i] = f;
 }
 
 hipError_t launch_oct_child_flags(const uint32_t* blocks, const float* U, uint32_t n, const double* stats,
-                                  const float* logits, uint32_t n1, uint32_t n2, float logit_thr,
+                                  const float* logits, uint32_t n1, uint32_t n2, float logit_thr, float keep,
                                   uint32_t* flags, hipStream_t s) {
     const uint32_t m = n * 8u;
     if (m == 0) return hipSuccess;
     hipLaunchKernelGGL(oct_child_flags_kernel, dim3((m + 255u) / 256u), dim3(256), 0, s, blocks, U, n, stats,
-                       logits, n1, n2, logit_thr, flags);
+                       logits, n1, n2, logit_thr, keep, flags);
     return hipGetLastError();
 }
 

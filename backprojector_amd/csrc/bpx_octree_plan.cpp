@@ -44,14 +44,17 @@ void octree_set_options(OctreeState& st, const bpx_octree_options& o) {
         prob[l] = p != 0.0f ? p : o.proposal_threshold;
     }
     if (o.flags & ~BPX_OCTREE_BOUNDS_ONLY) fail(BPX_ERR_INVALID_ARGUMENT, "unknown octree flags");
+    if (!(o.keep_bound_ratio >= 0.0f) || !std::isfinite(o.keep_bound_ratio))
+        fail(BPX_ERR_INVALID_ARGUMENT, "keep_bound_ratio must be finite and >= 0");
     st.tau = o.threshold;
+    st.keep = o.keep_bound_ratio;
     std::memcpy(st.prob, prob, sizeof(prob));
     st.bounds_only = (o.flags & BPX_OCTREE_BOUNDS_ONLY) != 0;
 }
 
 PlanImpl* make_plan_octree(const CaptureIn& c, const uint32_t n[3], const float zero[3],
                            const float inc[9], const bpx_octree_options* oo) {
-    bpx_octree_options o{4u, 0.25f, 0.5f, 0u, {}};
+    bpx_octree_options o{4u, 0.25f, 0.5f, 0u, {}, 0.0f};
     if (oo) o = *oo;
     if (o.levels == 0 || o.levels > 8) fail(BPX_ERR_INVALID_ARGUMENT, "octree levels must be 1..8");
     for (int k = 0; k < 3; ++k) {
@@ -204,7 +207,7 @@ void execute_octree(PlanImpl& pl, hipStream_t s) {
         // child flags: fixed rule (bound >= tau * running max), narrowed by learned proposals
         if (!net_child_flags(pl, l, s))
             hip_check(launch_oct_child_flags(cur, a.U, n, static_cast<const double*>(st.stats.ptr), nullptr,
-                                             lv.n[1], lv.n[2], 0.0f, static_cast<uint32_t*>(st.flags.ptr), s),
+                                             lv.n[1], lv.n[2], 0.0f, 0.0f, static_cast<uint32_t*>(st.flags.ptr), s),
                       "octree child flags");
         uint32_t* next = l + 1 < st.L ? static_cast<uint32_t*>(st.levels[l + 1].list.ptr)
                                       : static_cast<uint32_t*>(st.fine_list.ptr);

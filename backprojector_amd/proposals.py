@@ -131,6 +131,14 @@ def rel_l2_occupied(vol, dense):
     return float(np.linalg.norm((vol - dense)[occ]) / max(np.linalg.norm(dense[occ]), 1e-300))
 
 
+def apply_operating_point(plan, net, tau, cfg):
+    """Attaches net and sets one proposal configuration on an octree plan: a probability, one
+    per level, or {"p": either, "keep": keep_bound_ratio}."""
+    plan.attach_net(net)
+    p, keep = (cfg["p"], cfg.get("keep", 0.0)) if isinstance(cfg, dict) else (cfg, 0.0)
+    plan.set_octree_options(tau, p, keep_bound_ratio=keep)
+
+
 def compare(cap, voxels, net, configs, tau=0.25, levels=4, reps=3):
     """Dense vs bound-rule octree vs learned octree (each proposal configuration) on one
     capture; one octree plan is reused with its thresholds replaced."""
@@ -144,8 +152,7 @@ def compare(cap, voxels, net, configs, tau=0.25, levels=4, reps=3):
             if cfg is None:
                 op.attach_net(None)
             else:
-                op.attach_net(net)
-                op.set_octree_options(tau, cfg)
+                apply_operating_point(op, net, tau, cfg)
             t = op.benchmark(1, reps)["per_exec_ms"]
             vol = op.volume()
             st = op.octree_stats()

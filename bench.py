@@ -75,8 +75,7 @@ def octree_comparison(bp, cap, side, tau, proposal=None, reps=3):
     with bp.Plan.octree(*args, levels=4, threshold=tau) as oct_:
         for name, p in (("bound_rule", None), ("learned", proposal)):
             if p is not None:
-                oct_.attach_net(net)
-                oct_.set_octree_options(tau, p)
+                proposals.apply_operating_point(oct_, net, tau, p)
             t = oct_.benchmark(1, reps)
             vo = oct_.volume()
             st = oct_.octree_stats()

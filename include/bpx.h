@@ -202,6 +202,10 @@ typedef struct bpx_octree_options {
     float proposal_threshold; /* learned proposals: refine iff probability > this (0.5) */
     uint32_t flags;           /* BPX_OCTREE_*                                            */
     float level_proposal_threshold[8]; /* per coarse level (0 = use proposal_threshold)  */
+    float keep_bound_ratio;   /* learned proposals: a child whose block bound U reaches
+                                 keep_bound_ratio x the running max is refined whatever the
+                                 net says (the sound rule as the fallback where the bound is
+                                 high); 0 = the net alone decides                           */
 } bpx_octree_options;
 
 /* Run the coarse levels only and skip the exact finest level (training data for the learned

@@ -22,8 +22,7 @@ def test_learned_proposals_gain(gpu, seed):
     with gpu.Plan.octree(*args, levels=4, threshold=0.25) as op:
         for name in ("bound_rule", "learned"):
             if name == "learned":
-                op.attach_net(net)
-                op.set_octree_options(0.25, p)
+                proposals.apply_operating_point(op, net, 0.25, p)
             t = op.benchmark(1, 2)["per_exec_ms"]
             vol = op.volume()
             st = op.octree_stats()

@@ -187,6 +187,15 @@ def test_attached_net_narrows_and_stays_exact(gpu):
         lo.execute()
         assert lo.octree_stats()["active"] == [8 ** 3, 0, 0, 0]
         assert not lo.volume().any()
+        # the bound override: an always-no net with keep ratio ~0 refines every block the bound
+        # rule refines; with a keep ratio above every bound it vetoes everything again
+        lo.set_octree_options(0.25, 0.5, keep_bound_ratio=1e-30)
+        lo.execute()
+        assert np.array_equal(lo.volume(), v0) and lo.octree_stats()["active"] == a0
+        lo.set_octree_options(0.25, 0.5, keep_bound_ratio=1e30)
+        lo.execute()
+        assert lo.octree_stats()["active"] == [8 ** 3, 0, 0, 0]
+        lo.set_octree_options(0.25, 0.5)
         lo.attach_net(random_net(gpu, 21))
         lo.execute()
         v1 = lo.volume()
