@@ -40,11 +40,24 @@ struct SqrtPred {
     __device__ bool operator()(uint32_t key) const { return exact_sqrt(fkey_inv(key)) >= d; }
 };
 
+// Guesses in double: d ~ (z - lw - cw) / 2, and for the second search the square of the midpoint
+// between d* and the float below it (the smallest s whose correctly rounded root reaches d*
+// lies just above that square), rounded up to a float.  A good guess leaves the gallop two
+// predicate evaluations; the searches keep the result exact whatever the guess.
 __device__ __forceinline__ float stable_threshold(float lw, float cw, float z) {
-    const float g = 0.5f * ((z - lw) - cw);
-    const float dstar = fkey_inv(min_true_key(ChainPred{lw, cw, z}, fkey(g > 0.0f ? g : 0.0f), fkey(0.0f),
-                                              fkey(__builtin_inff())));
-    return fkey_inv(min_true_key(SqrtPred{dstar}, fkey(dstar * dstar), fkey(0.0f), fkey(__builtin_inff())));
+    const double gd = 0.5 * (((double)z - (double)lw) - (double)cw);
+    const float g = gd > 0.0 ? (float)gd : 0.0f;
+    const float dstar = fkey_inv(min_true_key(ChainPred{lw, cw, z}, fkey(g), fkey(0.0f), fkey(__builtin_inff())));
+    float sg = dstar * dstar;
+    if (dstar > 0.0f && dstar < __builtin_inff()) {
+        // neighbours of non-negative finite floats by their bit patterns
+        const float below = __uint_as_float(__float_as_uint(dstar) - 1u);
+        const double m = 0.5 * ((double)dstar + (double)below);
+        const double msq = m * m;
+        sg = (float)msq;
+        if ((double)sg < msq) sg = __uint_as_float(__float_as_uint(sg) + 1u);
+    }
+    return fkey_inv(min_true_key(SqrtPred{dstar}, fkey(sg), fkey(0.0f), fkey(__builtin_inff())));
 }
 
 // S[p][j] is written at float index p*pitch + j*stride + offset (stride 1, offset 0 for a
