@@ -154,6 +154,10 @@ struct CaptureIn {
     DeviceBuffer* device_rows = nullptr;
 };
 
+// Host -> device copy of a (possibly pageable) host array, enqueued on s and complete when
+// this returns; large arrays are pipelined through pinned staging buffers.
+void upload_host(void* dst, const void* src, size_t bytes, hipStream_t s);
+
 // Executes a plan, copies its volume to host memory and destroys the plan.
 void run_plan_to_host(PlanImpl* plan, float* out);
 

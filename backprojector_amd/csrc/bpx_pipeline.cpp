@@ -343,7 +343,7 @@ DeviceBuffer phasor_rows_device(const float* src, uint64_t rows, uint32_t src_T,
     if (rows == 0) return out;
     full.alloc(sizeof(float) * rows * src_T);
     chunk.alloc(sizeof(float) * rows * w.Tc);
-    hip_check(hipMemcpy(full.ptr, src, sizeof(float) * rows * src_T, hipMemcpyHostToDevice), "hipMemcpy(transient)");
+    upload_host(full.ptr, src, sizeof(float) * rows * src_T, nullptr);
     hip_check(launch_crop_rows(static_cast<const float*>(full.ptr), src_T, nullptr, (uint32_t)rows, w.Tc, w.from,
                                w.cfrom, w.ncopy, static_cast<float*>(chunk.ptr), nullptr),
               "crop kernel");

@@ -4,6 +4,8 @@
 #include <cstring>
 #include <limits>
 #include <memory>
+#include <mutex>
+#include <thread>
 
 #include "bpx_host.hpp"
 
@@ -155,6 +157,56 @@ static std::vector<uint32_t> used_pairs(const CaptureIn& c, uint32_t flags) {
     return idx;
 }
 
+// ------------------------------------------------------------------ host -> device uploads
+// Large pageable host arrays go through two pinned staging buffers (allocated once per
+// process): host threads copy chunk k + 1 into one buffer while the DMA engine copies chunk k
+// from the other.  Measured on MI355X: pageable hipMemcpy ~30 GB/s, pinned ~57 GB/s, 4-thread
+// host copies into pinned memory ~60 GB/s.
+namespace {
+constexpr size_t kStageChunk = 32u << 20;
+constexpr int kCopyThreads = 4;
+std::mutex g_upload_mutex;
+void* g_pinned[2] = {nullptr, nullptr};
+hipEvent_t g_pinned_ev[2] = {nullptr, nullptr};
+
+void parallel_copy(void* dst, const void* src, size_t bytes) {
+    std::thread th[kCopyThreads - 1];
+    const size_t part = (bytes / kCopyThreads + 63) & ~size_t(63);
+    auto piece = [&](int t) {
+        const size_t a = std::min(bytes, part * (size_t)t), b = std::min(bytes, part * (size_t)(t + 1));
+        if (b > a) std::memcpy(static_cast<char*>(dst) + a, static_cast<const char*>(src) + a, b - a);
+    };
+    for (int t = 1; t < kCopyThreads; ++t) th[t - 1] = std::thread(piece, t);
+    piece(0);
+    for (auto& x : th) x.join();
+}
+}  // namespace
+
+void upload_host(void* dst, const void* src, size_t bytes, hipStream_t s) {
+    if (bytes < 2 * kStageChunk) {
+        hip_check(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "hipMemcpy(upload)");
+        return;
+    }
+    std::lock_guard<std::mutex> lock(g_upload_mutex);
+    for (int b = 0; b < 2; ++b) {
+        if (!g_pinned[b]) hip_check(hipHostMalloc(&g_pinned[b], kStageChunk, hipHostMallocDefault), "hipHostMalloc");
+        if (!g_pinned_ev[b]) hip_check(hipEventCreateWithFlags(&g_pinned_ev[b], hipEventDisableTiming), "hipEventCreate");
+    }
+    size_t off = 0;
+    for (int k = 0; off < bytes; ++k, off += kStageChunk) {
+        const int b = k & 1;
+        const size_t n = std::min(kStageChunk, bytes - off);
+        hip_check(hipEventSynchronize(g_pinned_ev[b]), "hipEventSynchronize");   // its previous DMA is done
+        parallel_copy(g_pinned[b], static_cast<const char*>(src) + off, n);
+        hip_check(hipMemcpyAsync(static_cast<char*>(dst) + off, g_pinned[b], n, hipMemcpyHostToDevice, s),
+                  "hipMemcpyAsync(upload)");
+        hip_check(hipEventRecord(g_pinned_ev[b], s), "hipEventRecord");
+    }
+    // the buffers may be reused as soon as this returns
+    hip_check(hipEventSynchronize(g_pinned_ev[0]), "hipEventSynchronize");
+    hip_check(hipEventSynchronize(g_pinned_ev[1]), "hipEventSynchronize");
+}
+
 // Stages the raw rows of the used pairs on the device (compacting when pairs were dropped).
 static void stage_raw(PlanImpl& pl, const CaptureIn& c, const std::vector<uint32_t>& idx) {
     const size_t comp = c.cplx ? 2 : 1;
@@ -183,16 +235,14 @@ static void stage_raw(PlanImpl& pl, const CaptureIn& c, const std::vector<uint32
     if (!c.chunk) {
         if (idx.empty()) return;
         if (idx.size() == c.P && c.from == 0 && c.cfrom == 0 && c.ncopy == c.T && c.src_T == c.T) {
-            hip_check(hipMemcpy(pl.raw.ptr, c.src, sizeof(float) * row * c.P, hipMemcpyHostToDevice),
-                      "hipMemcpy(transient)");
+            upload_host(pl.raw.ptr, c.src, sizeof(float) * row * c.P, pl.stream);
             return;
         }
         // the source rows in one copy, then the crop (and compaction of dropped pairs) on the device
         DeviceBuffer full, map;
         full.alloc(sizeof(float) * (size_t)c.P * c.src_T);
         map.alloc(sizeof(uint32_t) * idx.size());
-        hip_check(hipMemcpy(full.ptr, c.src, sizeof(float) * (size_t)c.P * c.src_T, hipMemcpyHostToDevice),
-                  "hipMemcpy(transient)");
+        upload_host(full.ptr, c.src, sizeof(float) * (size_t)c.P * c.src_T, pl.stream);
         hip_check(hipMemcpy(map.ptr, idx.data(), sizeof(uint32_t) * idx.size(), hipMemcpyHostToDevice),
                   "hipMemcpy(row map)");
         hip_check(launch_crop_rows(static_cast<const float*>(full.ptr), c.src_T,
@@ -203,8 +253,7 @@ static void stage_raw(PlanImpl& pl, const CaptureIn& c, const std::vector<uint32
         return;
     }
     if (idx.size() == c.P) {
-        hip_check(hipMemcpy(pl.raw.ptr, c.chunk, sizeof(float) * row * c.P, hipMemcpyHostToDevice),
-                  "hipMemcpy(transient)");
+        upload_host(pl.raw.ptr, c.chunk, sizeof(float) * row * c.P, pl.stream);
     } else {
         std::vector<float> tmp(row * idx.size());
         for (size_t i = 0; i < idx.size(); ++i)
