@@ -91,6 +91,8 @@ struct PlanImpl {
     bool all_confocal = false;
     DeviceBuffer raw, H, S, Z, Z2, pair8, pair4, lasers, spads, vol;
     DeviceBuffer stage;                  // one StageJob (fused confocal layouts)
+    std::vector<uint32_t> used;          // source rows of the staged pairs (dropped pairs skipped)
+    uint32_t rows_in = 0;                // rows of the capture the plan was made from
     uint32_t stage_zb = 0, stage_sb = 0; // its blocks in the two staging passes
     hipEvent_t ev[8] = {};
     std::unique_ptr<OctreeState> oct;
@@ -153,6 +155,11 @@ struct CaptureIn {
     // buffer over (leaving it empty) when every pair is used, else gathers the used rows.
     DeviceBuffer* device_rows = nullptr;
 };
+
+// Replaces a plan's rows with new device rows of the same capture layout ([rows_in][T], the
+// layout of CaptureIn::chunk): the plan takes the buffer over when every row is used, else
+// gathers the used rows.  The next execution stages and reconstructs them.
+void plan_replace_rows(PlanImpl& pl, DeviceBuffer& rows);
 
 // Host -> device copy of a (possibly pageable) host array, enqueued on s and complete when
 // this returns; large arrays are pipelined through pinned staging buffers.

@@ -331,25 +331,32 @@ void phasor_filter_host(const float* in, uint64_t rows, uint32_t T, const std::v
     hip_check(hipMemcpy(out, dout.ptr, sizeof(float) * 2 * rows * T, hipMemcpyDeviceToHost), "hipMemcpy(phasor)");
 }
 
-// The phasor-filtered chunk on the device: the source rows in one copy, the chunk window cropped
-// (get_transient_chunk), then the FIR into [rows][Tc] complex rows -- no host chunk and no
-// host round trip of the complex samples.
-DeviceBuffer phasor_rows_device(const float* src, uint64_t rows, uint32_t src_T, const pipeline::ChunkWindow& w,
-                                const std::vector<float>& c, const std::vector<float>& s) {
-    DeviceBuffer full, chunk, out, dc, ds;
-    upload_taps(c, s, dc, ds);
+// The chunk on the device: the source rows in one copy, the chunk window cropped
+// (get_transient_chunk) into [rows][Tc] real rows.
+static DeviceBuffer chunk_rows_device(const float* src, uint64_t rows, uint32_t src_T, const pipeline::ChunkWindow& w) {
     if (rows >= (1ull << 32) || rows * w.Tc >= (1ull << 38)) fail(BPX_ERR_UNSUPPORTED, "phasor input too large");
-    out.alloc(sizeof(float) * 2 * std::max<uint64_t>(rows * w.Tc, 1));
-    if (rows == 0) return out;
+    DeviceBuffer full, chunk;
+    chunk.alloc(sizeof(float) * std::max<uint64_t>(rows * w.Tc, 1));
+    if (rows == 0) return chunk;
     full.alloc(sizeof(float) * rows * src_T);
-    chunk.alloc(sizeof(float) * rows * w.Tc);
     upload_host(full.ptr, src, sizeof(float) * rows * src_T, nullptr);
     hip_check(launch_crop_rows(static_cast<const float*>(full.ptr), src_T, nullptr, (uint32_t)rows, w.Tc, w.from,
                                w.cfrom, w.ncopy, static_cast<float*>(chunk.ptr), nullptr),
               "crop kernel");
-    hip_check(launch_phasor_fir(static_cast<const float*>(chunk.ptr), (uint32_t)rows, w.Tc,
+    hip_check(hipStreamSynchronize(nullptr), "hipStreamSynchronize");
+    return chunk;
+}
+
+// The phasor FIR of device chunk rows into [rows][Tc] complex rows (no host round trip).
+static DeviceBuffer phasor_rows_device(const DeviceBuffer& chunk, uint64_t rows, uint32_t Tc,
+                                       const std::vector<float>& c, const std::vector<float>& s) {
+    DeviceBuffer out, dc, ds;
+    upload_taps(c, s, dc, ds);
+    out.alloc(sizeof(float) * 2 * std::max<uint64_t>(rows * Tc, 1));
+    if (rows == 0) return out;
+    hip_check(launch_phasor_fir(static_cast<const float*>(chunk.ptr), (uint32_t)rows, Tc,
                                 static_cast<const float*>(dc.ptr), static_cast<const float*>(ds.ptr),
-                                (int)c.size(), out.ptr, w.Tc, 0u, kC64, 1.0f, nullptr),
+                                (int)c.size(), out.ptr, Tc, 0u, kC64, 1.0f, nullptr),
               "phasor FIR kernel");
     hip_check(hipStreamSynchronize(nullptr), "hipStreamSynchronize");
     return out;
@@ -554,9 +561,25 @@ bpx_status bpx_nlos_phasor(const float* transient_data, const int64_t* data_shap
                            const float volume_position[3], const float volume_size[3],
                            const uint32_t voxels_per_side[3], float wavelength,
                            int assume_row_major, int use_cpu, float* complex_volume_out) {
+    return bpx_nlos_phasor_multi(transient_data, data_shape, data_ndim, camera_grid_positions, camera_grid_shape,
+                                 laser_grid_positions, laser_grid_shape, camera_position, laser_position, t0,
+                                 deltaT, is_confocal, volume_position, volume_size, voxels_per_side, &wavelength, 1u,
+                                 assume_row_major, use_cpu, complex_volume_out);
+}
+
+bpx_status bpx_nlos_phasor_multi(const float* transient_data, const int64_t* data_shape, int data_ndim,
+                                 const float* camera_grid_positions, const int64_t* camera_grid_shape,
+                                 const float* laser_grid_positions, const int64_t* laser_grid_shape,
+                                 const float camera_position[3], const float laser_position[3],
+                                 float t0, float deltaT, int is_confocal,
+                                 const float volume_position[3], const float volume_size[3],
+                                 const uint32_t voxels_per_side[3], const float* wavelengths, uint32_t count,
+                                 int assume_row_major, int use_cpu, float* complex_volumes_out) {
     clear_error();
     try {
-        if (!complex_volume_out) fail(BPX_ERR_INVALID_ARGUMENT, "NULL output");
+        if (!complex_volumes_out) fail(BPX_ERR_INVALID_ARGUMENT, "NULL output");
+        if (count == 0) return BPX_OK;
+        if (!wavelengths) fail(BPX_ERR_INVALID_ARGUMENT, "NULL wavelengths");
         PipelineGeometry g = prepare(transient_data, data_shape, data_ndim, camera_grid_positions,
                                      camera_grid_shape, laser_grid_positions, laser_grid_shape,
                                      camera_position, laser_position, t0, deltaT, is_confocal,
@@ -564,34 +587,50 @@ bpx_status bpx_nlos_phasor(const float* transient_data, const int64_t* data_shap
         // :962-969 -- the float limits are truncated to int
         const float imin = (float)(int)g.min_idx, imax = (float)(int)g.max_idx;
         const pipeline::ChunkWindow w = pipeline::chunk_window(g.T, imin, imax);
-        std::vector<float> cs, sn;
-        if (pipeline::phasor_taps(wavelength, deltaT, 4, 0.0f, cs, sn) <= 0)
-            fail(BPX_ERR_INVALID_ARGUMENT, "the phasor pulse has no taps for this wavelength");
+        std::vector<std::vector<float>> cs(count), sn(count);
+        for (uint32_t i = 0; i < count; ++i)
+            if (pipeline::phasor_taps(wavelengths[i], deltaT, 4, 0.0f, cs[i], sn[i]) <= 0)
+                fail(BPX_ERR_INVALID_ARGUMENT, "the phasor pulse has no taps for wavelength " +
+                                                   std::to_string(wavelengths[i]));
         if (!device_ready()) fail(BPX_ERR_NO_DEVICE, "no gfx950 device available");
         if (g.rows != g.pairs.size())
             fail(BPX_ERR_INVALID_ARGUMENT, "transient_size (" + std::to_string(g.rows * w.Tc) +
                                                ") != num_pairs * T (" +
                                                std::to_string((uint64_t)g.pairs.size() * w.Tc) + ")");
-        DeviceBuffer rows = phasor_rows_device(g.arr.data_p, g.rows, g.T, w, cs, sn);
+        if (!use_cpu)
+            for (int k = 0; k < 3; ++k)
+                if (voxels_per_side[k] < 2)
+                    fail(BPX_ERR_INVALID_ARGUMENT,
+                         "the GPU path's grid spacing size/(N-1) needs at least 2 voxels per side");
+        // one upload and crop; per wavelength the FIR, then the same plan re-staged and run
+        const DeviceBuffer chunk = chunk_rows_device(g.arr.data_p, g.rows, g.T, w);
         const float chunked_t0 = t0 + imin * deltaT;  // :1011
-        CaptureIn c{nullptr, g.rows * w.Tc, w.Tc, g.pairs.data(), (uint32_t)g.pairs.size(), chunked_t0, deltaT,
-                    true};
-        c.device_rows = &rows;
-        if (use_cpu) {
-            if (!volume_position || !volume_size) fail(BPX_ERR_INVALID_ARGUMENT, "NULL volume argument");
-            run_plan_to_host(make_plan_window(c, voxels_per_side, volume_position, volume_size),
-                             complex_volume_out);
-            return BPX_OK;
+        const size_t nvox = (size_t)voxels_per_side[0] * voxels_per_side[1] * voxels_per_side[2];
+        std::unique_ptr<PlanImpl> plan;
+        for (uint32_t i = 0; i < count; ++i) {
+            DeviceBuffer rows = phasor_rows_device(chunk, g.rows, w.Tc, cs[i], sn[i]);
+            if (!plan) {
+                CaptureIn c{nullptr, g.rows * w.Tc, w.Tc, g.pairs.data(), (uint32_t)g.pairs.size(), chunked_t0,
+                            deltaT, true};
+                c.device_rows = &rows;
+                if (use_cpu) {
+                    if (!volume_position || !volume_size) fail(BPX_ERR_INVALID_ARGUMENT, "NULL volume argument");
+                    plan.reset(make_plan_window(c, voxels_per_side, volume_position, volume_size));
+                } else {
+                    const float zeros3[3] = {0, 0, 0};
+                    float zero[3], inc[9];
+                    pipeline::basis(volume_position, volume_size, voxels_per_side, zeros3, zero, inc);
+                    plan.reset(make_plan_single_tap(c, voxels_per_side, zero, inc, nullptr));
+                }
+            } else {
+                plan_replace_rows(*plan, rows);
+            }
+            plan->execute(plan->stream);
+            hip_check(hipMemcpyAsync(complex_volumes_out + 2 * nvox * i, plan->vol.ptr, nvox * 8,
+                                     hipMemcpyDeviceToHost, plan->stream),
+                      "hipMemcpyAsync(volume)");
+            hip_check(hipStreamSynchronize(plan->stream), "hipStreamSynchronize");
         }
-        for (int k = 0; k < 3; ++k)
-            if (voxels_per_side[k] < 2)
-                fail(BPX_ERR_INVALID_ARGUMENT,
-                     "the GPU path's grid spacing size/(N-1) needs at least 2 voxels per side");
-        const float zeros3[3] = {0, 0, 0};
-        float zero[3], inc[9];
-        pipeline::basis(volume_position, volume_size, voxels_per_side, zeros3, zero, inc);
-        run_plan_to_host(make_plan_single_tap(c, voxels_per_side, zero, inc, nullptr),
-                         complex_volume_out);
         return BPX_OK;
     } catch (const Failure& e) {
         return set_error(e.status, e.msg);

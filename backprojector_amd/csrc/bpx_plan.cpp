@@ -208,27 +208,49 @@ void upload_host(void* dst, const void* src, size_t bytes, hipStream_t s) {
 }
 
 // Stages the raw rows of the used pairs on the device (compacting when pairs were dropped).
+// Device rows [rows_in][row floats] -> the plan's raw rows of the used pairs: taken over when
+// every row is used, else gathered.
+static void adopt_device_rows(PlanImpl& pl, DeviceBuffer& rows, uint32_t rows_in, size_t row,
+                              const std::vector<uint32_t>& idx) {
+    if (idx.size() == rows_in && rows_in) {
+        pl.raw.release();
+        std::swap(pl.raw.ptr, rows.ptr);
+        std::swap(pl.raw.bytes, rows.bytes);
+        return;
+    }
+    pl.raw.alloc(sizeof(float) * row * std::max<size_t>(idx.size(), 1));
+    if (idx.empty()) return;
+    DeviceBuffer map;
+    map.alloc(sizeof(uint32_t) * idx.size());
+    hip_check(hipMemcpy(map.ptr, idx.data(), sizeof(uint32_t) * idx.size(), hipMemcpyHostToDevice),
+              "hipMemcpy(row map)");
+    hip_check(launch_crop_rows(static_cast<const float*>(rows.ptr), (uint32_t)row, static_cast<const uint32_t*>(map.ptr),
+                               (uint32_t)idx.size(), (uint32_t)row, 0u, 0u, (uint32_t)row,
+                               static_cast<float*>(pl.raw.ptr), pl.stream),
+              "row gather kernel");
+    hip_check(hipStreamSynchronize(pl.stream), "hipStreamSynchronize");
+}
+
+void plan_replace_rows(PlanImpl& pl, DeviceBuffer& rows) {
+    if (pl.path == kPathOctree) fail(BPX_ERR_UNSUPPORTED, "octree plans keep their rows");
+    if (pl.P == 0 || !pl.raw.ptr) return;   // nothing staged: every execution writes zeros
+    const size_t row = (size_t)pl.T * (pl.cplx ? 2 : 1);
+    if (rows.bytes < sizeof(float) * row * pl.rows_in) fail(BPX_ERR_INVALID_ARGUMENT, "replacement rows too small");
+    hip_check(hipStreamSynchronize(pl.stream), "hipStreamSynchronize");
+    adopt_device_rows(pl, rows, pl.rows_in, row, pl.used);
+    if (pl.stage.ptr) {   // the staging descriptor names the raw rows: point it at the new ones
+        const StageJob job = stage_job(pl);
+        hip_check(hipMemcpy(pl.stage.ptr, &job, sizeof(StageJob), hipMemcpyHostToDevice), "hipMemcpy(staging job)");
+    }
+}
+
 static void stage_raw(PlanImpl& pl, const CaptureIn& c, const std::vector<uint32_t>& idx) {
     const size_t comp = c.cplx ? 2 : 1;
     const size_t row = (size_t)c.T * comp;
+    pl.used = idx;
+    pl.rows_in = c.P;
     if (!c.chunk && c.device_rows) {
-        if (idx.size() == c.P && c.P) {
-            pl.raw.release();
-            std::swap(pl.raw.ptr, c.device_rows->ptr);
-            std::swap(pl.raw.bytes, c.device_rows->bytes);
-            return;
-        }
-        pl.raw.alloc(sizeof(float) * row * std::max<size_t>(idx.size(), 1));
-        if (idx.empty()) return;
-        DeviceBuffer map;
-        map.alloc(sizeof(uint32_t) * idx.size());
-        hip_check(hipMemcpy(map.ptr, idx.data(), sizeof(uint32_t) * idx.size(), hipMemcpyHostToDevice),
-                  "hipMemcpy(row map)");
-        hip_check(launch_crop_rows(static_cast<const float*>(c.device_rows->ptr), (uint32_t)row,
-                                   static_cast<const uint32_t*>(map.ptr), (uint32_t)idx.size(), (uint32_t)row, 0u, 0u,
-                                   (uint32_t)row, static_cast<float*>(pl.raw.ptr), pl.stream),
-                  "row gather kernel");
-        hip_check(hipStreamSynchronize(pl.stream), "hipStreamSynchronize");
+        adopt_device_rows(pl, *c.device_rows, c.P, row, idx);
         return;
     }
     pl.raw.alloc(sizeof(float) * row * std::max<size_t>(idx.size(), 1));

@@ -26,7 +26,7 @@ import numpy as np
 
 from ._native import check, lib, ptr
 
-__all__ = ["backproject", "phasor"]
+__all__ = ["backproject", "phasor", "phasor_multi"]
 
 
 def _f32(a):
@@ -120,4 +120,24 @@ def phasor(transient_data, camera_grid_positions, laser_grid_positions, camera_p
                                 1 if is_confocal else 0, ptr(vp), ptr(vs), ptr(N),
                                 float(wavelength), 1 if assume_row_major else 0,
                                 1 if use_cpu else 0, ptr(out)))
+    return out
+
+
+def phasor_multi(transient_data, camera_grid_positions, laser_grid_positions, camera_position,
+                 laser_position, t0, deltaT, is_confocal, volume_position, volume_size,
+                 voxels_per_side, wavelengths, assume_row_major=False, use_cpu=False):
+    """phasor() at several wavelengths of one capture -> complex64 [len(wavelengths), N0, N1, N2]
+    (an extension with no counterpart in the reference module; volume i equals
+    phasor(..., wavelengths[i], ...)).  The capture is copied to the GPU and cropped once."""
+    H, cg, lg, cp, lp, vp, vs, N = _prep(transient_data, camera_grid_positions,
+                                         laser_grid_positions, camera_position, laser_position,
+                                         volume_position, volume_size, voxels_per_side)
+    wl = np.ascontiguousarray(np.asarray(wavelengths, dtype=np.float32).reshape(-1))
+    out = np.zeros((len(wl),) + tuple(int(x) for x in N), np.complex64)
+    dshape, cshape, lshape = _shape(H), _shape(cg), _shape(lg)
+    check(lib().bpx_nlos_phasor_multi(ptr(H), ptr(dshape), H.ndim, ptr(cg), ptr(cshape), ptr(lg),
+                                      ptr(lshape), ptr(cp), ptr(lp), float(t0), float(deltaT),
+                                      1 if is_confocal else 0, ptr(vp), ptr(vs), ptr(N), ptr(wl),
+                                      len(wl), 1 if assume_row_major else 0, 1 if use_cpu else 0,
+                                      ptr(out)))
     return out

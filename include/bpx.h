@@ -159,6 +159,18 @@ bpx_status bpx_nlos_phasor(const float* transient_data, const int64_t* data_shap
                            const uint32_t voxels_per_side[3], float wavelength,
                            int assume_row_major, int use_cpu, float* complex_volume_out);
 
+/* bpx_nlos_phasor at several wavelengths of one capture (no reference counterpart; equal to
+ * `count` bpx_nlos_phasor calls, volume i at complex_volumes_out + 2 * N0*N1*N2 * i): the
+ * capture is copied and cropped once, and one plan is re-staged per wavelength. */
+bpx_status bpx_nlos_phasor_multi(const float* transient_data, const int64_t* data_shape, int data_ndim,
+                                 const float* camera_grid_positions, const int64_t* camera_grid_shape,
+                                 const float* laser_grid_positions, const int64_t* laser_grid_shape,
+                                 const float camera_position[3], const float laser_position[3],
+                                 float t0, float deltaT, int is_confocal,
+                                 const float volume_position[3], const float volume_size[3],
+                                 const uint32_t voxels_per_side[3], const float* wavelengths, uint32_t count,
+                                 int assume_row_major, int use_cpu, float* complex_volumes_out);
+
 /* Intermediates of the reference pipeline up to the device call (backproject.hpp:779-847):
  * the transient chunk [rows][Tc], the pair table, the GPU-path basis and chunked t0.  Arrays
  * are allocated by the library; release them with bpx_nlos_prepared_free. */

@@ -418,6 +418,24 @@ def test_centred_wall_basis(gpu):
     assert np.array_equal(along_x, np.transpose(ident[::-1], (1, 0, 2)))
 
 
+@pytest.mark.parametrize("use_cpu,nan_pair", [(False, False), (True, True)])
+def test_nlosbpy_phasor_multi_equals_single_calls(gpu, use_cpu, nan_pair):
+    """Several wavelengths of one capture (one copy and crop, one plan re-staged per
+    wavelength) give exactly the single-wavelength volumes."""
+    cap = scenes.render(scenes.config_scene(1), 12, 2048, t0=2.0, seed=17)
+    args = list(cap.nlos_args((10, 12, 9)))
+    if nan_pair:
+        args[1] = args[1].copy()
+        args[1][4, 2, 0] = np.nan
+    wls = [0.004, 0.005, 0.04, 0.0075]
+    multi = gpu.nlosbpy.phasor_multi(*args, wls, assume_row_major=True, use_cpu=use_cpu)
+    assert multi.shape == (4, 10, 12, 9) and multi.dtype == np.complex64
+    for i, wl in enumerate(wls):
+        one = gpu.nlosbpy.phasor(*args, wl, assume_row_major=True, use_cpu=use_cpu)
+        assert np.array_equal(multi[i], one), wl
+    assert np.abs(multi).max() > 0
+
+
 def test_nlosbpy_column_major_equals_row_major(gpu):
     cap = scenes.render(scenes.config_scene(1), 16, 2048, t0=2.0, seed=3)
     N = (12, 12, 12)
