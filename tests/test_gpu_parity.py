@@ -168,6 +168,39 @@ def test_metric_size_bins_on_sampled_voxels(gpu, oracle):
     assert np.array_equal(got, ref.astype(f32))
 
 
+def test_exhaustive_full_size_bins_on_sampled_voxels(gpu, oracle):
+    """The 16x16 lasers x 16x16 SPADs (65,536 pairs), T = 4096, 64^3 workload at full size with
+    integer sample codes (every sum exact): 1,024 random voxels equal the oracle's exactly."""
+    lg = scenes.wall_grid(16)
+    pr = oracle.precompute_pairs(scenes.ORIGIN, lg, scenes.ORIGIN, lg, confocal=False)
+    T = 4096
+    P = len(pr)
+    assert P == 65536
+    H = (np.add.outer(13 * np.arange(P, dtype=np.int32), 7 * np.arange(T, dtype=np.int32)) % 61 + 1).astype(f32)
+    N = (64, 64, 64)
+    zero, inc = grid_basis(N)
+    vol = gpu.backprojection(H, pr, N, zero, inc, 2.0, 0.001)
+    vox = sample_voxels(N, 1024, seed=6)
+    ref = oracle.a1_voxels_f64acc(H, pr, vox, zero, inc, 2.0, 0.001)
+    assert np.array_equal(vol[vox[:, 0], vox[:, 1], vox[:, 2]], ref.astype(f32))
+    assert np.count_nonzero(ref) > 900
+
+
+def test_headline_size_linearity(gpu):
+    """128x128 confocal pairs -> 128^3, T = 4096 at full size: with integer codes every voxel sum
+    is exact, so the reconstruction is exactly linear (vol(a) + vol(b) == vol(a + b))."""
+    pr = confocal_pairs(128)
+    T = 4096
+    a = coded_samples(len(pr), T, seed=3)
+    b = np.roll(coded_samples(len(pr), T, seed=8), 5, axis=1)
+    N = (128, 128, 128)
+    zero, inc = grid_basis(N)
+    va = gpu.backprojection(a, pr, N, zero, inc, 2.0, 0.001)
+    vb = gpu.backprojection(b, pr, N, zero, inc, 2.0, 0.001)
+    vab = gpu.backprojection(a + b, pr, N, zero, inc, 2.0, 0.001)
+    assert np.abs(va).max() > 0 and np.array_equal(va + vb, vab)
+
+
 # ----------------------------------------------------------------- real-valued volumes
 def test_confocal_volume_parity_rendered(gpu, oracle):
     cap = scenes.render(scenes.config_scene(1), 32, 4096, t0=2.0, seed=1001)
