@@ -149,7 +149,14 @@ def config_measurements(bp, cap, reps=5):
     c2 = scenes.render(scenes.config_scene(2), 16, 4096, confocal=False, laser_side=16, seed=1002)
     with bp.Plan.from_nlos(*c2.nlos_args(N2), wall_normal=(0, 0, 0), assume_row_major=True) as pl:
         out["C2"] = _plan_line(pl, N2, 4, "exhaustive 16x16 lasers x 16x16 SPADs -> 64^3, T=4096, f32", reps)
-    del c2
+    # the same pairs in a shuffled order: the list loses its laser x SPAD structure, so the
+    # arbitrary-pair kernel (both legs exact per update) runs instead of the grid kernel
+    pre2 = bp.nlosbpy.prepare(*c2.nlos_args(N2), (0, 0, 0), assume_row_major=True)
+    perm = np.random.default_rng(7).permutation(len(pre2["pairs"]))
+    with bp.Plan(pre2["chunk"][perm], pre2["pairs"][perm], N2, pre2["zero"], pre2["inc"], pre2["chunked_t0"],
+                 c2.deltaT) as pl:
+        out["C2_unstructured"] = _plan_line(pl, N2, 4, "C2 pairs in shuffled order (no grid structure), f32", reps)
+    del c2, pre2
     N = (128, 128, 128)
     for st, b in (("f16", 2), ("bf16", 2)):
         with bp.Plan.from_nlos(*cap.nlos_args(N), wall_normal=(0, 0, 0), assume_row_major=True,
