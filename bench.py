@@ -192,8 +192,8 @@ def log(*a):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: every box here has a single MI355X; --gpus {args.gpus} "
-              "is not supported (multi-GPU sharding is described in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one MI355X: --gpus {args.gpus} is not supported (this library runs "
+              "on a single GPU; sharding across GPUs is a design note, not code)", file=sys.stderr)
         sys.exit(2)
     import backprojector_amd as bp
     from backprojector_amd import scenes

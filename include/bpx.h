@@ -35,7 +35,7 @@ typedef enum bpx_status {
     BPX_ERR_INVALID_ARGUMENT = 1, /* shapes, sizes, non-finite geometry, bad enum values */
     BPX_ERR_OUT_OF_MEMORY = 2,    /* device or host allocation failed */
     BPX_ERR_DEVICE = 3,           /* a HIP call failed (message names it) */
-    BPX_ERR_UNSUPPORTED = 4,      /* valid request outside what this build implements */
+    BPX_ERR_UNSUPPORTED = 4,      /* valid request outside what this library implements */
     BPX_ERR_NO_DEVICE = 5         /* no gfx950 device visible */
 } bpx_status;
 
