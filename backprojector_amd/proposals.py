@@ -164,7 +164,8 @@ def compare(cap, voxels, net, configs, tau=0.25, levels=4, reps=3):
 
 
 def evaluate(net, seeds, voxels=256, configs=CANDIDATES, side=128, tau=0.25, label="validation"):
-    """C5-like captures (128^2 confocal, T=4096, 8 points + 2 patches) for each seed."""
+    """Captures like the benchmark's (128^2 confocal, T=4096, 8 points + 2 patches, 256^3) for
+    each seed."""
     per = []
     for k, seed in enumerate(seeds):
         cap = scenes.metric_capture(side=side, seed=seed)

@@ -48,7 +48,7 @@ def parse():
     ap.add_argument("--octree-proposal", type=float, default=None,
                     help="proposal threshold of the learned octree (default: the shipped choice)")
     ap.add_argument("--configs", type=int, default=1,
-                    help="also time the exhaustive (C2) and complex / 16-bit (C3) workloads (0 = skip)")
+                    help="also time the exhaustive 65,536-pair, 16-bit and complex workloads (0 = skip)")
     ap.add_argument("--one-shot", type=int, default=3,
                     help="timed one-shot nlosbpy.backproject calls (0 = skip)")
     ap.add_argument("--batch", type=int, default=64,
@@ -140,28 +140,30 @@ def _plan_line(pl, N, sample_bytes, workload, reps):
 
 
 def config_measurements(bp, cap, reps=5):
-    """The other workloads of the scope table, one plan each (device-resident, timed with HIP
-    events): C2 exhaustive pairs with exact bins; C3 complex (phasor-filtered) and 16-bit
-    storage at 128^3 from the headline capture."""
+    """The other workloads, one plan each (device-resident, timed with HIP events): the
+    exhaustive 16x16 x 16x16-pair capture into 64^3 with exact bins (grid kernel, and the same
+    pairs shuffled for the arbitrary-pair kernel); the headline capture at 128^3 with 16-bit
+    storage, in the CPU path's semantics, and phasor-filtered (complex) samples."""
     from backprojector_amd import scenes
     out = {}
     N2 = (64, 64, 64)
     c2 = scenes.render(scenes.config_scene(2), 16, 4096, confocal=False, laser_side=16, seed=1002)
     with bp.Plan.from_nlos(*c2.nlos_args(N2), wall_normal=(0, 0, 0), assume_row_major=True) as pl:
-        out["C2"] = _plan_line(pl, N2, 4, "exhaustive 16x16 lasers x 16x16 SPADs -> 64^3, T=4096, f32", reps)
+        out["exhaustive_64cube"] = _plan_line(pl, N2, 4, "exhaustive 16x16 lasers x 16x16 SPADs -> 64^3, T=4096, f32", reps)
     # the same pairs in a shuffled order: the list loses its laser x SPAD structure, so the
     # arbitrary-pair kernel (both legs exact per update) runs instead of the grid kernel
     pre2 = bp.nlosbpy.prepare(*c2.nlos_args(N2), (0, 0, 0), assume_row_major=True)
     perm = np.random.default_rng(7).permutation(len(pre2["pairs"]))
     with bp.Plan(pre2["chunk"][perm], pre2["pairs"][perm], N2, pre2["zero"], pre2["inc"], pre2["chunked_t0"],
                  c2.deltaT) as pl:
-        out["C2_unstructured"] = _plan_line(pl, N2, 4, "C2 pairs in shuffled order (no grid structure), f32", reps)
+        out["exhaustive_64cube_shuffled_pairs"] = _plan_line(
+            pl, N2, 4, "exhaustive 16x16 x 16x16 pairs in shuffled order (no grid structure) -> 64^3, f32", reps)
     del c2, pre2
     N = (128, 128, 128)
     for st, b in (("f16", 2), ("bf16", 2)):
         with bp.Plan.from_nlos(*cap.nlos_args(N), wall_normal=(0, 0, 0), assume_row_major=True,
                                storage=st) as pl:
-            out[f"C3_real_{st}"] = _plan_line(pl, N, b, f"confocal 128x128 -> 128^3, T=4096, {st} storage", reps)
+            out[f"confocal_128cube_{st}"] = _plan_line(pl, N, b, f"confocal 128x128 -> 128^3, T=4096, {st} storage", reps)
     pre = bp.nlosbpy.prepare(*cap.nlos_args(N), (0, 0, 0), assume_row_major=True)
     # the reference CPU path's semantics (voxel-diagonal window, double distances) on the GPU:
     # one-shot call from host arrays, so the copy of the chunk is included
@@ -172,7 +174,7 @@ def config_measurements(bp, cap, reps=5):
     bp.window_backprojection(*wargs)
     wms = (time.perf_counter() - t1) * 1e3
     upd = len(pre["pairs"]) * N[0] * N[1] * N[2]
-    out["cpu_path_semantics"] = {"workload": "confocal 128x128 -> 128^3, T=4096, window mode (CPU path), "
+    out["confocal_128cube_cpu_path_semantics"] = {"workload": "confocal 128x128 -> 128^3, T=4096, window mode (CPU path), "
                                              "one-shot from host arrays", "exec_ms": wms,
                                  "updates": upd, "updates_per_s": upd / (wms * 1e-3)}
     cplx = bp.phasor_pulse(pre["chunk"], 0.004, cap.deltaT)
@@ -180,7 +182,7 @@ def config_measurements(bp, cap, reps=5):
         with bp.Plan(cplx, pre["pairs"], N, pre["zero"], pre["inc"], pre["chunked_t0"], cap.deltaT,
                      storage=st) as pl:
             name = {"f32": "complex64", "f16": "complex f16", "bf16": "complex bf16"}[st]
-            out[f"C3_{name.replace(' ', '_')}"] = _plan_line(
+            out[f"phasor_128cube_{name.replace(' ', '_')}"] = _plan_line(
                 pl, N, b, f"confocal 128x128 -> 128^3, T=4096, phasor-filtered (4 mm), {name} storage", reps)
     return out
 

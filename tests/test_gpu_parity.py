@@ -348,7 +348,7 @@ def test_phasor_filter_explicit_sigma_vs_fp64(gpu):
 
 
 # ----------------------------------------------------------------- reference pipeline
-def test_config_c1_literal_and_shifted(gpu, oracle):
+def test_confocal_32x32_t2048_workload_literal_and_shifted(gpu, oracle):
     """The 32x32 confocal, T = 2048, t0 = 0, 32^3 workload as specified.  Its rendered capture is
     empty: every path through the hidden points is longer than T * deltaT = 2.048 m.  Voxel bins
     start at 2028 (the voxels nearest the wall), so with non-zero samples on the raw rows only a
