@@ -121,6 +121,7 @@ class BatchTiming(ctypes.Structure):
 
 FLAG_REFERENCE_PAIR_TRUNCATION = 0x1
 FLAG_CENTRED_WALL_BASIS = 0x2
+FLAG_COMPLEX_SAMPLES = 0x4
 OCTREE_BOUNDS_ONLY = 0x1
 
 _lib = None

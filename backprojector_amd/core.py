@@ -340,14 +340,19 @@ class Batch:
     hipGraph (no reference counterpart; each volume equals backprojection() of that capture
     alone with the same storage).  captures: iterable of (transient_chunk, pairs,
     voxels_per_side, volume_zero_pos, voxel_inc, t0, deltaT); storage "f32", "f16" or "bf16"
-    for all of them."""
+    for all of them.  Complex chunks (all or none) give complex volumes."""
 
     def __init__(self, captures, kahan_block=0, reference_pair_truncation=False, storage="f32"):
         self._keep = []
         descs = []
         self.shapes = []
+        captures = list(captures)
+        kinds = {bool(np.iscomplexobj(c[0])) for c in captures}
+        if len(kinds) > 1:
+            raise ValueError("a batch takes real or complex captures, not both")
+        self.complex = kinds == {True}
         for (chunk, pairs, N, zero, inc, t0, dT) in captures:
-            H, n = _samples(chunk, False)
+            H, n = _samples(chunk, self.complex)
             pr = as_pairs(pairs)
             P = len(pr)
             Nv = _u32(N).reshape(3)
@@ -367,6 +372,8 @@ class Batch:
             self.shapes.append(tuple(int(x) for x in Nv))
         arr = (nat.CaptureDesc * max(len(descs), 1))(*descs)
         flags = nat.FLAG_REFERENCE_PAIR_TRUNCATION if reference_pair_truncation else 0
+        if self.complex:
+            flags |= nat.FLAG_COMPLEX_SAMPLES
         opt = nat.make_options(storage, kahan_block, flags, -1)
         h = ctypes.c_void_p()
         check(lib().bpx_batch_create(arr, len(descs), ctypes.byref(opt), ctypes.byref(h)))
@@ -397,7 +404,7 @@ class Batch:
         check(lib().bpx_batch_synchronize(self._h))
 
     def volume(self, i):
-        out = np.zeros(self.shapes[i], np.float32)
+        out = np.zeros(self.shapes[i], np.complex64 if self.complex else np.float32)
         check(lib().bpx_batch_read_volume(self._h, int(i), ptr(out)))
         return out
 

@@ -27,6 +27,7 @@ struct BatchImpl {
     uint32_t n_stage = 0, stage_zb = 0, stage_sb = 0;
     uint32_t n_items = 0;
     int format = kF32;     // storage of every capture (fused rows)
+    bool cplx = false;     // complex captures and volumes
     bpx_batch_config cfg{2u, 0u, 0u};
     uint32_t groups = 0;
     hipGraph_t graph = nullptr;
@@ -128,7 +129,8 @@ void BatchImpl::build(const bpx_batch_config& c) {
         for (auto& pl : plans) {
             if (pl->P != 0) continue;
             const size_t nvox = (size_t)pl->g.n[0] * pl->g.n[1] * pl->g.n[2];
-            hip_check(launch_fill(static_cast<float*>(pl->vol.ptr), nvox, 0.0f, stream), "fill kernel");
+            hip_check(launch_fill(static_cast<float*>(pl->vol.ptr), nvox * (cplx ? 2u : 1u), 0.0f, stream),
+                      "fill kernel");
         }
         hip_check(launch_stage(static_cast<const StageJob*>(stage_jobs.ptr), n_stage, stage_zb, stage_sb, stream),
                   "batch staging kernels");
@@ -204,7 +206,12 @@ bpx_status bpx_batch_create(const bpx_capture_desc* captures, uint32_t count, co
         if (o.storage > BPX_STORAGE_BF16) fail(BPX_ERR_INVALID_ARGUMENT, "unknown storage format");
         o.variant = 5;  // fused rows: the layout the batch kernel reads
         auto b = std::make_unique<BatchImpl>();
-        b->format = o.storage == BPX_STORAGE_F16 ? kF16 : o.storage == BPX_STORAGE_BF16 ? kBF16 : kF32;
+        b->cplx = (o.flags & BPX_FLAG_COMPLEX_SAMPLES) != 0;
+        o.flags &= ~BPX_FLAG_COMPLEX_SAMPLES;
+        if (b->cplx)
+            b->format = o.storage == BPX_STORAGE_F16 ? kCF16 : o.storage == BPX_STORAGE_BF16 ? kCBF16 : kC64;
+        else
+            b->format = o.storage == BPX_STORAGE_F16 ? kF16 : o.storage == BPX_STORAGE_BF16 ? kBF16 : kF32;
         hip_check(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate");
         for (auto& e : b->ev) hip_check(hipEventCreate(&e), "hipEventCreate");
         int dev = 0, rate = 0;
@@ -213,7 +220,8 @@ bpx_status bpx_batch_create(const bpx_capture_desc* captures, uint32_t count, co
         b->wall_rate_khz = rate > 0 ? (double)rate : 1.0e5;
         for (uint32_t i = 0; i < count; ++i) {
             const bpx_capture_desc& d = captures[i];
-            CaptureIn c{d.transient_chunk, d.transient_size, d.T, d.scanned_pairs, d.num_pairs, d.t0, d.deltaT, false};
+            CaptureIn c{d.transient_chunk, d.transient_size, d.T, d.scanned_pairs, d.num_pairs, d.t0, d.deltaT,
+                        b->cplx};
             std::unique_ptr<PlanImpl> pl(make_plan_single_tap(c, d.voxels_per_side, d.volume_zero_pos, d.voxel_inc, &o));
             if (pl->P && (pl->path != kPathConfocal || !pl->fused || pl->format != b->format))
                 fail(BPX_ERR_UNSUPPORTED, "capture " + std::to_string(i) + " is not a confocal capture");
@@ -303,7 +311,7 @@ bpx_status bpx_batch_read_volume(bpx_batch batch, uint32_t index, float* voxel_v
         hip_check(hipStreamSynchronize(b.stream), "hipStreamSynchronize");
         const PlanImpl& pl = *b.plans[index];
         const size_t nvox = (size_t)pl.g.n[0] * pl.g.n[1] * pl.g.n[2];
-        hip_check(hipMemcpy(voxel_volume, pl.vol.ptr, 4u * nvox, hipMemcpyDeviceToHost), "hipMemcpy");
+        hip_check(hipMemcpy(voxel_volume, pl.vol.ptr, (b.cplx ? 8u : 4u) * nvox, hipMemcpyDeviceToHost), "hipMemcpy");
     });
 }
 

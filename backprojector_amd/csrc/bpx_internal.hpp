@@ -238,7 +238,7 @@ hipError_t launch_grid(const GridArgs& a, int format, int variant, hipStream_t s
 hipError_t launch_pairs(const PairsArgs& a, int format, int variant, hipStream_t stream);
 hipError_t launch_batch_reset(uint32_t* next, uint32_t* remaining, const uint32_t* tiles, uint32_t n,
                               uint64_t* start, hipStream_t s);
-// format: kF32, kF16 or kBF16 (fused rows); K: 2 or 4 voxels per lane.
+// format: kF32, kF16, kBF16, kC64, kCF16 or kCBF16 (fused rows); K: 2 or 4 voxels per lane.
 hipError_t launch_batch_confocal(const BatchArgs& b, int K, int format, uint32_t groups, hipStream_t s);
 int batch_resident_groups(int K, int format);  // work-groups resident at once (0 if unknown)
 hipError_t launch_window(const WindowArgs& a, int complex_samples, hipStream_t stream);

@@ -687,6 +687,9 @@ static const void* batch_kernel_of(int format) {
         case kF32: return reinterpret_cast<const void*>(batch_confocal_kernel<K, FusedSource<FusedF32>>);
         case kF16: return reinterpret_cast<const void*>(batch_confocal_kernel<K, FusedSource<FusedH16<false>>>);
         case kBF16: return reinterpret_cast<const void*>(batch_confocal_kernel<K, FusedSource<FusedH16<true>>>);
+        case kC64: return reinterpret_cast<const void*>(batch_confocal_kernel<K, FusedSource<FusedC64>>);
+        case kCF16: return reinterpret_cast<const void*>(batch_confocal_kernel<K, FusedSource<FusedC16<false>>>);
+        case kCBF16: return reinterpret_cast<const void*>(batch_confocal_kernel<K, FusedSource<FusedC16<true>>>);
         default: return nullptr;
     }
 }
@@ -719,6 +722,12 @@ hipError_t launch_batch_confocal(const BatchArgs& b, int K, int format, uint32_t
     else if (K == 2 && format == kF16) BPX_BATCH_LAUNCH(2, FusedSource<FusedH16<false>>);
     else if (K == 4 && format == kBF16) BPX_BATCH_LAUNCH(4, FusedSource<FusedH16<true>>);
     else if (K == 2 && format == kBF16) BPX_BATCH_LAUNCH(2, FusedSource<FusedH16<true>>);
+    else if (K == 4 && format == kC64) BPX_BATCH_LAUNCH(4, FusedSource<FusedC64>);
+    else if (K == 2 && format == kC64) BPX_BATCH_LAUNCH(2, FusedSource<FusedC64>);
+    else if (K == 4 && format == kCF16) BPX_BATCH_LAUNCH(4, FusedSource<FusedC16<false>>);
+    else if (K == 2 && format == kCF16) BPX_BATCH_LAUNCH(2, FusedSource<FusedC16<false>>);
+    else if (K == 4 && format == kCBF16) BPX_BATCH_LAUNCH(4, FusedSource<FusedC16<true>>);
+    else if (K == 2 && format == kCBF16) BPX_BATCH_LAUNCH(2, FusedSource<FusedC16<true>>);
     else return hipErrorInvalidValue;
 #undef BPX_BATCH_LAUNCH
     return hipGetLastError();

@@ -84,6 +84,9 @@ typedef struct bpx_options {
  * common normal (0, -1, 0) (README.md:116) and NaN axes for a normal parallel to z.  No effect
  * when the normal is zero; the identity basis results for (0, -1, 0). */
 #define BPX_FLAG_CENTRED_WALL_BASIS 0x2u
+/* bpx_batch_create: every capture's transient_chunk holds interleaved complex samples
+ * (transient_size counts complex elements) and every volume is complex (2 floats per voxel). */
+#define BPX_FLAG_COMPLEX_SAMPLES 0x4u
 
 /* ------------------------------------------------------------------ diagnostics */
 const char* bpx_last_error(void);   /* thread-local; "" when the last call succeeded */
@@ -341,7 +344,8 @@ typedef struct bpx_batch_timing {
 } bpx_batch_timing;
 typedef struct bpx_batch_s* bpx_batch;
 /* options: storage F32, F16 or BF16 for every capture (16-bit rows with a per-capture
- * power-of-two scale, as bpx_backprojection_ex); kahan_block and flags as there. */
+ * power-of-two scale, as bpx_backprojection_ex); kahan_block and flags as there, plus
+ * BPX_FLAG_COMPLEX_SAMPLES for complex captures (complex64 / complex fp16 / complex bf16 rows). */
 bpx_status bpx_batch_create(const bpx_capture_desc* captures, uint32_t count, const bpx_options* options,
                             bpx_batch* out);
 uint32_t bpx_batch_size(bpx_batch batch);

@@ -138,3 +138,23 @@ def test_cli_end_to_end(tmp_path, gpu, mode):
     else:
         ref = nlosbpy.backproject(*args, (0, 0, 0), assume_row_major=True, use_cpu=(mode == "cpu"))
         assert np.array_equal(vol, ref)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("order", ["row-major", "column-major"])
+def test_cli_exhaustive_end_to_end(tmp_path, gpu, order):
+    """A non-confocal dataset (5x5 lasers x 6x6 SPADs) through the command line, stored row- or
+    column-major: the volume equals nlosbpy.backproject on the same arrays."""
+    from backprojector_amd import nlosbpy, scenes
+    cap = scenes.render(scenes.config_scene(2), 6, 2048, t0=2.0, confocal=False, laser_side=5, seed=21)
+    data = cap.transient[:, :, :, :, None, :, None]          # [lx][ly][cx][cy][bounce][T][channel]
+    p = str(tmp_path / f"ex_{order}.hdf5")
+    h5lite.nlos_dataset(p, data, cap.camera_grid, cap.laser_grid, cap.camera_position, cap.laser_position,
+                        cap.t0, cap.deltaT, False, cap.volume_position, cap.volume_size, order=order,
+                        third_bounce=0)
+    out = str(tmp_path / "vol.hdf5")
+    r = run([p, "-r", "24,24,24", "-o", out])
+    assert f"Backprojected to {out}" in r.stdout
+    vol, _ = h5lite.read_volume(out)
+    ref = nlosbpy.backproject(*cap.nlos_args((24, 24, 24)), (0, 0, 0), assume_row_major=True)
+    assert np.abs(ref).max() > 0 and np.array_equal(vol, ref)

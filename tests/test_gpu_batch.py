@@ -63,6 +63,27 @@ def test_batch_16bit_storage_equals_single_captures(gpu, storage):
     assert np.linalg.norm(alone[4] - f32_alone) <= 2e-3 * np.linalg.norm(f32_alone)
 
 
+@pytest.mark.parametrize("storage", ["f32", "bf16"])
+def test_batch_complex_captures_equal_single_captures(gpu, storage):
+    """Phasor-filtered (complex) captures in one batch: complex volumes, each equal to the
+    capture reconstructed alone (complex64 or complex bf16 rows)."""
+    caps = []
+    for c in uneven(3)[:4]:
+        H = gpu.phasor_pulse(c[0], 0.04, c[6])
+        caps.append((H,) + tuple(c[1:]))
+    alone = [gpu.backprojection(*c, storage=storage) for c in caps]
+    with gpu.Batch(caps, storage=storage) as b:
+        for k in (2, 4):
+            b.set_config(lane_voxels=k, groups=0, order=0)
+            b.execute()
+            b.synchronize()
+            for i, ref in enumerate(alone):
+                v = b.volume(i)
+                assert v.dtype == np.complex64 and np.array_equal(v, ref), (k, i)
+    with pytest.raises(ValueError):
+        gpu.Batch([caps[0], uneven(3)[1]])
+
+
 @pytest.mark.parametrize("cfg", [dict(lane_voxels=4, groups=7, order=1),
                                  dict(lane_voxels=2, groups=1, order=0),
                                  dict(lane_voxels=4, groups=0, order=0)])
