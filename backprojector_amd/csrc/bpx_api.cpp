@@ -235,7 +235,7 @@ bpx_status bpx_octree_plan_stats(bpx_plan plan, bpx_octree_stats* stats) {
     return guarded([&] {
         if (!plan || !stats) fail(BPX_ERR_INVALID_ARGUMENT, "NULL argument");
         if (!plan->impl->oct) fail(BPX_ERR_INVALID_ARGUMENT, "not an octree plan");
-        hip_check(hipStreamSynchronize(plan->impl->stream), "hipStreamSynchronize");
+        plan_wait(*plan->impl);
         fill_octree_stats(*plan->impl, stats);
     });
 }
@@ -244,7 +244,7 @@ bpx_status bpx_octree_plan_set_options(bpx_plan plan, const bpx_octree_options* 
     return guarded([&] {
         if (!plan || !options) fail(BPX_ERR_INVALID_ARGUMENT, "NULL argument");
         if (!plan->impl->oct) fail(BPX_ERR_INVALID_ARGUMENT, "not an octree plan");
-        hip_check(hipStreamSynchronize(plan->impl->stream), "hipStreamSynchronize");
+        plan_wait(*plan->impl);
         octree_set_options(*plan->impl->oct, *options);
     });
 }
@@ -252,14 +252,24 @@ bpx_status bpx_octree_plan_set_options(bpx_plan plan, const bpx_octree_options* 
 bpx_status bpx_plan_execute(bpx_plan plan, void* hip_stream) {
     return guarded([&] {
         if (!plan) fail(BPX_ERR_INVALID_ARGUMENT, "NULL plan");
-        plan->impl->execute(hip_stream ? (hipStream_t)hip_stream : plan->impl->stream);
+        PlanImpl& pl = *plan->impl;
+        const hipStream_t s = hip_stream ? (hipStream_t)hip_stream : pl.stream;
+        // Executions of one plan share its buffers: order this one after the work queued on the
+        // plan's own stream and after the previous execution, whichever stream that used.
+        if (s != pl.stream) {
+            hip_check(hipEventRecord(pl.ready, pl.stream), "hipEventRecord");
+            hip_check(hipStreamWaitEvent(s, pl.ready, 0), "hipStreamWaitEvent");
+        }
+        hip_check(hipStreamWaitEvent(s, pl.done, 0), "hipStreamWaitEvent");
+        pl.execute(s);
+        hip_check(hipEventRecord(pl.done, s), "hipEventRecord");
     });
 }
 
 bpx_status bpx_plan_synchronize(bpx_plan plan) {
     return guarded([&] {
         if (!plan) fail(BPX_ERR_INVALID_ARGUMENT, "NULL plan");
-        hip_check(hipStreamSynchronize(plan->impl->stream), "hipStreamSynchronize");
+        plan_wait(*plan->impl);
     });
 }
 
@@ -268,7 +278,7 @@ bpx_status bpx_plan_read_volume(bpx_plan plan, float* voxel_volume) {
         if (!plan || !voxel_volume) fail(BPX_ERR_INVALID_ARGUMENT, "NULL argument");
         PlanImpl& pl = *plan->impl;
         const size_t nvox = (size_t)pl.g.n[0] * pl.g.n[1] * pl.g.n[2];
-        hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        plan_wait(pl);
         hip_check(hipMemcpy(voxel_volume, pl.vol.ptr, nvox * (pl.cplx ? 8 : 4), hipMemcpyDeviceToHost),
                   "hipMemcpy(volume)");
     });
@@ -283,7 +293,7 @@ bpx_status bpx_plan_read_thresholds(bpx_plan plan, float* out, uint64_t capacity
         const uint64_t n = (uint64_t)pl.P * (pl.T + 1u);
         if (capacity < n) fail(BPX_ERR_INVALID_ARGUMENT, "output holds fewer than pairs_used * (Tc + 1) floats");
         if (n == 0) return;
-        hip_check(hipStreamSynchronize(pl.stream), "hipStreamSynchronize");
+        plan_wait(pl);
         const DeviceBuffer& src = pl.fused ? pl.H : pl.S;
         const size_t stride = pl.fused ? (size_t)fused_elem_bytes(pl.format) / 4 : 1;
         const size_t pitch = pl.fused ? (size_t)pl.ldh * stride : pl.lds;
@@ -312,6 +322,7 @@ bpx_status bpx_plan_benchmark(bpx_plan plan, int32_t warmup, int32_t iterations,
     return guarded([&] {
         if (!plan || !out || iterations <= 0 || warmup < 0) fail(BPX_ERR_INVALID_ARGUMENT, "bad arguments");
         PlanImpl& pl = *plan->impl;
+        plan_wait(pl);
         for (int i = 0; i < warmup; ++i) pl.execute(pl.stream);
         hip_check(hipStreamSynchronize(pl.stream), "hipStreamSynchronize");
         double table_ms = 0, gather_ms = 0;
@@ -339,6 +350,7 @@ bpx_status bpx_plan_benchmark(bpx_plan plan, int32_t warmup, int32_t iterations,
 void bpx_plan_destroy(bpx_plan plan) {
     if (!plan) return;
     (void)hipStreamSynchronize(plan->impl->stream);
+    if (plan->impl->done) (void)hipEventSynchronize(plan->impl->done);
     delete plan;
 }
 

@@ -21,6 +21,7 @@ struct BatchImpl {
     std::vector<uint64_t> updates;   // voxel-pair updates per capture
     hipStream_t stream = nullptr;
     hipEvent_t ev[2] = {};
+    hipEvent_t done = nullptr, ready = nullptr;   // as PlanImpl::done / ready
     DeviceBuffer caps, flags, items, next, remaining, tiles;
     DeviceBuffer stamps;   // uint64 [1 + captures]: batch start, then each capture's completion
     DeviceBuffer stage_jobs;                      // StageJob per capture with pairs
@@ -38,9 +39,16 @@ struct BatchImpl {
         if (graph) (void)hipGraphDestroy(graph);
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
+        if (done) (void)hipEventDestroy(done);
+        if (ready) (void)hipEventDestroy(ready);
         if (stream) (void)hipStreamDestroy(stream);
     }
     void build(const bpx_batch_config& c);
+    // Waits for the batch's own stream and for its last execution on a caller's stream.
+    void wait() const {
+        hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        hip_check(hipEventSynchronize(done), "hipEventSynchronize");
+    }
 };
 
 void BatchImpl::build(const bpx_batch_config& c) {
@@ -214,6 +222,8 @@ bpx_status bpx_batch_create(const bpx_capture_desc* captures, uint32_t count, co
             b->format = o.storage == BPX_STORAGE_F16 ? kF16 : o.storage == BPX_STORAGE_BF16 ? kBF16 : kF32;
         hip_check(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate");
         for (auto& e : b->ev) hip_check(hipEventCreate(&e), "hipEventCreate");
+        hip_check(hipEventCreateWithFlags(&b->done, hipEventDisableTiming), "hipEventCreate");
+        hip_check(hipEventCreateWithFlags(&b->ready, hipEventDisableTiming), "hipEventCreate");
         int dev = 0, rate = 0;
         hip_check(hipGetDevice(&dev), "hipGetDevice");
         hip_check(hipDeviceGetAttribute(&rate, hipDeviceAttributeWallClockRate, dev), "hipDeviceGetAttribute");
@@ -242,13 +252,14 @@ bpx_status bpx_batch_create(const bpx_capture_desc* captures, uint32_t count, co
 void bpx_batch_destroy(bpx_batch batch) {
     if (!batch) return;
     (void)hipStreamSynchronize(batch->impl->stream);
+    if (batch->impl->done) (void)hipEventSynchronize(batch->impl->done);
     delete batch;
 }
 
 bpx_status bpx_batch_set_config(bpx_batch batch, const bpx_batch_config* config) {
     return batch_guard([&] {
         if (!batch || !config) fail(BPX_ERR_INVALID_ARGUMENT, "NULL argument");
-        hip_check(hipStreamSynchronize(batch->impl->stream), "hipStreamSynchronize");
+        batch->impl->wait();
         batch->impl->build(*config);
     });
 }
@@ -264,7 +275,7 @@ bpx_status bpx_batch_tune(bpx_batch batch, int32_t reps, bpx_batch_config* chose
     return batch_guard([&] {
         if (!batch) fail(BPX_ERR_INVALID_ARGUMENT, "NULL batch");
         BatchImpl& b = *batch->impl;
-        hip_check(hipStreamSynchronize(b.stream), "hipStreamSynchronize");
+        b.wait();
         reps = std::max(reps, 1);
         bpx_batch_config best{2u, 0u, 0u};
         double best_ms = INFINITY;
@@ -291,15 +302,22 @@ bpx_status bpx_batch_execute(bpx_batch batch, void* hip_stream) {
     return batch_guard([&] {
         if (!batch) fail(BPX_ERR_INVALID_ARGUMENT, "NULL batch");
         BatchImpl& b = *batch->impl;
-        hipStream_t s = hip_stream ? (hipStream_t)hip_stream : b.stream;
+        const hipStream_t s = hip_stream ? (hipStream_t)hip_stream : b.stream;
+        // one set of buffers: after the work on the batch's stream and the previous execution
+        if (s != b.stream) {
+            hip_check(hipEventRecord(b.ready, b.stream), "hipEventRecord");
+            hip_check(hipStreamWaitEvent(s, b.ready, 0), "hipStreamWaitEvent");
+        }
+        hip_check(hipStreamWaitEvent(s, b.done, 0), "hipStreamWaitEvent");
         hip_check(hipGraphLaunch(b.exec, s), "hipGraphLaunch");
+        hip_check(hipEventRecord(b.done, s), "hipEventRecord");
     });
 }
 
 bpx_status bpx_batch_synchronize(bpx_batch batch) {
     return batch_guard([&] {
         if (!batch) fail(BPX_ERR_INVALID_ARGUMENT, "NULL batch");
-        hip_check(hipStreamSynchronize(batch->impl->stream), "hipStreamSynchronize");
+        batch->impl->wait();
     });
 }
 
@@ -308,7 +326,7 @@ bpx_status bpx_batch_read_volume(bpx_batch batch, uint32_t index, float* voxel_v
         if (!batch || !voxel_volume) fail(BPX_ERR_INVALID_ARGUMENT, "NULL argument");
         BatchImpl& b = *batch->impl;
         if (index >= b.plans.size()) fail(BPX_ERR_INVALID_ARGUMENT, "capture index out of range");
-        hip_check(hipStreamSynchronize(b.stream), "hipStreamSynchronize");
+        b.wait();
         const PlanImpl& pl = *b.plans[index];
         const size_t nvox = (size_t)pl.g.n[0] * pl.g.n[1] * pl.g.n[2];
         hip_check(hipMemcpy(voxel_volume, pl.vol.ptr, (b.cplx ? 8u : 4u) * nvox, hipMemcpyDeviceToHost), "hipMemcpy");
@@ -319,7 +337,7 @@ bpx_status bpx_batch_latencies(bpx_batch batch, double* ms) {
     return batch_guard([&] {
         if (!batch || !ms) fail(BPX_ERR_INVALID_ARGUMENT, "NULL argument");
         BatchImpl& b = *batch->impl;
-        hip_check(hipStreamSynchronize(b.stream), "hipStreamSynchronize");
+        b.wait();
         const size_t n = b.plans.size();
         std::vector<uint64_t> st(n + 1);
         hip_check(hipMemcpy(st.data(), b.stamps.ptr, 8u * (n + 1), hipMemcpyDeviceToHost), "hipMemcpy");
@@ -332,7 +350,7 @@ bpx_status bpx_batch_benchmark(bpx_batch batch, int32_t warmup, int32_t iteratio
     return batch_guard([&] {
         if (!batch || !out) fail(BPX_ERR_INVALID_ARGUMENT, "NULL argument");
         BatchImpl& b = *batch->impl;
-        hip_check(hipStreamSynchronize(b.stream), "hipStreamSynchronize");
+        b.wait();
         if (warmup > 0) (void)run_timed(b, warmup);
         const int32_t it = std::max(iterations, 1);
         out->per_exec_ms = run_timed(b, it);

@@ -95,10 +95,15 @@ struct PlanImpl {
     uint32_t rows_in = 0;                // rows of the capture the plan was made from
     uint32_t stage_zb = 0, stage_sb = 0; // its blocks in the two staging passes
     hipEvent_t ev[8] = {};
+    hipEvent_t done = nullptr;           // recorded after each bpx_plan_execute, on its stream
+    hipEvent_t ready = nullptr;          // work queued on the plan's own stream (caller-stream runs)
     std::unique_ptr<OctreeState> oct;
     ~PlanImpl();
     void execute(hipStream_t s);
 };
+
+// Waits for the plan's own stream and for its last execution on a caller's stream.
+void plan_wait(const PlanImpl& pl);
 
 void execute_octree(PlanImpl& pl, hipStream_t s);
 

@@ -32,6 +32,8 @@ void DeviceBuffer::release() {
 PlanImpl::~PlanImpl() {
     for (auto& e : ev)
         if (e) (void)hipEventDestroy(e);
+    if (done) (void)hipEventDestroy(done);
+    if (ready) (void)hipEventDestroy(ready);
     if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -289,6 +291,13 @@ static void stage_raw(PlanImpl& pl, const CaptureIn& c, const std::vector<uint32
 
 static void make_events(PlanImpl& pl) {
     for (auto& e : pl.ev) hip_check(hipEventCreate(&e), "hipEventCreate");
+    hip_check(hipEventCreateWithFlags(&pl.done, hipEventDisableTiming), "hipEventCreate");
+    hip_check(hipEventCreateWithFlags(&pl.ready, hipEventDisableTiming), "hipEventCreate");
+}
+
+void plan_wait(const PlanImpl& pl) {
+    hip_check(hipStreamSynchronize(pl.stream), "hipStreamSynchronize");
+    if (pl.done) hip_check(hipEventSynchronize(pl.done), "hipEventSynchronize");
 }
 
 // Exhaustive-grid structure: p = l*C + c with laser data depending only on l and camera

@@ -191,6 +191,20 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def host_cpu():
+    """Host processor model and logical CPU count (from /proc/cpuinfo where present)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus": os.cpu_count()}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -286,7 +300,23 @@ def main():
                "kind": "port",
                "sample": f"reference CPU path (voxel-diagonal window, double distances) over "
                          f"{slabs} of {N[0]} x-slabs ({slab * slabs} voxels x {P} pairs) in "
-                         f"{dt:.1f} s on {nth} threads"}
+                         f"{dt:.1f} s on {nth} threads",
+               "host": host_cpu()}
+        # like-for-like: the single-tap formula of the GPU path, restated on the CPU (one bin per
+        # update, fp32 taps, double accumulation), over a bounded voxel sample
+        vox_all = np.stack(np.meshgrid(*[np.arange(n) for n in N], indexing="ij"), -1).reshape(-1, 3)
+        n1 = min(len(vox_all), slab)
+        t1 = time.perf_counter()
+        orc.a1_voxels_f64acc(H, pairs, vox_all[:n1], zero, inc, ct0, cap.deltaT)
+        one = time.perf_counter() - t1
+        n2 = int(max(n1, min(len(vox_all), n1 * (0.5 * args.cpu_seconds) / max(one, 1e-3))))
+        t1 = time.perf_counter()
+        orc.a1_voxels_f64acc(H, pairs, vox_all[:n2], zero, inc, ct0, cap.deltaT)
+        dt = time.perf_counter() - t1
+        cpu["single_tap"] = {"value": n2 * P / dt, "unit": "voxel-pair updates/s", "cores": nth,
+                             "sample": f"single-tap formula of the GPU path (fp32 bins, double sums) "
+                                       f"over the first {n2} voxels x {P} pairs in {dt:.1f} s on "
+                                       f"{nth} threads"}
         log(f"cpu baseline: {cpu}")
 
     configs = None

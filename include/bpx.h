@@ -4,7 +4,10 @@
  * Plain C: POD structs, pointers and sizes, status codes.  No function calls exit(); every
  * failure returns a bpx_status and leaves a message for bpx_last_error().  Host-buffer entry
  * points are synchronous; device-buffer entry points (plans, batches) enqueue on the stream
- * given (NULL = the library's own stream) and synchronise only where documented.
+ * given (NULL = the library's own stream) and synchronise only where documented.  Executions
+ * of one plan or batch share its buffers, so each is ordered (stream-side, no host wait) after
+ * the previous one, whatever stream that used; the synchronize, read, stats and destroy calls
+ * wait for the last execution on any stream.
  *
  * Reference interfaces replaced (mjgalindo/backprojector):
  *   bpx_backprojection          <- call_cuda_backprojection          backproject_cuda.hpp:19-28

@@ -31,3 +31,23 @@ def gpu(bpx):
     n = bpx.device_count()
     assert n >= 1, "no gfx950 device visible: GPU tests need the HIP path"
     return bpx
+
+
+@pytest.fixture
+def hip_streams(gpu):
+    """Two HIP streams of the caller (default priority), created through the HIP runtime that
+    libbpx is linked against (already loaded, so the same runtime instance)."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    hip.hipStreamCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+    hip.hipStreamSynchronize.argtypes = [ctypes.c_void_p]
+    hip.hipStreamDestroy.argtypes = [ctypes.c_void_p]
+    streams = []
+    for _ in range(2):
+        s = ctypes.c_void_p()
+        assert hip.hipStreamCreate(ctypes.byref(s)) == 0
+        streams.append(s.value)
+    yield streams
+    for s in streams:
+        assert hip.hipStreamSynchronize(s) == 0
+        assert hip.hipStreamDestroy(s) == 0

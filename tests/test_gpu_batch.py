@@ -46,6 +46,20 @@ def test_batch_equals_single_captures(gpu):
         assert np.all(lat > 0) and np.all(np.isfinite(lat))
 
 
+def test_batch_execute_on_caller_stream(gpu, hip_streams):
+    caps = uneven(5)[:3]
+    alone = [gpu.backprojection(*c) for c in caps]
+    s1, s2 = hip_streams
+    with gpu.Batch(caps) as b:
+        for rep in range(2):
+            b.execute(s1)
+            b.execute(s2)
+            for i, ref in enumerate(alone):   # the read waits for the execution on s2
+                assert np.array_equal(b.volume(i), ref), (rep, i)
+            assert np.all(b.latencies() > 0)
+        b.execute(s1)
+
+
 @pytest.mark.parametrize("storage", ["f16", "bf16"])
 def test_batch_16bit_storage_equals_single_captures(gpu, storage):
     """fp16 / bf16 rows with a per-capture scale: each batched volume equals the capture

@@ -612,3 +612,24 @@ def test_determinism_and_plan_reuse(gpu):
     assert np.array_equal(a, b) and np.array_equal(b, c)
     assert info["path"] == "confocal" and info["pairs_used"] == len(pr)
     assert t["gather_kernel_ms"] > 0 and t["table_kernel_ms"] > 0
+
+
+def test_plan_execute_on_caller_stream(gpu, hip_streams):
+    """An execution enqueued on a caller's HIP stream is ordered after the plan's earlier work
+    and is waited for by the plan's read, synchronize and destroy calls."""
+    pr = confocal_pairs(32)
+    rng = np.random.default_rng(3)
+    H = rng.poisson(3.0, (len(pr), 2048)).astype(f32)
+    N = (32, 32, 32)
+    zero, inc = grid_basis(N)
+    ref = gpu.backprojection(H, pr, N, zero, inc, 2.0, 0.001)
+    s1, s2 = hip_streams
+    with gpu.Plan(H, pr, N, zero, inc, 2.0, 0.001) as plan:
+        plan.execute(s1)
+        plan.execute(s2)          # after the execution on s1
+        assert np.array_equal(plan.volume(), ref)
+        plan.execute()                        # own stream, after the one on s2
+        plan.execute(s1)
+        plan.synchronize()
+        assert np.array_equal(plan.volume(), ref)
+        plan.execute(s2)          # destroyed while possibly still running
