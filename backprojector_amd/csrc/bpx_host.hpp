@@ -83,6 +83,7 @@ struct PlanImpl {
     Grid g{};
     Tiles tiles{};
     bool sep = false, flat = false, fused = false;
+    bool lds_tiles = false;              // the LDS-staged confocal kernel runs (16 voxels per lane)
     float lz = 0;
     uint32_t kahan_block = 32;
     float inv2 = 0, inv = 0, c1 = 0;

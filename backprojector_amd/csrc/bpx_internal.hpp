@@ -246,7 +246,13 @@ hipError_t launch_abs_max(const float* src, uint64_t n, float* result_dev, hipSt
 hipError_t launch_copy(void* dst, const void* src, uint64_t bytes, hipStream_t stream);
 
 // Voxels per lane (K) of each tuned variant; variant -1 picks the default.
-constexpr int kDefaultVariant = 5;  // fused layout, 2 voxels per lane (fastest measured)
+constexpr int kDefaultVariant = 5;  // fused layout, 2 voxels per lane (where the LDS kernel does not apply)
+constexpr int kLdsVariant = 6;      // LDS-staged 16 x 16 x 16 tiles (fused fp32, separable, flat);
+                                    // kLdsVariant + 1: 16 x 16 x 8 tiles
+constexpr uint64_t kLdsDeepTiles = 1024;  // 16-deep tiles from this many 16^3 tiles up
+// Whether the LDS-staged kernel's per-pair bin windows fit its segment for this grid spacing.
+bool confocal_lds_fits(const float inc[9], float inv2, int kz);
+int lds_variant_kz(int variant);
 int variant_k(int variant);
 int variant_count();
 

@@ -11,6 +11,7 @@
 // host guarantees |approximate - exact quotient| < 0.5 (bpx_api.cpp, tap_error_bound), so the
 // selected bin is bit-identical to the fp32 formula for every voxel and pair.
 #include <algorithm>
+#include <cmath>
 #include <type_traits>
 
 #include "bpx_device.hpp"
@@ -612,6 +613,189 @@ __global__ __launch_bounds__(256) void confocal_kernel(ConfocalArgs a) {
     confocal_tile<Src, K, SEP, FLAT>(a, xcd_remap(blockIdx.x, gridDim.x), threadIdx.x >> 6);
 }
 
+// ------------------------------------------------------------------- LDS-staged confocal gather
+// fp32 fused rows, separable grid, flat wall.  A 256-lane work-group owns a 16 x 16 x KZ voxel
+// tile (lane = one (x, y) column, KZ voxels along z).  For every pair, the bin guesses of all
+// the tile's voxels lie in a short window [jlo, jhi]: the tile's bounding box gives the nearest
+// and farthest squared distances, and the window is widened by 2 bins on each side, far more
+// than the approximate quotient's error.  The window's fused elements are copied to LDS with
+// coalesced loads and the voxel lookups read LDS instead of the texture path.  Four pairs per
+// stage (one per wave), double-buffered: the loads for stage b + 1 are in flight while the
+// waves gather stage b.
+//
+// The guesses are not clamped to [0, Tc]: elements outside the row (index < 0 or > Tc + 1) are
+// staged as zeros.  A guess j picks sample j or j - 1; a guess below 0 (quotient <= -1) or
+// above Tc (quotient >= Tc + 1) can only stand for a bin outside the capture, whose value is 0,
+// and inside [0, Tc] the guess equals the clamped one.  So every update adds the same value as
+// in confocal_kernel, in the same pair order with the same Kahan blocks: the volumes are
+// bit-identical.
+constexpr uint32_t kSegElems = 512;   // staged fused elements per pair (8 B each)
+constexpr uint32_t kSegPairs = 4;     // pairs per stage (one per wave)
+constexpr uint32_t kSegTileXY = 16;   // tile edge in x and y
+
+template <int KZ>
+__global__ __launch_bounds__(256) void confocal_lds_kernel(ConfocalArgs a) {
+    __shared__ uint2 seg[2][kSegPairs][kSegElems];
+    __shared__ int meta[2][kSegPairs][2];   // window start, staged element count
+    const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63u;
+    const uint32_t n0 = a.g.n[0], n1 = a.g.n[1], n2 = a.g.n[2];
+    const uint32_t ntz = (n2 + KZ - 1u) / KZ, nty = (n1 + kSegTileXY - 1u) / kSegTileXY;
+    const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t tz = tile % ntz, txy = tile / ntz, ty = txy % nty, tx = txy / nty;
+    const uint32_t ix = tx * kSegTileXY + (w & 1u) * 8u + (lane & 7u);
+    const uint32_t iy = ty * kSegTileXY + (w >> 1) * 8u + (lane >> 3);
+    const uint32_t cx = min(ix, n0 - 1u), cy = min(iy, n1 - 1u);
+    float vx = 0.0f, vy = 0.0f, dz2[KZ];
+#pragma unroll
+    for (int r = 0; r < KZ; ++r) {
+        float v[3];
+        voxel_position(a.g, cx, cy, min(tz * KZ + r, n2 - 1u), v);
+        if (r == 0) {
+            vx = v[0];
+            vy = v[1];
+        }
+        const float dz = a.lz - v[2];
+        dz2[r] = dz * dz;
+    }
+    // bounding box of the tile's voxel positions: each coordinate is a sum of terms monotone
+    // in one index each (rounding is monotone), so its extremes are at the 8 corners
+    const uint32_t x0 = min(tx * kSegTileXY, n0 - 1u), x1 = min(tx * kSegTileXY + kSegTileXY - 1u, n0 - 1u);
+    const uint32_t y0 = min(ty * kSegTileXY, n1 - 1u), y1 = min(ty * kSegTileXY + kSegTileXY - 1u, n1 - 1u);
+    const uint32_t z0 = min(tz * KZ, n2 - 1u), z1 = min(tz * KZ + KZ - 1u, n2 - 1u);
+    float bmin[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
+    float bmax[3] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        float v[3];
+        voxel_position(a.g, (c & 1) ? x1 : x0, (c & 2) ? y1 : y0, (c & 4) ? z1 : z0, v);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            bmin[k] = fminf(bmin[k], v[k]);
+            bmax[k] = fmaxf(bmax[k], v[k]);
+        }
+    }
+    const int last_elem = (int)a.Tc + 1;   // H[p][Tc + 1] = 0 ends every row
+    const uint32_t row_bytes = (a.Tc + 2u) * 8u;
+    const __amdgpu_buffer_rsrc_t er = make_rsrc(a.H, a.h_bytes);
+    // the wall is flat: the z terms of the box distances are the same for every pair
+    const float ez = a.lz - fminf(fmaxf(a.lz, bmin[2]), bmax[2]);
+    const float fz = fmaxf(fabsf(a.lz - bmin[2]), fabsf(a.lz - bmax[2]));
+    const float ez2 = ez * ez, fz2 = fz * fz;
+
+    // Stage registers of this wave's pair: window and kSegElems / 64 elements per lane.
+    uint2 pre[kSegElems / 64u];
+    int pjlo = 0, pcnt = 0;
+    auto fetch = [&](uint32_t p0) {
+        const uint32_t p = p0 + w;
+        pjlo = 0;
+        pcnt = 0;
+        if (p < a.P) {
+            const float4 pr = a.pairs[p];
+            const float ex = pr.x - fminf(fmaxf(pr.x, bmin[0]), bmax[0]);
+            const float ey = pr.y - fminf(fmaxf(pr.y, bmin[1]), bmax[1]);
+            const float fx = fmaxf(fabsf(pr.x - bmin[0]), fabsf(pr.x - bmax[0]));
+            const float fy = fmaxf(fabsf(pr.y - bmin[1]), fabsf(pr.y - bmax[1]));
+            const float smin = (ex * ex + ey * ey) + ez2;
+            const float smax = (fx * fx + fy * fy) + fz2;
+            const int jl = (int)__builtin_fmaf(__builtin_amdgcn_sqrtf(smin), a.inv2, pr.w) - 2;
+            const int jh = (int)__builtin_fmaf(__builtin_amdgcn_sqrtf(smax), a.inv2, pr.w) + 2;
+            pjlo = __builtin_amdgcn_readfirstlane(jl);
+            pcnt = __builtin_amdgcn_readfirstlane(min(jh + 2 - jl, (int)kSegElems));   // elements jl .. jh + 1
+        }
+        if (p < a.P && pjlo >= 0) {
+            // Window inside or past the row: a descriptor over row p alone returns 0 for every
+            // element past H[p][Tc + 1], so no lane needs a bounds test; elements past the
+            // window are loaded too and never read.
+            const __amdgpu_buffer_rsrc_t rr =
+                make_rsrc(static_cast<const char*>(a.H) + (size_t)p * a.ldh * 8u, row_bytes);
+            const int vo = (pjlo + (int)lane) * 8;
+#pragma unroll
+            for (uint32_t m = 0; m < kSegElems / 64u; ++m) {
+                const auto t = __builtin_amdgcn_raw_buffer_load_b64(rr, vo + (int)(512u * m), 0, 0);
+                pre[m] = make_uint2(t[0], t[1]);
+            }
+        } else {
+            // window starting before the row (or no pair): elements below 0 are zeros
+            const uint32_t soff = p < a.P ? p * a.ldh * 8u : 0u;
+#pragma unroll
+            for (uint32_t m = 0; m < kSegElems / 64u; ++m) {
+                const int e = (int)(lane + 64u * m), idx = pjlo + e;
+                uint2 v = make_uint2(0u, 0u);
+                if (e < pcnt && idx >= 0 && idx <= last_elem) {
+                    const auto t = __builtin_amdgcn_raw_buffer_load_b64(er, idx * 8, (int)soff, 0);
+                    v = make_uint2(t[0], t[1]);
+                }
+                pre[m] = v;
+            }
+        }
+    };
+    auto commit = [&](uint32_t buf) {
+#pragma unroll
+        for (uint32_t m = 0; m < kSegElems / 64u; ++m) seg[buf][w][lane + 64u * m] = pre[m];
+        if (lane == 0) {
+            meta[buf][w][0] = pjlo;
+            meta[buf][w][1] = pcnt;
+        }
+    };
+
+    float sum[KZ], comp[KZ], blk[KZ];
+#pragma unroll
+    for (int r = 0; r < KZ; ++r) sum[r] = comp[r] = blk[r] = 0.0f;
+    const uint32_t nstage = (a.P + kSegPairs - 1u) / kSegPairs;
+    const char* segb = reinterpret_cast<const char*>(&seg[0][0][0]);
+    uint32_t in_block = 0;
+    fetch(0u);
+    commit(0u);
+    __syncthreads();
+    for (uint32_t b = 0; b < nstage; ++b) {
+        const uint32_t buf = b & 1u;
+        if (b + 1u < nstage) fetch((b + 1u) * kSegPairs);
+        const uint32_t p0 = b * kSegPairs, np = min(kSegPairs, a.P - p0);
+        for (uint32_t i = 0; i < np; ++i) {
+            const float4 pr = a.pairs[p0 + i];
+            // byte offset of element j of this pair's window = 8 j + base (unsigned wrap-around);
+            // the window start is the same for every lane, so base lives in a scalar register
+            const uint32_t base = __builtin_amdgcn_readfirstlane(
+                ((buf * kSegPairs + i) * kSegElems - (uint32_t)meta[buf][i][0]) * 8u);
+            const float dx = pr.x - vx, dy = pr.y - vy;
+            const float axy = dx * dx + dy * dy;
+#pragma unroll
+            for (int r = 0; r < KZ; ++r) {
+                const float s = axy + dz2[r];
+                const int j = (int)__builtin_fmaf(__builtin_amdgcn_sqrtf(s), a.inv2, pr.w);
+                const char* e = segb + ((uint32_t)j * 8u + base);
+                const uint2 e0 = *reinterpret_cast<const uint2*>(e);
+                const float h1 = *reinterpret_cast<const float*>(e + 8);
+                blk[r] = blk[r] + (s >= __uint_as_float(e0.y) ? h1 : __uint_as_float(e0.x));
+            }
+            if (++in_block == a.kahan_block) {
+#pragma unroll
+                for (int r = 0; r < KZ; ++r) {
+                    kahan_add(sum[r], comp[r], blk[r]);
+                    blk[r] = 0.0f;
+                }
+                in_block = 0;
+            }
+        }
+        if (b + 1u < nstage) commit(buf ^ 1u);
+        __syncthreads();
+    }
+    if (in_block) {
+#pragma unroll
+        for (int r = 0; r < KZ; ++r) kahan_add(sum[r], comp[r], blk[r]);
+    }
+    if (ix < n0 && iy < n1) {
+#pragma unroll
+        for (int r = 0; r < KZ; ++r) {
+            const uint32_t iz = tz * KZ + r;
+            if (iz < n2) {
+                const size_t id = ((size_t)ix * n1 + iy) * n2 + iz;
+                store_voxel(a.out, id, kahan_result(sum[r], comp[r]) * a.scale);
+            }
+        }
+    }
+}
+
 // ------------------------------------------------------------------- batched captures
 // Persistent work queue over (capture, tile quarter) items of many confocal fp32 captures:
 // every WAVE takes the next item, runs the same tile body as the single-capture kernel (each
@@ -1020,7 +1204,9 @@ hipError_t launch_window(const WindowArgs& a, int complex_samples, hipStream_t s
 
 // ------------------------------------------------------------------- dispatch
 // Variants v = 3*layout + k: layout 0 = split tables, 1 = fused layout; k: 0 -> K = 4,
-// 1 -> K = 8, 2 -> K = 2 voxels per lane.
+// 1 -> K = 8, 2 -> K = 2 voxels per lane.  Variants 6 and 7: the LDS-staged kernel with
+// 16 x 16 x 16 / 16 x 16 x 8 tiles for fp32 fused rows on a separable grid and a flat wall when
+// the bin windows fit, otherwise the fused layout with K = 4 / 8.
 int variant_k(int variant) {
     switch (variant % 3) {
         case 1: return 8;
@@ -1029,7 +1215,8 @@ int variant_k(int variant) {
     }
 }
 bool variant_fused(int variant) { return variant >= 3; }
-int variant_count() { return 6; }
+int variant_count() { return 8; }
+int lds_variant_kz(int variant) { return variant == 7 ? 8 : 16; }
 int fused_elem_bytes(int format) {
     switch (format) {
         case kF32: case kF16: case kBF16: return 8;
@@ -1063,8 +1250,31 @@ static hipError_t confocal_dispatch(const ConfocalArgs& a, int variant, bool sep
     }
 }
 
+// Whether every pair's bin window over a 16 x 16 x kz tile fits the staged segment: the
+// window spans at most (farthest - nearest distance) * inv2 <= tile diagonal * inv2 bins, plus
+// the 2-bin margins on each side and the extra element for the upper sample.
+bool confocal_lds_fits(const float inc[9], float inv2, int kz) {
+    double d2 = 0.0;
+    for (int k = 0; k < 3; ++k) {
+        const double r = (kSegTileXY - 1.0) * (std::fabs((double)inc[k]) + std::fabs((double)inc[3 + k])) +
+                         (kz - 1.0) * std::fabs((double)inc[6 + k]);
+        d2 += r * r;
+    }
+    const double need = std::ceil(std::sqrt(d2) * std::fabs((double)inv2) * (1.0 + 1e-6)) + 8.0;
+    return std::isfinite(need) && need <= (double)kSegElems;
+}
+
 hipError_t launch_confocal(const ConfocalArgs& a, int format, int variant, bool sep, bool flat,
                            hipStream_t stream) {
+    if (variant >= kLdsVariant && format == kF32 && sep && flat && confocal_lds_fits(a.g.inc, a.inv2, lds_variant_kz(variant))) {
+        const uint32_t kz = (uint32_t)lds_variant_kz(variant);
+        const uint32_t n = ((a.g.n[0] + kSegTileXY - 1u) / kSegTileXY) * ((a.g.n[1] + kSegTileXY - 1u) / kSegTileXY) *
+                           ((a.g.n[2] + kz - 1u) / kz);
+        const dim3 grid(n), block(256);
+        if (kz == 16) hipLaunchKernelGGL((confocal_lds_kernel<16>), grid, block, 0, stream, a);
+        else hipLaunchKernelGGL((confocal_lds_kernel<8>), grid, block, 0, stream, a);
+        return hipGetLastError();
+    }
     if (variant_fused(variant)) {
         switch (format) {
             case kF32: return confocal_dispatch<FusedSource<FusedF32>>(a, variant, sep, flat, stream);

@@ -238,7 +238,7 @@ void plan_replace_rows(PlanImpl& pl, DeviceBuffer& rows) {
     if (pl.P == 0 || !pl.raw.ptr) return;   // nothing staged: every execution writes zeros
     const size_t row = (size_t)pl.T * (pl.cplx ? 2 : 1);
     if (rows.bytes < sizeof(float) * row * pl.rows_in) fail(BPX_ERR_INVALID_ARGUMENT, "replacement rows too small");
-    hip_check(hipStreamSynchronize(pl.stream), "hipStreamSynchronize");
+    plan_wait(pl);
     adopt_device_rows(pl, rows, pl.rows_in, row, pl.used);
     if (pl.stage.ptr) {   // the staging descriptor names the raw rows: point it at the new ones
         const StageJob job = stage_job(pl);
@@ -429,6 +429,17 @@ PlanImpl* make_plan_single_tap(const CaptureIn& c, const uint32_t n[3], const fl
         pl->lz = pr[0].laser_point[2];
         pl->sep = inc[6] == 0.0f && inc[7] == 0.0f;
         if (!pl->sep) pl->flat = false;
+        // Default kernel: the LDS-staged one where it applies (fp32, separable grid, flat wall,
+        // short bin windows); 16-deep tiles when there are enough of them to fill the GPU
+        // (4 work-groups per CU), else 8-deep tiles (twice the work-groups).
+        if (o.variant < 0 && pl->format == kF32 && pl->fused && pl->sep && pl->flat) {
+            const uint64_t t16 = (uint64_t)((n[0] + 15) / 16) * ((n[1] + 15) / 16) * ((n[2] + 15) / 16);
+            const int deep = t16 >= kLdsDeepTiles ? kLdsVariant : kLdsVariant + 1;
+            if (confocal_lds_fits(inc, pl->inv2, lds_variant_kz(deep))) pl->variant = deep;
+            else if (confocal_lds_fits(inc, pl->inv2, lds_variant_kz(kLdsVariant + 1))) pl->variant = kLdsVariant + 1;
+        }
+        pl->lds_tiles = pl->variant >= kLdsVariant && pl->format == kF32 && pl->fused && pl->sep && pl->flat &&
+                        confocal_lds_fits(inc, pl->inv2, lds_variant_kz(pl->variant));
         pl->pair4.alloc(sizeof(float) * p4.size());
         hip_check(hipMemcpy(pl->pair4.ptr, p4.data(), sizeof(float) * p4.size(), hipMemcpyHostToDevice),
                   "hipMemcpy(pair table)");

@@ -74,7 +74,10 @@ typedef struct bpx_options {
     uint32_t storage;     /* bpx_storage (default F32)                                       */
     uint32_t kahan_block; /* pairs summed plainly between compensated adds (0 -> 32)          */
     uint32_t flags;       /* BPX_FLAG_*                                                       */
-    int32_t variant;      /* gather-kernel variant (-1 -> autotuned/default)                  */
+    int32_t variant;      /* gather-kernel variant: -1 default (the LDS-staged kernel where it
+                             applies: fp32 real samples, separable grid, flat wall, short bin
+                             windows; else 5); 0-2 split tables, 3-5 fused rows with 4, 8, 2
+                             voxels per lane; 6 / 7 LDS-staged 16x16x16 / 16x16x8 tiles    */
 } bpx_options;
 
 /* Drop the last P mod min(256, P) pairs exactly as the reference launcher does
@@ -286,7 +289,8 @@ bpx_status bpx_plan_execute(bpx_plan plan, void* hip_stream);
 bpx_status bpx_plan_synchronize(bpx_plan plan);
 /* Copies the device volume to host (float or interleaved complex). */
 bpx_status bpx_plan_read_volume(bpx_plan plan, float* voxel_volume);
-/* Describes the plan: kernel path (0 confocal, 1 grid, 2 pairs), voxels per lane, number of
+/* Describes the plan: kernel path (0 confocal, 1 grid, 2 pairs), voxels per lane (the tile
+ * depth, 8 or 16, for the LDS-staged kernel), number of
  * pairs used, chunk length, per-capture scale. */
 bpx_status bpx_plan_info(bpx_plan plan, int32_t* path, int32_t* lane_voxels, uint32_t* pairs_used,
                          uint32_t* Tc, float* scale);

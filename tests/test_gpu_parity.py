@@ -71,7 +71,7 @@ def test_confocal_bins_bit_exact_all_variants_and_non_separable_basis(gpu, oracl
     N = (18, 17, 19)
     zero, inc = grid_basis(N)
     ref = oracle.a1_volume_f64acc(H, pr, N, zero, inc, 2.05, 0.001)
-    for v in range(6):
+    for v in range(8):
         vol = gpu.backprojection(H, pr, N, zero, inc, 2.05, 0.001, variant=v)
         assert np.array_equal(vol, ref.astype(f32)), f"variant {v}"
     # a rotated basis (wall normal) makes the grid non-separable and the wall non-flat
@@ -83,6 +83,52 @@ def test_confocal_bins_bit_exact_all_variants_and_non_separable_basis(gpu, oracl
     for v in (2, 5):
         vol2 = gpu.backprojection(H, pr2, N, zero2, inc2, 2.05, 0.001, variant=v)
         assert np.array_equal(vol2, ref2.astype(f32)), f"variant {v}"
+
+
+@pytest.mark.parametrize("variant,kz", [(6, 16), (7, 8)])
+@pytest.mark.parametrize("N,size,kahan", [((35, 33, 40), (0.25, 0.22, 0.3), 0), ((16, 16, 16), (0.1, 0.1, 0.1), 3),
+                                          ((50, 17, 9), (0.25, 0.08, 0.03), 0), ((1, 1, 1), (0.1, 0.1, 0.1), 0)])
+def test_lds_staged_kernel_bit_exact(gpu, oracle, N, size, kahan, variant, kz):
+    """The LDS-staged kernels (16 x 16 x kz tiles, each pair's bin window copied to LDS) at a
+    voxel pitch whose windows fit: bins exact against the oracle, and volumes identical to the
+    texture-path kernel (same element per update, same order, same Kahan blocks).  Partial
+    tiles in every axis, a one-voxel volume, and bins below 0 and above Tc."""
+    pr = confocal_pairs(16, jitter=0.01, seed=7)          # flat wall
+    T = 3000
+    H = coded_samples(len(pr), T, seed=2)
+    zero, inc = grid_basis(N, pos=(0.05, -0.02, 0.7), size=size)
+    if N == (1, 1, 1):
+        inc = np.diag([0.004, 0.004, 0.004]).astype(f32).ravel()
+    t0 = 1.2
+    with gpu.Plan(H, pr, N, zero, inc, t0, 0.001, kahan_block=kahan, variant=variant) as plan:
+        assert plan.info()["lane_voxels"] == kz   # the LDS-staged kernel runs
+        plan.execute()
+        v6 = plan.volume()
+    ref = oracle.a1_volume_f64acc(H, pr, N, zero, inc, t0, 0.001)
+    assert np.array_equal(v6, ref.astype(f32))
+    Hr = np.random.default_rng(1).gamma(0.5, 2.0, H.shape).astype(f32)
+    a = gpu.backprojection(Hr, pr, N, zero, inc, t0, 0.001, kahan_block=kahan, variant=variant)
+    b = gpu.backprojection(Hr, pr, N, zero, inc, t0, 0.001, kahan_block=kahan, variant=5)
+    assert np.array_equal(a, b)
+
+
+def test_lds_staged_kernel_falls_back(gpu, oracle):
+    """Variant 6 where the LDS kernel does not apply (coarse pitch: windows too long; a wall
+    that is not flat) runs the fused texture-path kernel and stays exact."""
+    T = 2500
+    for jitter_z, N, size in [(False, (20, 18, 22), (1, 1, 1)), (True, (35, 33, 40), (0.25, 0.22, 0.3))]:
+        pr = confocal_pairs(12, jitter=0.02, seed=3)
+        if jitter_z:
+            pr["laser_point"][:, 2] = np.linspace(0, 0.05, len(pr)).astype(f32)
+            pr["camera_point"] = pr["laser_point"]
+        H = coded_samples(len(pr), T, seed=5)
+        zero, inc = grid_basis(N, size=size)
+        with gpu.Plan(H, pr, N, zero, inc, 2.05, 0.001, variant=6) as plan:
+            assert plan.info()["lane_voxels"] == 4
+            plan.execute()
+            vol = plan.volume()
+        ref = oracle.a1_volume_f64acc(H, pr, N, zero, inc, 2.05, 0.001)
+        assert np.array_equal(vol, ref.astype(f32)), (jitter_z, N)
 
 
 @pytest.mark.parametrize("variant,storage,cplx", [(-1, "f32", False), (0, "f32", False), (2, "f32", False),
