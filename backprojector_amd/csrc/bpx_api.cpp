@@ -311,7 +311,7 @@ bpx_status bpx_plan_info(bpx_plan plan, int32_t* path, int32_t* lane_voxels, uin
         if (!plan) fail(BPX_ERR_INVALID_ARGUMENT, "NULL plan");
         const PlanImpl& pl = *plan->impl;
         if (path) *path = pl.path;
-        if (lane_voxels) *lane_voxels = pl.lds_tiles ? lds_variant_kz(pl.variant) : pl.K;
+        if (lane_voxels) *lane_voxels = pl.lds_tiles ? lds_variant_kz(pl.variant, pl.format) : pl.K;
         if (pairs_used) *pairs_used = pl.P;
         if (Tc) *Tc = pl.T;
         if (scale) *scale = pl.scale;

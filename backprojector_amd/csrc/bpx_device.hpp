@@ -156,18 +156,28 @@ struct SmpCBF16 {
 //   C64      element 16 B (re[j], im[j], S[j], 0)        loads 16 B at 16j and 8 B at 16(j+1)
 //   CF16/CBF16 element 16 B (S[j], c[j], c[j+1], 0)      load 12 B at 16j
 // `pick` returns the sample of bin j-1+[s >= S[j]].
+// kLdsBytes: bytes of an element as the LDS-staged kernel copies it (the leading bytes of the
+// row element; its lds_pick reads element j at e and the next element at e + kLdsBytes).
 struct FusedF32 {
     static constexpr int kElemBytes = 8;
+    static constexpr int kLdsBytes = 8;
     static constexpr int kSOffset = 1;  // float slot of S inside the element
     using acc_t = float;
     __device__ static __forceinline__ float pick(__amdgpu_buffer_rsrc_t r, uint32_t j, uint32_t soff, float s) {
         const auto v = __builtin_amdgcn_raw_buffer_load_b96(r, (int)(j * 8u), (int)soff, 0);
         return s >= __uint_as_float(v[1]) ? __uint_as_float(v[2]) : __uint_as_float(v[0]);
     }
+    // the same from a copy of the row in LDS; e = element j
+    __device__ static __forceinline__ float lds_pick(const char* e, float s) {
+        const uint2 a = *reinterpret_cast<const uint2*>(e);
+        const float h1 = *reinterpret_cast<const float*>(e + 8);
+        return s >= __uint_as_float(a.y) ? h1 : __uint_as_float(a.x);
+    }
 };
 template <bool BF>
 struct FusedH16 {
     static constexpr int kElemBytes = 8;
+    static constexpr int kLdsBytes = 8;
     static constexpr int kSOffset = 0;
     using acc_t = float;
     __device__ static __forceinline__ float cvt(uint32_t b) {
@@ -178,9 +188,14 @@ struct FusedH16 {
         const uint32_t h = s >= __uint_as_float(v[0]) ? (v[1] >> 16) : (v[1] & 0xffffu);
         return cvt(h);
     }
+    __device__ static __forceinline__ float lds_pick(const char* e, float s) {
+        const uint2 v = *reinterpret_cast<const uint2*>(e);
+        return cvt(s >= __uint_as_float(v.x) ? (v.y >> 16) : (v.y & 0xffffu));
+    }
 };
 struct FusedC64 {
     static constexpr int kElemBytes = 16;
+    static constexpr int kLdsBytes = 16;
     static constexpr int kSOffset = 2;
     using acc_t = float2;
     __device__ static __forceinline__ float2 pick(__amdgpu_buffer_rsrc_t r, uint32_t j, uint32_t soff, float s) {
@@ -189,10 +204,17 @@ struct FusedC64 {
         const bool up = s >= __uint_as_float(a[2]);
         return make_float2(__uint_as_float(up ? b[0] : a[0]), __uint_as_float(up ? b[1] : a[1]));
     }
+    __device__ static __forceinline__ float2 lds_pick(const char* e, float s) {
+        const uint4 a = *reinterpret_cast<const uint4*>(e);
+        const uint2 b = *reinterpret_cast<const uint2*>(e + 16);
+        const bool up = s >= __uint_as_float(a.z);
+        return make_float2(__uint_as_float(up ? b.x : a.x), __uint_as_float(up ? b.y : a.y));
+    }
 };
 template <bool BF>
 struct FusedC16 {
     static constexpr int kElemBytes = 16;
+    static constexpr int kLdsBytes = 8;    // (S[j], c[j]); c[j + 1] is the next element's
     static constexpr int kSOffset = 0;
     using acc_t = float2;
     __device__ static __forceinline__ float cvt(uint32_t b) {
@@ -201,6 +223,12 @@ struct FusedC16 {
     __device__ static __forceinline__ float2 pick(__amdgpu_buffer_rsrc_t r, uint32_t j, uint32_t soff, float s) {
         const auto v = __builtin_amdgcn_raw_buffer_load_b96(r, (int)(j * 16u), (int)soff, 0);
         const uint32_t c = s >= __uint_as_float(v[0]) ? v[2] : v[1];
+        return make_float2(cvt(c & 0xffffu), cvt(c >> 16));
+    }
+    __device__ static __forceinline__ float2 lds_pick(const char* e, float s) {
+        const uint2 v = *reinterpret_cast<const uint2*>(e);
+        const uint32_t c1 = *reinterpret_cast<const uint32_t*>(e + 12);
+        const uint32_t c = s >= __uint_as_float(v.x) ? c1 : v.y;
         return make_float2(cvt(c & 0xffffu), cvt(c >> 16));
     }
 };

@@ -247,12 +247,15 @@ hipError_t launch_copy(void* dst, const void* src, uint64_t bytes, hipStream_t s
 
 // Voxels per lane (K) of each tuned variant; variant -1 picks the default.
 constexpr int kDefaultVariant = 5;  // fused layout, 2 voxels per lane (where the LDS kernel does not apply)
-constexpr int kLdsVariant = 6;      // LDS-staged 16 x 16 x 16 tiles (fused fp32, separable, flat);
+constexpr int kLdsVariant = 6;      // LDS-staged 16 x 16 x 16 tiles (fused rows, separable, flat);
                                     // kLdsVariant + 1: 16 x 16 x 8 tiles
 constexpr uint64_t kLdsDeepTiles = 1024;  // 16-deep tiles from this many 16^3 tiles up
-// Whether the LDS-staged kernel's per-pair bin windows fit its segment for this grid spacing.
-bool confocal_lds_fits(const float inc[9], float inv2, int kz);
-int lds_variant_kz(int variant);
+// Whether the LDS-staged kernel's per-pair bin windows over 16 x 16 x kz tiles fit a segment of
+// seg elements for this grid spacing; the segment length of a storage format; the tile depth a
+// variant (6 or 7) gives for a storage format (complex formats: always 8).
+bool confocal_lds_fits(const float inc[9], float inv2, int kz, uint32_t seg);
+uint32_t lds_seg_elems(int format);
+int lds_variant_kz(int variant, int format);
 int variant_k(int variant);
 int variant_count();
 

@@ -614,7 +614,7 @@ __global__ __launch_bounds__(256) void confocal_kernel(ConfocalArgs a) {
 }
 
 // ------------------------------------------------------------------- LDS-staged confocal gather
-// fp32 fused rows, separable grid, flat wall.  A 256-lane work-group owns a 16 x 16 x KZ voxel
+// Fused rows of any storage format, separable grid, flat wall.  A 256-lane work-group owns a 16 x 16 x KZ voxel
 // tile (lane = one (x, y) column, KZ voxels along z).  For every pair, the bin guesses of all
 // the tile's voxels lie in a short window [jlo, jhi]: the tile's bounding box gives the nearest
 // and farthest squared distances, and the window is widened by 2 bins on each side, far more
@@ -629,13 +629,18 @@ __global__ __launch_bounds__(256) void confocal_kernel(ConfocalArgs a) {
 // and inside [0, Tc] the guess equals the clamped one.  So every update adds the same value as
 // in confocal_kernel, in the same pair order with the same Kahan blocks: the volumes are
 // bit-identical.
-constexpr uint32_t kSegElems = 512;   // staged fused elements per pair (8 B each)
 constexpr uint32_t kSegPairs = 4;     // pairs per stage (one per wave)
 constexpr uint32_t kSegTileXY = 16;   // tile edge in x and y
+// staged elements per pair: 512 of 8 B, or 384 of 16 B (complex64)
+__host__ __device__ constexpr uint32_t seg_elems(int elem_bytes) { return elem_bytes == 8 ? 512u : 384u; }
 
-template <int KZ>
+template <class Fz, int KZ>
 __global__ __launch_bounds__(256) void confocal_lds_kernel(ConfocalArgs a) {
-    __shared__ uint2 seg[2][kSegPairs][kSegElems];
+    using acc_t = typename Fz::acc_t;
+    // EG: bytes of a row element in HBM; E: bytes staged per element (its leading part)
+    constexpr uint32_t EG = Fz::kElemBytes, E = Fz::kLdsBytes, SEG = seg_elems(Fz::kLdsBytes), NL = SEG / 64u;
+    using elem_t = std::conditional_t<E == 8, uint2, uint4>;
+    __shared__ elem_t seg[2][kSegPairs][SEG];
     __shared__ int meta[2][kSegPairs][2];   // window start, staged element count
     const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63u;
     const uint32_t n0 = a.g.n[0], n1 = a.g.n[1], n2 = a.g.n[2];
@@ -674,16 +679,25 @@ __global__ __launch_bounds__(256) void confocal_lds_kernel(ConfocalArgs a) {
             bmax[k] = fmaxf(bmax[k], v[k]);
         }
     }
-    const int last_elem = (int)a.Tc + 1;   // H[p][Tc + 1] = 0 ends every row
-    const uint32_t row_bytes = (a.Tc + 2u) * 8u;
+    const int last_elem = (int)a.Tc + 1;   // sample Tc + 1 (padded index) = 0 ends every row
+    const uint32_t row_bytes = (a.Tc + 2u) * EG;
     const __amdgpu_buffer_rsrc_t er = make_rsrc(a.H, a.h_bytes);
     // the wall is flat: the z terms of the box distances are the same for every pair
     const float ez = a.lz - fminf(fmaxf(a.lz, bmin[2]), bmax[2]);
     const float fz = fmaxf(fabsf(a.lz - bmin[2]), fabsf(a.lz - bmax[2]));
     const float ez2 = ez * ez, fz2 = fz * fz;
 
-    // Stage registers of this wave's pair: window and kSegElems / 64 elements per lane.
-    uint2 pre[kSegElems / 64u];
+    auto load = [&](__amdgpu_buffer_rsrc_t r, int off, int soff) {
+        if constexpr (E == 8) {
+            const auto t = __builtin_amdgcn_raw_buffer_load_b64(r, off, soff, 0);
+            return make_uint2(t[0], t[1]);
+        } else {
+            const auto t = __builtin_amdgcn_raw_buffer_load_b128(r, off, soff, 0);
+            return make_uint4(t[0], t[1], t[2], t[3]);
+        }
+    };
+    // Stage registers of this wave's pair: window and NL elements per lane.
+    elem_t pre[NL];
     int pjlo = 0, pcnt = 0;
     auto fetch = [&](uint32_t p0) {
         const uint32_t p = p0 + w;
@@ -700,47 +714,45 @@ __global__ __launch_bounds__(256) void confocal_lds_kernel(ConfocalArgs a) {
             const int jl = (int)__builtin_fmaf(__builtin_amdgcn_sqrtf(smin), a.inv2, pr.w) - 2;
             const int jh = (int)__builtin_fmaf(__builtin_amdgcn_sqrtf(smax), a.inv2, pr.w) + 2;
             pjlo = __builtin_amdgcn_readfirstlane(jl);
-            pcnt = __builtin_amdgcn_readfirstlane(min(jh + 2 - jl, (int)kSegElems));   // elements jl .. jh + 1
+            pcnt = __builtin_amdgcn_readfirstlane(min(jh + 2 - jl, (int)SEG));   // elements jl .. jh + 1
         }
         if (p < a.P && pjlo >= 0) {
             // Window inside or past the row: a descriptor over row p alone returns 0 for every
-            // element past H[p][Tc + 1], so no lane needs a bounds test; elements past the
-            // window are loaded too and never read.
+            // element past its end, so no lane needs a bounds test; elements past the window
+            // are loaded too and never read.
             const __amdgpu_buffer_rsrc_t rr =
-                make_rsrc(static_cast<const char*>(a.H) + (size_t)p * a.ldh * 8u, row_bytes);
-            const int vo = (pjlo + (int)lane) * 8;
+                make_rsrc(static_cast<const char*>(a.H) + (size_t)p * a.ldh * EG, row_bytes);
+            const int vo = (pjlo + (int)lane) * (int)EG;
 #pragma unroll
-            for (uint32_t m = 0; m < kSegElems / 64u; ++m) {
-                const auto t = __builtin_amdgcn_raw_buffer_load_b64(rr, vo + (int)(512u * m), 0, 0);
-                pre[m] = make_uint2(t[0], t[1]);
-            }
+            for (uint32_t m = 0; m < NL; ++m) pre[m] = load(rr, vo + (int)(64u * EG * m), 0);
         } else {
             // window starting before the row (or no pair): elements below 0 are zeros
-            const uint32_t soff = p < a.P ? p * a.ldh * 8u : 0u;
+            const uint32_t soff = p < a.P ? p * a.ldh * EG : 0u;
 #pragma unroll
-            for (uint32_t m = 0; m < kSegElems / 64u; ++m) {
+            for (uint32_t m = 0; m < NL; ++m) {
                 const int e = (int)(lane + 64u * m), idx = pjlo + e;
-                uint2 v = make_uint2(0u, 0u);
-                if (e < pcnt && idx >= 0 && idx <= last_elem) {
-                    const auto t = __builtin_amdgcn_raw_buffer_load_b64(er, idx * 8, (int)soff, 0);
-                    v = make_uint2(t[0], t[1]);
-                }
+                elem_t v{};
+                if (e < pcnt && idx >= 0 && idx <= last_elem) v = load(er, idx * (int)EG, (int)soff);
                 pre[m] = v;
             }
         }
     };
     auto commit = [&](uint32_t buf) {
 #pragma unroll
-        for (uint32_t m = 0; m < kSegElems / 64u; ++m) seg[buf][w][lane + 64u * m] = pre[m];
+        for (uint32_t m = 0; m < NL; ++m) seg[buf][w][lane + 64u * m] = pre[m];
         if (lane == 0) {
             meta[buf][w][0] = pjlo;
             meta[buf][w][1] = pcnt;
         }
     };
 
-    float sum[KZ], comp[KZ], blk[KZ];
+    acc_t sum[KZ], comp[KZ], blk[KZ];
 #pragma unroll
-    for (int r = 0; r < KZ; ++r) sum[r] = comp[r] = blk[r] = 0.0f;
+    for (int r = 0; r < KZ; ++r) {
+        acc_zero(sum[r]);
+        acc_zero(comp[r]);
+        acc_zero(blk[r]);
+    }
     const uint32_t nstage = (a.P + kSegPairs - 1u) / kSegPairs;
     const char* segb = reinterpret_cast<const char*>(&seg[0][0][0]);
     uint32_t in_block = 0;
@@ -753,26 +765,23 @@ __global__ __launch_bounds__(256) void confocal_lds_kernel(ConfocalArgs a) {
         const uint32_t p0 = b * kSegPairs, np = min(kSegPairs, a.P - p0);
         for (uint32_t i = 0; i < np; ++i) {
             const float4 pr = a.pairs[p0 + i];
-            // byte offset of element j of this pair's window = 8 j + base (unsigned wrap-around);
+            // byte offset of element j of this pair's window = E j + base (unsigned wrap-around);
             // the window start is the same for every lane, so base lives in a scalar register
-            const uint32_t base = __builtin_amdgcn_readfirstlane(
-                ((buf * kSegPairs + i) * kSegElems - (uint32_t)meta[buf][i][0]) * 8u);
+            const uint32_t base =
+                __builtin_amdgcn_readfirstlane(((buf * kSegPairs + i) * SEG - (uint32_t)meta[buf][i][0]) * E);
             const float dx = pr.x - vx, dy = pr.y - vy;
             const float axy = dx * dx + dy * dy;
 #pragma unroll
             for (int r = 0; r < KZ; ++r) {
                 const float s = axy + dz2[r];
                 const int j = (int)__builtin_fmaf(__builtin_amdgcn_sqrtf(s), a.inv2, pr.w);
-                const char* e = segb + ((uint32_t)j * 8u + base);
-                const uint2 e0 = *reinterpret_cast<const uint2*>(e);
-                const float h1 = *reinterpret_cast<const float*>(e + 8);
-                blk[r] = blk[r] + (s >= __uint_as_float(e0.y) ? h1 : __uint_as_float(e0.x));
+                acc_add(blk[r], Fz::lds_pick(segb + ((uint32_t)j * E + base), s));
             }
             if (++in_block == a.kahan_block) {
 #pragma unroll
                 for (int r = 0; r < KZ; ++r) {
                     kahan_add(sum[r], comp[r], blk[r]);
-                    blk[r] = 0.0f;
+                    acc_zero(blk[r]);
                 }
                 in_block = 0;
             }
@@ -790,7 +799,7 @@ __global__ __launch_bounds__(256) void confocal_lds_kernel(ConfocalArgs a) {
             const uint32_t iz = tz * KZ + r;
             if (iz < n2) {
                 const size_t id = ((size_t)ix * n1 + iy) * n2 + iz;
-                store_voxel(a.out, id, kahan_result(sum[r], comp[r]) * a.scale);
+                store_voxel(a.out, id, scaled(kahan_result(sum[r], comp[r]), a.scale));
             }
         }
     }
@@ -1205,8 +1214,9 @@ hipError_t launch_window(const WindowArgs& a, int complex_samples, hipStream_t s
 // ------------------------------------------------------------------- dispatch
 // Variants v = 3*layout + k: layout 0 = split tables, 1 = fused layout; k: 0 -> K = 4,
 // 1 -> K = 8, 2 -> K = 2 voxels per lane.  Variants 6 and 7: the LDS-staged kernel with
-// 16 x 16 x 16 / 16 x 16 x 8 tiles for fp32 fused rows on a separable grid and a flat wall when
-// the bin windows fit, otherwise the fused layout with K = 4 / 8.
+// 16 x 16 x 16 / 16 x 16 x 8 tiles (complex formats: 16 x 16 x 8 for both) for fused rows on a
+// separable grid and a flat wall when the bin windows fit, otherwise the fused layout with
+// K = 4 / 8.
 int variant_k(int variant) {
     switch (variant % 3) {
         case 1: return 8;
@@ -1216,7 +1226,6 @@ int variant_k(int variant) {
 }
 bool variant_fused(int variant) { return variant >= 3; }
 int variant_count() { return 8; }
-int lds_variant_kz(int variant) { return variant == 7 ? 8 : 16; }
 int fused_elem_bytes(int format) {
     switch (format) {
         case kF32: case kF16: case kBF16: return 8;
@@ -1250,10 +1259,10 @@ static hipError_t confocal_dispatch(const ConfocalArgs& a, int variant, bool sep
     }
 }
 
-// Whether every pair's bin window over a 16 x 16 x kz tile fits the staged segment: the
+// Whether every pair's bin window over a 16 x 16 x kz tile fits a segment of seg elements: the
 // window spans at most (farthest - nearest distance) * inv2 <= tile diagonal * inv2 bins, plus
 // the 2-bin margins on each side and the extra element for the upper sample.
-bool confocal_lds_fits(const float inc[9], float inv2, int kz) {
+bool confocal_lds_fits(const float inc[9], float inv2, int kz, uint32_t seg) {
     double d2 = 0.0;
     for (int k = 0; k < 3; ++k) {
         const double r = (kSegTileXY - 1.0) * (std::fabs((double)inc[k]) + std::fabs((double)inc[3 + k])) +
@@ -1261,19 +1270,39 @@ bool confocal_lds_fits(const float inc[9], float inv2, int kz) {
         d2 += r * r;
     }
     const double need = std::ceil(std::sqrt(d2) * std::fabs((double)inv2) * (1.0 + 1e-6)) + 8.0;
-    return std::isfinite(need) && need <= (double)kSegElems;
+    return std::isfinite(need) && need <= (double)seg;
+}
+uint32_t lds_seg_elems(int format) { return seg_elems(format == kC64 ? 16 : 8); }
+int lds_variant_kz(int variant, int format) {
+    const bool cplx = format == kC64 || format == kCF16 || format == kCBF16;
+    return cplx || variant == kLdsVariant + 1 ? 8 : 16;   // complex: 2 accumulators per voxel
+}
+bool confocal_lds_applies(const ConfocalArgs& a, int format, int variant, bool sep, bool flat) {
+    return variant >= kLdsVariant && sep && flat &&
+           confocal_lds_fits(a.g.inc, a.inv2, lds_variant_kz(variant, format), lds_seg_elems(format));
+}
+
+template <class Fz, int KZ>
+static hipError_t lds_launch(const ConfocalArgs& a, hipStream_t stream) {
+    const uint32_t n = ((a.g.n[0] + kSegTileXY - 1u) / kSegTileXY) * ((a.g.n[1] + kSegTileXY - 1u) / kSegTileXY) *
+                       ((a.g.n[2] + KZ - 1u) / KZ);
+    hipLaunchKernelGGL((confocal_lds_kernel<Fz, KZ>), dim3(n), dim3(256), 0, stream, a);
+    return hipGetLastError();
 }
 
 hipError_t launch_confocal(const ConfocalArgs& a, int format, int variant, bool sep, bool flat,
                            hipStream_t stream) {
-    if (variant >= kLdsVariant && format == kF32 && sep && flat && confocal_lds_fits(a.g.inc, a.inv2, lds_variant_kz(variant))) {
-        const uint32_t kz = (uint32_t)lds_variant_kz(variant);
-        const uint32_t n = ((a.g.n[0] + kSegTileXY - 1u) / kSegTileXY) * ((a.g.n[1] + kSegTileXY - 1u) / kSegTileXY) *
-                           ((a.g.n[2] + kz - 1u) / kz);
-        const dim3 grid(n), block(256);
-        if (kz == 16) hipLaunchKernelGGL((confocal_lds_kernel<16>), grid, block, 0, stream, a);
-        else hipLaunchKernelGGL((confocal_lds_kernel<8>), grid, block, 0, stream, a);
-        return hipGetLastError();
+    if (confocal_lds_applies(a, format, variant, sep, flat)) {
+        const bool deep = lds_variant_kz(variant, format) == 16;
+        switch (format) {
+            case kF32: return deep ? lds_launch<FusedF32, 16>(a, stream) : lds_launch<FusedF32, 8>(a, stream);
+            case kF16: return deep ? lds_launch<FusedH16<false>, 16>(a, stream) : lds_launch<FusedH16<false>, 8>(a, stream);
+            case kBF16: return deep ? lds_launch<FusedH16<true>, 16>(a, stream) : lds_launch<FusedH16<true>, 8>(a, stream);
+            case kC64: return lds_launch<FusedC64, 8>(a, stream);
+            case kCF16: return lds_launch<FusedC16<false>, 8>(a, stream);
+            case kCBF16: return lds_launch<FusedC16<true>, 8>(a, stream);
+            default: return hipErrorInvalidValue;
+        }
     }
     if (variant_fused(variant)) {
         switch (format) {

@@ -432,14 +432,15 @@ PlanImpl* make_plan_single_tap(const CaptureIn& c, const uint32_t n[3], const fl
         // Default kernel: the LDS-staged one where it applies (fp32, separable grid, flat wall,
         // short bin windows); 16-deep tiles when there are enough of them to fill the GPU
         // (4 work-groups per CU), else 8-deep tiles (twice the work-groups).
-        if (o.variant < 0 && pl->format == kF32 && pl->fused && pl->sep && pl->flat) {
+        const uint32_t seg = lds_seg_elems(pl->format);
+        if (o.variant < 0 && pl->fused && pl->sep && pl->flat) {
             const uint64_t t16 = (uint64_t)((n[0] + 15) / 16) * ((n[1] + 15) / 16) * ((n[2] + 15) / 16);
             const int deep = t16 >= kLdsDeepTiles ? kLdsVariant : kLdsVariant + 1;
-            if (confocal_lds_fits(inc, pl->inv2, lds_variant_kz(deep))) pl->variant = deep;
-            else if (confocal_lds_fits(inc, pl->inv2, lds_variant_kz(kLdsVariant + 1))) pl->variant = kLdsVariant + 1;
+            if (confocal_lds_fits(inc, pl->inv2, lds_variant_kz(deep, pl->format), seg)) pl->variant = deep;
+            else if (confocal_lds_fits(inc, pl->inv2, 8, seg)) pl->variant = kLdsVariant + 1;
         }
-        pl->lds_tiles = pl->variant >= kLdsVariant && pl->format == kF32 && pl->fused && pl->sep && pl->flat &&
-                        confocal_lds_fits(inc, pl->inv2, lds_variant_kz(pl->variant));
+        pl->lds_tiles = pl->variant >= kLdsVariant && pl->fused && pl->sep && pl->flat &&
+                        confocal_lds_fits(inc, pl->inv2, lds_variant_kz(pl->variant, pl->format), seg);
         pl->pair4.alloc(sizeof(float) * p4.size());
         hip_check(hipMemcpy(pl->pair4.ptr, p4.data(), sizeof(float) * p4.size(), hipMemcpyHostToDevice),
                   "hipMemcpy(pair table)");

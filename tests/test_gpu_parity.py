@@ -112,6 +112,30 @@ def test_lds_staged_kernel_bit_exact(gpu, oracle, N, size, kahan, variant, kz):
     assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("storage,cplx", [("f16", False), ("bf16", False), ("f32", True), ("f16", True),
+                                          ("bf16", True)])
+def test_lds_staged_kernel_formats(gpu, storage, cplx):
+    """Every storage format through the LDS-staged kernel equals the texture-path kernel bit for
+    bit (same element per update, same order); complex formats use 16 x 16 x 8 tiles."""
+    pr = confocal_pairs(16, jitter=0.01, seed=7)
+    T = 3000
+    N = (35, 33, 40)
+    zero, inc = grid_basis(N, pos=(0.05, -0.02, 0.7), size=(0.25, 0.22, 0.3))
+    rng = np.random.default_rng(4)
+    H = rng.gamma(0.5, 2.0, (len(pr), T)).astype(f32)
+    if cplx:
+        H = (H - 1j * rng.gamma(0.5, 2.0, H.shape)).astype(np.complex64)
+    vols = {}
+    for v in (6, 7, 5):
+        with gpu.Plan(H, pr, N, zero, inc, 1.2, 0.001, storage=storage, variant=v) as plan:
+            lv = plan.info()["lane_voxels"]
+            assert lv == {6: 8 if cplx else 16, 7: 8, 5: 2}[v], (v, lv)
+            plan.execute()
+            vols[v] = plan.volume()
+    assert np.array_equal(vols[6], vols[5]) and np.array_equal(vols[7], vols[5])
+    assert np.abs(vols[5]).max() > 0
+
+
 def test_lds_staged_kernel_falls_back(gpu, oracle):
     """Variant 6 where the LDS kernel does not apply (coarse pitch: windows too long; a wall
     that is not flat) runs the fused texture-path kernel and stays exact."""
