@@ -249,7 +249,8 @@ hipError_t launch_copy(void* dst, const void* src, uint64_t bytes, hipStream_t s
 constexpr int kDefaultVariant = 5;  // fused layout, 2 voxels per lane (where the LDS kernel does not apply)
 constexpr int kLdsVariant = 6;      // LDS-staged 16 x 16 x 16 tiles (fused rows, separable, flat);
                                     // kLdsVariant + 1: 16 x 16 x 8 tiles
-constexpr uint64_t kLdsDeepTiles = 1024;  // 16-deep tiles from this many 16^3 tiles up
+constexpr uint64_t kLdsDeepTiles = 1024;  // 16-deep LDS tiles by default from this many up
+constexpr uint64_t kLdsMinTiles = 512;    // 8-deep LDS tiles by default from this many up
 // Whether the LDS-staged kernel's per-pair bin windows over 16 x 16 x kz tiles fit a segment of
 // seg elements for this grid spacing; the segment length of a storage format; the tile depth a
 // variant (6 or 7) gives for a storage format (complex formats: always 8).

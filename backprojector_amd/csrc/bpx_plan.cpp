@@ -429,15 +429,19 @@ PlanImpl* make_plan_single_tap(const CaptureIn& c, const uint32_t n[3], const fl
         pl->lz = pr[0].laser_point[2];
         pl->sep = inc[6] == 0.0f && inc[7] == 0.0f;
         if (!pl->sep) pl->flat = false;
-        // Default kernel: the LDS-staged one where it applies (fp32, separable grid, flat wall,
-        // short bin windows); 16-deep tiles when there are enough of them to fill the GPU
-        // (4 work-groups per CU), else 8-deep tiles (twice the work-groups).
+        // Default kernel: the LDS-staged one where it applies (separable grid, flat wall, short
+        // bin windows) and has enough tiles to fill the GPU: 16-deep tiles from 4 work-groups
+        // per CU, else 8-deep tiles (twice the work-groups) from 2 per CU.  Measured at 16,384
+        // pairs: 96^3 (432 8-deep tiles) ties with the texture path, 112^3 (686) runs 12.1 ms
+        // against 19.4 ms.
         const uint32_t seg = lds_seg_elems(pl->format);
         if (o.variant < 0 && pl->fused && pl->sep && pl->flat) {
             const uint64_t t16 = (uint64_t)((n[0] + 15) / 16) * ((n[1] + 15) / 16) * ((n[2] + 15) / 16);
-            const int deep = t16 >= kLdsDeepTiles ? kLdsVariant : kLdsVariant + 1;
-            if (confocal_lds_fits(inc, pl->inv2, lds_variant_kz(deep, pl->format), seg)) pl->variant = deep;
-            else if (confocal_lds_fits(inc, pl->inv2, 8, seg)) pl->variant = kLdsVariant + 1;
+            const uint64_t t8 = (uint64_t)((n[0] + 15) / 16) * ((n[1] + 15) / 16) * ((n[2] + 7) / 8);
+            const int deep = t16 >= kLdsDeepTiles && lds_variant_kz(kLdsVariant, pl->format) == 16
+                                 ? kLdsVariant : kLdsVariant + 1;
+            if (deep == kLdsVariant && confocal_lds_fits(inc, pl->inv2, 16, seg)) pl->variant = deep;
+            else if (t8 >= kLdsMinTiles && confocal_lds_fits(inc, pl->inv2, 8, seg)) pl->variant = kLdsVariant + 1;
         }
         pl->lds_tiles = pl->variant >= kLdsVariant && pl->fused && pl->sep && pl->flat &&
                         confocal_lds_fits(inc, pl->inv2, lds_variant_kz(pl->variant, pl->format), seg);
