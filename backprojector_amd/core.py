@@ -239,8 +239,10 @@ class Plan:
         check(lib().bpx_plan_info(self._h, ctypes.byref(path), ctypes.byref(k), ctypes.byref(used),
                                   ctypes.byref(tc), ctypes.byref(sc)))
         names = {0: "confocal", 1: "grid", 2: "pairs", 3: "window"}
+        kname = ctypes.create_string_buffer(160)
+        check(lib().bpx_plan_kernel_name(self._h, kname, len(kname)))
         return dict(path=names.get(path.value, path.value), lane_voxels=k.value,
-                    pairs_used=used.value, Tc=tc.value, scale=sc.value)
+                    pairs_used=used.value, Tc=tc.value, scale=sc.value, kernel=kname.value.decode())
 
     def benchmark(self, warmup=2, iterations=10):
         t = nat.Timing()

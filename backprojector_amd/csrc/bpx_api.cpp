@@ -2,6 +2,7 @@
 // all failures and converts them to a bpx_status plus a thread-local message.
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -311,10 +312,28 @@ bpx_status bpx_plan_info(bpx_plan plan, int32_t* path, int32_t* lane_voxels, uin
         if (!plan) fail(BPX_ERR_INVALID_ARGUMENT, "NULL plan");
         const PlanImpl& pl = *plan->impl;
         if (path) *path = pl.path;
-        if (lane_voxels) *lane_voxels = pl.lds_tiles ? lds_variant_kz(pl.variant, pl.format) : pl.K;
+        if (lane_voxels) *lane_voxels = pl.lds_tiles ? lds_shape(pl.variant, pl.format).kz : pl.K;
         if (pairs_used) *pairs_used = pl.P;
         if (Tc) *Tc = pl.T;
         if (scale) *scale = pl.scale;
+    });
+}
+
+bpx_status bpx_plan_kernel_name(bpx_plan plan, char* out, uint64_t capacity) {
+    return guarded([&] {
+        if (!plan || !out || capacity == 0) fail(BPX_ERR_INVALID_ARGUMENT, "NULL argument");
+        const PlanImpl& pl = *plan->impl;
+        std::string name;
+        if (pl.lds_tiles) {
+            const LdsShape sh = lds_shape(pl.variant, pl.format);
+            name = "confocal_lds_kernel<KZ=" + std::to_string(sh.kz) + ", waves=" + std::to_string(sh.nw) +
+                   "> (16x16x" + std::to_string(sh.depth()) + " tiles)";
+        } else {
+            static const char* paths[] = {"confocal_kernel", "grid_kernel", "pairs_kernel", "window_kernel",
+                                          "octree kernels"};
+            name = std::string(paths[pl.path]) + "<K=" + std::to_string(pl.K) + ">";
+        }
+        std::snprintf(out, (size_t)capacity, "%s", name.c_str());
     });
 }
 

@@ -247,16 +247,22 @@ hipError_t launch_copy(void* dst, const void* src, uint64_t bytes, hipStream_t s
 
 // Voxels per lane (K) of each tuned variant; variant -1 picks the default.
 constexpr int kDefaultVariant = 5;  // fused layout, 2 voxels per lane (where the LDS kernel does not apply)
-constexpr int kLdsVariant = 6;      // LDS-staged 16 x 16 x 16 tiles (fused rows, separable, flat);
-                                    // kLdsVariant + 1: 16 x 16 x 8 tiles
-constexpr uint64_t kLdsDeepTiles = 1024;  // 16-deep LDS tiles by default from this many up
-constexpr uint64_t kLdsMinTiles = 512;    // 8-deep LDS tiles by default from this many up
+constexpr int kLdsVariant = 6;      // LDS-staged 16^3 tiles, 256 lanes (fused rows, separable, flat);
+                                    // + 1: 16 x 16 x 8 tiles, 256 lanes; + 2: 16^3 tiles, 512 lanes
+constexpr uint64_t kLdsDeepTiles = 1024;  // 16^3 tiles of 256 lanes by default from this many up
+constexpr uint64_t kLdsMinTiles = 512;    // the other LDS shapes by default from this many up
 // Whether the LDS-staged kernel's per-pair bin windows over 16 x 16 x kz tiles fit a segment of
 // seg elements for this grid spacing; the segment length of a storage format; the tile depth a
 // variant (6 or 7) gives for a storage format (complex formats: always 8).
 bool confocal_lds_fits(const float inc[9], float inv2, int kz, uint32_t seg);
 uint32_t lds_seg_elems(int format);
-int lds_variant_kz(int variant, int format);
+// Tile of an LDS-staged variant (6, 7, 8) for a storage format: kz voxels per lane along z,
+// nw waves per work-group (= pairs per stage); tiles are 16 x 16 x depth().
+struct LdsShape {
+    int kz, nw;
+    int depth() const { return kz * nw / 4; }
+};
+LdsShape lds_shape(int variant, int format);
 int variant_k(int variant);
 int variant_count();
 

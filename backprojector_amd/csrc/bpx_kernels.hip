@@ -629,32 +629,34 @@ __global__ __launch_bounds__(256) void confocal_kernel(ConfocalArgs a) {
 // and inside [0, Tc] the guess equals the clamped one.  So every update adds the same value as
 // in confocal_kernel, in the same pair order with the same Kahan blocks: the volumes are
 // bit-identical.
-constexpr uint32_t kSegPairs = 4;     // pairs per stage (one per wave)
 constexpr uint32_t kSegTileXY = 16;   // tile edge in x and y
 // staged elements per pair: 512 of 8 B, or 384 of 16 B (complex64)
 __host__ __device__ constexpr uint32_t seg_elems(int elem_bytes) { return elem_bytes == 8 ? 512u : 384u; }
 
-template <class Fz, int KZ>
-__global__ __launch_bounds__(256) void confocal_lds_kernel(ConfocalArgs a) {
+template <class Fz, int KZ, int NW>
+__global__ __launch_bounds__(NW * 64) void confocal_lds_kernel(ConfocalArgs a) {
     using acc_t = typename Fz::acc_t;
     // EG: bytes of a row element in HBM; E: bytes staged per element (its leading part)
     constexpr uint32_t EG = Fz::kElemBytes, E = Fz::kLdsBytes, SEG = seg_elems(Fz::kLdsBytes), NL = SEG / 64u;
+    // NW waves (= pairs per stage); tile depth TZ: NW / 4 z-slabs of KZ voxels per lane
+    constexpr uint32_t TZ = KZ * NW / 4;
     using elem_t = std::conditional_t<E == 8, uint2, uint4>;
-    __shared__ elem_t seg[2][kSegPairs][SEG];
-    __shared__ int meta[2][kSegPairs][2];   // window start, staged element count
+    __shared__ elem_t seg[2][NW][SEG];
+    __shared__ int meta[2][NW][2];   // window start, staged element count
     const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63u;
     const uint32_t n0 = a.g.n[0], n1 = a.g.n[1], n2 = a.g.n[2];
-    const uint32_t ntz = (n2 + KZ - 1u) / KZ, nty = (n1 + kSegTileXY - 1u) / kSegTileXY;
+    const uint32_t ntz = (n2 + TZ - 1u) / TZ, nty = (n1 + kSegTileXY - 1u) / kSegTileXY;
     const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
     const uint32_t tz = tile % ntz, txy = tile / ntz, ty = txy % nty, tx = txy / nty;
     const uint32_t ix = tx * kSegTileXY + (w & 1u) * 8u + (lane & 7u);
-    const uint32_t iy = ty * kSegTileXY + (w >> 1) * 8u + (lane >> 3);
+    const uint32_t iy = ty * kSegTileXY + ((w >> 1) & 1u) * 8u + (lane >> 3);
+    const uint32_t iz0 = tz * TZ + (w >> 2) * KZ;
     const uint32_t cx = min(ix, n0 - 1u), cy = min(iy, n1 - 1u);
     float vx = 0.0f, vy = 0.0f, dz2[KZ];
 #pragma unroll
     for (int r = 0; r < KZ; ++r) {
         float v[3];
-        voxel_position(a.g, cx, cy, min(tz * KZ + r, n2 - 1u), v);
+        voxel_position(a.g, cx, cy, min(iz0 + r, n2 - 1u), v);
         if (r == 0) {
             vx = v[0];
             vy = v[1];
@@ -666,7 +668,7 @@ __global__ __launch_bounds__(256) void confocal_lds_kernel(ConfocalArgs a) {
     // in one index each (rounding is monotone), so its extremes are at the 8 corners
     const uint32_t x0 = min(tx * kSegTileXY, n0 - 1u), x1 = min(tx * kSegTileXY + kSegTileXY - 1u, n0 - 1u);
     const uint32_t y0 = min(ty * kSegTileXY, n1 - 1u), y1 = min(ty * kSegTileXY + kSegTileXY - 1u, n1 - 1u);
-    const uint32_t z0 = min(tz * KZ, n2 - 1u), z1 = min(tz * KZ + KZ - 1u, n2 - 1u);
+    const uint32_t z0 = min(tz * TZ, n2 - 1u), z1 = min(tz * TZ + TZ - 1u, n2 - 1u);
     float bmin[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
     float bmax[3] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
 #pragma unroll
@@ -753,7 +755,7 @@ __global__ __launch_bounds__(256) void confocal_lds_kernel(ConfocalArgs a) {
         acc_zero(comp[r]);
         acc_zero(blk[r]);
     }
-    const uint32_t nstage = (a.P + kSegPairs - 1u) / kSegPairs;
+    const uint32_t nstage = (a.P + NW - 1u) / NW;
     const char* segb = reinterpret_cast<const char*>(&seg[0][0][0]);
     uint32_t in_block = 0;
     fetch(0u);
@@ -761,14 +763,14 @@ __global__ __launch_bounds__(256) void confocal_lds_kernel(ConfocalArgs a) {
     __syncthreads();
     for (uint32_t b = 0; b < nstage; ++b) {
         const uint32_t buf = b & 1u;
-        if (b + 1u < nstage) fetch((b + 1u) * kSegPairs);
-        const uint32_t p0 = b * kSegPairs, np = min(kSegPairs, a.P - p0);
+        if (b + 1u < nstage) fetch((b + 1u) * NW);
+        const uint32_t p0 = b * NW, np = min((uint32_t)NW, a.P - p0);
         for (uint32_t i = 0; i < np; ++i) {
             const float4 pr = a.pairs[p0 + i];
             // byte offset of element j of this pair's window = E j + base (unsigned wrap-around);
             // the window start is the same for every lane, so base lives in a scalar register
             const uint32_t base =
-                __builtin_amdgcn_readfirstlane(((buf * kSegPairs + i) * SEG - (uint32_t)meta[buf][i][0]) * E);
+                __builtin_amdgcn_readfirstlane(((buf * NW + i) * SEG - (uint32_t)meta[buf][i][0]) * E);
             const float dx = pr.x - vx, dy = pr.y - vy;
             const float axy = dx * dx + dy * dy;
 #pragma unroll
@@ -796,7 +798,7 @@ __global__ __launch_bounds__(256) void confocal_lds_kernel(ConfocalArgs a) {
     if (ix < n0 && iy < n1) {
 #pragma unroll
         for (int r = 0; r < KZ; ++r) {
-            const uint32_t iz = tz * KZ + r;
+            const uint32_t iz = iz0 + r;
             if (iz < n2) {
                 const size_t id = ((size_t)ix * n1 + iy) * n2 + iz;
                 store_voxel(a.out, id, scaled(kahan_result(sum[r], comp[r]), a.scale));
@@ -1225,7 +1227,7 @@ int variant_k(int variant) {
     }
 }
 bool variant_fused(int variant) { return variant >= 3; }
-int variant_count() { return 8; }
+int variant_count() { return 9; }
 int fused_elem_bytes(int format) {
     switch (format) {
         case kF32: case kF16: case kBF16: return 8;
@@ -1273,34 +1275,43 @@ bool confocal_lds_fits(const float inc[9], float inv2, int kz, uint32_t seg) {
     return std::isfinite(need) && need <= (double)seg;
 }
 uint32_t lds_seg_elems(int format) { return seg_elems(format == kC64 ? 16 : 8); }
-int lds_variant_kz(int variant, int format) {
+LdsShape lds_shape(int variant, int format) {
     const bool cplx = format == kC64 || format == kCF16 || format == kCBF16;
-    return cplx || variant == kLdsVariant + 1 ? 8 : 16;   // complex: 2 accumulators per voxel
+    if (variant == kLdsVariant + 2 && format != kC64) return {8, 8};   // 512 lanes, 16^3 tiles
+    if (cplx || variant != kLdsVariant) return {8, 4};                 // complex: 2 accumulators per voxel
+    return {16, 4};
 }
 bool confocal_lds_applies(const ConfocalArgs& a, int format, int variant, bool sep, bool flat) {
     return variant >= kLdsVariant && sep && flat &&
-           confocal_lds_fits(a.g.inc, a.inv2, lds_variant_kz(variant, format), lds_seg_elems(format));
+           confocal_lds_fits(a.g.inc, a.inv2, lds_shape(variant, format).depth(), lds_seg_elems(format));
 }
 
-template <class Fz, int KZ>
+template <class Fz, int KZ, int NW>
 static hipError_t lds_launch(const ConfocalArgs& a, hipStream_t stream) {
+    constexpr uint32_t TZ = KZ * NW / 4;
     const uint32_t n = ((a.g.n[0] + kSegTileXY - 1u) / kSegTileXY) * ((a.g.n[1] + kSegTileXY - 1u) / kSegTileXY) *
-                       ((a.g.n[2] + KZ - 1u) / KZ);
-    hipLaunchKernelGGL((confocal_lds_kernel<Fz, KZ>), dim3(n), dim3(256), 0, stream, a);
+                       ((a.g.n[2] + TZ - 1u) / TZ);
+    hipLaunchKernelGGL((confocal_lds_kernel<Fz, KZ, NW>), dim3(n), dim3(NW * 64), 0, stream, a);
     return hipGetLastError();
+}
+template <class Fz>
+static hipError_t lds_launch_shape(const ConfocalArgs& a, LdsShape sh, hipStream_t stream) {
+    if (sh.kz == 16) return lds_launch<Fz, 16, 4>(a, stream);
+    if (sh.nw == 8) return lds_launch<Fz, 8, 8>(a, stream);
+    return lds_launch<Fz, 8, 4>(a, stream);
 }
 
 hipError_t launch_confocal(const ConfocalArgs& a, int format, int variant, bool sep, bool flat,
                            hipStream_t stream) {
     if (confocal_lds_applies(a, format, variant, sep, flat)) {
-        const bool deep = lds_variant_kz(variant, format) == 16;
+        const LdsShape sh = lds_shape(variant, format);
         switch (format) {
-            case kF32: return deep ? lds_launch<FusedF32, 16>(a, stream) : lds_launch<FusedF32, 8>(a, stream);
-            case kF16: return deep ? lds_launch<FusedH16<false>, 16>(a, stream) : lds_launch<FusedH16<false>, 8>(a, stream);
-            case kBF16: return deep ? lds_launch<FusedH16<true>, 16>(a, stream) : lds_launch<FusedH16<true>, 8>(a, stream);
-            case kC64: return lds_launch<FusedC64, 8>(a, stream);
-            case kCF16: return lds_launch<FusedC16<false>, 8>(a, stream);
-            case kCBF16: return lds_launch<FusedC16<true>, 8>(a, stream);
+            case kF32: return lds_launch_shape<FusedF32>(a, sh, stream);
+            case kF16: return lds_launch_shape<FusedH16<false>>(a, sh, stream);
+            case kBF16: return lds_launch_shape<FusedH16<true>>(a, sh, stream);
+            case kC64: return lds_launch<FusedC64, 8, 4>(a, stream);
+            case kCF16: return lds_launch<FusedC16<false>, 8, 4>(a, stream);
+            case kCBF16: return lds_launch<FusedC16<true>, 8, 4>(a, stream);
             default: return hipErrorInvalidValue;
         }
     }

@@ -430,21 +430,24 @@ PlanImpl* make_plan_single_tap(const CaptureIn& c, const uint32_t n[3], const fl
         pl->sep = inc[6] == 0.0f && inc[7] == 0.0f;
         if (!pl->sep) pl->flat = false;
         // Default kernel: the LDS-staged one where it applies (separable grid, flat wall, short
-        // bin windows) and has enough tiles to fill the GPU: 16-deep tiles from 4 work-groups
-        // per CU, else 8-deep tiles (twice the work-groups) from 2 per CU.  Measured at 16,384
-        // pairs: 96^3 (432 8-deep tiles) ties with the texture path, 112^3 (686) runs 12.1 ms
-        // against 19.4 ms.
+        // bin windows) and has enough tiles to fill the GPU.  Real formats: 16^3 tiles of 256
+        // lanes (16 voxels per lane) from 1,024 tiles, 16^3 tiles of 512 lanes (8 per lane)
+        // from 512; else, and for complex formats, 16 x 16 x 8 tiles of 256 lanes from 512.
+        // Measured at 16,384 pairs (gather ms, the three shapes in that order, texture path
+        // last): 256^3 81.7 / 85.8 / 96.7 / 141; 128^3 16.1 / 12.8 / 14.4 / 20.0; 112^3 - /
+        // 13.2 / 12.1 / 19.4; 96^3 - / - / 10.2 / 10.0.
         const uint32_t seg = lds_seg_elems(pl->format);
         if (o.variant < 0 && pl->fused && pl->sep && pl->flat) {
             const uint64_t t16 = (uint64_t)((n[0] + 15) / 16) * ((n[1] + 15) / 16) * ((n[2] + 15) / 16);
             const uint64_t t8 = (uint64_t)((n[0] + 15) / 16) * ((n[1] + 15) / 16) * ((n[2] + 7) / 8);
-            const int deep = t16 >= kLdsDeepTiles && lds_variant_kz(kLdsVariant, pl->format) == 16
-                                 ? kLdsVariant : kLdsVariant + 1;
-            if (deep == kLdsVariant && confocal_lds_fits(inc, pl->inv2, 16, seg)) pl->variant = deep;
-            else if (t8 >= kLdsMinTiles && confocal_lds_fits(inc, pl->inv2, 8, seg)) pl->variant = kLdsVariant + 1;
+            const bool cplx = pl->format == kC64 || pl->format == kCF16 || pl->format == kCBF16;
+            const bool fits16 = confocal_lds_fits(inc, pl->inv2, 16, seg), fits8 = confocal_lds_fits(inc, pl->inv2, 8, seg);
+            if (!cplx && t16 >= kLdsDeepTiles && fits16) pl->variant = kLdsVariant;
+            else if (!cplx && t16 >= kLdsMinTiles && fits16) pl->variant = kLdsVariant + 2;
+            else if (t8 >= kLdsMinTiles && fits8) pl->variant = kLdsVariant + 1;
         }
         pl->lds_tiles = pl->variant >= kLdsVariant && pl->fused && pl->sep && pl->flat &&
-                        confocal_lds_fits(inc, pl->inv2, lds_variant_kz(pl->variant, pl->format), seg);
+                        confocal_lds_fits(inc, pl->inv2, lds_shape(pl->variant, pl->format).depth(), seg);
         pl->pair4.alloc(sizeof(float) * p4.size());
         hip_check(hipMemcpy(pl->pair4.ptr, p4.data(), sizeof(float) * p4.size(), hipMemcpyHostToDevice),
                   "hipMemcpy(pair table)");

@@ -66,7 +66,7 @@ def octree_comparison(bp, cap, side, tau, proposal=None, reps=3):
     with bp.Plan(*args) as plain:
         tp = plain.benchmark(1, reps)
         vp = plain.volume()
-        plain_lane_voxels = plain.info()["lane_voxels"]
+        plain_kernel = plain.info()["kernel"]
     # the texture-path gather (the machinery the octree's exact voxels use), bit-identical
     with bp.Plan(*args, variant=5) as plain_tex:
         tt = plain_tex.benchmark(1, reps)
@@ -95,7 +95,7 @@ def octree_comparison(bp, cap, side, tau, proposal=None, reps=3):
             if p is not None:
                 rows[name]["proposal_threshold"] = p
     return {"side": side, "levels": 4, "tau": tau, "plain_ms": tp["per_exec_ms"],
-            "plain_kernel": kernel_name(plain_lane_voxels, -1),
+            "plain_kernel": plain_kernel,
             "plain_texture_gather_ms": tt["per_exec_ms"],
             "occupied_voxels": int(occ.sum()),
             "occupied_definition": f"plain value >= {occ_frac} x max",
@@ -141,7 +141,8 @@ def _plan_line(pl, N, sample_bytes, workload, reps):
     info = pl.info()
     upd = info["pairs_used"] * N[0] * N[1] * N[2]
     g = t["gather_kernel_ms"]
-    return {"workload": workload, "kernel_path": info["path"], "lane_voxels": info["lane_voxels"],
+    return {"workload": workload, "kernel_path": info["path"], "kernel": info["kernel"],
+            "lane_voxels": info["lane_voxels"],
             "updates": upd, "exec_ms": t["per_exec_ms"], "gather_ms": g, "staging_ms": t["table_kernel_ms"],
             "updates_per_s": upd / (t["per_exec_ms"] * 1e-3), "gather_updates_per_s": upd / (g * 1e-3),
             "bytes_per_update": sample_bytes,
@@ -198,14 +199,6 @@ def config_measurements(bp, cap, reps=5):
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
-
-
-def kernel_name(lane_voxels, variant):
-    """The gather kernel a fused fp32 confocal plan runs: the LDS-staged kernel reports its
-    tile depth (8 or 16) as voxels per lane."""
-    if lane_voxels in (8, 16) and variant in (-1, 6, 7):
-        return f"confocal_lds_kernel<{lane_voxels}>"
-    return f"confocal_kernel<K={lane_voxels}>"
 
 
 def host_cpu():
@@ -357,7 +350,7 @@ def main():
     # summary (FETCH_SIZE / WRITE_SIZE scaled by the copy-kernel calibration)
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "roofline_traffic.json")
-    key = f"confocal_{info['path']}_lv{info['lane_voxels']}_{args.storage}_{args.side}_{N[0]}_{args.T}"
+    key = f"{info['kernel']}|{args.storage}|{args.side}|{N[0]}|{args.T}"
     if os.path.exists(tfile):
         tab = json.load(open(tfile))
         if key in tab:
@@ -384,7 +377,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "traffic_key": key,
-                     "kernel": kernel_name(info["lane_voxels"], args.variant), "kernel_ms": gather_ms,
+                     "kernel": info["kernel"], "kernel_ms": gather_ms,
                      "note": "achieved = 4 B per update (algorithmic) / kernel time; the LDS-staged "
                              "kernel fetches each staged sample once per tile and reads it from LDS for "
                              "every voxel of the tile, so this rate can exceed the HBM peak; its real "

@@ -77,7 +77,8 @@ typedef struct bpx_options {
     int32_t variant;      /* gather-kernel variant: -1 default (the LDS-staged kernel where it
                              applies: fp32 real samples, separable grid, flat wall, short bin
                              windows; else 5); 0-2 split tables, 3-5 fused rows with 4, 8, 2
-                             voxels per lane; 6 / 7 LDS-staged 16x16x16 / 16x16x8 tiles    */
+                             voxels per lane; 6 / 7 / 8 LDS-staged 16^3 tiles of 256 lanes,
+                             16x16x8 of 256, 16^3 of 512                                    */
 } bpx_options;
 
 /* Drop the last P mod min(256, P) pairs exactly as the reference launcher does
@@ -294,6 +295,10 @@ bpx_status bpx_plan_read_volume(bpx_plan plan, float* voxel_volume);
  * pairs used, chunk length, per-capture scale. */
 bpx_status bpx_plan_info(bpx_plan plan, int32_t* path, int32_t* lane_voxels, uint32_t* pairs_used,
                          uint32_t* Tc, float* scale);
+/* Name of the gather kernel an execution of the plan launches, with its tile shape (for
+ * profiles and reports), as a NUL-terminated string of at most `capacity` bytes.  No reference
+ * counterpart (the reference has one kernel per path). */
+bpx_status bpx_plan_kernel_name(bpx_plan plan, char* out, uint64_t capacity);
 
 /* Copies the threshold table of a confocal plan's last execution to host: out[p][j],
  * p < pairs_used, j <= Tc, = the smallest squared wall-point-to-voxel distance whose single-tap

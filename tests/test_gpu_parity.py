@@ -71,7 +71,7 @@ def test_confocal_bins_bit_exact_all_variants_and_non_separable_basis(gpu, oracl
     N = (18, 17, 19)
     zero, inc = grid_basis(N)
     ref = oracle.a1_volume_f64acc(H, pr, N, zero, inc, 2.05, 0.001)
-    for v in range(8):
+    for v in range(9):
         vol = gpu.backprojection(H, pr, N, zero, inc, 2.05, 0.001, variant=v)
         assert np.array_equal(vol, ref.astype(f32)), f"variant {v}"
     # a rotated basis (wall normal) makes the grid non-separable and the wall non-flat
@@ -85,11 +85,12 @@ def test_confocal_bins_bit_exact_all_variants_and_non_separable_basis(gpu, oracl
         assert np.array_equal(vol2, ref2.astype(f32)), f"variant {v}"
 
 
-@pytest.mark.parametrize("variant,kz", [(6, 16), (7, 8)])
+@pytest.mark.parametrize("variant,kz", [(6, 16), (7, 8), (8, 8)])
 @pytest.mark.parametrize("N,size,kahan", [((35, 33, 40), (0.25, 0.22, 0.3), 0), ((16, 16, 16), (0.1, 0.1, 0.1), 3),
                                           ((50, 17, 9), (0.25, 0.08, 0.03), 0), ((1, 1, 1), (0.1, 0.1, 0.1), 0)])
 def test_lds_staged_kernel_bit_exact(gpu, oracle, N, size, kahan, variant, kz):
-    """The LDS-staged kernels (16 x 16 x kz tiles, each pair's bin window copied to LDS) at a
+    """The LDS-staged kernels (16 x 16 x 16 / 16 x 16 x 8 tiles with 256 lanes, 16^3 tiles with
+    512 lanes; each pair's bin window copied to LDS) at a
     voxel pitch whose windows fit: bins exact against the oracle, and volumes identical to the
     texture-path kernel (same element per update, same order, same Kahan blocks).  Partial
     tiles in every axis, a one-voxel volume, and bins below 0 and above Tc."""
@@ -126,13 +127,13 @@ def test_lds_staged_kernel_formats(gpu, storage, cplx):
     if cplx:
         H = (H - 1j * rng.gamma(0.5, 2.0, H.shape)).astype(np.complex64)
     vols = {}
-    for v in (6, 7, 5):
+    for v in (6, 7, 8, 5):
         with gpu.Plan(H, pr, N, zero, inc, 1.2, 0.001, storage=storage, variant=v) as plan:
             lv = plan.info()["lane_voxels"]
-            assert lv == {6: 8 if cplx else 16, 7: 8, 5: 2}[v], (v, lv)
+            assert lv == {6: 8 if cplx else 16, 7: 8, 8: 8, 5: 2}[v], (v, lv)
             plan.execute()
             vols[v] = plan.volume()
-    assert np.array_equal(vols[6], vols[5]) and np.array_equal(vols[7], vols[5])
+    assert all(np.array_equal(vols[v], vols[5]) for v in (6, 7, 8))
     assert np.abs(vols[5]).max() > 0
 
 
