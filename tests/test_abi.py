@@ -75,6 +75,15 @@ def test_size_mismatch_is_invalid(bpx):
     assert st == 1 and b"transient_size" in _native.lib().bpx_last_error()
 
 
+def test_plan_queries_reject_null(bpx):
+    from backprojector_amd import _native
+    L = _native.lib()
+    buf = ctypes.create_string_buffer(64)
+    assert L.bpx_plan_kernel_name(None, buf, len(buf)) == 1
+    assert b"NULL" in L.bpx_last_error()
+    assert L.bpx_plan_synchronize(None) == 1 and L.bpx_plan_execute(None, None) == 1
+
+
 def test_nlosbpy_signature_matches_reference(bpx):
     import inspect
     sig = inspect.signature(bpx.nlosbpy.backproject)
