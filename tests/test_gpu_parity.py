@@ -137,6 +137,29 @@ def test_lds_staged_kernel_formats(gpu, storage, cplx):
     assert np.abs(vols[5]).max() > 0
 
 
+@pytest.mark.parametrize("variant", [6, 7, 8])
+def test_lds_staged_kernel_windows_before_the_row(gpu, oracle, variant):
+    """Pair count not a multiple of any stage size (15 x 15 = 225) and t0 late enough that the
+    bin windows of near voxels start before bin 0 (the staging path that zero-fills elements
+    below the row) while far voxels fall past Tc: bins exact against the oracle."""
+    pr = confocal_pairs(15, jitter=0.01, seed=11)
+    T = 2000
+    H = coded_samples(len(pr), T, seed=9)
+    N = (33, 35, 31)
+    zero, inc = grid_basis(N, pos=(0.0, 0.05, 0.45), size=(0.25, 0.25, 0.25))
+    ref = oracle.a1_volume_f64acc(H, pr, N, zero, inc, 3.3, 0.001)
+    with gpu.Plan(H, pr, N, zero, inc, 3.3, 0.001, variant=variant) as plan:
+        assert plan.info()["kernel"].startswith("confocal_lds_kernel")
+        plan.execute()
+        vol = plan.volume()
+    assert np.array_equal(vol, ref.astype(f32))
+    # the geometry does reach both ends: raw bins below 0 and past Tc for some voxel-pairs
+    vox = np.array([[0, 0, 0], [16, 17, 15], [32, 34, 30], [0, 34, 0]], np.uint32)
+    _, raw = oracle.a1_bins(pr, np.tile(vox, (len(pr), 1)), np.repeat(np.arange(len(pr)), len(vox)),
+                            zero, inc, 3.3, 0.001, T)
+    assert raw.min() < 0 and raw.max() >= T
+
+
 def test_lds_staged_kernel_falls_back(gpu, oracle):
     """Variant 6 where the LDS kernel does not apply (coarse pitch: windows too long; a wall
     that is not flat) runs the fused texture-path kernel and stays exact."""
