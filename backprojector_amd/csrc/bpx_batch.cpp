@@ -1,7 +1,8 @@
 // Batched uneven confocal captures: one device-resident plan per capture, the staging of every
-// capture (rows and threshold tables) in one pair of launches, all gathers in ONE persistent
-// work-queue launch, and the whole per-batch sequence (staging, queue reset, persistent
-// gather) captured in a hipGraph that bpx_batch_execute replays.  A project-written autotuner times a few launch
+// capture (rows and threshold tables) in one pair of launches, all gathers in at most two
+// persistent work-queue launches (the LDS-staged kernel for captures whose bin windows fit its
+// segment, the texture-path kernel for the rest), and the whole per-batch sequence (staging,
+// queue reset, persistent gathers) captured in a hipGraph that bpx_batch_execute replays.  A project-written autotuner times a few launch
 // configurations (voxels per lane, resident groups, queue order) on the batch itself.
 // No reference counterpart: the reference reconstructs one capture per call
 // (call_cuda_backprojection, backproject_cuda.cu:370-500).
@@ -22,11 +23,13 @@ struct BatchImpl {
     hipStream_t stream = nullptr;
     hipEvent_t ev[2] = {};
     hipEvent_t done = nullptr, ready = nullptr;   // as PlanImpl::done / ready
-    DeviceBuffer caps, flags, items, next, remaining, tiles;
+    DeviceBuffer caps, flags, items, items_lds, next, remaining, tiles;
     DeviceBuffer stamps;   // uint64 [1 + captures]: batch start, then each capture's completion
     DeviceBuffer stage_jobs;                      // StageJob per capture with pairs
     uint32_t n_stage = 0, stage_zb = 0, stage_sb = 0;
-    uint32_t n_items = 0;
+    uint32_t n_items = 0;          // texture-path queue: (capture, tile * 4 + wave quarter)
+    uint32_t n_lds_items = 0;      // LDS-staged queue: (capture, 16 x 16 x 8 tile)
+    uint32_t lds_groups = 0;
     int format = kF32;     // storage of every capture (fused rows)
     bool cplx = false;     // complex captures and volumes
     bpx_batch_config cfg{2u, 0u, 0u};
@@ -59,6 +62,7 @@ void BatchImpl::build(const bpx_batch_config& c) {
     // per-capture arguments and tiles for this K
     std::vector<ConfocalArgs> args(n);
     std::vector<uint32_t> fl(n), nt(n);
+    std::vector<char> lds(n, 0);
     for (uint32_t i = 0; i < n; ++i) {
         const PlanImpl& pl = *plans[i];
         ConfocalArgs& a = args[i];
@@ -76,7 +80,10 @@ void BatchImpl::build(const bpx_batch_config& c) {
         a.lz = pl.lz;
         a.kahan_block = pl.kahan_block;
         fl[i] = (pl.sep ? 1u : 0u) | (pl.sep && pl.flat ? 2u : 0u);
-        nt[i] = pl.P ? 4u * a.t.count : 0u;  // one item per wave quarter of every tile
+        // captures whose bin windows fit the LDS segment go to the LDS-staged queue (one item
+        // per work-group tile), the others to the texture-path queue (one per wave quarter)
+        lds[i] = pl.P && batch_lds_applies(pl.g, pl.inv2, format, pl.sep, pl.flat);
+        nt[i] = !pl.P ? 0u : lds[i] ? batch_lds_tiles(pl.g) : 4u * a.t.count;
     }
     // queue: (capture, tile * 4 + quarter), the quarters of a tile adjacent; order 0 = captures
     // with most pairs first, order 1 = capture order
@@ -86,13 +93,15 @@ void BatchImpl::build(const bpx_batch_config& c) {
         std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
             return (uint64_t)plans[x]->P * c.lane_voxels > (uint64_t)plans[y]->P * c.lane_voxels;
         });
-    std::vector<uint32_t> it;
+    std::vector<uint32_t> it, itl;
     for (uint32_t i : order)
         for (uint32_t t = 0; t < nt[i]; ++t) {
-            it.push_back(i);
-            it.push_back(t);
+            std::vector<uint32_t>& q = lds[i] ? itl : it;
+            q.push_back(i);
+            q.push_back(t);
         }
     n_items = (uint32_t)(it.size() / 2);
+    n_lds_items = (uint32_t)(itl.size() / 2);
     // staging of every capture in one pair of launches: each job's blocks follow the previous job's
     std::vector<StageJob> jobs;
     uint64_t zb = 0, sb = 0;
@@ -117,16 +126,25 @@ void BatchImpl::build(const bpx_batch_config& c) {
     flags.alloc(4u * std::max<uint32_t>(n, 1));
     tiles.alloc(4u * std::max<uint32_t>(n, 1));
     items.alloc(8u * std::max<uint32_t>(n_items, 1));
+    items_lds.alloc(8u * std::max<uint32_t>(n_lds_items, 1));
     hip_check(hipMemcpy(caps.ptr, args.data(), sizeof(ConfocalArgs) * n, hipMemcpyHostToDevice), "hipMemcpy");
     hip_check(hipMemcpy(flags.ptr, fl.data(), 4u * n, hipMemcpyHostToDevice), "hipMemcpy");
     hip_check(hipMemcpy(tiles.ptr, nt.data(), 4u * n, hipMemcpyHostToDevice), "hipMemcpy");
     if (n_items) hip_check(hipMemcpy(items.ptr, it.data(), 8u * n_items, hipMemcpyHostToDevice), "hipMemcpy");
+    if (n_lds_items)
+        hip_check(hipMemcpy(items_lds.ptr, itl.data(), 8u * n_lds_items, hipMemcpyHostToDevice), "hipMemcpy");
+    if (n_lds_items) {
+        const int rl = batch_lds_resident_groups(format);
+        if (rl <= 0) fail(BPX_ERR_DEVICE, "cannot query the occupancy of the batch kernel");
+        lds_groups = std::min<uint32_t>((uint32_t)rl, n_lds_items);
+    }
     const int resident = batch_resident_groups(K, format);
     if (resident <= 0) fail(BPX_ERR_DEVICE, "cannot query the occupancy of the batch kernel");
     groups = c.groups ? std::min<uint32_t>(c.groups, (uint32_t)resident) : (uint32_t)resident;
     groups = std::max<uint32_t>(1u, std::min<uint32_t>(groups, std::max<uint32_t>(n_items, 1u)));
     cfg = c;
     cfg.groups = groups;
+    cfg.lds_captures = (uint32_t)std::count(lds.begin(), lds.end(), (char)1);
     // the per-batch sequence as one graph (no memset nodes: the queue reset is a kernel)
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
@@ -146,13 +164,24 @@ void BatchImpl::build(const bpx_batch_config& c) {
                                      static_cast<const uint32_t*>(tiles.ptr), n, static_cast<uint64_t*>(stamps.ptr),
                                      stream),
                   "batch reset kernel");
+        if (n_lds_items) {
+            BatchArgs b{};
+            b.caps = static_cast<const ConfocalArgs*>(caps.ptr);
+            b.flags = static_cast<const uint32_t*>(flags.ptr);
+            b.items = static_cast<const uint2*>(items_lds.ptr);
+            b.n_items = n_lds_items;
+            b.next = static_cast<uint32_t*>(next.ptr);
+            b.remaining = static_cast<uint32_t*>(remaining.ptr);
+            b.done = static_cast<uint64_t*>(stamps.ptr) + 1;
+            hip_check(launch_batch_lds(b, format, lds_groups, stream), "batch LDS-staged gather kernel");
+        }
         if (n_items) {
             BatchArgs b{};
             b.caps = static_cast<const ConfocalArgs*>(caps.ptr);
             b.flags = static_cast<const uint32_t*>(flags.ptr);
             b.items = static_cast<const uint2*>(items.ptr);
             b.n_items = n_items;
-            b.next = static_cast<uint32_t*>(next.ptr);
+            b.next = static_cast<uint32_t*>(next.ptr) + 1;
             b.remaining = static_cast<uint32_t*>(remaining.ptr);
             b.done = static_cast<uint64_t*>(stamps.ptr) + 1;
             hip_check(launch_batch_confocal(b, K, format, groups, stream), "batch gather kernel");
@@ -239,7 +268,7 @@ bpx_status bpx_batch_create(const bpx_capture_desc* captures, uint32_t count, co
             b->plans.push_back(std::move(pl));
         }
         const uint32_t n = std::max<uint32_t>(count, 1);
-        b->next.alloc(4);
+        b->next.alloc(8);   // queue heads: LDS-staged, texture path
         b->remaining.alloc(4u * n);
         b->stamps.alloc(8u * (n + 1u));
         b->build(bpx_batch_config{2u, 0u, 0u});

@@ -241,6 +241,12 @@ hipError_t launch_batch_reset(uint32_t* next, uint32_t* remaining, const uint32_
 // format: kF32, kF16, kBF16, kC64, kCF16 or kCBF16 (fused rows); K: 2 or 4 voxels per lane.
 hipError_t launch_batch_confocal(const BatchArgs& b, int K, int format, uint32_t groups, hipStream_t s);
 int batch_resident_groups(int K, int format);  // work-groups resident at once (0 if unknown)
+// The LDS-staged persistent batch kernel (16 x 16 x 8 tiles, one item per work-group and tile):
+// whether a capture's grid fits it, its tiles, resident work-groups, launch (queue head next[0]).
+bool batch_lds_applies(const Grid& g, float inv2, int format, bool sep, bool flat);
+uint32_t batch_lds_tiles(const Grid& g);
+int batch_lds_resident_groups(int format);
+hipError_t launch_batch_lds(const BatchArgs& b, int format, uint32_t groups, hipStream_t s);
 hipError_t launch_window(const WindowArgs& a, int complex_samples, hipStream_t stream);
 hipError_t launch_abs_max(const float* src, uint64_t n, float* result_dev, hipStream_t stream);
 hipError_t launch_copy(void* dst, const void* src, uint64_t bytes, hipStream_t stream);

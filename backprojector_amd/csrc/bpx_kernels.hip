@@ -633,8 +633,10 @@ constexpr uint32_t kSegTileXY = 16;   // tile edge in x and y
 // staged elements per pair: 512 of 8 B, or 384 of 16 B (complex64)
 __host__ __device__ constexpr uint32_t seg_elems(int elem_bytes) { return elem_bytes == 8 ? 512u : 384u; }
 
+// One tile of the LDS-staged gather for the whole work-group (NW waves).  It ends with a
+// work-group barrier, so a persistent caller may run the next tile right after it.
 template <class Fz, int KZ, int NW>
-__global__ __launch_bounds__(NW * 64) void confocal_lds_kernel(ConfocalArgs a) {
+__device__ __forceinline__ void lds_tile(const ConfocalArgs& a, uint32_t tile) {
     using acc_t = typename Fz::acc_t;
     // EG: bytes of a row element in HBM; E: bytes staged per element (its leading part)
     constexpr uint32_t EG = Fz::kElemBytes, E = Fz::kLdsBytes, SEG = seg_elems(Fz::kLdsBytes), NL = SEG / 64u;
@@ -646,7 +648,6 @@ __global__ __launch_bounds__(NW * 64) void confocal_lds_kernel(ConfocalArgs a) {
     const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63u;
     const uint32_t n0 = a.g.n[0], n1 = a.g.n[1], n2 = a.g.n[2];
     const uint32_t ntz = (n2 + TZ - 1u) / TZ, nty = (n1 + kSegTileXY - 1u) / kSegTileXY;
-    const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
     const uint32_t tz = tile % ntz, txy = tile / ntz, ty = txy % nty, tx = txy / nty;
     const uint32_t ix = tx * kSegTileXY + (w & 1u) * 8u + (lane & 7u);
     const uint32_t iy = ty * kSegTileXY + ((w >> 1) & 1u) * 8u + (lane >> 3);
@@ -807,6 +808,11 @@ __global__ __launch_bounds__(NW * 64) void confocal_lds_kernel(ConfocalArgs a) {
     }
 }
 
+template <class Fz, int KZ, int NW>
+__global__ __launch_bounds__(NW * 64) void confocal_lds_kernel(ConfocalArgs a) {
+    lds_tile<Fz, KZ, NW>(a, xcd_remap(blockIdx.x, gridDim.x));
+}
+
 // ------------------------------------------------------------------- batched captures
 // Persistent work queue over (capture, tile quarter) items of many confocal fp32 captures:
 // every WAVE takes the next item, runs the same tile body as the single-capture kernel (each
@@ -853,6 +859,35 @@ __global__ __launch_bounds__(256) void batch_confocal_kernel(BatchArgs b) {
     }
 }
 
+// Persistent work queue over (capture, 16 x 16 x 8 tile) items of the captures whose bin
+// windows fit the LDS segment: every WORK-GROUP takes the next item (one atomic by thread 0,
+// broadcast through LDS between two barriers, so the exit test is the same for all its
+// waves) and runs the LDS-staged tile body, which ends with a barrier before the next item
+// reuses the LDS buffers.  Volumes are bit-identical to the single-capture kernels'.
+template <class Fz>
+__global__ __launch_bounds__(256) void batch_lds_kernel(BatchArgs b) {
+    __shared__ uint32_t s_item;
+    // a work-group can take at most every item once: the bound makes termination unconditional
+    for (uint32_t taken = 0; taken <= b.n_items; ++taken) {
+        if (threadIdx.x == 0) s_item = atomicAdd(b.next, 1u);
+        __syncthreads();
+        const uint32_t it = __builtin_amdgcn_readfirstlane(s_item);
+        __syncthreads();
+        if (it >= b.n_items) return;
+        const uint2 item = b.items[it];
+        const uint32_t c = __builtin_amdgcn_readfirstlane(item.x);
+        const uint32_t tile = __builtin_amdgcn_readfirstlane(item.y);
+        const ConfocalArgs a = wave_uniform_load(b.caps + c);
+        lds_tile<Fz, 8, 4>(a, tile);
+        __threadfence();
+        if (threadIdx.x == 0) {
+            const uint32_t left = atomicSub(&b.remaining[c], 1u);
+            if (left == 1u) atomicMax(reinterpret_cast<unsigned long long*>(&b.done[c]),
+                                      (unsigned long long)wall_clock64());
+        }
+    }
+}
+
 // Resets the queue (next = 0, remaining = items per capture, completion stamps 0) and stamps
 // the start time: stamps[0] = start, stamps[1 + i] = completion of capture i.
 __global__ void batch_reset_kernel(uint32_t* next, uint32_t* remaining, const uint32_t* tiles, uint32_t n,
@@ -863,7 +898,8 @@ __global__ void batch_reset_kernel(uint32_t* next, uint32_t* remaining, const ui
         start[1 + i] = 0ull;  // completion stamps (done) follow the start stamp
     }
     if (i == 0) {
-        *next = 0u;
+        next[0] = 0u;   // LDS-staged queue
+        next[1] = 0u;   // texture-path queue
         *start = wall_clock64();
     }
 }
@@ -893,8 +929,19 @@ static const void* batch_kernel(int K, int format) {
     return K == 4 ? batch_kernel_of<4>(format) : K == 2 ? batch_kernel_of<2>(format) : nullptr;
 }
 
-int batch_resident_groups(int K, int format) {
-    const void* fn = batch_kernel(K, format);
+static const void* batch_lds_kernel_of(int format) {
+    switch (format) {
+        case kF32: return reinterpret_cast<const void*>(batch_lds_kernel<FusedF32>);
+        case kF16: return reinterpret_cast<const void*>(batch_lds_kernel<FusedH16<false>>);
+        case kBF16: return reinterpret_cast<const void*>(batch_lds_kernel<FusedH16<true>>);
+        case kC64: return reinterpret_cast<const void*>(batch_lds_kernel<FusedC64>);
+        case kCF16: return reinterpret_cast<const void*>(batch_lds_kernel<FusedC16<false>>);
+        case kCBF16: return reinterpret_cast<const void*>(batch_lds_kernel<FusedC16<true>>);
+        default: return nullptr;
+    }
+}
+
+static int resident_of(const void* fn) {
     int dev = 0, cus = 0, per_cu = 0;
     if (!fn || hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
@@ -907,6 +954,25 @@ int batch_resident_groups(int K, int format) {
     }
     return cus * per_cu;
 }
+
+int batch_lds_resident_groups(int format) { return resident_of(batch_lds_kernel_of(format)); }
+
+hipError_t launch_batch_lds(const BatchArgs& b, int format, uint32_t groups, hipStream_t s) {
+    if (groups == 0u) return hipErrorInvalidValue;
+    const void* fn = batch_lds_kernel_of(format);
+    if (!fn) return hipErrorInvalidValue;
+    void* args[] = {const_cast<BatchArgs*>(&b)};
+    return hipLaunchKernel(fn, dim3(groups), dim3(256), args, 0, s);
+}
+
+bool batch_lds_applies(const Grid& g, float inv2, int format, bool sep, bool flat) {
+    return sep && flat && confocal_lds_fits(g.inc, inv2, 8, lds_seg_elems(format));
+}
+uint32_t batch_lds_tiles(const Grid& g) {
+    return ((g.n[0] + kSegTileXY - 1u) / kSegTileXY) * ((g.n[1] + kSegTileXY - 1u) / kSegTileXY) * ((g.n[2] + 7u) / 8u);
+}
+
+int batch_resident_groups(int K, int format) { return resident_of(batch_kernel(K, format)); }
 
 hipError_t launch_batch_confocal(const BatchArgs& b, int K, int format, uint32_t groups, hipStream_t s) {
     if (groups == 0u) return hipErrorInvalidValue;

@@ -347,6 +347,8 @@ typedef struct bpx_batch_config {
     uint32_t lane_voxels;  /* voxels per lane: 2 or 4                                   */
     uint32_t groups;       /* persistent work-groups (0 = all that are resident at once) */
     uint32_t order;        /* queue: 0 = captures with most pairs first, 1 = given order */
+    uint32_t lds_captures; /* out (get/tune): captures on the LDS-staged queue (those whose
+                              bin windows fit its segment); ignored by set_config       */
 } bpx_batch_config;
 typedef struct bpx_batch_timing {
     double per_exec_ms;    /* mean device time of one batch execution (HIP events)       */

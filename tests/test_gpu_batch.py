@@ -46,6 +46,29 @@ def test_batch_equals_single_captures(gpu):
         assert np.all(lat > 0) and np.all(np.isfinite(lat))
 
 
+def test_batch_lds_staged_queue(gpu, oracle):
+    """Captures whose bin windows fit the LDS segment (4 mm bins, fine enough voxel pitch) run on
+    the LDS-staged persistent queue, the others on the texture-path queue, in the same batch;
+    every volume equals its capture reconstructed alone, and the coded ones the oracle."""
+    rng = np.random.default_rng(7)
+    spec = [(8, 1024, 24), (12, 1024, 30), (16, 2048, 20), (8, 1024, 40), (10, 1024, 33), (6, 512, 12)]
+    caps = [capture(rng, *sp, coded=(i % 2 == 0)) for i, sp in enumerate(spec)]
+    alone = [gpu.backprojection(*c) for c in caps]
+    with gpu.Batch(caps) as b:
+        cfg = b.config()
+        assert 3 <= cfg["lds_captures"] < len(caps), cfg
+        for rep in range(2):
+            b.execute()
+            b.synchronize()
+            for i, ref in enumerate(alone):
+                assert np.array_equal(b.volume(i), ref), (rep, i)
+        assert np.all(b.latencies() > 0)
+    for i in (0, 2, 4):
+        H, pr, Nv, zero, inc, t0, dT = caps[i]
+        ref = oracle.a1_volume_f64acc(H, pr, Nv, zero, inc, t0, dT)
+        assert np.array_equal(alone[i], ref.astype(np.float32)), i
+
+
 def test_batch_execute_on_caller_stream(gpu, hip_streams):
     caps = uneven(5)[:3]
     alone = [gpu.backprojection(*c) for c in caps]
