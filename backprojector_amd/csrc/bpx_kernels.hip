@@ -634,8 +634,10 @@ constexpr uint32_t kSegTileXY = 16;   // tile edge in x and y
 __host__ __device__ constexpr uint32_t seg_elems(int elem_bytes) { return elem_bytes == 8 ? 512u : 384u; }
 
 // One tile of the LDS-staged gather for the whole work-group (NW waves).  It ends with a
-// work-group barrier, so a persistent caller may run the next tile right after it.
-template <class Fz, int KZ, int NW>
+// work-group barrier, so a persistent caller may run the next tile right after it.  BOUNDED:
+// stage only the 64-element groups a window touches (pays off for short windows, costs
+// branches for long ones).
+template <class Fz, int KZ, int NW, bool BOUNDED = false>
 __device__ __forceinline__ void lds_tile(const ConfocalArgs& a, uint32_t tile) {
     using acc_t = typename Fz::acc_t;
     // EG: bytes of a row element in HBM; E: bytes staged per element (its leading part)
@@ -721,13 +723,14 @@ __device__ __forceinline__ void lds_tile(const ConfocalArgs& a, uint32_t tile) {
         }
         if (p < a.P && pjlo >= 0) {
             // Window inside or past the row: a descriptor over row p alone returns 0 for every
-            // element past its end, so no lane needs a bounds test; elements past the window
-            // are loaded too and never read.
+            // element past its end, so no lane needs a bounds test (unbounded: elements past the
+            // window are loaded too and never read).
             const __amdgpu_buffer_rsrc_t rr =
                 make_rsrc(static_cast<const char*>(a.H) + (size_t)p * a.ldh * EG, row_bytes);
             const int vo = (pjlo + (int)lane) * (int)EG;
 #pragma unroll
-            for (uint32_t m = 0; m < NL; ++m) pre[m] = load(rr, vo + (int)(64u * EG * m), 0);
+            for (uint32_t m = 0; m < NL; ++m)
+                if (!BOUNDED || (int)(64u * m) < pcnt) pre[m] = load(rr, vo + (int)(64u * EG * m), 0);
         } else {
             // window starting before the row (or no pair): elements below 0 are zeros
             const uint32_t soff = p < a.P ? p * a.ldh * EG : 0u;
@@ -741,8 +744,10 @@ __device__ __forceinline__ void lds_tile(const ConfocalArgs& a, uint32_t tile) {
         }
     };
     auto commit = [&](uint32_t buf) {
+        // (bounded: only the groups the window touches; no lookup reads past element pcnt - 1)
 #pragma unroll
-        for (uint32_t m = 0; m < NL; ++m) seg[buf][w][lane + 64u * m] = pre[m];
+        for (uint32_t m = 0; m < NL; ++m)
+            if (!BOUNDED || (int)(64u * m) < pcnt) seg[buf][w][lane + 64u * m] = pre[m];
         if (lane == 0) {
             meta[buf][w][0] = pjlo;
             meta[buf][w][1] = pcnt;
@@ -878,7 +883,7 @@ __global__ __launch_bounds__(256) void batch_lds_kernel(BatchArgs b) {
         const uint32_t c = __builtin_amdgcn_readfirstlane(item.x);
         const uint32_t tile = __builtin_amdgcn_readfirstlane(item.y);
         const ConfocalArgs a = wave_uniform_load(b.caps + c);
-        lds_tile<Fz, 8, 4>(a, tile);
+        lds_tile<Fz, 8, 4, true>(a, tile);
         __threadfence();
         if (threadIdx.x == 0) {
             const uint32_t left = atomicSub(&b.remaining[c], 1u);
