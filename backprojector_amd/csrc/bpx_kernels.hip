@@ -42,10 +42,16 @@ struct SqrtPred {
     __device__ bool operator()(uint32_t key) const { return exact_sqrt(fkey_inv(key)) >= d; }
 };
 
-// Guesses in double: d ~ (z - lw - cw) / 2, and for the second search the square of the midpoint
-// between d* and the float below it (the smallest s whose correctly rounded root reaches d*
-// lies just above that square), rounded up to a float.  A good guess leaves the gallop two
-// predicate evaluations; the searches keep the result exact whatever the guess.
+// The first search starts from a guess in double, d ~ (z - lw - cw) / 2; a good guess leaves
+// the gallop two predicate evaluations and the search keeps the result exact whatever the guess.
+//
+// The second needs no search for d* in [2^-60, 2^60]: with d- the float below d* and
+// m = (d- + d*) / 2 (exact in double: 26 significant bits), sqrt_rn(s) >= d* iff sqrt(s) > m, or
+// sqrt(s) = m and the tie rounds to d* (round half to even: d*'s last mantissa bit is 0).  m^2
+// has at most 52 significant bits, so it is exact in double, and sqrt(s) > m iff s > m^2.  The
+// smallest such float is m^2 rounded up to a float, one float higher when m^2 is itself a float
+// and the tie goes to d- (then sqrt_rn(m^2) = d-).  Outside that range (0, tiny, huge or
+// infinite d*) the search below runs from the same guess.
 __device__ __forceinline__ float stable_threshold(float lw, float cw, float z) {
     const double gd = 0.5 * (((double)z - (double)lw) - (double)cw);
     const float g = gd > 0.0 ? (float)gd : 0.0f;
@@ -58,6 +64,10 @@ __device__ __forceinline__ float stable_threshold(float lw, float cw, float z) {
         const double msq = m * m;
         sg = (float)msq;
         if ((double)sg < msq) sg = __uint_as_float(__float_as_uint(sg) + 1u);
+        if (dstar >= 0x1p-60f && dstar <= 0x1p60f) {
+            if ((double)sg == msq && (__float_as_uint(dstar) & 1u)) sg = __uint_as_float(__float_as_uint(sg) + 1u);
+            return sg;
+        }
     }
     return fkey_inv(min_true_key(SqrtPred{dstar}, fkey(sg), fkey(0.0f), fkey(__builtin_inff())));
 }
