@@ -328,6 +328,7 @@ bpx_status bpx_plan_kernel_name(bpx_plan plan, char* out, uint64_t capacity) {
             const LdsShape sh = lds_shape(pl.variant, pl.format);
             name = "confocal_lds_kernel<KZ=" + std::to_string(sh.kz) + ", waves=" + std::to_string(sh.nw) +
                    "> (16x16x" + std::to_string(sh.depth()) + " tiles)";
+            if (pl.format == kC64) name += " x2 (real and imaginary passes)";
         } else {
             static const char* paths[] = {"confocal_kernel", "grid_kernel", "pairs_kernel", "window_kernel",
                                           "octree kernels"};

@@ -211,6 +211,26 @@ struct FusedC64 {
         return make_float2(__uint_as_float(up ? b.x : a.x), __uint_as_float(up ? b.y : a.y));
     }
 };
+// One component (COMP 0 = real, 1 = imaginary) of complex64 fused rows, as the LDS-staged
+// kernel stages it: (c[j], S[j]) 8-B elements, picked like fp32 rows.  Two passes (one per
+// component) give the complex volume; each component's sum is the complex kernel's.
+template <int COMP>
+struct FusedC64Part {
+    static constexpr int kElemBytes = 16;
+    static constexpr int kLdsBytes = 8;
+    static constexpr int kComponent = COMP;
+    using acc_t = float;
+    __device__ static __forceinline__ uint2 stage_load(__amdgpu_buffer_rsrc_t r, int off, int soff) {
+        const auto t = __builtin_amdgcn_raw_buffer_load_b96(r, off, soff, 0);
+        return make_uint2(COMP ? t[1] : t[0], t[2]);
+    }
+    __device__ static __forceinline__ float lds_pick(const char* e, float s) {
+        const uint2 a = *reinterpret_cast<const uint2*>(e);
+        const float h1 = *reinterpret_cast<const float*>(e + 8);
+        return s >= __uint_as_float(a.y) ? h1 : __uint_as_float(a.x);
+    }
+};
+
 template <bool BF>
 struct FusedC16 {
     static constexpr int kElemBytes = 16;

@@ -259,7 +259,8 @@ constexpr uint64_t kLdsDeepTiles = 1024;  // 16^3 tiles of 256 lanes by default 
 constexpr uint64_t kLdsMinTiles = 512;    // the other LDS shapes by default from this many up
 // Whether the LDS-staged kernel's per-pair bin windows over 16 x 16 x kz tiles fit a segment of
 // seg elements for this grid spacing; the segment length of a storage format; the tile depth a
-// variant (6 or 7) gives for a storage format (complex formats: always 8).
+// variant (6, 7 or 8) gives for a storage format (complex 16-bit formats: 16 x 16 x 8 tiles;
+// complex64 runs as two real passes with the real formats' shapes).
 bool confocal_lds_fits(const float inc[9], float inv2, int kz, uint32_t seg);
 uint32_t lds_seg_elems(int format);
 // Tile of an LDS-staged variant (6, 7, 8) for a storage format: kz voxels per lane along z,

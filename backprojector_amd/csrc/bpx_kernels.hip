@@ -647,6 +647,12 @@ __host__ __device__ constexpr uint32_t seg_elems(int elem_bytes) { return elem_b
 // work-group barrier, so a persistent caller may run the next tile right after it.  BOUNDED:
 // stage only the 64-element groups a window touches (pays off for short windows, costs
 // branches for long ones).
+// Sample formats that stage one component of complex rows (FusedC64Part).
+template <class T, class = void>
+struct is_component : std::false_type {};
+template <class T>
+struct is_component<T, std::void_t<decltype(T::kComponent)>> : std::true_type {};
+
 template <class Fz, int KZ, int NW, bool BOUNDED = false>
 __device__ __forceinline__ void lds_tile(const ConfocalArgs& a, uint32_t tile) {
     using acc_t = typename Fz::acc_t;
@@ -703,7 +709,9 @@ __device__ __forceinline__ void lds_tile(const ConfocalArgs& a, uint32_t tile) {
     const float ez2 = ez * ez, fz2 = fz * fz;
 
     auto load = [&](__amdgpu_buffer_rsrc_t r, int off, int soff) {
-        if constexpr (E == 8) {
+        if constexpr (is_component<Fz>::value) {
+            return Fz::stage_load(r, off, soff);
+        } else if constexpr (E == 8) {
             const auto t = __builtin_amdgcn_raw_buffer_load_b64(r, off, soff, 0);
             return make_uint2(t[0], t[1]);
         } else {
@@ -817,7 +825,10 @@ __device__ __forceinline__ void lds_tile(const ConfocalArgs& a, uint32_t tile) {
             const uint32_t iz = iz0 + r;
             if (iz < n2) {
                 const size_t id = ((size_t)ix * n1 + iy) * n2 + iz;
-                store_voxel(a.out, id, scaled(kahan_result(sum[r], comp[r]), a.scale));
+                if constexpr (is_component<Fz>::value)   // one component of a complex volume
+                    static_cast<float*>(a.out)[2 * id + Fz::kComponent] = kahan_result(sum[r], comp[r]) * a.scale;
+                else
+                    store_voxel(a.out, id, scaled(kahan_result(sum[r], comp[r]), a.scale));
             }
         }
     }
@@ -981,7 +992,9 @@ hipError_t launch_batch_lds(const BatchArgs& b, int format, uint32_t groups, hip
 }
 
 bool batch_lds_applies(const Grid& g, float inv2, int format, bool sep, bool flat) {
-    return sep && flat && confocal_lds_fits(g.inc, inv2, 8, lds_seg_elems(format));
+    // the batch kernel stages complex64 rows as 16-B elements (both components in one pass)
+    const uint32_t seg = seg_elems(format == kC64 ? FusedC64::kLdsBytes : 8);
+    return sep && flat && confocal_lds_fits(g.inc, inv2, 8, seg);
 }
 uint32_t batch_lds_tiles(const Grid& g) {
     return ((g.n[0] + kSegTileXY - 1u) / kSegTileXY) * ((g.n[1] + kSegTileXY - 1u) / kSegTileXY) * ((g.n[2] + 7u) / 8u);
@@ -1296,10 +1309,10 @@ hipError_t launch_window(const WindowArgs& a, int complex_samples, hipStream_t s
 
 // ------------------------------------------------------------------- dispatch
 // Variants v = 3*layout + k: layout 0 = split tables, 1 = fused layout; k: 0 -> K = 4,
-// 1 -> K = 8, 2 -> K = 2 voxels per lane.  Variants 6 and 7: the LDS-staged kernel with
-// 16 x 16 x 16 / 16 x 16 x 8 tiles (complex formats: 16 x 16 x 8 for both) for fused rows on a
-// separable grid and a flat wall when the bin windows fit, otherwise the fused layout with
-// K = 4 / 8.
+// 1 -> K = 8, 2 -> K = 2 voxels per lane.  Variants 6, 7 and 8: the LDS-staged kernel with
+// 16^3 tiles of 256 lanes, 16 x 16 x 8 tiles of 256 lanes, 16^3 tiles of 512 lanes (complex
+// 16-bit rows: 16 x 16 x 8 for 6 and 7) for fused rows on a separable grid and a flat wall when
+// the bin windows fit, otherwise the fused layout with K = 4 / 8 / 2.
 int variant_k(int variant) {
     switch (variant % 3) {
         case 1: return 8;
@@ -1355,11 +1368,12 @@ bool confocal_lds_fits(const float inc[9], float inv2, int kz, uint32_t seg) {
     const double need = std::ceil(std::sqrt(d2) * std::fabs((double)inv2) * (1.0 + 1e-6)) + 8.0;
     return std::isfinite(need) && need <= (double)seg;
 }
-uint32_t lds_seg_elems(int format) { return seg_elems(format == kC64 ? 16 : 8); }
+uint32_t lds_seg_elems(int) { return seg_elems(8); }   // every format stages 8-B elements
 LdsShape lds_shape(int variant, int format) {
-    const bool cplx = format == kC64 || format == kCF16 || format == kCBF16;
-    if (variant == kLdsVariant + 2 && format != kC64) return {8, 8};   // 512 lanes, 16^3 tiles
-    if (cplx || variant != kLdsVariant) return {8, 4};                 // complex: 2 accumulators per voxel
+    // complex 16-bit rows accumulate two components per voxel; complex64 runs as two real passes
+    const bool cplx = format == kCF16 || format == kCBF16;
+    if (variant == kLdsVariant + 2) return {8, 8};                     // 512 lanes, 16^3 tiles
+    if (cplx || variant != kLdsVariant) return {8, 4};
     return {16, 4};
 }
 bool confocal_lds_applies(const ConfocalArgs& a, int format, int variant, bool sep, bool flat) {
@@ -1390,7 +1404,10 @@ hipError_t launch_confocal(const ConfocalArgs& a, int format, int variant, bool 
             case kF32: return lds_launch_shape<FusedF32>(a, sh, stream);
             case kF16: return lds_launch_shape<FusedH16<false>>(a, sh, stream);
             case kBF16: return lds_launch_shape<FusedH16<true>>(a, sh, stream);
-            case kC64: return lds_launch<FusedC64, 8, 4>(a, stream);
+            case kC64: {   // two real passes, one per component, at the real formats' shapes
+                const hipError_t e = lds_launch_shape<FusedC64Part<0>>(a, sh, stream);
+                return e != hipSuccess ? e : lds_launch_shape<FusedC64Part<1>>(a, sh, stream);
+            }
             case kCF16: return lds_launch<FusedC16<false>, 8, 4>(a, stream);
             case kCBF16: return lds_launch<FusedC16<true>, 8, 4>(a, stream);
             default: return hipErrorInvalidValue;

@@ -440,7 +440,7 @@ PlanImpl* make_plan_single_tap(const CaptureIn& c, const uint32_t n[3], const fl
         if (o.variant < 0 && pl->fused && pl->sep && pl->flat) {
             const uint64_t t16 = (uint64_t)((n[0] + 15) / 16) * ((n[1] + 15) / 16) * ((n[2] + 15) / 16);
             const uint64_t t8 = (uint64_t)((n[0] + 15) / 16) * ((n[1] + 15) / 16) * ((n[2] + 7) / 8);
-            const bool cplx = pl->format == kC64 || pl->format == kCF16 || pl->format == kCBF16;
+            const bool cplx = pl->format == kCF16 || pl->format == kCBF16;   // two accumulators per voxel
             const bool fits16 = confocal_lds_fits(inc, pl->inv2, 16, seg), fits8 = confocal_lds_fits(inc, pl->inv2, 8, seg);
             if (!cplx && t16 >= kLdsDeepTiles && fits16) pl->variant = kLdsVariant;
             else if (!cplx && t16 >= kLdsMinTiles && fits16) pl->variant = kLdsVariant + 2;
