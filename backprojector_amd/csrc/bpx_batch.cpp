@@ -1,8 +1,9 @@
 // Batched uneven confocal captures: one device-resident plan per capture, the staging of every
 // capture (rows and threshold tables) in one pair of launches, all gathers in at most two
-// persistent work-queue launches (the LDS-staged kernel for captures whose bin windows fit its
-// segment, the texture-path kernel for the rest), and the whole per-batch sequence (staging,
-// queue reset, persistent gathers) captured in a hipGraph that bpx_batch_execute replays.  A project-written autotuner times a few launch
+// persistent work-queue launches running side by side (the LDS-staged kernel for captures whose
+// bin windows fit its segment, the texture-path kernel for the rest), and the whole per-batch
+// sequence (staging, queue reset, persistent gathers) captured in a hipGraph that
+// bpx_batch_execute replays.  A project-written autotuner times a few launch
 // configurations (voxels per lane, resident groups, queue order) on the batch itself.
 // No reference counterpart: the reference reconstructs one capture per call
 // (call_cuda_backprojection, backproject_cuda.cu:370-500).
@@ -23,6 +24,10 @@ struct BatchImpl {
     hipStream_t stream = nullptr;
     hipEvent_t ev[2] = {};
     hipEvent_t done = nullptr, ready = nullptr;   // as PlanImpl::done / ready
+    // the texture-path queue runs beside the LDS-staged one: a second stream (default
+    // priority) forked from and joined back into the captured sequence
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
     DeviceBuffer caps, flags, items, items_lds, next, remaining, tiles;
     DeviceBuffer stamps;   // uint64 [1 + captures]: batch start, then each capture's completion
     DeviceBuffer stage_jobs;                      // StageJob per capture with pairs
@@ -44,6 +49,9 @@ struct BatchImpl {
             if (e) (void)hipEventDestroy(e);
         if (done) (void)hipEventDestroy(done);
         if (ready) (void)hipEventDestroy(ready);
+        if (fork) (void)hipEventDestroy(fork);
+        if (join) (void)hipEventDestroy(join);
+        if (side) (void)hipStreamDestroy(side);
         if (stream) (void)hipStreamDestroy(stream);
     }
     void build(const bpx_batch_config& c);
@@ -164,6 +172,15 @@ void BatchImpl::build(const bpx_batch_config& c) {
                                      static_cast<const uint32_t*>(tiles.ptr), n, static_cast<uint64_t*>(stamps.ptr),
                                      stream),
                   "batch reset kernel");
+        // both queues: the texture-path one on the side stream, forked right after the reset
+        // so that it runs beside the LDS-staged one (they share nothing but the per-capture
+        // counters, and every capture is on exactly one queue)
+        const bool fork_side = n_lds_items && n_items;
+        const hipStream_t ts = fork_side ? side : stream;
+        if (fork_side) {
+            hip_check(hipEventRecord(fork, stream), "hipEventRecord");
+            hip_check(hipStreamWaitEvent(side, fork, 0), "hipStreamWaitEvent");
+        }
         if (n_lds_items) {
             BatchArgs b{};
             b.caps = static_cast<const ConfocalArgs*>(caps.ptr);
@@ -184,7 +201,11 @@ void BatchImpl::build(const bpx_batch_config& c) {
             b.next = static_cast<uint32_t*>(next.ptr) + 1;
             b.remaining = static_cast<uint32_t*>(remaining.ptr);
             b.done = static_cast<uint64_t*>(stamps.ptr) + 1;
-            hip_check(launch_batch_confocal(b, K, format, groups, stream), "batch gather kernel");
+            hip_check(launch_batch_confocal(b, K, format, groups, ts), "batch gather kernel");
+        }
+        if (fork_side) {
+            hip_check(hipEventRecord(join, side), "hipEventRecord");
+            hip_check(hipStreamWaitEvent(stream, join, 0), "hipStreamWaitEvent");
         }
     } catch (...) {
         hipGraph_t g = nullptr;
@@ -250,6 +271,9 @@ bpx_status bpx_batch_create(const bpx_capture_desc* captures, uint32_t count, co
         else
             b->format = o.storage == BPX_STORAGE_F16 ? kF16 : o.storage == BPX_STORAGE_BF16 ? kBF16 : kF32;
         hip_check(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate");
+        hip_check(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking), "hipStreamCreate");
+        hip_check(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming), "hipEventCreate");
+        hip_check(hipEventCreateWithFlags(&b->join, hipEventDisableTiming), "hipEventCreate");
         for (auto& e : b->ev) hip_check(hipEventCreate(&e), "hipEventCreate");
         hip_check(hipEventCreateWithFlags(&b->done, hipEventDisableTiming), "hipEventCreate");
         hip_check(hipEventCreateWithFlags(&b->ready, hipEventDisableTiming), "hipEventCreate");
