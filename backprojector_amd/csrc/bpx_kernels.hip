@@ -250,9 +250,12 @@ __global__ __launch_bounds__(256) void stage_rows_kernel(const StageJob* __restr
                                                          uint32_t blocks) {
     for (uint32_t blk = blockIdx.x; blk < blocks; blk += gridDim.x) {
         const StageJob& jb = jobs[stage_job_of<false>(jobs, n_jobs, blk)];
-        const uint64_t e = (uint64_t)(blk - jb.s_first) * 256u + threadIdx.x;
-        if (e >= (uint64_t)jb.rows * jb.lde) continue;
-        const uint32_t p = (uint32_t)(e / jb.lde), j = (uint32_t)(e % jb.lde);
+        // each block covers 256 elements of one row: row and segment from block-uniform values
+        // (no per-lane division)
+        const uint32_t per_row = (jb.lde + 255u) / 256u, local = blk - jb.s_first;
+        const uint32_t p = local / per_row, j = (local - p * per_row) * 256u + threadIdx.x;
+        if (j >= jb.lde) continue;
+        const uint64_t e = (uint64_t)p * jb.lde + j;
         const int cplx = format_complex(jb.format) ? 1 : 0;
         const float* row = jb.src + (size_t)p * jb.Tc * (cplx ? 2u : 1u);
         float S = 0.0f;
@@ -286,7 +289,7 @@ hipError_t launch_crop_rows(const float* src, uint32_t src_T, const uint32_t* ma
 
 uint32_t stage_z_blocks(const StageJob& j) { return (j.Tc + 1u + 255u) / 256u; }
 uint32_t stage_row_blocks(const StageJob& j) {
-    return (uint32_t)(((uint64_t)j.rows * j.lde + 255u) / 256u);
+    return (uint32_t)((uint64_t)j.rows * ((j.lde + 255u) / 256u));   // whole blocks per row
 }
 
 hipError_t launch_stage(const StageJob* jobs, uint32_t n_jobs, uint32_t z_blocks, uint32_t row_blocks,
