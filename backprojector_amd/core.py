@@ -21,6 +21,22 @@ def _u32(a):
     return np.ascontiguousarray(a, dtype=np.uint32)
 
 
+def _vec3(a, name="vector"):
+    """A float32 3-vector for the C ABI, which reads exactly 3 floats; any other size raises."""
+    v = _f32(a).reshape(-1)
+    if v.size != 3:
+        raise ValueError(f"{name} must have 3 elements, got {v.size}")
+    return v
+
+
+def _mat9(a, name="voxel_inc"):
+    """The 3x3 row-major voxel increment matrix as 9 float32 values."""
+    v = _f32(a).reshape(-1)
+    if v.size != 9:
+        raise ValueError(f"{name} must have 9 elements, got {v.size}")
+    return v
+
+
 def as_pairs(pairs) -> np.ndarray:
     """Coerces to the CameraLaserPair record layout (backproject_cuda.hpp:9-13)."""
     pr = np.ascontiguousarray(pairs)
@@ -58,7 +74,7 @@ def backprojection(transient_chunk, pairs, voxels_per_side, volume_zero_pos, vox
     flags = nat.FLAG_REFERENCE_PAIR_TRUNCATION if reference_pair_truncation else 0
     opt = nat.make_options(storage, kahan_block, flags, variant)
     check(lib().bpx_backprojection_ex(ptr(H), n, T, ptr(pr), P, ptr(out), ptr(N),
-                                      ptr(_f32(volume_zero_pos)), ptr(_f32(voxel_inc).reshape(9)),
+                                      ptr(_vec3(volume_zero_pos, "volume_zero_pos")), ptr(_mat9(voxel_inc)),
                                       float(t0), float(deltaT), 1 if cplx else 0,
                                       ctypes.byref(opt)))
     return out
@@ -76,7 +92,7 @@ def window_backprojection(transient_chunk, pairs, voxels_per_side, volume_positi
     N = _u32(voxels_per_side).reshape(3)
     out = np.zeros(tuple(int(x) for x in N), np.complex64 if cplx else np.float32)
     check(lib().bpx_window_backprojection(ptr(H), n, T, ptr(pr), P, ptr(out), ptr(N),
-                                          ptr(_f32(volume_position)), ptr(_f32(volume_size)),
+                                          ptr(_vec3(volume_position, "volume_position")), ptr(_vec3(volume_size, "volume_size")),
                                           float(t0), float(deltaT), 1 if cplx else 0))
     return out
 
@@ -117,8 +133,8 @@ def octree_backprojection(transient_chunk, pairs, voxels_per_side, volume_zero_p
     o = nat.make_octree_options(levels, threshold, proposal_threshold)
     st = nat.OctreeStats()
     check(lib().bpx_octree_backprojection(ptr(H), n, T, ptr(pr), P, ptr(out), ptr(N),
-                                          ptr(_f32(volume_zero_pos)),
-                                          ptr(_f32(voxel_inc).reshape(9)), float(t0),
+                                          ptr(_vec3(volume_zero_pos, "volume_zero_pos")),
+                                          ptr(_mat9(voxel_inc)), float(t0),
                                           float(deltaT), ctypes.byref(o), ctypes.byref(st)))
     return out, st.to_dict()
 
@@ -140,7 +156,7 @@ class Plan:
         opt = nat.make_options(storage, kahan_block, flags, variant)
         h = ctypes.c_void_p()
         check(lib().bpx_plan_create(ptr(H), n, T, ptr(pr), P, ptr(_u32(self.N)),
-                                    ptr(_f32(volume_zero_pos)), ptr(_f32(voxel_inc).reshape(9)),
+                                    ptr(_vec3(volume_zero_pos, "volume_zero_pos")), ptr(_mat9(voxel_inc)),
                                     float(t0), float(deltaT), 1 if cplx else 0, ctypes.byref(opt),
                                     ctypes.byref(h)))
         self._h = h
@@ -166,10 +182,10 @@ class Plan:
         self.complex = False
         h = ctypes.c_void_p()
         check(lib().bpx_nlos_plan_create(ptr(H), ptr(ds), H.ndim, ptr(cg), ptr(cs), ptr(lg), ptr(ls),
-                                         ptr(_f32(camera_position)), ptr(_f32(laser_position)),
+                                         ptr(_vec3(camera_position, "camera_position")), ptr(_vec3(laser_position, "laser_position")),
                                          float(t0), float(deltaT), 1 if is_confocal else 0,
-                                         ptr(_f32(volume_position)), ptr(_f32(volume_size)),
-                                         ptr(N), ptr(_f32(wall_normal)),
+                                         ptr(_vec3(volume_position, "volume_position")), ptr(_vec3(volume_size, "volume_size")),
+                                         ptr(N), ptr(_vec3(wall_normal, "wall_normal")),
                                          1 if assume_row_major else 0, ctypes.byref(opt),
                                          ctypes.byref(h)))
         self._h = h
@@ -191,8 +207,8 @@ class Plan:
         self.levels = int(levels)
         h = ctypes.c_void_p()
         check(lib().bpx_octree_plan_create(ptr(H), n, T, ptr(pr), P, ptr(_u32(self.N)),
-                                           ptr(_f32(volume_zero_pos)),
-                                           ptr(_f32(voxel_inc).reshape(9)), float(t0),
+                                           ptr(_vec3(volume_zero_pos, "volume_zero_pos")),
+                                           ptr(_mat9(voxel_inc)), float(t0),
                                            float(deltaT), ctypes.byref(o), ctypes.byref(h)))
         self._h = h
         return self
@@ -365,8 +381,8 @@ class Batch:
             d.scanned_pairs = pr.ctypes.data
             d.num_pairs = P
             d.voxels_per_side[:] = [int(x) for x in Nv]
-            d.volume_zero_pos[:] = [float(x) for x in _f32(zero).reshape(3)]
-            d.voxel_inc[:] = [float(x) for x in _f32(inc).reshape(9)]
+            d.volume_zero_pos[:] = [float(x) for x in _vec3(zero, "volume_zero_pos")]
+            d.voxel_inc[:] = [float(x) for x in _mat9(inc)]
             d.t0 = float(t0)
             d.deltaT = float(dT)
             self._keep += [H, pr]

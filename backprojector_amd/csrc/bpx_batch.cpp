@@ -159,6 +159,7 @@ void BatchImpl::build(const bpx_batch_config& c) {
     exec = nullptr;
     graph = nullptr;
     hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    bool side_joined = false;   // the side stream has joined the capture and not yet left it
     try {
         for (auto& pl : plans) {
             if (pl->P != 0) continue;
@@ -180,6 +181,7 @@ void BatchImpl::build(const bpx_batch_config& c) {
         if (fork_side) {
             hip_check(hipEventRecord(fork, stream), "hipEventRecord");
             hip_check(hipStreamWaitEvent(side, fork, 0), "hipStreamWaitEvent");
+            side_joined = true;
         }
         if (n_lds_items) {
             BatchArgs b{};
@@ -206,8 +208,13 @@ void BatchImpl::build(const bpx_batch_config& c) {
         if (fork_side) {
             hip_check(hipEventRecord(join, side), "hipEventRecord");
             hip_check(hipStreamWaitEvent(stream, join, 0), "hipStreamWaitEvent");
+            side_joined = false;
         }
     } catch (...) {
+        // a side stream left in the capture would make the capture fail as unjoined and stay in
+        // capture mode: join it back first, then end (and drop) the capture
+        if (side_joined && hipEventRecord(join, side) == hipSuccess)
+            (void)hipStreamWaitEvent(stream, join, 0);
         hipGraph_t g = nullptr;
         (void)hipStreamEndCapture(stream, &g);
         if (g) (void)hipGraphDestroy(g);

@@ -97,6 +97,23 @@ def test_nlosbpy_signature_matches_reference(bpx):
     assert list(sig.parameters)[11] == "wavelength"
 
 
+def test_top_level_nlosbpy_module_is_the_reference_name(bpx):
+    """`import nlosbpy` works as with the reference's pybind11 module (nlosbpy.cpp:81) and
+    exposes the same function objects, argument names and defaults (nlosbpy.cpp:89-119)."""
+    import inspect
+    import nlosbpy
+    assert nlosbpy.backproject is bpx.nlosbpy.backproject
+    assert nlosbpy.phasor is bpx.nlosbpy.phasor
+    assert {"backproject", "phasor"} <= set(nlosbpy.__all__)
+    sig = inspect.signature(nlosbpy.phasor)
+    assert list(sig.parameters) == [
+        "transient_data", "camera_grid_positions", "laser_grid_positions", "camera_position",
+        "laser_position", "t0", "deltaT", "is_confocal", "volume_position", "volume_size",
+        "voxels_per_side", "wavelength", "assume_row_major", "use_cpu"]
+    assert sig.parameters["assume_row_major"].default is False
+    assert sig.parameters["use_cpu"].default is False
+
+
 def test_nlosbpy_shape_errors(bpx):
     H = np.zeros((4, 4, 32), np.float32)
     g = np.zeros((4, 5, 3), np.float32)

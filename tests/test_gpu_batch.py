@@ -50,9 +50,7 @@ def test_batch_lds_staged_queue(gpu, oracle):
     """Captures whose bin windows fit the LDS segment (4 mm bins, fine enough voxel pitch) run on
     the LDS-staged persistent queue, the others on the texture-path queue, in the same batch;
     every volume equals its capture reconstructed alone, and the coded ones the oracle."""
-    rng = np.random.default_rng(7)
-    spec = [(8, 1024, 24), (12, 1024, 30), (16, 2048, 20), (8, 1024, 40), (10, 1024, 33), (6, 512, 12)]
-    caps = [capture(rng, *sp, coded=(i % 2 == 0)) for i, sp in enumerate(spec)]
+    caps = mixed()
     alone = [gpu.backprojection(*c) for c in caps]
     with gpu.Batch(caps) as b:
         cfg = b.config()
@@ -69,11 +67,21 @@ def test_batch_lds_staged_queue(gpu, oracle):
         assert np.array_equal(alone[i], ref.astype(np.float32)), i
 
 
+def mixed(seed=7):
+    """Captures of which some fit the LDS-staged queue and the others go to the texture path."""
+    rng = np.random.default_rng(seed)
+    spec = [(8, 1024, 24), (12, 1024, 30), (16, 2048, 20), (8, 1024, 40), (10, 1024, 33), (6, 512, 12)]
+    return [capture(rng, *sp, coded=(i % 2 == 0)) for i, sp in enumerate(spec)]
+
+
 def test_batch_execute_on_caller_stream(gpu, hip_streams):
-    caps = uneven(5)[:3]
+    """Both persistent queues (the graph's fork/join branch) replayed on two caller streams."""
+    caps = mixed()
     alone = [gpu.backprojection(*c) for c in caps]
     s1, s2 = hip_streams
     with gpu.Batch(caps) as b:
+        cfg = b.config()
+        assert 1 <= cfg["lds_captures"] <= len(caps) - 1, cfg
         for rep in range(2):
             b.execute(s1)
             b.execute(s2)
@@ -180,3 +188,53 @@ def test_batch_edge_cases(gpu):
     with pytest.raises(BpxError):
         with gpu.Batch(caps) as b:
             b.set_config(lane_voxels=3)
+
+
+def _codes(P, T, salt):
+    """Integer sample codes 1..61 for [P][T]: every voxel sum is an integer below 2^24, so any
+    summation order is exact and one wrong bin changes the sum."""
+    cb = (7 * np.arange(T, dtype=np.int64) + salt) % 61
+    cp = (13 * np.arange(P, dtype=np.int64)) % 61
+    return ((cp[:, None] + cb[None, :]) % 61 + 1).astype(f32)
+
+
+@pytest.mark.timeout(900)
+def test_c4_64_uneven_captures(gpu, oracle):
+    """The 64 seeded uneven captures (SeedSequence(1004).spawn(64): wall side 16-128, T 1024-4096,
+    volume side 16-96), i.e. one call_cuda_backprojection per capture in the reference
+    (backproject_cuda.cu:370-500), as one autotuned persistent batch.  Every volume equals its
+    capture's own plan exactly; on integer-coded samples of the same geometry, 256 sampled
+    voxels of 8 captures (the largest included) equal the oracle's fp32 formula exactly."""
+    from backprojector_amd import proposals
+    caps = scenes.batch_captures(64, 1004)
+    args = [proposals.plan_args(c, N) for c, N in caps]
+    del caps
+    print(f"C4: {len(args)} captures prepared", flush=True)
+    with gpu.Batch(args) as b:
+        cfg = b.tune(reps=1)
+        assert 1 <= cfg["lds_captures"] < len(args), cfg
+        b.execute()
+        b.synchronize()
+        lat = b.latencies()
+        assert np.all(lat > 0) and np.all(np.isfinite(lat))
+        for i, a in enumerate(args):
+            with gpu.Plan(*a) as pl:
+                pl.execute()
+                assert np.array_equal(b.volume(i), pl.volume()), i
+    print("C4: batch volumes equal single plans", flush=True)
+    work = [len(a[1]) * int(np.prod(a[2])) for a in args]
+    largest = int(np.argmax(work))
+    assert len(args[largest][1]) == 128 * 128 and args[largest][2] == (96, 96, 96)
+    check = [largest] + [i for i in np.random.default_rng(1004).permutation(len(args)) if i != largest][:7]
+    coded = [(_codes(len(a[1]), a[0].shape[1], i),) + tuple(a[1:]) for i, a in enumerate(args)]
+    del args
+    with gpu.Batch(coded) as b:
+        b.tune(reps=1)
+        b.execute()
+        rng = np.random.default_rng(4)
+        for i in check:
+            H, pr, N, zero, inc, t0, dT = coded[i]
+            vox = np.stack([rng.integers(0, n, 256) for n in N], 1).astype(np.uint32)
+            ref = oracle.a1_voxels_f64acc(H, pr, vox, zero, inc, t0, dT)
+            got = b.volume(i)[vox[:, 0], vox[:, 1], vox[:, 2]]
+            assert ref.max() > 0 and np.array_equal(got.astype(np.float64), ref), i

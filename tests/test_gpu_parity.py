@@ -487,6 +487,28 @@ def test_nlosbpy_backproject_matches_oracle_pipeline(gpu, oracle):
     assert rel_l2(vol, exact) <= 1e-6 and rel_l2(vol, ref) <= 1e-5
 
 
+def test_import_nlosbpy_matches_oracle_pipeline(gpu, oracle):
+    """A reference caller's `import nlosbpy` (nlosbpy.cpp:81) runs the HIP path: plain and
+    phasor volumes against the oracle's bp::backproject / bp::phasor_reconstruction."""
+    import nlosbpy
+    cap = scenes.render(scenes.config_scene(1), 16, 2048, t0=2.0, seed=1011)
+    N = (16, 12, 20)
+    vol = nlosbpy.backproject(*cap.nlos_args(N), np.zeros(3, f32), assume_row_major=True)
+    ref, inter = oracle.backproject_pipeline(cap.transient, cap.camera_grid, cap.laser_grid,
+                                             cap.camera_position, cap.laser_position, cap.t0,
+                                             cap.deltaT, True, cap.volume_position,
+                                             cap.volume_size, N)
+    exact = oracle.a1_volume_f64acc(inter["H"], inter["pairs"], N, inter["zero"], inter["inc"],
+                                    float(inter["t0"]), cap.deltaT)
+    assert vol.dtype == np.float32 and vol.shape == N and np.abs(ref).max() > 0
+    assert rel_l2(vol, exact) <= 1e-6 and rel_l2(vol, ref) <= 1e-5
+    cv = nlosbpy.phasor(*cap.nlos_args(N), 0.004, assume_row_major=True)
+    cref, _ = oracle.phasor_pipeline(cap.transient, cap.camera_grid, cap.laser_grid,
+                                     cap.camera_position, cap.laser_position, cap.t0, cap.deltaT,
+                                     True, cap.volume_position, cap.volume_size, N, 0.004)
+    assert cv.dtype == np.complex64 and rel_l2(cv, cref) <= 1e-5
+
+
 @pytest.mark.parametrize("t0,T,nan_pair,confocal", [
     (2.0, 2048, False, True),     # window starts inside the rows and runs past the last bin
     (4.0, 2048, True, True),      # window starts before bin 0 (zero head), one NaN pair dropped
