@@ -129,7 +129,7 @@ _lib = None
 # Every symbol include/bpx.h declares (checked by the CPU test suite).
 EXPORTED = [
     "bpx_last_error", "bpx_version", "bpx_abi_version", "bpx_device_count",
-    "bpx_backprojection", "bpx_complex_backprojection", "bpx_backprojection_ex",
+    "bpx_backprojection", "bpx_complex_backprojection", "bpx_backprojection_ex", "bpx_backprojection_noalloc",
     "bpx_window_backprojection", "bpx_phasor_pulse", "bpx_nlos_backproject", "bpx_nlos_phasor", "bpx_nlos_phasor_multi",
     "bpx_plan_create", "bpx_nlos_plan_create", "bpx_plan_execute", "bpx_plan_synchronize", "bpx_plan_read_volume",
     "bpx_plan_info", "bpx_plan_kernel_name", "bpx_plan_read_thresholds", "bpx_plan_benchmark", "bpx_plan_destroy", "bpx_render_transient",
@@ -162,6 +162,7 @@ def lib():
         getattr(L, name).argtypes = [c_p, u64, u32, c_p, u32, c_p, c_p, c_p, c_p, f32, f32]
     L.bpx_backprojection_ex.argtypes = [c_p, u64, u32, c_p, u32, c_p, c_p, c_p, c_p, f32, f32,
                                         ctypes.c_int, c_p]
+    L.bpx_backprojection_noalloc.argtypes = [c_p, u64, u32, c_p, u32, c_p, c_p, c_p, c_p, f32, f32, c_p]
     L.bpx_window_backprojection.argtypes = [c_p, u64, u32, c_p, u32, c_p, c_p, c_p, c_p, f32,
                                             f32, ctypes.c_int]
     L.bpx_phasor_pulse.argtypes = [c_p, u64, u32, f32, f32, u32, f32, c_p]

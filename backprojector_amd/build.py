@@ -32,6 +32,7 @@ SOURCES = [
     "bpx_net.hip",
     "bpx_net.cpp",
     "bpx_batch.cpp",
+    "bpx_noalloc.hip",
 ]
 
 COMMON = [

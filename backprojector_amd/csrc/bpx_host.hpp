@@ -23,6 +23,10 @@ struct Failure {
 [[noreturn]] void fail(bpx_status s, const std::string& msg);
 void hip_check(hipError_t e, const char* what);
 bool device_ready();  // a gfx950 device is visible and selected
+// Bound on |approximate - exact quotient| of the table bin lookup for these pairs and grid; the
+// lookup is exact below 0.25 (allocates nothing).
+double single_tap_error_bound(const bpx_pair* pairs, uint32_t P, const uint32_t n[3], const float zero[3],
+                              const float inc[9], float t0, float dT, uint32_t T);
 
 // ------------------------------------------------------------------ plan
 enum Path : int { kPathConfocal = 0, kPathGrid = 1, kPathPairs = 2, kPathWindow = 3, kPathOctree = 4 };

@@ -236,6 +236,8 @@ hipError_t launch_confocal(const ConfocalArgs& a, int format, int variant, bool 
                            hipStream_t stream);
 hipError_t launch_grid(const GridArgs& a, int format, int variant, hipStream_t stream);
 hipError_t launch_pairs(const PairsArgs& a, int format, int variant, hipStream_t stream);
+// fp32 samples read straight from unpadded rows [P][Tc] (ldh = Tc); no padded copy needed.
+hipError_t launch_pairs_raw(const PairsArgs& a, hipStream_t stream);
 hipError_t launch_batch_reset(uint32_t* next, uint32_t* remaining, const uint32_t* tiles, uint32_t n,
                               uint64_t* start, hipStream_t s);
 // format: kF32, kF16, kBF16, kC64, kCF16 or kCBF16 (fused rows); K: 2 or 4 voxels per lane.

@@ -1123,7 +1123,9 @@ __global__ __launch_bounds__(256) void grid_kernel(GridArgs a) {
 }
 
 // ------------------------------------------------------------------- arbitrary pairs
-template <class Smp, int K>
+// RAW: the samples are the caller's unpadded rows [P][Tc] (ldh = Tc) instead of padded rows;
+// padded index hj is raw sample hj - 1, and hj = 0 or Tc + 1 (outside the row) reads 0.
+template <class Smp, int K, bool RAW = false>
 __global__ __launch_bounds__(256) void pairs_kernel(PairsArgs a) {
     using acc_t = typename Smp::acc_t;
     extern __shared__ float zs[];
@@ -1160,7 +1162,14 @@ __global__ __launch_bounds__(256) void pairs_kernel(PairsArgs a) {
             const float q1 = __builtin_fmaf(total, a.inv, a.c1);
             const uint32_t j = (uint32_t)(int)__builtin_amdgcn_fmed3f(q1, 0.0f, fTc);
             const uint32_t hj = j + (total >= zs[j] ? 1u : 0u);
-            val[r] = Smp::load(hr, hj, soff_h);
+            if constexpr (RAW) {
+                const uint32_t b = hj - 1u;   // wraps to 2^32 - 1 for hj = 0
+                const bool in_row = b < a.Tc;
+                const acc_t x = Smp::load(hr, in_row ? b : 0u, soff_h);
+                val[r] = in_row ? x : acc_t{};
+            } else {
+                val[r] = Smp::load(hr, hj, soff_h);
+            }
         }
     };
     uint32_t p = 0;
@@ -1464,6 +1473,12 @@ template <class Smp>
 static hipError_t pairs_dispatch(const PairsArgs& a, hipStream_t s) {
     const size_t lds = sizeof(float) * (a.Tc + 1u);
     hipLaunchKernelGGL((pairs_kernel<Smp, 4>), dim3(a.t.count), dim3(256), lds, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_pairs_raw(const PairsArgs& a, hipStream_t s) {
+    const size_t lds = sizeof(float) * (a.Tc + 1u);
+    hipLaunchKernelGGL((pairs_kernel<SmpF32, 4, true>), dim3(a.t.count), dim3(256), lds, s, a);
     return hipGetLastError();
 }
 

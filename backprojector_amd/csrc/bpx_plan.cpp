@@ -73,9 +73,10 @@ static void grid_corners(const uint32_t n[3], const float zero[3], const float i
     }
 }
 
-static double max_total_path(const std::vector<bpx_pair>& pr, const double corners[8][3]) {
+static double max_total_path(const bpx_pair* pr, size_t count, const double corners[8][3]) {
     double mx = 0.0;
-    for (const auto& p : pr) {
+    for (size_t i = 0; i < count; ++i) {
+        const bpx_pair& p = pr[i];
         double dl = 0, dc = 0;
         for (int m = 0; m < 8; ++m) {
             dl = std::max(dl, dist3(p.laser_point, corners[m]));
@@ -84,6 +85,13 @@ static double max_total_path(const std::vector<bpx_pair>& pr, const double corne
         mx = std::max(mx, std::fabs((double)p.laser_wall) + std::fabs((double)p.camera_wall) + dl + dc);
     }
     return mx;
+}
+
+double single_tap_error_bound(const bpx_pair* pairs, uint32_t P, const uint32_t n[3], const float zero[3],
+                              const float inc[9], float t0, float dT, uint32_t T) {
+    double corners[8][3];
+    grid_corners(n, zero, inc, corners);
+    return tap_error_bound(max_total_path(pairs, P, corners), t0, dT, T);
 }
 
 // Power-of-two scale s so that max|x| / s lies in [2^14, 2^15) (exact rescaling).
@@ -368,7 +376,7 @@ PlanImpl* make_plan_single_tap(const CaptureIn& c, const uint32_t n[3], const fl
     // exactness precondition of the table lookup
     double corners[8][3];
     grid_corners(n, zero, inc, corners);
-    const double bound = tap_error_bound(max_total_path(pr, corners), c.t0, c.dT, c.T);
+    const double bound = tap_error_bound(max_total_path(pr.data(), pr.size(), corners), c.t0, c.dT, c.T);
     if (bound >= 0.25)
         fail(BPX_ERR_UNSUPPORTED,
              "path lengths / t0 too large relative to deltaT for the exact bin lookup (bound " +
@@ -670,7 +678,7 @@ PlanImpl* make_plan_window(const CaptureIn& c, const uint32_t n[3], const float 
                              (m & 1) ? (double)(n[2] - 1) : 0.0};
         for (int k = 0; k < 3; ++k) corners[m][k] = (double)pl->wzero[k] + (double)pl->wvs[k] * i[k];
     }
-    const double bound = tap_error_bound(max_total_path(pr, corners) + width, c.t0, c.dT, c.T);
+    const double bound = tap_error_bound(max_total_path(pr.data(), pr.size(), corners) + width, c.t0, c.dT, c.T);
     if (bound >= 0.25)
         fail(BPX_ERR_UNSUPPORTED, "path lengths / t0 too large relative to deltaT for the exact bin lookup");
     pl->inv = (float)(1.0 / (double)c.dT);

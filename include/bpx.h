@@ -128,6 +128,25 @@ bpx_status bpx_backprojection_ex(const float* transient_chunk, uint64_t transien
                                  float t0, float deltaT, int complex_samples,
                                  const bpx_options* options);
 
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the volume of
+ * bpx_backprojection (fp32 samples, every pair used) computed on caller-owned DEVICE buffers,
+ * with no hipMalloc / hipFree / allocating library call.  The library's working tables live in
+ * one fixed device array of BPX_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends.
+ *   transient_chunk: device pointer, [num_pairs][T] fp32 samples (transient_size = num_pairs * T)
+ *   scanned_pairs:   HOST array (copied into the fixed array; finite values only)
+ *   voxel_volume:    device pointer, [N0][N1][N2] fp32, overwritten
+ * Work runs on hip_stream (NULL = the null stream); the call returns when it has finished.
+ * Limits (else BPX_ERR_UNSUPPORTED): roundup(num_pairs * 32, 256) + (T + 1) * 4 <=
+ * BPX_NOALLOC_SCRATCH_BYTES, T <= 16383.  Every call on one GPU uses that GPU's single array:
+ * calls that overlap on one GPU (other threads, other streams) race. */
+#define BPX_NOALLOC_SCRATCH_BYTES (8u << 20)
+bpx_status bpx_backprojection_noalloc(const float* transient_chunk, uint64_t transient_size, uint32_t T,
+                                      const bpx_pair* scanned_pairs, uint32_t num_pairs,
+                                      float* voxel_volume, const uint32_t voxels_per_side[3],
+                                      const float volume_zero_pos[3], const float voxel_inc[9],
+                                      float t0, float deltaT, void* hip_stream);
+
 /* CPU-path semantics on the GPU: the grid is given as the reference CPU path takes it
  * (volume_position = centre, volume_size), spacing size/N, voxel-diagonal window. */
 bpx_status bpx_window_backprojection(const float* transient_chunk, uint64_t transient_size,
