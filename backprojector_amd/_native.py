@@ -64,14 +64,22 @@ class NlosPrepared(ctypes.Structure):
 class OctreeOptions(ctypes.Structure):
     _fields_ = [("levels", ctypes.c_uint32), ("threshold", ctypes.c_float),
                 ("proposal_threshold", ctypes.c_float), ("flags", ctypes.c_uint32),
-                ("level_proposal_threshold", ctypes.c_float * 8), ("keep_bound_ratio", ctypes.c_float)]
+                ("level_proposal_threshold", ctypes.c_float * 8), ("keep_bound_ratio", ctypes.c_float),
+                ("level_fallback_threshold", ctypes.c_float * 8), ("coarse_pair_stride", ctypes.c_uint32)]
 
 
 def make_octree_options(levels=4, threshold=0.25, proposal_threshold=0.5, bounds_only=False,
-                        keep_bound_ratio=0.0):
-    """proposal_threshold: one probability, or one per coarse level."""
+                        keep_bound_ratio=0.0, fallback_threshold=0.0, coarse_pair_stride=1):
+    """proposal_threshold / fallback_threshold: one probability, or one per coarse level."""
     o = OctreeOptions(int(levels), float(threshold), 0.5, OCTREE_BOUNDS_ONLY if bounds_only else 0)
     o.keep_bound_ratio = float(keep_bound_ratio)
+    o.coarse_pair_stride = int(coarse_pair_stride)
+    fb = [float(fallback_threshold)] * int(levels) if np.ndim(fallback_threshold) == 0 else \
+        [float(x) for x in fallback_threshold]
+    if len(fb) != int(levels):
+        raise ValueError(f"expected {levels} level fallback thresholds, got {len(fb)}")
+    for i, x in enumerate(fb):
+        o.level_fallback_threshold[i] = x
     if np.ndim(proposal_threshold) == 0:
         o.proposal_threshold = float(proposal_threshold)
     else:
@@ -87,14 +95,16 @@ class OctreeStats(ctypes.Structure):
     _fields_ = [("levels", ctypes.c_uint32), ("active", ctypes.c_uint32 * 9),
                 ("running_max", ctypes.c_double * 8), ("bound_max", ctypes.c_double * 8),
                 ("bound_norm", ctypes.c_double * 8), ("threshold", ctypes.c_double * 8),
-                ("fine_updates", ctypes.c_uint64), ("bound_updates", ctypes.c_uint64)]
+                ("fine_updates", ctypes.c_uint64), ("bound_updates", ctypes.c_uint64),
+                ("fallback_voxels", ctypes.c_uint32), ("coarse_pairs", ctypes.c_uint32)]
 
     def to_dict(self):
         L = self.levels
         return dict(levels=L, active=list(self.active[:L + 1]),
                     running_max=list(self.running_max[:L]), bound_max=list(self.bound_max[:L]),
                     bound_norm=list(self.bound_norm[:L]), threshold=list(self.threshold[:L]),
-                    fine_updates=int(self.fine_updates), bound_updates=int(self.bound_updates))
+                    fine_updates=int(self.fine_updates), bound_updates=int(self.bound_updates),
+                    fallback_voxels=int(self.fallback_voxels), coarse_pairs=int(self.coarse_pairs))
 
 
 class CaptureDesc(ctypes.Structure):
@@ -107,11 +117,11 @@ class CaptureDesc(ctypes.Structure):
 
 class BatchConfig(ctypes.Structure):
     _fields_ = [("lane_voxels", ctypes.c_uint32), ("groups", ctypes.c_uint32),
-                ("order", ctypes.c_uint32), ("lds_captures", ctypes.c_uint32)]
+                ("order", ctypes.c_uint32), ("lds_captures", ctypes.c_uint32), ("lds_depth", ctypes.c_uint32)]
 
     def to_dict(self):
         return dict(lane_voxels=int(self.lane_voxels), groups=int(self.groups),
-                    order=int(self.order), lds_captures=int(self.lds_captures))
+                    order=int(self.order), lds_captures=int(self.lds_captures), lds_depth=int(self.lds_depth))
 
 
 class BatchTiming(ctypes.Structure):

@@ -214,11 +214,15 @@ class Plan:
         return self
 
     def set_octree_options(self, threshold=0.25, proposal_threshold=0.5, bounds_only=False,
-                           keep_bound_ratio=0.0):
+                           keep_bound_ratio=0.0, fallback_threshold=0.0, coarse_pair_stride=1):
         """Replaces an octree plan's thresholds (proposal_threshold: one value or one per level;
         keep_bound_ratio > 0: children of blocks whose bound reaches that fraction of the running
-        max are refined whatever an attached net says)."""
-        o = nat.make_octree_options(self.levels, threshold, proposal_threshold, bounds_only, keep_bound_ratio)
+        max are refined whatever an attached net says; fallback_threshold (one value or one per
+        level): children the net does not refine but rates above this are computed at full
+        resolution; coarse_pair_stride s: with a net attached, the coarse levels use every s-th
+        wall point along each grid axis)."""
+        o = nat.make_octree_options(self.levels, threshold, proposal_threshold, bounds_only, keep_bound_ratio,
+                                    fallback_threshold, coarse_pair_stride)
         check(lib().bpx_octree_plan_set_options(self._h, ctypes.byref(o)))
 
     def attach_net(self, net):
@@ -411,8 +415,8 @@ class Batch:
         check(lib().bpx_batch_get_config(self._h, ctypes.byref(c)))
         return c.to_dict()
 
-    def set_config(self, lane_voxels=2, groups=0, order=0):
-        c = nat.BatchConfig(int(lane_voxels), int(groups), int(order))
+    def set_config(self, lane_voxels=2, groups=0, order=0, lds_depth=8):
+        c = nat.BatchConfig(int(lane_voxels), int(groups), int(order), 0, int(lds_depth))
         check(lib().bpx_batch_set_config(self._h, ctypes.byref(c)))
 
     def execute(self, stream=None):

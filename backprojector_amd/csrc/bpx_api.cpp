@@ -191,9 +191,11 @@ static void fill_octree_stats(const PlanImpl& pl, bpx_octree_stats* s) {
         s->threshold[l] = st.level_stats[5 * l + 4];
     }
     uint64_t b = 0;
-    for (uint32_t l = 0; l < st.L && l < st.level_active.size(); ++l) b += (uint64_t)st.level_active[l] * pl.P;
+    for (uint32_t l = 0; l < st.L && l < st.level_active.size(); ++l) b += (uint64_t)st.level_active[l] * st.coarse_P;
     s->bound_updates = b;
     s->fine_updates = st.level_active.size() > st.L ? (uint64_t)st.level_active[st.L] * pl.P : 0;
+    for (uint32_t v : st.fallback_voxels) s->fallback_voxels += v;
+    s->coarse_pairs = st.coarse_P;
 }
 
 bpx_status bpx_octree_backprojection(const float* transient_chunk, uint64_t transient_size,

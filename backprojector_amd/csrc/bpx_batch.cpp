@@ -1,12 +1,15 @@
-// Batched uneven confocal captures: one device-resident plan per capture, the staging of every
-// capture (rows and threshold tables) in one pair of launches, all gathers in at most two
-// persistent work-queue launches running side by side (the LDS-staged kernel for captures whose
-// bin windows fit its segment, the texture-path kernel for the rest), and the whole per-batch
-// sequence (staging, queue reset, persistent gathers) captured in a hipGraph that
-// bpx_batch_execute replays.  A project-written autotuner times a few launch
-// configurations (voxels per lane, resident groups, queue order) on the batch itself.
+// Batched uneven confocal captures: one device-resident plan per capture and one persistent
+// launch over a single work queue of staging items (fused rows and thresholds), LDS-staged
+// tiles (captures whose bin windows fit the LDS segment) and texture-path tiles (the rest),
+// after a prologue launch that builds every capture's path-length table and resets the queue.
+// A capture's gather items follow its staging items in the queue and wait for them, and the
+// staging of the next capture is queued ahead of the current capture's gathers, so staging
+// overlaps the gathers.  The sequence is captured in a hipGraph that bpx_batch_execute replays.
+// A project-written autotuner times launch configurations (texture-path voxels per lane, LDS
+// tile depth, queue order, resident work-groups) on the batch itself and keeps the fastest.
 // No reference counterpart: the reference reconstructs one capture per call
-// (call_cuda_backprojection, backproject_cuda.cu:370-500).
+// (call_cuda_backprojection, backproject_cuda.cu:370-500), with a launch shape chosen from
+// occupancy alone (cudaOccupancyMaxPotentialBlockSize, :406).
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -24,20 +27,14 @@ struct BatchImpl {
     hipStream_t stream = nullptr;
     hipEvent_t ev[2] = {};
     hipEvent_t done = nullptr, ready = nullptr;   // as PlanImpl::done / ready
-    // the texture-path queue runs beside the LDS-staged one: a second stream (default
-    // priority) forked from and joined back into the captured sequence
-    hipStream_t side = nullptr;
-    hipEvent_t fork = nullptr, join = nullptr;
-    DeviceBuffer caps, flags, items, items_lds, next, remaining, tiles;
+    DeviceBuffer caps, flags, items, jobs, init, next, staged, remaining, error;
     DeviceBuffer stamps;   // uint64 [1 + captures]: batch start, then each capture's completion
-    DeviceBuffer stage_jobs;                      // StageJob per capture with pairs
-    uint32_t n_stage = 0, stage_zb = 0, stage_sb = 0;
-    uint32_t n_items = 0;          // texture-path queue: (capture, tile * 4 + wave quarter)
-    uint32_t n_lds_items = 0;      // LDS-staged queue: (capture, 16 x 16 x 8 tile)
-    uint32_t lds_groups = 0;
+    DeviceBuffer zjobs;    // StageJob per capture with pairs (path-length tables, prologue)
+    uint32_t n_zjobs = 0, z_blocks = 0;
+    uint32_t n_items = 0;
     int format = kF32;     // storage of every capture (fused rows)
     bool cplx = false;     // complex captures and volumes
-    bpx_batch_config cfg{2u, 0u, 0u};
+    bpx_batch_config cfg{2u, 0u, 0u, 0u, 8u};
     uint32_t groups = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -49,27 +46,32 @@ struct BatchImpl {
             if (e) (void)hipEventDestroy(e);
         if (done) (void)hipEventDestroy(done);
         if (ready) (void)hipEventDestroy(ready);
-        if (fork) (void)hipEventDestroy(fork);
-        if (join) (void)hipEventDestroy(join);
-        if (side) (void)hipStreamDestroy(side);
         if (stream) (void)hipStreamDestroy(stream);
     }
     void build(const bpx_batch_config& c);
-    // Waits for the batch's own stream and for its last execution on a caller's stream.
+    // Waits for the batch's own stream and for its last execution on a caller's stream, then
+    // checks the queue's error word.
     void wait() const {
         hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
         hip_check(hipEventSynchronize(done), "hipEventSynchronize");
+        uint32_t e = 0;
+        hip_check(hipMemcpy(&e, error.ptr, 4, hipMemcpyDeviceToHost), "hipMemcpy(error word)");
+        if (e) fail(BPX_ERR_DEVICE, "batch queue: a capture's staging did not finish in time");
     }
 };
 
 void BatchImpl::build(const bpx_batch_config& c) {
     if (c.lane_voxels != 2u && c.lane_voxels != 4u) fail(BPX_ERR_INVALID_ARGUMENT, "lane_voxels must be 2 or 4");
     if (c.order > 1u) fail(BPX_ERR_INVALID_ARGUMENT, "order must be 0 or 1");
+    const uint32_t kz_req = c.lds_depth ? c.lds_depth : 8u;
+    if (kz_req != 8u && kz_req != 16u) fail(BPX_ERR_INVALID_ARGUMENT, "lds_depth must be 8 or 16");
     const int K = (int)c.lane_voxels;
+    const int kz = batch_lds_depth((int)kz_req, format);
     const uint32_t n = (uint32_t)plans.size();
-    // per-capture arguments and tiles for this K
+    // per-capture arguments, tiles and staging jobs for this configuration
     std::vector<ConfocalArgs> args(n);
-    std::vector<uint32_t> fl(n), nt(n);
+    std::vector<StageJob> sj(std::max<uint32_t>(n, 1));
+    std::vector<uint32_t> fl(n), nt(n), ns(n);
     std::vector<char> lds(n, 0);
     for (uint32_t i = 0; i < n; ++i) {
         const PlanImpl& pl = *plans[i];
@@ -88,78 +90,100 @@ void BatchImpl::build(const bpx_batch_config& c) {
         a.lz = pl.lz;
         a.kahan_block = pl.kahan_block;
         fl[i] = (pl.sep ? 1u : 0u) | (pl.sep && pl.flat ? 2u : 0u);
-        // captures whose bin windows fit the LDS segment go to the LDS-staged queue (one item
-        // per work-group tile), the others to the texture-path queue (one per wave quarter)
-        lds[i] = pl.P && batch_lds_applies(pl.g, pl.inv2, format, pl.sep, pl.flat);
-        nt[i] = !pl.P ? 0u : lds[i] ? batch_lds_tiles(pl.g) : 4u * a.t.count;
+        // captures whose bin windows fit the LDS segment run LDS-staged tiles, the others
+        // texture-path tiles
+        lds[i] = pl.P && batch_lds_applies(pl.g, pl.inv2, format, pl.sep, pl.flat, kz);
+        nt[i] = !pl.P ? 0u : lds[i] ? batch_lds_tiles(pl.g, kz) : a.t.count;
+        if (pl.P) {
+            sj[i] = stage_job(pl);
+            ns[i] = (stage_row_blocks(sj[i]) + kBatchStageBlocks - 1u) / kBatchStageBlocks;
+        }
     }
-    // queue: (capture, tile * 4 + quarter), the quarters of a tile adjacent; order 0 = captures
-    // with most pairs first, order 1 = capture order
+    // capture order: 0 = most work first, 1 = given order; each capture's staging items are
+    // queued one capture ahead of its gather items
     std::vector<uint32_t> order(n);
     std::iota(order.begin(), order.end(), 0u);
     if (c.order == 0)
-        std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
-            return (uint64_t)plans[x]->P * c.lane_voxels > (uint64_t)plans[y]->P * c.lane_voxels;
-        });
-    std::vector<uint32_t> it, itl;
-    for (uint32_t i : order)
-        for (uint32_t t = 0; t < nt[i]; ++t) {
-            std::vector<uint32_t>& q = lds[i] ? itl : it;
-            q.push_back(i);
-            q.push_back(t);
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return updates[x] > updates[y]; });
+    std::vector<uint32_t> it;
+    auto push_stage = [&](uint32_t i) {
+        for (uint32_t k = 0; k < ns[i]; ++k) {
+            it.push_back(i | (kBatchItemStage << 28));
+            it.push_back(k * kBatchStageBlocks);
         }
-    n_items = (uint32_t)(it.size() / 2);
-    n_lds_items = (uint32_t)(itl.size() / 2);
-    // staging of every capture in one pair of launches: each job's blocks follow the previous job's
-    std::vector<StageJob> jobs;
-    uint64_t zb = 0, sb = 0;
-    for (auto& pl : plans) {
-        if (pl->P == 0) continue;
-        StageJob j = stage_job(*pl);
-        j.z_first = (uint32_t)zb;
-        j.s_first = (uint32_t)sb;
-        zb += stage_z_blocks(j);
-        sb += stage_row_blocks(j);
-        if (sb >= (1ull << 31)) fail(BPX_ERR_UNSUPPORTED, "batch staging above 2^39 row elements");
-        jobs.push_back(j);
+    };
+    auto push_gather = [&](uint32_t i) {
+        const uint32_t type = lds[i] ? kBatchItemLds : kBatchItemTex;
+        for (uint32_t t = 0; t < nt[i]; ++t) {
+            it.push_back(i | (type << 28));
+            it.push_back(t);
+        }
+    };
+    if (n) push_stage(order[0]);
+    for (uint32_t k = 0; k < n; ++k) {
+        if (k + 1 < n) push_stage(order[k + 1]);
+        push_gather(order[k]);
     }
-    n_stage = (uint32_t)jobs.size();
-    stage_zb = (uint32_t)zb;
-    stage_sb = (uint32_t)sb;
-    stage_jobs.alloc(sizeof(StageJob) * std::max<size_t>(jobs.size(), 1));
-    if (n_stage)
-        hip_check(hipMemcpy(stage_jobs.ptr, jobs.data(), sizeof(StageJob) * n_stage, hipMemcpyHostToDevice),
-                  "hipMemcpy");
+    n_items = (uint32_t)(it.size() / 2);
+    // path-length tables of every capture in the prologue: each job's blocks follow the previous one's
+    std::vector<StageJob> zj;
+    uint64_t zb = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (!plans[i]->P) continue;
+        StageJob j = sj[i];
+        j.z_first = (uint32_t)zb;
+        zb += stage_z_blocks(j);
+        zj.push_back(j);
+    }
+    if (zb >= (1ull << 31)) fail(BPX_ERR_UNSUPPORTED, "batch path-length tables too large");
+    n_zjobs = (uint32_t)zj.size();
+    z_blocks = (uint32_t)zb;
+    std::vector<uint32_t> in(2 * std::max<uint32_t>(n, 1), 0u);
+    for (uint32_t i = 0; i < n; ++i) {
+        in[2 * i] = ns[i];
+        in[2 * i + 1] = nt[i];
+    }
+    zjobs.alloc(sizeof(StageJob) * std::max<size_t>(zj.size(), 1));
+    jobs.alloc(sizeof(StageJob) * sj.size());
+    init.alloc(4u * in.size());
     caps.alloc(sizeof(ConfocalArgs) * std::max<uint32_t>(n, 1));
     flags.alloc(4u * std::max<uint32_t>(n, 1));
-    tiles.alloc(4u * std::max<uint32_t>(n, 1));
     items.alloc(8u * std::max<uint32_t>(n_items, 1));
-    items_lds.alloc(8u * std::max<uint32_t>(n_lds_items, 1));
-    hip_check(hipMemcpy(caps.ptr, args.data(), sizeof(ConfocalArgs) * n, hipMemcpyHostToDevice), "hipMemcpy");
-    hip_check(hipMemcpy(flags.ptr, fl.data(), 4u * n, hipMemcpyHostToDevice), "hipMemcpy");
-    hip_check(hipMemcpy(tiles.ptr, nt.data(), 4u * n, hipMemcpyHostToDevice), "hipMemcpy");
-    if (n_items) hip_check(hipMemcpy(items.ptr, it.data(), 8u * n_items, hipMemcpyHostToDevice), "hipMemcpy");
-    if (n_lds_items)
-        hip_check(hipMemcpy(items_lds.ptr, itl.data(), 8u * n_lds_items, hipMemcpyHostToDevice), "hipMemcpy");
-    if (n_lds_items) {
-        const int rl = batch_lds_resident_groups(format);
-        if (rl <= 0) fail(BPX_ERR_DEVICE, "cannot query the occupancy of the batch kernel");
-        lds_groups = std::min<uint32_t>((uint32_t)rl, n_lds_items);
+    if (n_zjobs)
+        hip_check(hipMemcpy(zjobs.ptr, zj.data(), sizeof(StageJob) * n_zjobs, hipMemcpyHostToDevice), "hipMemcpy");
+    hip_check(hipMemcpy(jobs.ptr, sj.data(), sizeof(StageJob) * sj.size(), hipMemcpyHostToDevice), "hipMemcpy");
+    hip_check(hipMemcpy(init.ptr, in.data(), 4u * in.size(), hipMemcpyHostToDevice), "hipMemcpy");
+    if (n) {
+        hip_check(hipMemcpy(caps.ptr, args.data(), sizeof(ConfocalArgs) * n, hipMemcpyHostToDevice), "hipMemcpy");
+        hip_check(hipMemcpy(flags.ptr, fl.data(), 4u * n, hipMemcpyHostToDevice), "hipMemcpy");
     }
-    const int resident = batch_resident_groups(K, format);
+    if (n_items) hip_check(hipMemcpy(items.ptr, it.data(), 8u * n_items, hipMemcpyHostToDevice), "hipMemcpy");
+    const int resident = batch_queue_resident_groups(K, kz, format);
     if (resident <= 0) fail(BPX_ERR_DEVICE, "cannot query the occupancy of the batch kernel");
     groups = c.groups ? std::min<uint32_t>(c.groups, (uint32_t)resident) : (uint32_t)resident;
     groups = std::max<uint32_t>(1u, std::min<uint32_t>(groups, std::max<uint32_t>(n_items, 1u)));
     cfg = c;
     cfg.groups = groups;
+    cfg.lds_depth = (uint32_t)kz;
     cfg.lds_captures = (uint32_t)std::count(lds.begin(), lds.end(), (char)1);
-    // the per-batch sequence as one graph (no memset nodes: the queue reset is a kernel)
+    BatchQueueArgs q{};
+    q.caps = static_cast<const ConfocalArgs*>(caps.ptr);
+    q.flags = static_cast<const uint32_t*>(flags.ptr);
+    q.jobs = static_cast<const StageJob*>(jobs.ptr);
+    q.items = static_cast<const uint2*>(items.ptr);
+    q.n_items = n_items;
+    q.spin_limit = 1u << 24;
+    q.next = static_cast<uint32_t*>(next.ptr);
+    q.staged = static_cast<uint32_t*>(staged.ptr);
+    q.remaining = static_cast<uint32_t*>(remaining.ptr);
+    q.done = static_cast<uint64_t*>(stamps.ptr) + 1;
+    q.error = static_cast<uint32_t*>(error.ptr);
+    // the per-batch sequence as one graph (no memset nodes: the reset is a kernel)
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     exec = nullptr;
     graph = nullptr;
     hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-    bool side_joined = false;   // the side stream has joined the capture and not yet left it
     try {
         for (auto& pl : plans) {
             if (pl->P != 0) continue;
@@ -167,54 +191,12 @@ void BatchImpl::build(const bpx_batch_config& c) {
             hip_check(launch_fill(static_cast<float*>(pl->vol.ptr), nvox * (cplx ? 2u : 1u), 0.0f, stream),
                       "fill kernel");
         }
-        hip_check(launch_stage(static_cast<const StageJob*>(stage_jobs.ptr), n_stage, stage_zb, stage_sb, stream),
-                  "batch staging kernels");
-        hip_check(launch_batch_reset(static_cast<uint32_t*>(next.ptr), static_cast<uint32_t*>(remaining.ptr),
-                                     static_cast<const uint32_t*>(tiles.ptr), n, static_cast<uint64_t*>(stamps.ptr),
-                                     stream),
-                  "batch reset kernel");
-        // both queues: the texture-path one on the side stream, forked right after the reset
-        // so that it runs beside the LDS-staged one (they share nothing but the per-capture
-        // counters, and every capture is on exactly one queue)
-        const bool fork_side = n_lds_items && n_items;
-        const hipStream_t ts = fork_side ? side : stream;
-        if (fork_side) {
-            hip_check(hipEventRecord(fork, stream), "hipEventRecord");
-            hip_check(hipStreamWaitEvent(side, fork, 0), "hipStreamWaitEvent");
-            side_joined = true;
-        }
-        if (n_lds_items) {
-            BatchArgs b{};
-            b.caps = static_cast<const ConfocalArgs*>(caps.ptr);
-            b.flags = static_cast<const uint32_t*>(flags.ptr);
-            b.items = static_cast<const uint2*>(items_lds.ptr);
-            b.n_items = n_lds_items;
-            b.next = static_cast<uint32_t*>(next.ptr);
-            b.remaining = static_cast<uint32_t*>(remaining.ptr);
-            b.done = static_cast<uint64_t*>(stamps.ptr) + 1;
-            hip_check(launch_batch_lds(b, format, lds_groups, stream), "batch LDS-staged gather kernel");
-        }
-        if (n_items) {
-            BatchArgs b{};
-            b.caps = static_cast<const ConfocalArgs*>(caps.ptr);
-            b.flags = static_cast<const uint32_t*>(flags.ptr);
-            b.items = static_cast<const uint2*>(items.ptr);
-            b.n_items = n_items;
-            b.next = static_cast<uint32_t*>(next.ptr) + 1;
-            b.remaining = static_cast<uint32_t*>(remaining.ptr);
-            b.done = static_cast<uint64_t*>(stamps.ptr) + 1;
-            hip_check(launch_batch_confocal(b, K, format, groups, ts), "batch gather kernel");
-        }
-        if (fork_side) {
-            hip_check(hipEventRecord(join, side), "hipEventRecord");
-            hip_check(hipStreamWaitEvent(stream, join, 0), "hipStreamWaitEvent");
-            side_joined = false;
-        }
+        hip_check(launch_batch_prologue(static_cast<const StageJob*>(zjobs.ptr), n_zjobs, z_blocks, q,
+                                        static_cast<const uint32_t*>(init.ptr), n,
+                                        static_cast<uint64_t*>(stamps.ptr), stream),
+                  "batch prologue kernel");
+        if (n_items) hip_check(launch_batch_queue(q, K, kz, format, groups, stream), "batch queue kernel");
     } catch (...) {
-        // a side stream left in the capture would make the capture fail as unjoined and stay in
-        // capture mode: join it back first, then end (and drop) the capture
-        if (side_joined && hipEventRecord(join, side) == hipSuccess)
-            (void)hipStreamWaitEvent(stream, join, 0);
         hipGraph_t g = nullptr;
         (void)hipStreamEndCapture(stream, &g);
         if (g) (void)hipGraphDestroy(g);
@@ -278,9 +260,6 @@ bpx_status bpx_batch_create(const bpx_capture_desc* captures, uint32_t count, co
         else
             b->format = o.storage == BPX_STORAGE_F16 ? kF16 : o.storage == BPX_STORAGE_BF16 ? kBF16 : kF32;
         hip_check(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate");
-        hip_check(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking), "hipStreamCreate");
-        hip_check(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming), "hipEventCreate");
-        hip_check(hipEventCreateWithFlags(&b->join, hipEventDisableTiming), "hipEventCreate");
         for (auto& e : b->ev) hip_check(hipEventCreate(&e), "hipEventCreate");
         hip_check(hipEventCreateWithFlags(&b->done, hipEventDisableTiming), "hipEventCreate");
         hip_check(hipEventCreateWithFlags(&b->ready, hipEventDisableTiming), "hipEventCreate");
@@ -299,10 +278,13 @@ bpx_status bpx_batch_create(const bpx_capture_desc* captures, uint32_t count, co
             b->plans.push_back(std::move(pl));
         }
         const uint32_t n = std::max<uint32_t>(count, 1);
-        b->next.alloc(8);   // queue heads: LDS-staged, texture path
+        b->next.alloc(4);
+        b->error.alloc(4);
+        hip_check(hipMemset(b->error.ptr, 0, 4), "hipMemset");
+        b->staged.alloc(4u * n);
         b->remaining.alloc(4u * n);
         b->stamps.alloc(8u * (n + 1u));
-        b->build(bpx_batch_config{2u, 0u, 0u});
+        b->build(bpx_batch_config{2u, 0u, 0u, 0u, 8u});
         auto h = std::make_unique<bpx_batch_s>();
         h->impl = std::move(b);
         *out = h.release();
@@ -337,22 +319,26 @@ bpx_status bpx_batch_tune(bpx_batch batch, int32_t reps, bpx_batch_config* chose
         BatchImpl& b = *batch->impl;
         b.wait();
         reps = std::max(reps, 1);
-        bpx_batch_config best{2u, 0u, 0u};
+        bpx_batch_config best{2u, 0u, 0u, 0u, 8u};
         double best_ms = INFINITY;
-        for (uint32_t k : {2u, 4u}) {
-            const int resident = batch_resident_groups((int)k, b.format);
-            for (uint32_t frac : {1u, 2u})
-                for (uint32_t order : {0u, 1u}) {
-                    const bpx_batch_config c{k, resident > 0 ? std::max(1u, (uint32_t)resident / frac) : 0u, order};
-                    b.build(c);
-                    (void)run_timed(b, 1);  // warm-up
-                    const double ms = run_timed(b, reps);
-                    if (ms < best_ms) {
-                        best_ms = ms;
-                        best = b.cfg;
+        const uint32_t depths[2] = {8u, 16u};
+        for (uint32_t k : {2u, 4u})
+            for (uint32_t kz : depths) {
+                if (kz == 16u && format_complex(b.format)) continue;   // complex formats: depth 8
+                const int resident = batch_queue_resident_groups((int)k, (int)kz, b.format);
+                for (uint32_t frac : {1u, 2u})
+                    for (uint32_t order : {0u, 1u}) {
+                        const bpx_batch_config c{k, resident > 0 ? std::max(1u, (uint32_t)resident / frac) : 0u,
+                                                 order, 0u, kz};
+                        b.build(c);
+                        (void)run_timed(b, 1);  // warm-up
+                        const double ms = run_timed(b, reps);
+                        if (ms < best_ms) {
+                            best_ms = ms;
+                            best = b.cfg;
+                        }
                     }
-                }
-        }
+            }
         b.build(best);
         if (chosen) *chosen = b.cfg;
     });

@@ -63,9 +63,18 @@ struct OctreeState {
     float tau = 0.25f;
     float prob[8] = {0.5f, 0.5f, 0.5f, 0.5f, 0.5f, 0.5f, 0.5f, 0.5f};  // per-level proposal thresholds
     float keep = 0.0f;    // U >= keep x running max overrides the net's veto (0: never)
+    float fallback[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // per-level full-resolution fallback probabilities
+    uint32_t stride = 1;  // coarse-level pair stride along each wall-grid axis (with a net)
     bool bounds_only = false;
     std::vector<Level> levels;
     DeviceBuffer ping, pong, fine_list, flags, offs, sums, total, stats, partial;
+    DeviceBuffer fb_list;                 // low-confidence children of one level (fallback)
+    DeviceBuffer fb_voxels;               // their fine voxels, gathered over the levels
+    // pair subset of the coarse levels: (x, y, z, c1) and the fused row of each subset pair
+    DeviceBuffer sub_pairs, sub_rows;
+    uint32_t P_sub = 0, sub_stride = 0;   // subset size and the stride it was built for (0: none)
+    uint32_t coarse_P = 0;                // pairs the coarse levels used in the last execution
+    std::vector<uint32_t> fallback_voxels;  // fine voxels added by the fallback, per level
     DeviceBuffer bound_part_u, bound_part_v;  // pair-segment partial bounds (kBoundPartCapacity floats)
     static constexpr uint64_t kBoundPartCapacity = 1ull << 20;
     uint32_t* host_total = nullptr;

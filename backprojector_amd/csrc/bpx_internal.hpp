@@ -52,15 +52,23 @@ struct ConfocalArgs {
 };
 
 // Factorised exhaustive-grid kernel parameters (threshold table in path-length space).
-// Batched confocal fp32 captures (persistent work queue, see batch_confocal_kernel).
-struct BatchArgs {
+// Batched confocal captures: one persistent launch over a queue of staging, LDS-staged and
+// texture-path items (batch_queue_kernel).
+constexpr uint32_t kBatchStageBlocks = 16;   // 256-element fused-row blocks per staging item
+enum : uint32_t { kBatchItemStage = 0, kBatchItemLds = 1, kBatchItemTex = 2 };
+struct StageJob;
+struct BatchQueueArgs {
     const ConfocalArgs* caps;   // per capture; tiles for the batch's voxels per lane
     const uint32_t* flags;      // per capture: bit 0 SEP, bit 1 FLAT
-    const uint2* items;         // (capture, tile * 4 + wave quarter) in queue order
+    const StageJob* jobs;       // per capture: its row staging (rows 0: nothing to stage)
+    const uint2* items;         // x = capture | type << 28; y = tile, or first row block (stage)
     uint32_t n_items;
+    uint32_t spin_limit;        // polls of a capture's staging counter before giving up
     uint32_t* next;             // queue head (0 at launch)
-    uint32_t* remaining;        // per capture: items not finished
-    uint64_t* done;             // per capture: wall clock when its last item finished (0 at launch)
+    uint32_t* staged;           // per capture: staging items not finished
+    uint32_t* remaining;        // per capture: gather items not finished
+    uint64_t* done;             // per capture: wall clock when its last gather item finished
+    uint32_t* error;            // nonzero when a wait gave up (never in a correct execution)
 };
 struct GridArgs {
     const void* H;          // padded rows [L*C][ldh], pair p = l*C + c
@@ -112,7 +120,8 @@ struct WindowArgs {
 struct OctBoundArgs {
     const float* M;         // window-max table of the level [P][ldm]
     const void* E;          // fused fp32 sample+threshold rows [P][lde]
-    const float4* pairs;    // (x, y, z, c1)
+    const float4* pairs;    // (x, y, z, c1) of the level's pairs (all, or a subset)
+    const uint32_t* erow;   // fused row of pair p in E (nullptr = row p)
     const uint32_t* blocks;
     uint32_t nblocks;
     float* U;               // upper bound per block
@@ -146,10 +155,11 @@ struct WinmaxLevels {
     uint32_t count;
 };
 bool winmax_fused_fits(uint32_t Tc);
+// row_index: source row of each table row (nullptr = the same row).
 hipError_t launch_winmax_fused(const float* in, uint32_t in_pitch, uint32_t rows, uint32_t Tc, uint32_t out_pitch,
-                               const WinmaxLevels& lv, hipStream_t s);
+                               const WinmaxLevels& lv, const uint32_t* row_index, hipStream_t s);
 hipError_t launch_winmax_double(const float* in, uint32_t in_pitch, float* out, uint32_t out_pitch,
-                                uint32_t rows, uint32_t Tc, uint32_t half, hipStream_t s);
+                                uint32_t rows, uint32_t Tc, uint32_t half, const uint32_t* row_index, hipStream_t s);
 hipError_t launch_winmax_combine(const float* t2, uint32_t pitch2, float* out, uint32_t out_pitch,
                                  uint32_t rows, uint32_t Tc, uint32_t shift, hipStream_t s);
 // Pair segments for a level of nblocks blocks (partials must fit part_capacity floats).
@@ -160,9 +170,11 @@ hipError_t launch_oct_reduce(const float* U, const float* V, uint32_t n, double*
 hipError_t launch_oct_fine(const OctFineArgs& a, hipStream_t s);
 hipError_t launch_fill(float* p, uint64_t n, float v, hipStream_t s);
 hipError_t launch_oct_all_blocks(uint32_t nx, uint32_t ny, uint32_t nz, uint32_t* out, hipStream_t s);
-hipError_t launch_oct_scan_children(const uint32_t* flags, uint32_t m, uint32_t* offs, uint32_t* sums,
+// Children (8 per block, flags[8 b + c]) whose flag equals want, compacted in block order.
+hipError_t launch_oct_scan_children(const uint32_t* flags, uint32_t m, uint32_t want, uint32_t* offs, uint32_t* sums,
                                     uint32_t* total, const uint32_t* blocks, uint32_t* children,
                                     hipStream_t s);
+hipError_t launch_oct_expand(const uint32_t* blocks, uint32_t n, uint32_t B, uint32_t* out, hipStream_t s);
 
 // ------------------------------------------------------------------ proposal net kernels
 hipError_t launch_conv3_fwd(const uint16_t* X, const uint16_t* W, const float* bias, uint16_t* yb,
@@ -187,8 +199,8 @@ hipError_t launch_child_labels(const float* pooled, uint32_t n0, uint32_t n1, ui
 hipError_t launch_max_f64(const float* x, uint64_t n, double* partial, uint32_t max_parts, double* out,
                           hipStream_t s);
 hipError_t launch_oct_child_flags(const uint32_t* blocks, const float* U, uint32_t n, const double* stats,
-                                  const float* logits, uint32_t n1, uint32_t n2, float logit_thr, float keep,
-                                  uint32_t* flags, hipStream_t s);
+                                  const float* logits, uint32_t n1, uint32_t n2, float logit_thr, float fallback_thr,
+                                  float keep, uint32_t* flags, hipStream_t s);
 // All launchers enqueue on `stream` and return the hipError_t of the launch.
 // Confocal threshold table S[p][j] (j = 0..Tc) from the pair-independent path-length table
 // Z[j] = smallest total path whose single-tap bin is >= j (launch_ztable kind 0, Tc + 1 entries).
@@ -238,17 +250,17 @@ hipError_t launch_grid(const GridArgs& a, int format, int variant, hipStream_t s
 hipError_t launch_pairs(const PairsArgs& a, int format, int variant, hipStream_t stream);
 // fp32 samples read straight from unpadded rows [P][Tc] (ldh = Tc); no padded copy needed.
 hipError_t launch_pairs_raw(const PairsArgs& a, hipStream_t stream);
-hipError_t launch_batch_reset(uint32_t* next, uint32_t* remaining, const uint32_t* tiles, uint32_t n,
-                              uint64_t* start, hipStream_t s);
-// format: kF32, kF16, kBF16, kC64, kCF16 or kCBF16 (fused rows); K: 2 or 4 voxels per lane.
-hipError_t launch_batch_confocal(const BatchArgs& b, int K, int format, uint32_t groups, hipStream_t s);
-int batch_resident_groups(int K, int format);  // work-groups resident at once (0 if unknown)
-// The LDS-staged persistent batch kernel (16 x 16 x 8 tiles, one item per work-group and tile):
-// whether a capture's grid fits it, its tiles, resident work-groups, launch (queue head next[0]).
-bool batch_lds_applies(const Grid& g, float inv2, int format, bool sep, bool flat);
-uint32_t batch_lds_tiles(const Grid& g);
-int batch_lds_resident_groups(int format);
-hipError_t launch_batch_lds(const BatchArgs& b, int format, uint32_t groups, hipStream_t s);
+// The batch queue kernel (K voxels per lane on the texture path, LDS tile depth kz, storage
+// format): resident work-groups, launch, the prologue (path-length tables, queue reset; init =
+// [staging items, gather items] per capture), and whether / in how many tiles a capture runs
+// LDS-staged.
+int batch_queue_resident_groups(int K, int kz, int format);
+int batch_lds_depth(int kz, int format);   // the depth the queue kernel uses for this format
+hipError_t launch_batch_queue(const BatchQueueArgs& b, int K, int kz, int format, uint32_t groups, hipStream_t s);
+hipError_t launch_batch_prologue(const StageJob* jobs, uint32_t n_jobs, uint32_t z_blocks, const BatchQueueArgs& b,
+                                 const uint32_t* init, uint32_t n, uint64_t* start, hipStream_t s);
+bool batch_lds_applies(const Grid& g, float inv2, int format, bool sep, bool flat, int kz);
+uint32_t batch_lds_tiles(const Grid& g, int kz);
 hipError_t launch_window(const WindowArgs& a, int complex_samples, hipStream_t stream);
 hipError_t launch_abs_max(const float* src, uint64_t n, float* result_dev, hipStream_t stream);
 hipError_t launch_copy(void* dst, const void* src, uint64_t bytes, hipStream_t stream);

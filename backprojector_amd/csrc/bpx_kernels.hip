@@ -234,15 +234,33 @@ __device__ __forceinline__ uint32_t stage_job_of(const StageJob* jobs, uint32_t 
     return lo;
 }
 
+// Entry j of a job's path-length table Z (kind 0: single tap).
+__device__ __forceinline__ void stage_z_entry(const StageJob& jb, uint32_t j) {
+    const ZPred pred{0, jb.t0, jb.dT, 0.0f, (float)j};
+    const float g = jb.t0 + ((float)j - 0.5f) * jb.dT;
+    jb.Z[j] = fkey_inv(min_true_key(pred, fkey(g), fkey(-__builtin_inff()), fkey(__builtin_inff())));
+}
+
+// Lane threadIdx.x of row block `local` of a job (256 elements of one fused row): the row and
+// segment come from block-uniform values (no per-lane division).
+__device__ __forceinline__ void stage_row_block(const StageJob& jb, uint32_t local) {
+    const uint32_t per_row = (jb.lde + 255u) / 256u;
+    const uint32_t p = local / per_row, j = (local - p * per_row) * 256u + threadIdx.x;
+    if (j >= jb.lde) return;
+    const uint64_t e = (uint64_t)p * jb.lde + j;
+    const int cplx = format_complex(jb.format) ? 1 : 0;
+    const float* row = jb.src + (size_t)p * jb.Tc * (cplx ? 2u : 1u);
+    float S = 0.0f;
+    if (j <= jb.Tc) S = stable_threshold(jb.pair8[8u * p + 7u], jb.pair8[8u * p + 6u], jb.Z[j]);
+    store_fused(jb.dst, e, jb.format, row, j, jb.Tc, jb.inv_scale, S);
+}
+
 __global__ __launch_bounds__(256) void stage_z_kernel(const StageJob* __restrict__ jobs, uint32_t n_jobs,
                                                       uint32_t blocks) {
     for (uint32_t blk = blockIdx.x; blk < blocks; blk += gridDim.x) {
         const StageJob& jb = jobs[stage_job_of<true>(jobs, n_jobs, blk)];
         const uint32_t j = (blk - jb.z_first) * 256u + threadIdx.x;
-        if (j > jb.Tc) continue;
-        const ZPred pred{0, jb.t0, jb.dT, 0.0f, (float)j};
-        const float g = jb.t0 + ((float)j - 0.5f) * jb.dT;
-        jb.Z[j] = fkey_inv(min_true_key(pred, fkey(g), fkey(-__builtin_inff()), fkey(__builtin_inff())));
+        if (j <= jb.Tc) stage_z_entry(jb, j);
     }
 }
 
@@ -250,17 +268,7 @@ __global__ __launch_bounds__(256) void stage_rows_kernel(const StageJob* __restr
                                                          uint32_t blocks) {
     for (uint32_t blk = blockIdx.x; blk < blocks; blk += gridDim.x) {
         const StageJob& jb = jobs[stage_job_of<false>(jobs, n_jobs, blk)];
-        // each block covers 256 elements of one row: row and segment from block-uniform values
-        // (no per-lane division)
-        const uint32_t per_row = (jb.lde + 255u) / 256u, local = blk - jb.s_first;
-        const uint32_t p = local / per_row, j = (local - p * per_row) * 256u + threadIdx.x;
-        if (j >= jb.lde) continue;
-        const uint64_t e = (uint64_t)p * jb.lde + j;
-        const int cplx = format_complex(jb.format) ? 1 : 0;
-        const float* row = jb.src + (size_t)p * jb.Tc * (cplx ? 2u : 1u);
-        float S = 0.0f;
-        if (j <= jb.Tc) S = stable_threshold(jb.pair8[8u * p + 7u], jb.pair8[8u * p + 6u], jb.Z[j]);
-        store_fused(jb.dst, e, jb.format, row, j, jb.Tc, jb.inv_scale, S);
+        stage_row_block(jb, blk - jb.s_first);
     }
 }
 
@@ -788,32 +796,51 @@ __device__ __forceinline__ void lds_tile(const ConfocalArgs& a, uint32_t tile) {
     fetch(0u);
     commit(0u);
     __syncthreads();
+    // One pair of a stage: slot = its LDS segment, start = its window start (wave-uniform).
+    auto pair_step = [&](const float4 pr, uint32_t slot, uint32_t start) {
+        // byte offset of element j of this pair's window = E j + base (unsigned wrap-around);
+        // the window start is the same for every lane, so base lives in a scalar register
+        const uint32_t base = (slot * SEG - start) * E;
+        const float dx = pr.x - vx, dy = pr.y - vy;
+        const float axy = dx * dx + dy * dy;
+#pragma unroll
+        for (int r = 0; r < KZ; ++r) {
+            const float s = axy + dz2[r];
+            const int j = (int)__builtin_fmaf(__builtin_amdgcn_sqrtf(s), a.inv2, pr.w);
+            acc_add(blk[r], Fz::lds_pick(segb + ((uint32_t)j * E + base), s));
+        }
+        if (++in_block == a.kahan_block) {
+#pragma unroll
+            for (int r = 0; r < KZ; ++r) {
+                kahan_add(sum[r], comp[r], blk[r]);
+                acc_zero(blk[r]);
+            }
+            in_block = 0;
+        }
+    };
     for (uint32_t b = 0; b < nstage; ++b) {
         const uint32_t buf = b & 1u;
         if (b + 1u < nstage) fetch((b + 1u) * NW);
-        const uint32_t p0 = b * NW, np = min((uint32_t)NW, a.P - p0);
-        for (uint32_t i = 0; i < np; ++i) {
-            const float4 pr = a.pairs[p0 + i];
-            // byte offset of element j of this pair's window = E j + base (unsigned wrap-around);
-            // the window start is the same for every lane, so base lives in a scalar register
-            const uint32_t base =
-                __builtin_amdgcn_readfirstlane(((buf * NW + i) * SEG - (uint32_t)meta[buf][i][0]) * E);
-            const float dx = pr.x - vx, dy = pr.y - vy;
-            const float axy = dx * dx + dy * dy;
+        const uint32_t p0 = b * NW;
+        // Full stages with 8 real voxels per lane: the stage's window starts and pair data read
+        // once into scalar registers and the pair loop unrolled.  Deeper tiles (16 voxels per
+        // lane) and complex accumulators keep the loop: unrolled, they need more registers than
+        // 4 waves per SIMD allow.
+        constexpr bool kUnrollStage = KZ == 8 && std::is_same<acc_t, float>::value;
+        if (kUnrollStage && p0 + NW <= a.P) {
+            uint32_t start[NW];
+            float4 prs[NW];
 #pragma unroll
-            for (int r = 0; r < KZ; ++r) {
-                const float s = axy + dz2[r];
-                const int j = (int)__builtin_fmaf(__builtin_amdgcn_sqrtf(s), a.inv2, pr.w);
-                acc_add(blk[r], Fz::lds_pick(segb + ((uint32_t)j * E + base), s));
+            for (uint32_t i = 0; i < NW; ++i) {
+                start[i] = __builtin_amdgcn_readfirstlane((uint32_t)meta[buf][i][0]);
+                prs[i] = a.pairs[p0 + i];
             }
-            if (++in_block == a.kahan_block) {
 #pragma unroll
-                for (int r = 0; r < KZ; ++r) {
-                    kahan_add(sum[r], comp[r], blk[r]);
-                    acc_zero(blk[r]);
-                }
-                in_block = 0;
-            }
+            for (uint32_t i = 0; i < NW; ++i) pair_step(prs[i], buf * NW + i, start[i]);
+        } else {
+            const uint32_t np = min((uint32_t)NW, a.P - p0);
+            for (uint32_t i = 0; i < np; ++i)
+                pair_step(a.pairs[p0 + i], buf * NW + i, __builtin_amdgcn_readfirstlane((uint32_t)meta[buf][i][0]));
         }
         if (b + 1u < nstage) commit(buf ^ 1u);
         __syncthreads();
@@ -843,14 +870,20 @@ __global__ __launch_bounds__(NW * 64) void confocal_lds_kernel(ConfocalArgs a) {
 }
 
 // ------------------------------------------------------------------- batched captures
-// Persistent work queue over (capture, tile quarter) items of many confocal fp32 captures:
-// every WAVE takes the next item, runs the same tile body as the single-capture kernel (each
-// volume is bit-identical to its own plan's), and the wave that finishes a capture's last
-// quarter stamps the wall clock.  Waves leave when the queue is empty.
-// The queue head and the completion counters are updated by every lane with an increment that
-// is 1 on lane 0 and 0 elsewhere, never under a lane-divergent branch: a branch on "lane 0"
-// around the head update inside this loop lets the compiler split the wave into lane 0 and the
-// rest (the rest then re-read a stale head and never leave).  There is no work-group barrier.
+// One persistent launch serves every item of a batch of confocal captures from a single queue:
+//   stage   -- kBatchStageBlocks row blocks (256 fused-row elements each) of one capture
+//   LDS     -- one 16 x 16 x KZ tile of a capture whose bin windows fit the LDS segment
+//   texture -- one 8 x 8 x 4K tile of any other capture (the 4 waves take its quarters)
+// Work-groups take items in queue order (one atomic by thread 0, broadcast through LDS between
+// two barriers, so every wave of a work-group sees the same item and they leave together).  A
+// capture's gather items wait until its staging items are done; the queue places a capture's
+// staging one capture ahead of its gathers, so the next capture's rows are built while the
+// current one is gathered.  Staging items are taken before the gathers that wait for them and
+// never wait themselves, so every wait ends; it is also bounded (spin_limit polls, then the
+// error word is set and the work-group goes on), and the item loop is bounded by the item
+// count, so the launch always drains.  Each tile runs the single-capture kernels' tile body:
+// every volume is bit-identical to its own plan's.  The work-group finishing a capture's last
+// gather item stamps the device wall clock (per-capture latency).
 // A struct every lane reads from the same address, moved word by word into scalar registers:
 // the buffer descriptors built from it are then wave-uniform (no per-pair waterfall loops).
 template <class T>
@@ -864,39 +897,10 @@ __device__ __forceinline__ T wave_uniform_load(const T* p) {
     return out;
 }
 
-template <int K, class Src>
-__global__ __launch_bounds__(256) void batch_confocal_kernel(BatchArgs b) {
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t one = lane == 0u ? 1u : 0u;
-    // a wave can take at most every item once: the bound makes termination unconditional
-    for (uint32_t taken = 0; taken <= b.n_items; ++taken) {
-        // all lanes active: the first active lane is lane 0, whose old value is the item
-        const uint32_t it = __builtin_amdgcn_readfirstlane(atomicAdd(b.next, one));
-        if (it >= b.n_items) return;
-        const uint2 item = b.items[it];
-        const uint32_t c = __builtin_amdgcn_readfirstlane(item.x);
-        const uint32_t q = __builtin_amdgcn_readfirstlane(item.y);  // tile * 4 + quarter
-        const ConfocalArgs a = wave_uniform_load(b.caps + c);
-        const uint32_t f = __builtin_amdgcn_readfirstlane(b.flags[c]);
-        if (f == 3u) confocal_tile<Src, K, true, true>(a, q >> 2, q & 3u);
-        else if (f == 1u) confocal_tile<Src, K, true, false>(a, q >> 2, q & 3u);
-        else confocal_tile<Src, K, false, false>(a, q >> 2, q & 3u);
-        __threadfence();
-        const uint32_t left = atomicSub(&b.remaining[c], one);
-        const unsigned long long stamp = (one && left == 1u) ? (unsigned long long)wall_clock64() : 0ull;
-        atomicMax(reinterpret_cast<unsigned long long*>(&b.done[c]), stamp);
-    }
-}
-
-// Persistent work queue over (capture, 16 x 16 x 8 tile) items of the captures whose bin
-// windows fit the LDS segment: every WORK-GROUP takes the next item (one atomic by thread 0,
-// broadcast through LDS between two barriers, so the exit test is the same for all its
-// waves) and runs the LDS-staged tile body, which ends with a barrier before the next item
-// reuses the LDS buffers.  Volumes are bit-identical to the single-capture kernels'.
-template <class Fz>
-__global__ __launch_bounds__(256) void batch_lds_kernel(BatchArgs b) {
+template <class Fz, class Src, int K, int KZ>
+__global__ __launch_bounds__(256) void batch_queue_kernel(BatchQueueArgs b) {
     __shared__ uint32_t s_item;
-    // a work-group can take at most every item once: the bound makes termination unconditional
+    const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     for (uint32_t taken = 0; taken <= b.n_items; ++taken) {
         if (threadIdx.x == 0) s_item = atomicAdd(b.next, 1u);
         __syncthreads();
@@ -904,70 +908,96 @@ __global__ __launch_bounds__(256) void batch_lds_kernel(BatchArgs b) {
         __syncthreads();
         if (it >= b.n_items) return;
         const uint2 item = b.items[it];
-        const uint32_t c = __builtin_amdgcn_readfirstlane(item.x);
-        const uint32_t tile = __builtin_amdgcn_readfirstlane(item.y);
+        const uint32_t c = __builtin_amdgcn_readfirstlane(item.x & 0x0fffffffu);
+        const uint32_t type = __builtin_amdgcn_readfirstlane(item.x >> 28);
+        const uint32_t y = __builtin_amdgcn_readfirstlane(item.y);
+        if (type == kBatchItemStage) {
+            const StageJob jb = wave_uniform_load(b.jobs + c);
+            const uint32_t total = jb.rows * ((jb.lde + 255u) / 256u);
+            const uint32_t end = min(y + kBatchStageBlocks, total);
+            for (uint32_t blk = y; blk < end; ++blk) stage_row_block(jb, blk);
+            __threadfence();
+            __syncthreads();
+            if (threadIdx.x == 0) atomicSub(&b.staged[c], 1u);
+            continue;
+        }
+        if (threadIdx.x == 0) {
+            uint32_t polls = 0;
+            while (__hip_atomic_load(&b.staged[c], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
+                if (++polls > b.spin_limit) {
+                    atomicOr(b.error, 1u);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(8);
+            }
+        }
+        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");   // the staged rows, for every wave
         const ConfocalArgs a = wave_uniform_load(b.caps + c);
-        lds_tile<Fz, 8, 4, true>(a, tile);
+        if (type == kBatchItemLds) {
+            lds_tile<Fz, KZ, 4, true>(a, y);
+        } else {
+            const uint32_t f = __builtin_amdgcn_readfirstlane(b.flags[c]);
+            if (f == 3u) confocal_tile<Src, K, true, true>(a, y, w);
+            else if (f == 1u) confocal_tile<Src, K, true, false>(a, y, w);
+            else confocal_tile<Src, K, false, false>(a, y, w);
+        }
         __threadfence();
+        __syncthreads();
         if (threadIdx.x == 0) {
             const uint32_t left = atomicSub(&b.remaining[c], 1u);
-            if (left == 1u) atomicMax(reinterpret_cast<unsigned long long*>(&b.done[c]),
-                                      (unsigned long long)wall_clock64());
+            if (left == 1u)
+                atomicMax(reinterpret_cast<unsigned long long*>(&b.done[c]), (unsigned long long)wall_clock64());
         }
     }
 }
 
-// Resets the queue (next = 0, remaining = items per capture, completion stamps 0) and stamps
-// the start time: stamps[0] = start, stamps[1 + i] = completion of capture i.
-__global__ void batch_reset_kernel(uint32_t* next, uint32_t* remaining, const uint32_t* tiles, uint32_t n,
-                                   uint64_t* start) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) {
-        remaining[i] = tiles[i];
-        start[1 + i] = 0ull;  // completion stamps (done) follow the start stamp
+// Before the queue: every capture's path-length table (blocks < z_blocks, as stage_z_kernel)
+// and, in the last block, the queue reset: head and error 0, per-capture counters from their
+// initial values, completion stamps 0, the start stamp.
+__global__ __launch_bounds__(256) void batch_prologue_kernel(const StageJob* __restrict__ jobs, uint32_t n_jobs,
+                                                             uint32_t z_blocks, BatchQueueArgs b,
+                                                             const uint32_t* __restrict__ init, uint32_t n,
+                                                             uint64_t* start) {
+    if (blockIdx.x < z_blocks) {
+        const StageJob& jb = jobs[stage_job_of<true>(jobs, n_jobs, blockIdx.x)];
+        const uint32_t j = (blockIdx.x - jb.z_first) * 256u + threadIdx.x;
+        if (j <= jb.Tc) stage_z_entry(jb, j);
+        return;
     }
-    if (i == 0) {
-        next[0] = 0u;   // LDS-staged queue
-        next[1] = 0u;   // texture-path queue
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        b.staged[i] = init[2u * i];
+        b.remaining[i] = init[2u * i + 1u];
+        b.done[i] = 0ull;
+    }
+    if (threadIdx.x == 0) {
+        *b.next = 0u;
+        *b.error = 0u;
         *start = wall_clock64();
     }
 }
 
-hipError_t launch_batch_reset(uint32_t* next, uint32_t* remaining, const uint32_t* tiles, uint32_t n,
-                              uint64_t* start, hipStream_t s) {
-    hipLaunchKernelGGL(batch_reset_kernel, dim3((n + 255u) / 256u + (n == 0u ? 1u : 0u)), dim3(256), 0, s, next,
-                       remaining, tiles, n, start);
-    return hipGetLastError();
+template <class Fz, class Src, int K, int KZ>
+static const void* batch_queue_fn() {
+    return reinterpret_cast<const void*>(batch_queue_kernel<Fz, Src, K, KZ>);
 }
-
-// The batch kernel of a storage format (every capture of a batch shares it) and K.
-template <int K>
-static const void* batch_kernel_of(int format) {
+template <int K, int KZ>
+static const void* batch_queue_of(int format) {
     switch (format) {
-        case kF32: return reinterpret_cast<const void*>(batch_confocal_kernel<K, FusedSource<FusedF32>>);
-        case kF16: return reinterpret_cast<const void*>(batch_confocal_kernel<K, FusedSource<FusedH16<false>>>);
-        case kBF16: return reinterpret_cast<const void*>(batch_confocal_kernel<K, FusedSource<FusedH16<true>>>);
-        case kC64: return reinterpret_cast<const void*>(batch_confocal_kernel<K, FusedSource<FusedC64>>);
-        case kCF16: return reinterpret_cast<const void*>(batch_confocal_kernel<K, FusedSource<FusedC16<false>>>);
-        case kCBF16: return reinterpret_cast<const void*>(batch_confocal_kernel<K, FusedSource<FusedC16<true>>>);
+        case kF32: return batch_queue_fn<FusedF32, FusedSource<FusedF32>, K, KZ>();
+        case kF16: return batch_queue_fn<FusedH16<false>, FusedSource<FusedH16<false>>, K, KZ>();
+        case kBF16: return batch_queue_fn<FusedH16<true>, FusedSource<FusedH16<true>>, K, KZ>();
+        case kC64: return batch_queue_fn<FusedC64, FusedSource<FusedC64>, K, KZ>();
+        case kCF16: return batch_queue_fn<FusedC16<false>, FusedSource<FusedC16<false>>, K, KZ>();
+        case kCBF16: return batch_queue_fn<FusedC16<true>, FusedSource<FusedC16<true>>, K, KZ>();
         default: return nullptr;
     }
 }
-
-static const void* batch_kernel(int K, int format) {
-    return K == 4 ? batch_kernel_of<4>(format) : K == 2 ? batch_kernel_of<2>(format) : nullptr;
-}
-
-static const void* batch_lds_kernel_of(int format) {
-    switch (format) {
-        case kF32: return reinterpret_cast<const void*>(batch_lds_kernel<FusedF32>);
-        case kF16: return reinterpret_cast<const void*>(batch_lds_kernel<FusedH16<false>>);
-        case kBF16: return reinterpret_cast<const void*>(batch_lds_kernel<FusedH16<true>>);
-        case kC64: return reinterpret_cast<const void*>(batch_lds_kernel<FusedC64>);
-        case kCF16: return reinterpret_cast<const void*>(batch_lds_kernel<FusedC16<false>>);
-        case kCBF16: return reinterpret_cast<const void*>(batch_lds_kernel<FusedC16<true>>);
-        default: return nullptr;
-    }
+static const void* batch_queue(int K, int kz, int format) {
+    if (format_complex(format)) kz = 8;   // two accumulators per voxel: 8 voxels per lane
+    if (K == 2) return kz == 16 ? batch_queue_of<2, 16>(format) : batch_queue_of<2, 8>(format);
+    if (K == 4) return kz == 16 ? batch_queue_of<4, 16>(format) : batch_queue_of<4, 8>(format);
+    return nullptr;
 }
 
 static int resident_of(const void* fn) {
@@ -984,45 +1014,31 @@ static int resident_of(const void* fn) {
     return cus * per_cu;
 }
 
-int batch_lds_resident_groups(int format) { return resident_of(batch_lds_kernel_of(format)); }
+int batch_queue_resident_groups(int K, int kz, int format) { return resident_of(batch_queue(K, kz, format)); }
+int batch_lds_depth(int kz, int format) { return format_complex(format) ? 8 : kz; }
 
-hipError_t launch_batch_lds(const BatchArgs& b, int format, uint32_t groups, hipStream_t s) {
-    if (groups == 0u) return hipErrorInvalidValue;
-    const void* fn = batch_lds_kernel_of(format);
-    if (!fn) return hipErrorInvalidValue;
-    void* args[] = {const_cast<BatchArgs*>(&b)};
+hipError_t launch_batch_queue(const BatchQueueArgs& b, int K, int kz, int format, uint32_t groups, hipStream_t s) {
+    const void* fn = batch_queue(K, kz, format);
+    if (!fn || groups == 0u) return hipErrorInvalidValue;
+    void* args[] = {const_cast<BatchQueueArgs*>(&b)};
     return hipLaunchKernel(fn, dim3(groups), dim3(256), args, 0, s);
 }
 
-bool batch_lds_applies(const Grid& g, float inv2, int format, bool sep, bool flat) {
+hipError_t launch_batch_prologue(const StageJob* jobs, uint32_t n_jobs, uint32_t z_blocks, const BatchQueueArgs& b,
+                                 const uint32_t* init, uint32_t n, uint64_t* start, hipStream_t s) {
+    hipLaunchKernelGGL(batch_prologue_kernel, dim3(z_blocks + 1u), dim3(256), 0, s, jobs, n_jobs, z_blocks, b, init,
+                       n, start);
+    return hipGetLastError();
+}
+
+bool batch_lds_applies(const Grid& g, float inv2, int format, bool sep, bool flat, int kz) {
     // the batch kernel stages complex64 rows as 16-B elements (both components in one pass)
     const uint32_t seg = seg_elems(format == kC64 ? FusedC64::kLdsBytes : 8);
-    return sep && flat && confocal_lds_fits(g.inc, inv2, 8, seg);
+    return sep && flat && confocal_lds_fits(g.inc, inv2, kz, seg);
 }
-uint32_t batch_lds_tiles(const Grid& g) {
-    return ((g.n[0] + kSegTileXY - 1u) / kSegTileXY) * ((g.n[1] + kSegTileXY - 1u) / kSegTileXY) * ((g.n[2] + 7u) / 8u);
-}
-
-int batch_resident_groups(int K, int format) { return resident_of(batch_kernel(K, format)); }
-
-hipError_t launch_batch_confocal(const BatchArgs& b, int K, int format, uint32_t groups, hipStream_t s) {
-    if (groups == 0u) return hipErrorInvalidValue;
-#define BPX_BATCH_LAUNCH(KK, SRC) hipLaunchKernelGGL((batch_confocal_kernel<KK, SRC>), dim3(groups), dim3(256), 0, s, b)
-    if (K == 4 && format == kF32) BPX_BATCH_LAUNCH(4, FusedSource<FusedF32>);
-    else if (K == 2 && format == kF32) BPX_BATCH_LAUNCH(2, FusedSource<FusedF32>);
-    else if (K == 4 && format == kF16) BPX_BATCH_LAUNCH(4, FusedSource<FusedH16<false>>);
-    else if (K == 2 && format == kF16) BPX_BATCH_LAUNCH(2, FusedSource<FusedH16<false>>);
-    else if (K == 4 && format == kBF16) BPX_BATCH_LAUNCH(4, FusedSource<FusedH16<true>>);
-    else if (K == 2 && format == kBF16) BPX_BATCH_LAUNCH(2, FusedSource<FusedH16<true>>);
-    else if (K == 4 && format == kC64) BPX_BATCH_LAUNCH(4, FusedSource<FusedC64>);
-    else if (K == 2 && format == kC64) BPX_BATCH_LAUNCH(2, FusedSource<FusedC64>);
-    else if (K == 4 && format == kCF16) BPX_BATCH_LAUNCH(4, FusedSource<FusedC16<false>>);
-    else if (K == 2 && format == kCF16) BPX_BATCH_LAUNCH(2, FusedSource<FusedC16<false>>);
-    else if (K == 4 && format == kCBF16) BPX_BATCH_LAUNCH(4, FusedSource<FusedC16<true>>);
-    else if (K == 2 && format == kCBF16) BPX_BATCH_LAUNCH(2, FusedSource<FusedC16<true>>);
-    else return hipErrorInvalidValue;
-#undef BPX_BATCH_LAUNCH
-    return hipGetLastError();
+uint32_t batch_lds_tiles(const Grid& g, int kz) {
+    return ((g.n[0] + kSegTileXY - 1u) / kSegTileXY) * ((g.n[1] + kSegTileXY - 1u) / kSegTileXY) *
+           ((g.n[2] + (uint32_t)kz - 1u) / (uint32_t)kz);
 }
 
 // ------------------------------------------------------------------- exhaustive grid

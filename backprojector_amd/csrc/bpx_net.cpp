@@ -141,10 +141,13 @@ bool net_child_flags(PlanImpl& pl, uint32_t l, hipStream_t s) {
     net.forward(lv.n[0], lv.n[1], lv.n[2], s);
     const float p = st.prob[l];
     const float logit_thr = std::log(p / (1.0f - p));
+    // low-confidence band (fallback probability, p]: full-resolution fallback; none when unset
+    const float fb = st.fallback[l];
+    const float fallback_thr = fb > 0.0f && fb < p ? std::log(fb / (1.0f - fb)) : logit_thr;
     hip_check(launch_oct_child_flags(static_cast<const uint32_t*>(lv.list.ptr), static_cast<const float*>(lv.U.ptr),
                                      lv.count, static_cast<const double*>(st.stats.ptr),
                                      static_cast<const float*>(net.logits.ptr), lv.n[1], lv.n[2], logit_thr,
-                                     st.keep, static_cast<uint32_t*>(st.flags.ptr), s),
+                                     fallback_thr, st.keep, static_cast<uint32_t*>(st.flags.ptr), s),
               "learned child flags");
     return true;
 }

@@ -377,33 +377,37 @@ hipError_t launch_max_f64(const float* x, uint64_t n, double* partial, uint32_t 
 
 // Learned child flags of a level: flag[8i + c] = (U[i] >= threshold) && sigmoid(logit) > p,
 // logits read from the dense level grid at the block's position.
-// Children of block b: refined iff U >= tau x running max (stats[4]) and, with a net, its logit
-// exceeds the threshold or U >= keep x running max (stats[0]; keep 0 never overrides).
+// Children of block b: 1 = refined, 2 = computed at full resolution (every fine voxel), 0 =
+// dropped.  Without a net: 1 iff U >= tau x running max (stats[4]).  With a net, a child that
+// passes that rule is refined iff its logit exceeds logit_thr or U >= keep x running max
+// (stats[0]; keep 0 never overrides); below logit_thr but above fallback_thr (a low-confidence
+// child) it takes the full-resolution fallback (fallback_thr >= logit_thr: never).
 __global__ void oct_child_flags_kernel(const uint32_t* __restrict__ blocks, const float* __restrict__ U,
                                        uint32_t n, const double* __restrict__ stats,
                                        const float* __restrict__ logits, uint32_t n1, uint32_t n2,
-                                       float logit_thr, float keep, uint32_t* __restrict__ flags) {
+                                       float logit_thr, float fallback_thr, float keep,
+                                       uint32_t* __restrict__ flags) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n * 8u) return;
     const uint32_t b = i / 8u, c = i % 8u;
     uint32_t f = (double)U[b] >= stats[4] ? 1u : 0u;
-    if (logits) {
+    if (logits && f) {
         const uint32_t cc = blocks[b];
         const uint32_t x = cc & 1023u, y = (cc >> 10) & 1023u, z = cc >> 20;
         const float lg = logits[(((size_t)x * n1 + y) * n2 + z) * 16u + c];
         const bool keep_bound = keep > 0.0f && (double)U[b] >= (double)keep * stats[0];
-        if (!(lg > logit_thr) && !keep_bound) f = 0u;
+        if (!(lg > logit_thr) && !keep_bound) f = lg > fallback_thr ? 2u : 0u;
     }
     flags[i] = f;
 }
 
 hipError_t launch_oct_child_flags(const uint32_t* blocks, const float* U, uint32_t n, const double* stats,
-                                  const float* logits, uint32_t n1, uint32_t n2, float logit_thr, float keep,
-                                  uint32_t* flags, hipStream_t s) {
+                                  const float* logits, uint32_t n1, uint32_t n2, float logit_thr, float fallback_thr,
+                                  float keep, uint32_t* flags, hipStream_t s) {
     const uint32_t m = n * 8u;
     if (m == 0) return hipSuccess;
     hipLaunchKernelGGL(oct_child_flags_kernel, dim3((m + 255u) / 256u), dim3(256), 0, s, blocks, U, n, stats,
-                       logits, n1, n2, logit_thr, keep, flags);
+                       logits, n1, n2, logit_thr, fallback_thr, keep, flags);
     return hipGetLastError();
 }
 

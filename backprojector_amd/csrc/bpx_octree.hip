@@ -25,12 +25,13 @@ namespace bpx {
 // floats.  Built by doubling: width 2w from two width-w tables.
 __global__ __launch_bounds__(256) void winmax_double_kernel(const float* __restrict__ in, uint32_t in_pitch,
                                                             float* __restrict__ out, uint32_t out_pitch,
-                                                            uint32_t rows, uint32_t Tc, uint32_t half) {
+                                                            uint32_t rows, uint32_t Tc, uint32_t half,
+                                                            const uint32_t* __restrict__ row_index) {
     const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t n = Tc + 1u;
     if (gid >= (uint64_t)rows * n) return;
     const uint32_t p = (uint32_t)(gid / n), b = (uint32_t)(gid % n);
-    const float* r = in + (size_t)p * in_pitch;
+    const float* r = in + (size_t)(row_index ? row_index[p] : p) * in_pitch;
     float m = 0.0f;
     if (b < Tc) {
         m = fmaxf(m, r[b]);
@@ -61,13 +62,15 @@ constexpr uint32_t kWinmaxLdsBytes = 64u * 1024u;
 
 bool winmax_fused_fits(uint32_t Tc) { return 2ull * 4ull * ((uint64_t)Tc + 1u) <= kWinmaxLdsBytes; }
 
+// rows: the source row of table row p (nullptr = row p).
 __global__ __launch_bounds__(1024) void winmax_fused_kernel(const float* __restrict__ in, uint32_t in_pitch,
-                                                            uint32_t Tc, uint32_t out_pitch, WinmaxLevels lv) {
+                                                            uint32_t Tc, uint32_t out_pitch, WinmaxLevels lv,
+                                                            const uint32_t* __restrict__ rows) {
     extern __shared__ float lds[];
     float* a = lds;
     float* b = lds + (Tc + 1u);
     const uint32_t p = blockIdx.x;
-    const float* r = in + (size_t)p * in_pitch;
+    const float* r = in + (size_t)(rows ? rows[p] : p) * in_pitch;
     const uint32_t n = Tc + 1u;
     for (uint32_t t = threadIdx.x; t < n; t += blockDim.x) a[t] = t < Tc ? fmaxf(0.0f, r[t]) : 0.0f;
     __syncthreads();
@@ -101,19 +104,21 @@ __global__ __launch_bounds__(1024) void winmax_fused_kernel(const float* __restr
 }
 
 hipError_t launch_winmax_fused(const float* in, uint32_t in_pitch, uint32_t rows, uint32_t Tc, uint32_t out_pitch,
-                               const WinmaxLevels& lv, hipStream_t s) {
+                               const WinmaxLevels& lv, const uint32_t* row_index, hipStream_t s) {
     if (!winmax_fused_fits(Tc) || lv.count == 0 || lv.count > 8) return hipErrorInvalidValue;
     if (rows == 0) return hipSuccess;
     const uint32_t lds = 2u * 4u * (Tc + 1u);
-    hipLaunchKernelGGL(winmax_fused_kernel, dim3(rows), dim3(1024), lds, s, in, in_pitch, Tc, out_pitch, lv);
+    hipLaunchKernelGGL(winmax_fused_kernel, dim3(rows), dim3(1024), lds, s, in, in_pitch, Tc, out_pitch, lv,
+                       row_index);
     return hipGetLastError();
 }
 
 hipError_t launch_winmax_double(const float* in, uint32_t in_pitch, float* out, uint32_t out_pitch,
-                                uint32_t rows, uint32_t Tc, uint32_t half, hipStream_t s) {
+                                uint32_t rows, uint32_t Tc, uint32_t half, const uint32_t* row_index, hipStream_t s) {
     const uint64_t n = (uint64_t)rows * (Tc + 1u);
+    if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(winmax_double_kernel, dim3((uint32_t)((n + 255u) / 256u)), dim3(256), 0, s, in,
-                       in_pitch, out, out_pitch, rows, Tc, half);
+                       in_pitch, out, out_pitch, rows, Tc, half, row_index);
     return hipGetLastError();
 }
 
@@ -173,7 +178,8 @@ __global__ __launch_bounds__(256) void oct_bound_kernel(OctBoundArgs a) {
     const uint32_t p0 = blockIdx.y * a.chunk, p1 = min(a.P, p0 + a.chunk);
     for (uint32_t p = p0; p < p1; ++p) {
         const float4 pr = a.pairs[p];
-        const uint32_t moff = p * a.ldm * 4u, eoff = p * a.lde * 8u;
+        // table row p of the level's pair set; fused sample row of the same pair in the capture
+        const uint32_t moff = p * a.ldm * 4u, eoff = (a.erow ? a.erow[p] : p) * a.lde * 8u;
         const float c1R = pr.w + a.cR;
 #pragma unroll
         for (int r = 0; r < K; ++r) {
@@ -365,11 +371,13 @@ hipError_t launch_oct_reduce(const float* U, const float* V, uint32_t n, double*
 
 // ------------------------------------------------------------------ selection
 // Exclusive scan of flags in chunks of 1024 (one work-group each), chunk totals in sums.
+// Items i with flags[i] == want are counted.
 __global__ __launch_bounds__(1024) void scan_chunk_kernel(const uint32_t* __restrict__ flags, uint32_t n,
-                                                          uint32_t* __restrict__ offs, uint32_t* __restrict__ sums) {
+                                                          uint32_t want, uint32_t* __restrict__ offs,
+                                                          uint32_t* __restrict__ sums) {
     __shared__ uint32_t sh[1024];
     const uint32_t i = blockIdx.x * 1024u + threadIdx.x;
-    const uint32_t f = i < n ? flags[i] : 0u;
+    const uint32_t f = (i < n && flags[i] == want) ? 1u : 0u;
     sh[threadIdx.x] = f;
     __syncthreads();
     for (uint32_t o = 1; o < 1024u; o <<= 1) {
@@ -410,12 +418,12 @@ __global__ __launch_bounds__(1024) void scan_sums_kernel(uint32_t* __restrict__ 
 // Per-child variant: flags has 8 entries per parent (child c of parent i at 8i + c); the
 // selected children are written in parent order.
 __global__ __launch_bounds__(256) void oct_children8_kernel(const uint32_t* __restrict__ blocks,
-                                                            const uint32_t* __restrict__ flags,
+                                                            const uint32_t* __restrict__ flags, uint32_t want,
                                                             const uint32_t* __restrict__ offs,
                                                             const uint32_t* __restrict__ sums, uint32_t m,
                                                             uint32_t* __restrict__ children) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= m || !flags[i]) return;
+    if (i >= m || flags[i] != want) return;
     const uint32_t o = sums[i / 1024u] + offs[i];
     uint32_t x, y, z;
     unpack_block(blocks[i / 8u], x, y, z);
@@ -424,15 +432,36 @@ __global__ __launch_bounds__(256) void oct_children8_kernel(const uint32_t* __re
     children[o] = cx | (cy << 10) | (cz << 20);
 }
 
-hipError_t launch_oct_scan_children(const uint32_t* flags, uint32_t m, uint32_t* offs, uint32_t* sums,
+hipError_t launch_oct_scan_children(const uint32_t* flags, uint32_t m, uint32_t want, uint32_t* offs, uint32_t* sums,
                                     uint32_t* total, const uint32_t* blocks, uint32_t* children,
                                     hipStream_t s) {
     if (m == 0) return hipMemsetAsync(total, 0, sizeof(uint32_t), s);
     const uint32_t chunks = (m + 1023u) / 1024u;
-    hipLaunchKernelGGL(scan_chunk_kernel, dim3(chunks), dim3(1024), 0, s, flags, m, offs, sums);
+    hipLaunchKernelGGL(scan_chunk_kernel, dim3(chunks), dim3(1024), 0, s, flags, m, want, offs, sums);
     hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(1024), 0, s, sums, chunks, total);
-    hipLaunchKernelGGL(oct_children8_kernel, dim3((m + 255u) / 256u), dim3(256), 0, s, blocks, flags, offs, sums,
-                       m, children);
+    hipLaunchKernelGGL(oct_children8_kernel, dim3((m + 255u) / 256u), dim3(256), 0, s, blocks, flags, want, offs,
+                       sums, m, children);
+    return hipGetLastError();
+}
+
+// Every fine voxel of n blocks of edge B (packed coordinates in block units), written from
+// out[0] in block order: the full-resolution fallback of low-confidence children.
+__global__ __launch_bounds__(256) void oct_expand_kernel(const uint32_t* __restrict__ blocks, uint32_t n,
+                                                         uint32_t B, uint32_t* __restrict__ out) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t cube = B * B * B;
+    if (i >= (uint64_t)n * cube) return;
+    const uint32_t b = (uint32_t)(i / cube), o = (uint32_t)(i % cube);
+    uint32_t x, y, z;
+    unpack_block(blocks[b], x, y, z);
+    const uint32_t ox = o / (B * B), oy = (o / B) % B, oz = o % B;
+    out[i] = (x * B + ox) | ((y * B + oy) << 10) | ((z * B + oz) << 20);
+}
+
+hipError_t launch_oct_expand(const uint32_t* blocks, uint32_t n, uint32_t B, uint32_t* out, hipStream_t s) {
+    const uint64_t m = (uint64_t)n * B * B * B;
+    if (m == 0) return hipSuccess;
+    hipLaunchKernelGGL(oct_expand_kernel, dim3((uint32_t)((m + 255u) / 256u)), dim3(256), 0, s, blocks, n, B, out);
     return hipGetLastError();
 }
 

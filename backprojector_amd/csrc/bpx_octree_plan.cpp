@@ -46,15 +46,56 @@ void octree_set_options(OctreeState& st, const bpx_octree_options& o) {
     if (o.flags & ~BPX_OCTREE_BOUNDS_ONLY) fail(BPX_ERR_INVALID_ARGUMENT, "unknown octree flags");
     if (!(o.keep_bound_ratio >= 0.0f) || !std::isfinite(o.keep_bound_ratio))
         fail(BPX_ERR_INVALID_ARGUMENT, "keep_bound_ratio must be finite and >= 0");
+    for (uint32_t l = 0; l < 8; ++l) {
+        const float f = o.level_fallback_threshold[l];
+        if (f != 0.0f && !valid_p(f)) fail(BPX_ERR_INVALID_ARGUMENT, "fallback thresholds must be 0 or lie in (0, 1)");
+    }
+    if (o.coarse_pair_stride > 64) fail(BPX_ERR_INVALID_ARGUMENT, "coarse_pair_stride must be 0..64");
     st.tau = o.threshold;
     st.keep = o.keep_bound_ratio;
     std::memcpy(st.prob, prob, sizeof(prob));
+    std::memcpy(st.fallback, o.level_fallback_threshold, sizeof(st.fallback));
+    st.stride = std::max<uint32_t>(1u, o.coarse_pair_stride);
     st.bounds_only = (o.flags & BPX_OCTREE_BOUNDS_ONLY) != 0;
+}
+
+// The coarse levels' pair subset for stride s: for a square wall grid (P = side^2 pairs in
+// row-major grid order, as the pipeline builds confocal captures) the pairs at every s-th grid
+// row and column; otherwise every (s^2)-th pair.  Rebuilt when the stride changes.
+static void build_pair_subset(PlanImpl& pl, uint32_t s) {
+    OctreeState& st = *pl.oct;
+    if (st.sub_stride == s) return;
+    const uint32_t P = pl.P;
+    std::vector<float> p4((size_t)P * 4);
+    if (P) hip_check(hipMemcpy(p4.data(), pl.pair4.ptr, sizeof(float) * p4.size(), hipMemcpyDeviceToHost),
+                     "hipMemcpy(pair table)");
+    std::vector<uint32_t> rows;
+    uint32_t side = (uint32_t)std::lround(std::sqrt((double)P));
+    if ((uint64_t)side * side == P && side > 0) {
+        for (uint32_t i = 0; i < side; i += s)
+            for (uint32_t j = 0; j < side; j += s) rows.push_back(i * side + j);
+    } else {
+        for (uint64_t p = 0; p < P; p += (uint64_t)s * s) rows.push_back((uint32_t)p);
+    }
+    std::vector<float> sub(rows.size() * 4);
+    for (size_t i = 0; i < rows.size(); ++i) std::memcpy(&sub[4 * i], &p4[4 * (size_t)rows[i]], 16);
+    st.sub_pairs.release();
+    st.sub_rows.release();
+    if (!rows.empty()) {
+        st.sub_pairs.alloc(sizeof(float) * sub.size());
+        st.sub_rows.alloc(sizeof(uint32_t) * rows.size());
+        hip_check(hipMemcpy(st.sub_pairs.ptr, sub.data(), sizeof(float) * sub.size(), hipMemcpyHostToDevice),
+                  "hipMemcpy(pair subset)");
+        hip_check(hipMemcpy(st.sub_rows.ptr, rows.data(), sizeof(uint32_t) * rows.size(), hipMemcpyHostToDevice),
+                  "hipMemcpy(pair subset rows)");
+    }
+    st.P_sub = (uint32_t)rows.size();
+    st.sub_stride = s;
 }
 
 PlanImpl* make_plan_octree(const CaptureIn& c, const uint32_t n[3], const float zero[3],
                            const float inc[9], const bpx_octree_options* oo) {
-    bpx_octree_options o{4u, 0.25f, 0.5f, 0u, {}, 0.0f};
+    bpx_octree_options o{4u, 0.25f, 0.5f, 0u, {}, 0.0f, {}, 1u};
     if (oo) o = *oo;
     if (o.levels == 0 || o.levels > 8) fail(BPX_ERR_INVALID_ARGUMENT, "octree levels must be 1..8");
     for (int k = 0; k < 3; ++k) {
@@ -96,10 +137,12 @@ PlanImpl* make_plan_octree(const CaptureIn& c, const uint32_t n[3], const float 
     }
     const uint64_t nvox = (uint64_t)n[0] * n[1] * n[2];
     st->fine_list.alloc(4 * nvox);
+    st->fb_list.alloc(4 * nvox);
+    st->fb_voxels.alloc(4 * nvox);
     st->flags.alloc(4 * nvox);
     st->offs.alloc(4 * nvox);
     st->sums.alloc(4 * ((nvox + 1023) / 1024 + 1));
-    st->total.alloc(sizeof(uint32_t));
+    st->total.alloc(2 * sizeof(uint32_t));   // refined children, fallback children
     st->stats.alloc(8 * sizeof(double));
     st->partial.alloc(4 * sizeof(double) * 1024);
     st->bound_part_u.alloc(4 * OctreeState::kBoundPartCapacity);
@@ -113,9 +156,9 @@ PlanImpl* make_plan_octree(const CaptureIn& c, const uint32_t n[3], const float 
 
 // Builds each level's window-max table from the raw rows by doubling widths (in LDS, one pass,
 // when a row fits; else through the ping/pong buffers).
-static void build_winmax(PlanImpl& pl, hipStream_t s) {
+static void build_winmax(PlanImpl& pl, uint32_t P, const uint32_t* rows, hipStream_t s) {
     OctreeState& st = *pl.oct;
-    const uint32_t P = pl.P, Tc = pl.T;
+    const uint32_t Tc = pl.T;
     if (!st.ping.ptr) {
         WinmaxLevels wl{};
         for (uint32_t l = 0; l < st.L; ++l) {
@@ -123,7 +166,7 @@ static void build_winmax(PlanImpl& pl, hipStream_t s) {
             wl.W[wl.count] = st.levels[l].W;
             ++wl.count;
         }
-        hip_check(launch_winmax_fused(static_cast<const float*>(pl.raw.ptr), Tc, P, Tc, st.ldm, wl, s),
+        hip_check(launch_winmax_fused(static_cast<const float*>(pl.raw.ptr), Tc, P, Tc, st.ldm, wl, rows, s),
                   "window-max kernel (LDS)");
         return;
     }
@@ -132,7 +175,7 @@ static void build_winmax(PlanImpl& pl, hipStream_t s) {
     float* cur = static_cast<float*>(st.ping.ptr);
     float* nxt = static_cast<float*>(st.pong.ptr);
     // width 1: max(0, H[b])
-    hip_check(launch_winmax_double(static_cast<const float*>(pl.raw.ptr), Tc, cur, st.ldm, P, Tc, 0u, s),
+    hip_check(launch_winmax_double(static_cast<const float*>(pl.raw.ptr), Tc, cur, st.ldm, P, Tc, 0u, rows, s),
               "window-max kernel");
     uint32_t w = 1;
     for (;;) {
@@ -142,7 +185,7 @@ static void build_winmax(PlanImpl& pl, hipStream_t s) {
                                                 lv.W - w, s),
                           "window-max combine kernel");
         if (2u * w > wmax) break;
-        hip_check(launch_winmax_double(cur, st.ldm, nxt, st.ldm, P, Tc, w, s), "window-max kernel");
+        hip_check(launch_winmax_double(cur, st.ldm, nxt, st.ldm, P, Tc, w, nullptr, s), "window-max kernel");
         std::swap(cur, nxt);
         w *= 2u;
     }
@@ -154,14 +197,23 @@ void execute_octree(PlanImpl& pl, hipStream_t s) {
     hip_check(hipGetLastError(), "HIP error pending before the octree execution");
     st.level_active.assign(st.L + 1, 0);
     st.level_stats.assign(5 * st.L, 0.0);
+    st.fallback_voxels.assign(st.L, 0);
     for (auto& lv : st.levels) lv.count = 0;
     hip_check(launch_fill(static_cast<float*>(pl.vol.ptr), nvox, 0.0f, s), "fill kernel");
     if (pl.P == 0) return;
+    // the coarse levels' pairs: every pair, or the strided subset with a net attached or for
+    // training data (bounds only); the bound rule alone keeps every pair, so it stays sound
+    const bool subset = (st.net || st.bounds_only) && st.stride > 1;
+    if (subset) build_pair_subset(pl, st.stride);
+    const uint32_t Pc = subset ? st.P_sub : pl.P;
+    const float4* coarse_pairs = static_cast<const float4*>(subset ? st.sub_pairs.ptr : pl.pair4.ptr);
+    const uint32_t* coarse_rows = subset ? static_cast<const uint32_t*>(st.sub_rows.ptr) : nullptr;
+    st.coarse_P = Pc;
     hip_check(hipEventRecord(pl.ev[0], s), "hipEventRecord");
     void* E = pl.H.ptr;
     const uint32_t e_bytes = (uint32_t)pl.H.bytes;
     stage_rows(pl, s);
-    build_winmax(pl, s);
+    build_winmax(pl, Pc, coarse_rows, s);
     hip_check(hipEventRecord(pl.ev[1], s), "hipEventRecord");
     hip_check(hipMemsetAsync(st.stats.ptr, 0, 8 * sizeof(double), s), "hipMemsetAsync");
     const OctreeState::Level& l0 = st.levels[0];
@@ -171,6 +223,9 @@ void execute_octree(PlanImpl& pl, hipStream_t s) {
                              "x" + std::to_string(l0.n[2]) + ")";
     hip_check(launch_oct_all_blocks(l0.n[0], l0.n[1], l0.n[2], static_cast<uint32_t*>(l0.list.ptr), s),
               what.c_str());
+    // full-resolution fallback voxels gathered over the levels (disjoint from the refined ones)
+    uint32_t* fb_voxels = static_cast<uint32_t*>(st.fb_voxels.ptr);
+    uint64_t fb_total = 0;
     for (uint32_t l = 0; l < st.L; ++l) {
         OctreeState::Level& lv = st.levels[l];
         lv.count = n;
@@ -180,12 +235,13 @@ void execute_octree(PlanImpl& pl, hipStream_t s) {
         OctBoundArgs a{};
         a.M = static_cast<const float*>(lv.M.ptr);
         a.E = E;
-        a.pairs = static_cast<const float4*>(pl.pair4.ptr);
+        a.pairs = coarse_pairs;
+        a.erow = coarse_rows;
         a.blocks = cur;
         a.nblocks = n;
         a.U = static_cast<float*>(lv.U.ptr);
         a.V = static_cast<float*>(lv.V.ptr);
-        a.P = pl.P;
+        a.P = Pc;
         a.Tc = pl.T;
         a.ldm = st.ldm;
         a.lde = pl.ldh;
@@ -197,7 +253,7 @@ void execute_octree(PlanImpl& pl, hipStream_t s) {
         a.inv2 = pl.inv2;
         a.partU = static_cast<float*>(st.bound_part_u.ptr);
         a.partV = static_cast<float*>(st.bound_part_v.ptr);
-        hip_check(launch_oct_bound(a, oct_bound_segments(n, pl.P, OctreeState::kBoundPartCapacity), s),
+        hip_check(launch_oct_bound(a, oct_bound_segments(n, Pc, OctreeState::kBoundPartCapacity), s),
                   "octree bound kernel");
         hip_check(launch_oct_reduce(a.U, a.V, n, static_cast<double*>(st.partial.ptr), 1024u,
                                     static_cast<double*>(st.stats.ptr), (double)st.tau, s),
@@ -205,25 +261,57 @@ void execute_octree(PlanImpl& pl, hipStream_t s) {
         hip_check(hipMemcpyAsync(lv.stats.ptr, st.stats.ptr, 5 * sizeof(double), hipMemcpyDeviceToDevice, s),
                   "hipMemcpyAsync");
         // child flags: fixed rule (bound >= tau * running max), narrowed by learned proposals
+        // (1 = refine, 2 = full-resolution fallback, 0 = drop)
         if (!net_child_flags(pl, l, s))
             hip_check(launch_oct_child_flags(cur, a.U, n, static_cast<const double*>(st.stats.ptr), nullptr,
-                                             lv.n[1], lv.n[2], 0.0f, 0.0f, static_cast<uint32_t*>(st.flags.ptr), s),
+                                             lv.n[1], lv.n[2], 0.0f, 0.0f, 0.0f,
+                                             static_cast<uint32_t*>(st.flags.ptr), s),
                       "octree child flags");
         uint32_t* next = l + 1 < st.L ? static_cast<uint32_t*>(st.levels[l + 1].list.ptr)
                                       : static_cast<uint32_t*>(st.fine_list.ptr);
-        hip_check(launch_oct_scan_children(static_cast<const uint32_t*>(st.flags.ptr), 8u * n,
+        hip_check(launch_oct_scan_children(static_cast<const uint32_t*>(st.flags.ptr), 8u * n, 1u,
                                            static_cast<uint32_t*>(st.offs.ptr),
                                            static_cast<uint32_t*>(st.sums.ptr),
                                            static_cast<uint32_t*>(st.total.ptr), cur, next, s),
                   "octree child scan");
         hip_check(hipMemcpyAsync(st.host_total, st.total.ptr, sizeof(uint32_t), hipMemcpyDeviceToHost, s),
                   "hipMemcpyAsync");
+        const bool fallback = st.net && st.fallback[l] > 0.0f;
+        if (fallback) {
+            hip_check(launch_oct_scan_children(static_cast<const uint32_t*>(st.flags.ptr), 8u * n, 2u,
+                                               static_cast<uint32_t*>(st.offs.ptr),
+                                               static_cast<uint32_t*>(st.sums.ptr),
+                                               static_cast<uint32_t*>(st.total.ptr) + 1, cur,
+                                               static_cast<uint32_t*>(st.fb_list.ptr), s),
+                      "octree fallback scan");
+            hip_check(hipMemcpyAsync(st.host_total + 1, static_cast<uint32_t*>(st.total.ptr) + 1, sizeof(uint32_t),
+                                     hipMemcpyDeviceToHost, s),
+                      "hipMemcpyAsync");
+        }
         hip_check(hipMemcpyAsync(st.host_stats, st.stats.ptr, 5 * sizeof(double), hipMemcpyDeviceToHost, s),
                   "hipMemcpyAsync");
         hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
         for (int k = 0; k < 5; ++k) st.level_stats[5 * l + k] = st.host_stats[k];
         n = st.host_total[0];
+        if (fallback && st.host_total[1]) {
+            // every fine voxel of each low-confidence child (edge B / 2 fine voxels)
+            const uint32_t Bc = lv.B / 2u;
+            const uint64_t add = (uint64_t)st.host_total[1] * Bc * Bc * Bc;
+            if (fb_total + add > nvox) fail(BPX_ERR_DEVICE, "octree fallback list overflow");
+            hip_check(launch_oct_expand(static_cast<const uint32_t*>(st.fb_list.ptr), st.host_total[1], Bc,
+                                        fb_voxels + fb_total, s),
+                      "octree fallback expansion");
+            fb_total += add;
+            st.fallback_voxels[l] = (uint32_t)add;
+        }
     }
+    if (st.bounds_only) fb_total = 0;
+    if ((uint64_t)n + fb_total > nvox) fail(BPX_ERR_DEVICE, "octree fine list overflow");
+    if (fb_total)
+        hip_check(hipMemcpyAsync(static_cast<uint32_t*>(st.fine_list.ptr) + n, fb_voxels, 4 * fb_total,
+                                 hipMemcpyDeviceToDevice, s),
+                  "hipMemcpyAsync(fallback voxels)");
+    n += (uint32_t)fb_total;
     st.level_active[st.L] = st.bounds_only ? 0u : n;
     hip_check(hipEventRecord(pl.ev[2], s), "hipEventRecord");
     if (!st.bounds_only) {

@@ -34,11 +34,21 @@ TRAIN_SEED_BASE = 50000         # training scenes 50000 ... 50000 + n - 1
 VALIDATION_SEED_BASE = 55000    # captures that choose the proposal thresholds
 TEST_SEED_BASE = 60000          # held-out test captures 60000 ...
 
-# Proposal-threshold configurations tried on the validation captures: one probability for
-# every level, or one per level (coarse levels cautious: a veto there drops a large block).
-CANDIDATES = [0.05, 0.1, 0.2, 0.3, 0.5,
-              [0.05, 0.1, 0.2, 0.4], [0.05, 0.1, 0.3, 0.5], [0.1, 0.2, 0.3, 0.5],
-              [0.05, 0.1, 0.2, 0.6], [0.1, 0.2, 0.4, 0.6], [0.05, 0.2, 0.4, 0.7]]
+# Proposal configurations tried on the validation captures: a probability for every level or
+# one per level (coarse levels cautious: a veto there drops a large block), the coarse-level
+# pair stride (features from every 2nd wall point per axis: a quarter of the pairs), and the
+# band of low-confidence children computed at full resolution instead (level 2: children of
+# 2 x 2 x 2 fine voxels; at the last coarse level a child is one voxel, so a band there equals
+# a lower probability).
+CANDIDATES = [0.2, 0.3,
+              {"p": 0.2, "stride": 2}, {"p": 0.3, "stride": 2},
+              {"p": 0.15, "stride": 2, "fallback": [0, 0, 0.08, 0]},
+              {"p": 0.2, "stride": 2, "fallback": [0, 0, 0.1, 0]},
+              {"p": 0.25, "stride": 2, "fallback": [0, 0, 0.1, 0]},
+              {"p": 0.3, "stride": 2, "fallback": [0, 0, 0.1, 0]},
+              {"p": 0.3, "stride": 2, "fallback": [0, 0, 0.15, 0]},
+              {"p": [0.1, 0.15, 0.2, 0.3], "stride": 2, "fallback": [0, 0, 0.1, 0]},
+              {"p": [0.1, 0.2, 0.3, 0.25], "stride": 2, "fallback": [0, 0.1, 0.15, 0]}]
 
 
 def log(*a):
@@ -77,7 +87,7 @@ def training_capture(index, side, voxels, T=4096):
 
 def train(scenes_n=2000, steps_per_scene=3, side=64, voxels=256, levels=4, lr=2e-3,
           pos_weight=8.0, seed=0, log_every=50, init=None, tau=0.25, on_policy=0.5,
-          policy_threshold=0.1, label_fraction=0.2):
+          policy_threshold=0.1, label_fraction=0.2, coarse_pair_stride=1):
     """Trains a net on `scenes_n` distinct synthetic scenes (Adam, cosine learning rate).
 
     Each scene's octree runs its coarse levels with the bound rule (threshold tau), so the net
@@ -87,7 +97,8 @@ def train(scenes_n=2000, steps_per_scene=3, side=64, voxels=256, levels=4, lr=2e
     occupied at label_fraction x max, below the evaluation's OCCUPIED_FRACTION, so that voxels
     near the occupancy boundary lie inside the positive class (a safety margin: one missed
     occupied voxel costs far more error than one extra computed voxel costs time).
-    Returns (net, history)."""
+    coarse_pair_stride: the coarse levels' features from every s-th wall point per grid axis,
+    as at inference with that stride.  Returns (net, history)."""
     net = core.Net(seed, params=init)
     hist = []
     total = scenes_n * steps_per_scene
@@ -104,6 +115,9 @@ def train(scenes_n=2000, steps_per_scene=3, side=64, voxels=256, levels=4, lr=2e
             with core.Plan.octree(*args, levels=levels, threshold=tau, bounds_only=True,
                                   proposal_threshold=policy_threshold) as op, \
                     core.Plan(*args) as dp:
+                if coarse_pair_stride > 1:
+                    op.set_octree_options(tau, policy_threshold, bounds_only=True,
+                                          coarse_pair_stride=coarse_pair_stride)
                 if attach:
                     op.attach_net(net)
                 try:
@@ -131,12 +145,39 @@ def rel_l2_occupied(vol, dense):
     return float(np.linalg.norm((vol - dense)[occ]) / max(np.linalg.norm(dense[occ]), 1e-300))
 
 
+def surface_support(cap, voxels, dilate=1):
+    """Voxels within `dilate` voxels (per axis) of the hidden geometry's surface samples: an
+    occupancy that does not depend on the reconstruction or on any threshold."""
+    N = np.asarray((voxels,) * 3)
+    lo = np.asarray(cap.volume_position, np.float64) - 0.5 * np.asarray(cap.volume_size, np.float64)
+    step = np.asarray(cap.volume_size, np.float64) / (N - 1)
+    idx = np.rint((np.asarray(cap.hidden[:, :3], np.float64) - lo) / step).astype(np.int64)
+    idx = idx[np.all((idx >= 0) & (idx < N), axis=1)]
+    m = np.zeros(tuple(N), bool)
+    for dx in range(-dilate, dilate + 1):
+        for dy in range(-dilate, dilate + 1):
+            for dz in range(-dilate, dilate + 1):
+                j = idx + (dx, dy, dz)
+                j = j[np.all((j >= 0) & (j < N), axis=1)]
+                m[j[:, 0], j[:, 1], j[:, 2]] = True
+    return m
+
+
+def rel_l2_on(vol, dense, mask):
+    return float(np.linalg.norm((vol - dense)[mask]) / max(np.linalg.norm(dense[mask]), 1e-300))
+
+
 def apply_operating_point(plan, net, tau, cfg):
     """Attaches net and sets one proposal configuration on an octree plan: a probability, one
-    per level, or {"p": either, "keep": keep_bound_ratio}."""
+    per level, or {"p": either, "keep": keep_bound_ratio, "fallback": full-resolution fallback
+    probability (one or per level), "stride": coarse-level pair stride}."""
     plan.attach_net(net)
-    p, keep = (cfg["p"], cfg.get("keep", 0.0)) if isinstance(cfg, dict) else (cfg, 0.0)
-    plan.set_octree_options(tau, p, keep_bound_ratio=keep)
+    if isinstance(cfg, dict):
+        plan.set_octree_options(tau, cfg["p"], keep_bound_ratio=cfg.get("keep", 0.0),
+                                fallback_threshold=cfg.get("fallback", 0.0),
+                                coarse_pair_stride=cfg.get("stride", 1))
+    else:
+        plan.set_octree_options(tau, cfg)
 
 
 def compare(cap, voxels, net, configs, tau=0.25, levels=4, reps=3):
@@ -146,6 +187,7 @@ def compare(cap, voxels, net, configs, tau=0.25, levels=4, reps=3):
     with core.Plan(*args) as dp:
         td = dp.benchmark(1, reps)["per_exec_ms"]
         dense = dp.volume()
+    support = surface_support(cap, voxels) if cap.hidden is not None else None
     rows = []
     with core.Plan.octree(*args, levels=levels, threshold=tau) as op:
         for cfg in [None] + list(configs):
@@ -158,6 +200,7 @@ def compare(cap, voxels, net, configs, tau=0.25, levels=4, reps=3):
             st = op.octree_stats()
             rows.append({"proposal_threshold": cfg, "ms": t, "speedup": td / t,
                          "rel_l2_occupied": rel_l2_occupied(vol, dense),
+                         "rel_l2_surface_support": rel_l2_on(vol, dense, support) if support is not None else None,
                          "fine_voxels": st["active"][-1], "active": st["active"]})
     return {"dense_ms": td, "occupied_voxels": int((dense >= OCCUPIED_FRACTION * dense.max()).sum()),
             "rows": rows}
@@ -178,9 +221,13 @@ def evaluate(net, seeds, voxels=256, configs=CANDIDATES, side=128, tau=0.25, lab
     for j, cfg in enumerate([None] + list(configs)):
         sp = [r["rows"][j]["speedup"] for r in per]
         er = [r["rows"][j]["rel_l2_occupied"] for r in per]
+        es = [r["rows"][j]["rel_l2_surface_support"] for r in per if r["rows"][j]["rel_l2_surface_support"] is not None]
         summary.append({"proposal_threshold": cfg, "mean_speedup": float(np.mean(sp)),
                         "min_speedup": float(np.min(sp)), "mean_err": float(np.mean(er)),
-                        "max_err": float(np.max(er))})
+                        "max_err": float(np.max(er)),
+                        "mean_err_surface_support": float(np.mean(es)) if es else None,
+                        "max_err_surface_support": float(np.max(es)) if es else None,
+                        "per_capture_err": [float(e) for e in er], "per_capture_speedup": [float(x) for x in sp]})
     return summary, per
 
 

@@ -46,17 +46,25 @@ def parse():
                     help="fine volume side of the octree-vs-plain comparison (0 = skip)")
     ap.add_argument("--octree-tau", type=float, default=0.25)
     ap.add_argument("--octree-proposal", type=float, default=None,
-                    help="proposal threshold of the learned octree (default: the shipped choice)")
+                    help="proposal threshold of the learned octree (default: the shipped operating point)")
+    ap.add_argument("--octree-weights", default=None,
+                    help="proposal-net weights and operating point (path without .npy/.json; default: shipped)")
     ap.add_argument("--configs", type=int, default=1,
                     help="also time the exhaustive 65,536-pair, 16-bit and complex workloads (0 = skip)")
     ap.add_argument("--one-shot", type=int, default=3,
                     help="timed one-shot nlosbpy.backproject calls (0 = skip)")
     ap.add_argument("--batch", type=int, default=64,
                     help="uneven captures of the batched comparison (0 = skip)")
+    ap.add_argument("--counters", choices=["headline", "all", "off"], default="headline",
+                    help="rocprofv3 --pmc passes run by this bench before its timed run: over the headline "
+                         "gather only (default), over every workload of this run, or none")
+    ap.add_argument("--counters-out", default=None,
+                    help="write the per-kernel counters and rooflines of the passes to this JSON file")
+    ap.add_argument("--counter-timeout", type=int, default=150, help="seconds per counter pass")
     return ap.parse_args()
 
 
-def octree_comparison(bp, cap, side, tau, proposal=None, reps=3):
+def octree_comparison(bp, cap, side, tau, proposal=None, reps=3, ctx=None, weights=None):
     """Plain dense reconstruction vs octree refinement at side^3 on the same capture: the
     sound bound rule alone, and with the shipped learned proposal net attached."""
     from backprojector_amd import proposals
@@ -73,8 +81,9 @@ def octree_comparison(bp, cap, side, tau, proposal=None, reps=3):
         if not np.array_equal(plain_tex.volume(), vp):
             raise RuntimeError("plain kernels disagree")
     occ = vp >= occ_frac * vp.max()
+    support = proposals.surface_support(cap, side) if cap.hidden is not None else None
     upd_plain = len(args[1]) * side ** 3
-    net, meta = proposals.load()
+    net, meta = proposals.load(weights) if weights else proposals.load()
     if proposal is None:
         proposal = meta.get("proposal_threshold") or 0.2
     rows = {}
@@ -91,6 +100,8 @@ def octree_comparison(bp, cap, side, tau, proposal=None, reps=3):
                           "rel_l2_occupied": err, "fine_voxels_computed": st["active"][-1],
                           "active_per_level": st["active"],
                           "work_fraction": (st["fine_updates"] + st["bound_updates"]) / upd_plain,
+                          "fallback_voxels": st["fallback_voxels"], "coarse_pairs": st["coarse_pairs"],
+                          "rel_l2_surface_support": proposals.rel_l2_on(vo, vp, support) if support is not None else None,
                           "fp64_running_max": st["running_max"]}
             if p is not None:
                 rows[name]["proposal_threshold"] = p
@@ -99,12 +110,30 @@ def octree_comparison(bp, cap, side, tau, proposal=None, reps=3):
             "plain_texture_gather_ms": tt["per_exec_ms"],
             "occupied_voxels": int(occ.sum()),
             "occupied_definition": f"plain value >= {occ_frac} x max",
+            "surface_support_voxels": int(support.sum()) if support is not None else None,
+            "surface_support_definition": "voxels within 1 voxel (per axis) of a hidden surface sample; "
+                                          "independent of the reconstruction and of tau",
             "net": {"trained_scenes": meta.get("scenes"), "description": meta.get("description"),
-                    "test": meta.get("test")},
-            **rows}
+                    "operating_point": meta.get("proposal_threshold"),
+                    "held_out_test": meta.get("test")},
+            **rows, "kernel_rooflines": octree_rooflines(ctx)}
 
 
-def batch_comparison(bp, count, seed=1004, serial=True):
+def octree_rooflines(ctx):
+    """Counters of the octree's sparse passes (their largest dispatch; durations under the
+    counter passes)."""
+    if ctx is None or not ctx.get("counters"):
+        return None
+    out = {}
+    for name in ("oct_bound_kernel<1>", "oct_bound_kernel<2>", "oct_fine_kernel<1>", "oct_fine_kernel<2>",
+                 "oct_tile_kernel", "winmax_fused_kernel", "conv3_fwd_kernel"):
+        r = roofline_of(ctx["counters"], name, None, None, ctx["fingerprint"], pick="max")
+        if r is not None:
+            out[name] = r
+    return out
+
+
+def batch_comparison(bp, count, seed=1004, serial=True, ctx=None):
     """Uneven confocal captures: one persistent work-queue launch (autotuned, hipGraph replay)
     against one plan execution per capture."""
     from backprojector_amd import proposals, scenes
@@ -133,23 +162,33 @@ def batch_comparison(bp, count, seed=1004, serial=True):
                 ms += pl.benchmark(1, 2)["per_exec_ms"]
         out["one_launch_per_capture_ms"] = ms
         out["speedup_vs_one_launch_per_capture"] = ms / t["per_exec_ms"]
+    if ctx is not None and ctx.get("counters"):
+        out["kernel_rooflines"] = {
+            name: roofline_of(ctx["counters"], name, None, None, ctx["fingerprint"], pick="max")
+            for name in ("batch_lds_kernel<FusedF32>", "batch_confocal_kernel<", "batch_kernel<")}
     return out
 
 
-def _plan_line(pl, N, sample_bytes, workload, reps):
+def _plan_line(pl, N, sample_bytes, workload, reps, storage="f32", cplx=False, ctx=None):
     t = pl.benchmark(1, reps)
     info = pl.info()
     upd = info["pairs_used"] * N[0] * N[1] * N[2]
     g = t["gather_kernel_ms"]
-    return {"workload": workload, "kernel_path": info["path"], "kernel": info["kernel"],
+    line = {"workload": workload, "kernel_path": info["path"], "kernel": info["kernel"],
             "lane_voxels": info["lane_voxels"],
             "updates": upd, "exec_ms": t["per_exec_ms"], "gather_ms": g, "staging_ms": t["table_kernel_ms"],
             "updates_per_s": upd / (t["per_exec_ms"] * 1e-3), "gather_updates_per_s": upd / (g * 1e-3),
             "bytes_per_update": sample_bytes,
-            "gather_frac_hbm": upd * sample_bytes / (g * 1e-3) / 1e9 / HBM_PEAK_GBS}
+            "effective_gather_GBps": upd * sample_bytes / (g * 1e-3) / 1e9}
+    if ctx is not None:
+        prefix = device_kernel(info, storage, cplx)
+        if prefix.startswith("confocal_lds_kernel<FusedC64Part<0>"):   # two passes, one per component
+            prefix = [prefix, prefix.replace("Part<0>", "Part<1>")]
+        line["roofline"] = roofline_of(ctx["counters"], prefix, g, upd * sample_bytes, ctx["fingerprint"])
+    return line
 
 
-def config_measurements(bp, cap, reps=5):
+def config_measurements(bp, cap, reps=5, ctx=None):
     """The other workloads, one plan each (device-resident, timed with HIP events): the
     exhaustive 16x16 x 16x16-pair capture into 64^3 with exact bins (grid kernel, and the same
     pairs shuffled for the arbitrary-pair kernel); the headline capture at 128^3 with 16-bit
@@ -159,7 +198,8 @@ def config_measurements(bp, cap, reps=5):
     N2 = (64, 64, 64)
     c2 = scenes.render(scenes.config_scene(2), 16, 4096, confocal=False, laser_side=16, seed=1002)
     with bp.Plan.from_nlos(*c2.nlos_args(N2), wall_normal=(0, 0, 0), assume_row_major=True) as pl:
-        out["exhaustive_64cube"] = _plan_line(pl, N2, 4, "exhaustive 16x16 lasers x 16x16 SPADs -> 64^3, T=4096, f32", reps)
+        out["exhaustive_64cube"] = _plan_line(pl, N2, 4, "exhaustive 16x16 lasers x 16x16 SPADs -> 64^3, T=4096, f32",
+                                              reps, ctx=ctx)
     # the same pairs in a shuffled order: the list loses its laser x SPAD structure, so the
     # arbitrary-pair kernel (both legs exact per update) runs instead of the grid kernel
     pre2 = bp.nlosbpy.prepare(*c2.nlos_args(N2), (0, 0, 0), assume_row_major=True)
@@ -167,13 +207,15 @@ def config_measurements(bp, cap, reps=5):
     with bp.Plan(pre2["chunk"][perm], pre2["pairs"][perm], N2, pre2["zero"], pre2["inc"], pre2["chunked_t0"],
                  c2.deltaT) as pl:
         out["exhaustive_64cube_shuffled_pairs"] = _plan_line(
-            pl, N2, 4, "exhaustive 16x16 x 16x16 pairs in shuffled order (no grid structure) -> 64^3, f32", reps)
+            pl, N2, 4, "exhaustive 16x16 x 16x16 pairs in shuffled order (no grid structure) -> 64^3, f32", reps,
+            ctx=ctx)
     del c2, pre2
     N = (128, 128, 128)
     for st, b in (("f16", 2), ("bf16", 2)):
         with bp.Plan.from_nlos(*cap.nlos_args(N), wall_normal=(0, 0, 0), assume_row_major=True,
                                storage=st) as pl:
-            out[f"confocal_128cube_{st}"] = _plan_line(pl, N, b, f"confocal 128x128 -> 128^3, T=4096, {st} storage", reps)
+            out[f"confocal_128cube_{st}"] = _plan_line(pl, N, b, f"confocal 128x128 -> 128^3, T=4096, {st} storage",
+                                                       reps, storage=st, ctx=ctx)
     pre = bp.nlosbpy.prepare(*cap.nlos_args(N), (0, 0, 0), assume_row_major=True)
     # the reference CPU path's semantics (voxel-diagonal window, double distances) on the GPU:
     # one-shot call from host arrays, so the copy of the chunk is included
@@ -187,13 +229,21 @@ def config_measurements(bp, cap, reps=5):
     out["confocal_128cube_cpu_path_semantics"] = {"workload": "confocal 128x128 -> 128^3, T=4096, window mode (CPU path), "
                                              "one-shot from host arrays", "exec_ms": wms,
                                  "updates": upd, "updates_per_s": upd / (wms * 1e-3)}
+    if ctx is not None:   # kernel time: its duration under the counter passes
+        out["confocal_128cube_cpu_path_semantics"]["roofline"] = roofline_of(
+            ctx["counters"], "window_kernel", None, None, ctx["fingerprint"])
     cplx = bp.phasor_pulse(pre["chunk"], 0.004, cap.deltaT)
+    if ctx is not None:   # the complex FIR filter (one-shot call; kernel time under the counter passes)
+        out["phasor_fir_4mm"] = {"workload": "phasor FIR (4 mm) of the 128x128 x Tc chunk, complex64 out",
+                                 "roofline": roofline_of(ctx["counters"], "phasor_fir_kernel", None,
+                                                         pre["chunk"].nbytes * 3, ctx["fingerprint"])}
     for st, b in (("f32", 8), ("f16", 4), ("bf16", 4)):
         with bp.Plan(cplx, pre["pairs"], N, pre["zero"], pre["inc"], pre["chunked_t0"], cap.deltaT,
                      storage=st) as pl:
             name = {"f32": "complex64", "f16": "complex f16", "bf16": "complex bf16"}[st]
             out[f"phasor_128cube_{name.replace(' ', '_')}"] = _plan_line(
-                pl, N, b, f"confocal 128x128 -> 128^3, T=4096, phasor-filtered (4 mm), {name} storage", reps)
+                pl, N, b, f"confocal 128x128 -> 128^3, T=4096, phasor-filtered (4 mm), {name} storage", reps,
+                storage=st, cplx=True, ctx=ctx)
     return out
 
 
@@ -215,12 +265,128 @@ def host_cpu():
     return {"model": model, "logical_cpus": os.cpu_count()}
 
 
+COUNTERS_FILE = os.path.join(ROOT, "profiles", "counters.json")
+
+
+def source_fingerprint():
+    """sha256 of the kernel sources: committed counter summaries older than the kernels are
+    reported as stale."""
+    import glob
+    import hashlib
+    h = hashlib.sha256()
+    for f in sorted(glob.glob(os.path.join(ROOT, "backprojector_amd", "csrc", "*"))):
+        if f.endswith((".hip", ".hpp", ".cpp")):
+            h.update(os.path.basename(f).encode())
+            h.update(open(f, "rb").read())
+    return h.hexdigest()[:16]
+
+
+def counter_passes(args):
+    """Runs this bench under rocprofv3 --pmc (one process per pass, a short run of the same
+    workloads) and returns {kernel: [per-dispatch counters]}, or None."""
+    from backprojector_amd import roofline as rl
+    child = [sys.executable, os.path.abspath(__file__), "--counters", "off", "--steps", "2", "--warmup", "1",
+             "--cpu-seconds", "0", "--check-voxels", "16", "--one-shot", "0", "--T", str(args.T), "--side",
+             str(args.side), "--voxels", str(args.voxels), "--storage", args.storage, "--variant", str(args.variant)]
+    if args.counters == "headline":
+        child += ["--octree", "0", "--configs", "0", "--batch", "0"]
+    else:
+        child += ["--octree", str(args.octree), "--octree-tau", str(args.octree_tau), "--configs", str(args.configs),
+                  "--batch", str(args.batch)]
+        if args.octree_proposal is not None:
+            child += ["--octree-proposal", str(args.octree_proposal)]
+        if args.octree_weights:
+            child += ["--octree-weights", args.octree_weights]
+    env = dict(os.environ)
+    env["PYTHONPATH"] = ROOT + os.pathsep + env.get("PYTHONPATH", "")
+    t = time.perf_counter()
+    counts, errors = rl.collect(child, timeout_s=args.counter_timeout, log=log, env=env)
+    log(f"counter passes: {len(rl.PASSES)} runs in {time.perf_counter() - t:.0f} s, {len(counts)} kernels"
+        + (f", errors: {errors}" if errors else ""))
+    return {"kernels": counts, "errors": errors, "passes": rl.PASSES,
+            "command": "rocprofv3 --pmc <pass> --output-format csv -d <dir> -o run -- " + " ".join(
+                ["python3", "bench.py"] + child[2:]),
+            "fingerprint": source_fingerprint()} if counts else None
+
+
+def committed_counters():
+    try:
+        return json.load(open(COUNTERS_FILE))
+    except (OSError, ValueError):
+        return None
+
+
+def device_kernel(info, storage, cplx):
+    """Signature prefix (roofline.signature form) of the gather kernel a plan launches."""
+    import re
+    k = info["kernel"]
+    m = re.match(r"confocal_lds_kernel<KZ=(\d+), waves=(\d+)>", k)
+    if m:
+        fz = {("f32", False): "FusedF32", ("f16", False): "FusedH16<false>", ("bf16", False): "FusedH16<true>",
+              ("f32", True): "FusedC64Part<0>", ("f16", True): "FusedC16<false>",
+              ("bf16", True): "FusedC16<true>"}[(storage, cplx)]
+        return f"confocal_lds_kernel<{fz}, {m.group(1)}, {m.group(2)}>"
+    smp = {("f32", False): "SmpF32", ("f16", False): "SmpF16", ("bf16", False): "SmpBF16",
+           ("f32", True): "SmpC64", ("f16", True): "SmpCF16", ("bf16", True): "SmpCBF16"}[(storage, cplx)]
+    m = re.match(r"(grid_kernel|pairs_kernel)<K=(\d+)>", k)
+    if m:
+        return f"{m.group(1)}<{smp}, {m.group(2)}"
+    m = re.match(r"confocal_kernel<K=(\d+)>", k)
+    if m:
+        return "confocal_kernel<FusedSource<"
+    return k.split("<")[0]
+
+
+def roofline_of(counters, prefix, kernel_ms, algorithmic_bytes, fingerprint, pick="mean"):
+    """Roofline block of one kernel from a counter set (live or committed), or None.  prefix: a
+    signature prefix, or a list of them whose counters are summed (one launch made of several
+    kernels).  kernel_ms None: the kernels' duration under the counter passes."""
+    from backprojector_amd import roofline as rl
+    if not counters:
+        return None
+    prefixes = prefix if isinstance(prefix, list) else [prefix]
+    names, c = [], {}
+    for pf in prefixes:
+        name, ci = rl.find_signature(counters["kernels"], pf, pick)
+        if ci is None:
+            return None
+        names.append(name)
+        for k, v in ci.items():
+            c[k] = c.get(k, 0.0) + v
+    if "VALUBusy" in c:
+        c["VALUBusy"] /= len(prefixes)
+    if kernel_ms is None:
+        if "duration_ns" not in c:
+            return None
+        kernel_ms = c["duration_ns"] * 1e-6
+    r = rl.kernel_roofline(c, kernel_ms, algorithmic_bytes)
+    r["kernels"] = names
+    r["counters_per_dispatch"] = {k: v for k, v in c.items() if k != "duration_ns"}
+    if "duration_ns" in c:
+        r["kernel_ms_under_counters"] = c["duration_ns"] * 1e-6
+    r["counters_source"] = counters.get("source", "live rocprofv3 --pmc passes of this bench run")
+    r["dispatch"] = "largest" if pick == "max" else "mean"
+    r["stale"] = counters.get("fingerprint") != fingerprint
+    return r
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X: --gpus {args.gpus} is not supported (this library runs "
               "on a single GPU; sharding across GPUs is a design note, not code)", file=sys.stderr)
         sys.exit(2)
+    # hardware counters first, in child processes, before this process touches the GPU
+    fp = source_fingerprint()
+    live = counter_passes(args) if args.counters != "off" else None
+    committed = committed_counters()
+    if committed is not None:
+        committed["source"] = "profiles/counters.json (committed rocprofv3 --pmc passes)"
+    if args.counters_out and live is not None:
+        with open(args.counters_out, "w") as f:
+            json.dump(live, f)
+    # workloads this run profiled use the live counters, the others the committed ones
+    ctx = {"counters": live if args.counters == "all" and live else committed, "fingerprint": fp}
     import backprojector_amd as bp
     from backprojector_amd import scenes
 
@@ -331,30 +497,49 @@ def main():
 
     configs = None
     if args.configs:
-        configs = config_measurements(bp, cap)
-        log("configs: " + "; ".join(f"{k} {v.get('gather_ms', v['exec_ms']):.2f} ms "
-                                    f"{v.get('gather_updates_per_s', v['updates_per_s']):.3g}/s"
+        configs = config_measurements(bp, cap, ctx=ctx)
+        log("configs: " + "; ".join(f"{k} {v.get('gather_ms', v.get('exec_ms', 0)):.2f} ms "
+                                    f"{v.get('gather_updates_per_s', v.get('updates_per_s', 0)):.3g}/s"
                                     for k, v in configs.items()))
 
     plan.close()
     octree = None
     if args.octree > 0:
-        octree = octree_comparison(bp, cap, args.octree, args.octree_tau, args.octree_proposal)
+        octree = octree_comparison(bp, cap, args.octree, args.octree_tau, args.octree_proposal, ctx=ctx,
+                                   weights=args.octree_weights)
         log(f"octree: {octree}")
     batch = None
     if args.batch > 0:
-        batch = batch_comparison(bp, args.batch)
+        batch = batch_comparison(bp, args.batch, ctx=ctx)
         log(f"batch: {batch}")
 
-    # ---- HBM traffic per launch of the gather kernel, from the committed rocprofv3 --pmc
-    # summary (FETCH_SIZE / WRITE_SIZE scaled by the copy-kernel calibration)
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", "roofline_traffic.json")
-    key = f"{info['kernel']}|{args.storage}|{args.side}|{N[0]}|{args.T}"
-    if os.path.exists(tfile):
-        tab = json.load(open(tfile))
-        if key in tab:
-            traffic = tab[key]["hbm_bytes_per_launch"]
+    # ---- the gather kernel against its rooflines: counters from the rocprofv3 --pmc passes of this
+    # run (else the committed ones), kernel time from this run's HIP events
+    prefix = device_kernel(info, args.storage, False)
+    rl_block = roofline_of(live if live else committed, prefix, gather_ms, updates * sample_bytes, fp)
+    if rl_block is not None and "valu" in rl_block:
+        roof = {"bound": rl_block["bound"],
+                "achieved": rl_block[rl_block["bound"]]["achieved"], "peak": rl_block[rl_block["bound"]]["peak"],
+                "unit": rl_block[rl_block["bound"]]["unit"], "frac": rl_block["frac"],
+                "traffic": rl_block.get("hbm", {}).get("traffic_bytes"),
+                "valu_frac": rl_block["valu"]["frac"], "valu_busy": rl_block.get("valu_busy"),
+                "sqrt_frac": rl_block.get("sqrt", {}).get("frac"), "hbm_frac": rl_block.get("hbm", {}).get("frac"),
+                "lds_bank_conflict_frac": rl_block.get("lds_bank_conflict_frac"),
+                "valu_instructions_per_update": rl_block["valu"]["instructions_per_dispatch"] / updates,
+                "effective_gather_GBps": achieved, "algorithmic_bytes_per_update": sample_bytes,
+                "kernel": rl_block["kernels"][0], "kernel_ms": gather_ms,
+                "table_kernel_ms": timing["table_kernel_ms"],
+                "counters_source": rl_block["counters_source"], "stale": rl_block["stale"],
+                "counters_per_dispatch": rl_block["counters_per_dispatch"],
+                "note": "bound = the roofline with the largest fraction; VALU: SQ_INSTS_VALU x 64 lanes / kernel "
+                        "time against 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz; sqrt: SQ_INSTS_VALU_TRANS_F32 x 64 "
+                        "against a quarter of that; HBM: (FETCH_SIZE x 2.000 + WRITE_SIZE) KiB (copy-kernel "
+                        "calibration) against 8 TB/s.  effective_gather_GBps = 4 B per update / kernel time "
+                        "(each staged sample is read from LDS for every voxel of a tile, so it exceeds HBM)"}
+    else:
+        roof = {"bound": None, "achieved": None, "peak": None, "unit": None, "frac": None, "traffic": None,
+                "effective_gather_GBps": achieved, "kernel_ms": gather_ms,
+                "note": "no counters (rocprofv3 unavailable and no committed summary for this kernel)"}
 
     out = {
         "metric": METRIC,
@@ -374,16 +559,7 @@ def main():
                    "pairs": P, "voxels": N[0] * N[1] * N[2], "chunk_bins": info["Tc"],
                    "storage": args.storage, "kernel_path": info["path"],
                    "lane_voxels": info["lane_voxels"]},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "traffic_key": key,
-                     "kernel": info["kernel"], "kernel_ms": gather_ms,
-                     "note": "achieved = 4 B per update (algorithmic) / kernel time; the LDS-staged "
-                             "kernel fetches each staged sample once per tile and reads it from LDS for "
-                             "every voxel of the tile, so this rate can exceed the HBM peak; its real "
-                             "HBM bytes per launch are `traffic`",
-                     "table_kernel_ms": timing["table_kernel_ms"],
-                     "bytes_per_update": sample_bytes},
+        "roofline": roof,
         "cpu_baseline": cpu,
         "check": check,
         "one_shot": one_shot,

@@ -247,6 +247,17 @@ typedef struct bpx_octree_options {
                                  keep_bound_ratio x the running max is refined whatever the
                                  net says (the sound rule as the fallback where the bound is
                                  high); 0 = the net alone decides                           */
+    float level_fallback_threshold[8]; /* learned proposals, per coarse level: a child the
+                                 net does not refine but whose probability exceeds this is
+                                 computed at full resolution instead (every fine voxel of it,
+                                 exactly: the plain reconstruction as the low-confidence
+                                 fallback); 0 = no fallback band                            */
+    uint32_t coarse_pair_stride; /* learned proposals: the coarse levels' features (U, V) use
+                                 every s-th wall point along each grid axis (s x s fewer
+                                 pairs; 1 = all pairs); finest-level voxels always use every
+                                 pair.  Used while a net is attached and for bounds-only
+                                 executions (training data); the bound rule alone always uses
+                                 every pair, so that it stays sound                          */
 } bpx_octree_options;
 
 /* Run the coarse levels only and skip the exact finest level (training data for the learned
@@ -262,6 +273,9 @@ typedef struct bpx_octree_stats {
     double threshold[8];      /* threshold applied at each level                        */
     uint64_t fine_updates;    /* voxel-pair updates of the exact finest level           */
     uint64_t bound_updates;   /* block-pair bound evaluations of all coarse levels       */
+    uint32_t fallback_voxels; /* fine voxels computed by the full-resolution fallback
+                                 (included in active[levels])                             */
+    uint32_t coarse_pairs;    /* pairs the coarse levels used                            */
 } bpx_octree_stats;
 
 /* Same arguments as call_cuda_octree_backprojection (backproject_cuda.hpp:30-39). */
@@ -366,8 +380,10 @@ typedef struct bpx_batch_config {
     uint32_t lane_voxels;  /* voxels per lane: 2 or 4                                   */
     uint32_t groups;       /* persistent work-groups (0 = all that are resident at once) */
     uint32_t order;        /* queue: 0 = captures with most pairs first, 1 = given order */
-    uint32_t lds_captures; /* out (get/tune): captures on the LDS-staged queue (those whose
-                              bin windows fit its segment); ignored by set_config       */
+    uint32_t lds_captures; /* out (get/tune): captures run as LDS-staged tiles (those whose
+                              bin windows fit the LDS segment); ignored by set_config   */
+    uint32_t lds_depth;    /* LDS-staged tile depth: 8 (16 x 16 x 8) or 16 (16^3); 0 = 8;
+                              complex formats always use 8                              */
 } bpx_batch_config;
 typedef struct bpx_batch_timing {
     double per_exec_ms;    /* mean device time of one batch execution (HIP events)       */

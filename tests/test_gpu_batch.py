@@ -55,12 +55,14 @@ def test_batch_lds_staged_queue(gpu, oracle):
     with gpu.Batch(caps) as b:
         cfg = b.config()
         assert 3 <= cfg["lds_captures"] < len(caps), cfg
-        for rep in range(2):
-            b.execute()
-            b.synchronize()
-            for i, ref in enumerate(alone):
-                assert np.array_equal(b.volume(i), ref), (rep, i)
-        assert np.all(b.latencies() > 0)
+        for depth in (8, 16, 8):     # 16 x 16 x 8 and 16^3 LDS tiles, staging in the same queue
+            b.set_config(lane_voxels=2, groups=0, order=0, lds_depth=depth)
+            for rep in range(2):
+                b.execute()
+                b.synchronize()
+                for i, ref in enumerate(alone):
+                    assert np.array_equal(b.volume(i), ref), (depth, rep, i)
+            assert np.all(b.latencies() > 0)
     for i in (0, 2, 4):
         H, pr, Nv, zero, inc, t0, dT = caps[i]
         ref = oracle.a1_volume_f64acc(H, pr, Nv, zero, inc, t0, dT)
@@ -131,7 +133,9 @@ def test_batch_complex_captures_equal_single_captures(gpu, storage):
 
 @pytest.mark.parametrize("cfg", [dict(lane_voxels=4, groups=7, order=1),
                                  dict(lane_voxels=2, groups=1, order=0),
-                                 dict(lane_voxels=4, groups=0, order=0)])
+                                 dict(lane_voxels=4, groups=0, order=0),
+                                 dict(lane_voxels=2, groups=0, order=1, lds_depth=16),
+                                 dict(lane_voxels=4, groups=3, order=0, lds_depth=16)])
 def test_batch_configs_identical(gpu, cfg):
     caps = uneven(1)
     alone = [gpu.backprojection(*c) for c in caps]
@@ -139,6 +143,7 @@ def test_batch_configs_identical(gpu, cfg):
         b.set_config(**cfg)
         got = b.config()
         assert got["lane_voxels"] == cfg["lane_voxels"] and got["order"] == cfg["order"]
+        assert got["lds_depth"] == cfg.get("lds_depth", 8)
         b.execute()
         for i, ref in enumerate(alone):
             assert np.array_equal(b.volume(i), ref), (cfg, i)
