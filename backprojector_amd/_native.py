@@ -117,11 +117,13 @@ class CaptureDesc(ctypes.Structure):
 
 class BatchConfig(ctypes.Structure):
     _fields_ = [("lane_voxels", ctypes.c_uint32), ("groups", ctypes.c_uint32),
-                ("order", ctypes.c_uint32), ("lds_captures", ctypes.c_uint32), ("lds_depth", ctypes.c_uint32)]
+                ("order", ctypes.c_uint32), ("lds_captures", ctypes.c_uint32), ("lds_depth", ctypes.c_uint32),
+                ("launches", ctypes.c_uint32)]
 
     def to_dict(self):
         return dict(lane_voxels=int(self.lane_voxels), groups=int(self.groups),
-                    order=int(self.order), lds_captures=int(self.lds_captures), lds_depth=int(self.lds_depth))
+                    order=int(self.order), lds_captures=int(self.lds_captures), lds_depth=int(self.lds_depth),
+                    launches=int(self.launches))
 
 
 class BatchTiming(ctypes.Structure):

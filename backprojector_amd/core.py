@@ -415,8 +415,8 @@ class Batch:
         check(lib().bpx_batch_get_config(self._h, ctypes.byref(c)))
         return c.to_dict()
 
-    def set_config(self, lane_voxels=2, groups=0, order=0, lds_depth=8):
-        c = nat.BatchConfig(int(lane_voxels), int(groups), int(order), 0, int(lds_depth))
+    def set_config(self, lane_voxels=2, groups=0, order=0, lds_depth=8, launches=1):
+        c = nat.BatchConfig(int(lane_voxels), int(groups), int(order), 0, int(lds_depth), int(launches))
         check(lib().bpx_batch_set_config(self._h, ctypes.byref(c)))
 
     def execute(self, stream=None):

@@ -2,9 +2,8 @@
 // launch over a single work queue of staging items (fused rows and thresholds), LDS-staged
 // tiles (captures whose bin windows fit the LDS segment) and texture-path tiles (the rest),
 // after a prologue launch that builds every capture's path-length table and resets the queue.
-// A capture's gather items follow its staging items in the queue and wait for them, and the
-// staging of the next capture is queued ahead of the current capture's gathers, so staging
-// overlaps the gathers.  The sequence is captured in a hipGraph that bpx_batch_execute replays.
+// The queue holds every staging item first and then the gather items; a capture's gathers wait
+// for its staging, so the first captures' gathers overlap the last captures' staging.  The sequence is captured in a hipGraph that bpx_batch_execute replays.
 // A project-written autotuner times launch configurations (texture-path voxels per lane, LDS
 // tile depth, queue order, resident work-groups) on the batch itself and keeps the fastest.
 // No reference counterpart: the reference reconstructs one capture per call
@@ -27,14 +26,19 @@ struct BatchImpl {
     hipStream_t stream = nullptr;
     hipEvent_t ev[2] = {};
     hipEvent_t done = nullptr, ready = nullptr;   // as PlanImpl::done / ready
-    DeviceBuffer caps, flags, items, jobs, init, next, staged, remaining, error;
+    DeviceBuffer caps, flags, items, items_tex, jobs, rjobs, init, next, staged, remaining, error;
     DeviceBuffer stamps;   // uint64 [1 + captures]: batch start, then each capture's completion
     DeviceBuffer zjobs;    // StageJob per capture with pairs (path-length tables, prologue)
-    uint32_t n_zjobs = 0, z_blocks = 0;
-    uint32_t n_items = 0;
+    uint32_t n_zjobs = 0, z_blocks = 0, row_blocks = 0;
+    uint32_t n_items = 0, n_tex_items = 0;
+    uint32_t tex_groups = 0;
+    // two-launch form: the texture-path queue runs beside the LDS-staged one on a second stream
+    // (default priority), forked from and joined back into the captured sequence
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
     int format = kF32;     // storage of every capture (fused rows)
     bool cplx = false;     // complex captures and volumes
-    bpx_batch_config cfg{2u, 0u, 0u, 0u, 8u};
+    bpx_batch_config cfg{2u, 0u, 0u, 0u, 8u, 1u};
     uint32_t groups = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -46,6 +50,9 @@ struct BatchImpl {
             if (e) (void)hipEventDestroy(e);
         if (done) (void)hipEventDestroy(done);
         if (ready) (void)hipEventDestroy(ready);
+        if (fork) (void)hipEventDestroy(fork);
+        if (join) (void)hipEventDestroy(join);
+        if (side) (void)hipStreamDestroy(side);
         if (stream) (void)hipStreamDestroy(stream);
     }
     void build(const bpx_batch_config& c);
@@ -62,11 +69,14 @@ struct BatchImpl {
 
 void BatchImpl::build(const bpx_batch_config& c) {
     if (c.lane_voxels != 2u && c.lane_voxels != 4u) fail(BPX_ERR_INVALID_ARGUMENT, "lane_voxels must be 2 or 4");
-    if (c.order > 1u) fail(BPX_ERR_INVALID_ARGUMENT, "order must be 0 or 1");
+    if (c.order > 2u) fail(BPX_ERR_INVALID_ARGUMENT, "order must be 0, 1 or 2");
     const uint32_t kz_req = c.lds_depth ? c.lds_depth : 8u;
     if (kz_req != 8u && kz_req != 16u) fail(BPX_ERR_INVALID_ARGUMENT, "lds_depth must be 8 or 16");
+    const uint32_t launches = c.launches ? c.launches : 1u;
+    if (launches > 2u) fail(BPX_ERR_INVALID_ARGUMENT, "launches must be 1 or 2");
     const int K = (int)c.lane_voxels;
     const int kz = batch_lds_depth((int)kz_req, format);
+    const bool two = launches == 2u;
     const uint32_t n = (uint32_t)plans.size();
     // per-capture arguments, tiles and staging jobs for this configuration
     std::vector<ConfocalArgs> args(n);
@@ -93,17 +103,20 @@ void BatchImpl::build(const bpx_batch_config& c) {
         // captures whose bin windows fit the LDS segment run LDS-staged tiles, the others
         // texture-path tiles
         lds[i] = pl.P && batch_lds_applies(pl.g, pl.inv2, format, pl.sep, pl.flat, kz);
-        nt[i] = !pl.P ? 0u : lds[i] ? batch_lds_tiles(pl.g, kz) : a.t.count;
+        // (two launches: a texture-path tile is 4 wave items, one per quarter)
+        nt[i] = !pl.P ? 0u : lds[i] ? batch_lds_tiles(pl.g, kz) : (two ? 4u : 1u) * a.t.count;
         if (pl.P) {
             sj[i] = stage_job(pl);
-            ns[i] = (stage_row_blocks(sj[i]) + kBatchStageBlocks - 1u) / kBatchStageBlocks;
+            ns[i] = two ? 0u : (stage_row_blocks(sj[i]) + kBatchStageBlocks - 1u) / kBatchStageBlocks;
         }
     }
-    // capture order: 0 = most work first, 1 = given order; each capture's staging items are
-    // queued one capture ahead of its gather items
+    // capture order: 0 = most work first, 1 = given order, 2 = most work first with the
+    // texture-path tiles spread evenly among the LDS-staged ones (the two kinds load different
+    // units -- texture addressing against VALU and LDS -- so work-groups of both kinds on one CU
+    // overlap).  Every staging item comes first (capture order), then the gather items.
     std::vector<uint32_t> order(n);
     std::iota(order.begin(), order.end(), 0u);
-    if (c.order == 0)
+    if (c.order != 1)
         std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return updates[x] > updates[y]; });
     std::vector<uint32_t> it;
     auto push_stage = [&](uint32_t i) {
@@ -119,25 +132,61 @@ void BatchImpl::build(const bpx_batch_config& c) {
             it.push_back(t);
         }
     };
-    if (n) push_stage(order[0]);
-    for (uint32_t k = 0; k < n; ++k) {
-        if (k + 1 < n) push_stage(order[k + 1]);
-        push_gather(order[k]);
+    for (uint32_t k = 0; k < n; ++k) push_stage(order[k]);
+    if (c.order == 2) {
+        std::vector<uint32_t> a, b;   // LDS-staged and texture-path gather items, capture order
+        for (uint32_t k = 0; k < n; ++k) {
+            const uint32_t i = order[k];
+            std::vector<uint32_t>& q = lds[i] ? a : b;
+            const uint32_t type = lds[i] ? kBatchItemLds : kBatchItemTex;
+            for (uint32_t t = 0; t < nt[i]; ++t) {
+                q.push_back(i | (type << 28));
+                q.push_back(t);
+            }
+        }
+        // proportional merge: after each item of the longer list, the share of the other
+        const size_t na = a.size() / 2, nb = b.size() / 2;
+        size_t ia = 0, ib = 0;
+        while (ia < na || ib < nb) {
+            const bool take_a = ib >= nb || (ia < na && (ia + 1) * nb <= (ib + 1) * na);
+            std::vector<uint32_t>& q = take_a ? a : b;
+            size_t& k = take_a ? ia : ib;
+            it.push_back(q[2 * k]);
+            it.push_back(q[2 * k + 1]);
+            ++k;
+        }
+    } else {
+        for (uint32_t k = 0; k < n; ++k) push_gather(order[k]);
+    }
+    // two launches: the texture-path items go to their own (wave-granular) queue
+    std::vector<uint32_t> itx;
+    if (two) {
+        std::vector<uint32_t> keep;
+        for (size_t k = 0; k < it.size(); k += 2) {
+            std::vector<uint32_t>& q = (it[k] >> 28) == kBatchItemTex ? itx : keep;
+            q.push_back(it[k]);
+            q.push_back(it[k + 1]);
+        }
+        it.swap(keep);
     }
     n_items = (uint32_t)(it.size() / 2);
+    n_tex_items = (uint32_t)(itx.size() / 2);
     // path-length tables of every capture in the prologue: each job's blocks follow the previous one's
     std::vector<StageJob> zj;
-    uint64_t zb = 0;
+    uint64_t zb = 0, rb = 0;
     for (uint32_t i = 0; i < n; ++i) {
         if (!plans[i]->P) continue;
         StageJob j = sj[i];
         j.z_first = (uint32_t)zb;
+        j.s_first = (uint32_t)rb;
         zb += stage_z_blocks(j);
+        rb += stage_row_blocks(j);
         zj.push_back(j);
     }
-    if (zb >= (1ull << 31)) fail(BPX_ERR_UNSUPPORTED, "batch path-length tables too large");
+    if (zb >= (1ull << 31) || rb >= (1ull << 31)) fail(BPX_ERR_UNSUPPORTED, "batch staging too large");
     n_zjobs = (uint32_t)zj.size();
     z_blocks = (uint32_t)zb;
+    row_blocks = (uint32_t)rb;
     std::vector<uint32_t> in(2 * std::max<uint32_t>(n, 1), 0u);
     for (uint32_t i = 0; i < n; ++i) {
         in[2 * i] = ns[i];
@@ -149,6 +198,7 @@ void BatchImpl::build(const bpx_batch_config& c) {
     caps.alloc(sizeof(ConfocalArgs) * std::max<uint32_t>(n, 1));
     flags.alloc(4u * std::max<uint32_t>(n, 1));
     items.alloc(8u * std::max<uint32_t>(n_items, 1));
+    items_tex.alloc(8u * std::max<uint32_t>(n_tex_items, 1));
     if (n_zjobs)
         hip_check(hipMemcpy(zjobs.ptr, zj.data(), sizeof(StageJob) * n_zjobs, hipMemcpyHostToDevice), "hipMemcpy");
     hip_check(hipMemcpy(jobs.ptr, sj.data(), sizeof(StageJob) * sj.size(), hipMemcpyHostToDevice), "hipMemcpy");
@@ -158,13 +208,22 @@ void BatchImpl::build(const bpx_batch_config& c) {
         hip_check(hipMemcpy(flags.ptr, fl.data(), 4u * n, hipMemcpyHostToDevice), "hipMemcpy");
     }
     if (n_items) hip_check(hipMemcpy(items.ptr, it.data(), 8u * n_items, hipMemcpyHostToDevice), "hipMemcpy");
-    const int resident = batch_queue_resident_groups(K, kz, format);
+    if (n_tex_items)
+        hip_check(hipMemcpy(items_tex.ptr, itx.data(), 8u * n_tex_items, hipMemcpyHostToDevice), "hipMemcpy");
+    const int resident = batch_queue_resident_groups(K, kz, format, (int)launches);
     if (resident <= 0) fail(BPX_ERR_DEVICE, "cannot query the occupancy of the batch kernel");
     groups = c.groups ? std::min<uint32_t>(c.groups, (uint32_t)resident) : (uint32_t)resident;
     groups = std::max<uint32_t>(1u, std::min<uint32_t>(groups, std::max<uint32_t>(n_items, 1u)));
+    if (two && n_tex_items) {
+        const int rt = batch_tex_resident_groups(K, format);
+        if (rt <= 0) fail(BPX_ERR_DEVICE, "cannot query the occupancy of the batch texture-path kernel");
+        tex_groups = c.groups ? std::min<uint32_t>(c.groups, (uint32_t)rt) : (uint32_t)rt;
+        tex_groups = std::max<uint32_t>(1u, std::min<uint32_t>(tex_groups, (n_tex_items + 3u) / 4u));
+    }
     cfg = c;
     cfg.groups = groups;
     cfg.lds_depth = (uint32_t)kz;
+    cfg.launches = launches;
     cfg.lds_captures = (uint32_t)std::count(lds.begin(), lds.end(), (char)1);
     BatchQueueArgs q{};
     q.caps = static_cast<const ConfocalArgs*>(caps.ptr);
@@ -183,7 +242,12 @@ void BatchImpl::build(const bpx_batch_config& c) {
     if (graph) (void)hipGraphDestroy(graph);
     exec = nullptr;
     graph = nullptr;
+    BatchQueueArgs qt = q;   // the texture-path queue of the two-launch form
+    qt.items = static_cast<const uint2*>(items_tex.ptr);
+    qt.n_items = n_tex_items;
+    qt.next = static_cast<uint32_t*>(next.ptr) + 1;
     hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    bool side_joined = false;   // the side stream has joined the capture and not yet left it
     try {
         for (auto& pl : plans) {
             if (pl->P != 0) continue;
@@ -195,8 +259,32 @@ void BatchImpl::build(const bpx_batch_config& c) {
                                         static_cast<const uint32_t*>(init.ptr), n,
                                         static_cast<uint64_t*>(stamps.ptr), stream),
                   "batch prologue kernel");
-        if (n_items) hip_check(launch_batch_queue(q, K, kz, format, groups, stream), "batch queue kernel");
+        if (two) {
+            // every row staged first (no waits in either queue), then both queues side by side
+            hip_check(launch_stage(static_cast<const StageJob*>(zjobs.ptr), n_zjobs, 0u, row_blocks, stream),
+                      "batch row staging kernel");
+            const bool fork_side = n_items && n_tex_items;
+            if (fork_side) {
+                hip_check(hipEventRecord(fork, stream), "hipEventRecord");
+                hip_check(hipStreamWaitEvent(side, fork, 0), "hipStreamWaitEvent");
+                side_joined = true;
+            }
+            if (n_items) hip_check(launch_batch_queue(q, K, kz, format, 2, groups, stream), "batch LDS-staged queue");
+            if (n_tex_items)
+                hip_check(launch_batch_tex(qt, K, format, tex_groups, fork_side ? side : stream),
+                          "batch texture-path queue");
+            if (fork_side) {
+                hip_check(hipEventRecord(join, side), "hipEventRecord");
+                hip_check(hipStreamWaitEvent(stream, join, 0), "hipStreamWaitEvent");
+                side_joined = false;
+            }
+        } else if (n_items) {
+            hip_check(launch_batch_queue(q, K, kz, format, 1, groups, stream), "batch queue kernel");
+        }
     } catch (...) {
+        // a side stream left in the capture would make the capture fail as unjoined and stay in
+        // capture mode: join it back first, then end (and drop) the capture
+        if (side_joined && hipEventRecord(join, side) == hipSuccess) (void)hipStreamWaitEvent(stream, join, 0);
         hipGraph_t g = nullptr;
         (void)hipStreamEndCapture(stream, &g);
         if (g) (void)hipGraphDestroy(g);
@@ -260,6 +348,9 @@ bpx_status bpx_batch_create(const bpx_capture_desc* captures, uint32_t count, co
         else
             b->format = o.storage == BPX_STORAGE_F16 ? kF16 : o.storage == BPX_STORAGE_BF16 ? kBF16 : kF32;
         hip_check(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate");
+        hip_check(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking), "hipStreamCreate");
+        hip_check(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming), "hipEventCreate");
+        hip_check(hipEventCreateWithFlags(&b->join, hipEventDisableTiming), "hipEventCreate");
         for (auto& e : b->ev) hip_check(hipEventCreate(&e), "hipEventCreate");
         hip_check(hipEventCreateWithFlags(&b->done, hipEventDisableTiming), "hipEventCreate");
         hip_check(hipEventCreateWithFlags(&b->ready, hipEventDisableTiming), "hipEventCreate");
@@ -278,13 +369,13 @@ bpx_status bpx_batch_create(const bpx_capture_desc* captures, uint32_t count, co
             b->plans.push_back(std::move(pl));
         }
         const uint32_t n = std::max<uint32_t>(count, 1);
-        b->next.alloc(4);
+        b->next.alloc(8);   // queue heads: the (LDS-staged) queue, the texture-path queue
         b->error.alloc(4);
         hip_check(hipMemset(b->error.ptr, 0, 4), "hipMemset");
         b->staged.alloc(4u * n);
         b->remaining.alloc(4u * n);
         b->stamps.alloc(8u * (n + 1u));
-        b->build(bpx_batch_config{2u, 0u, 0u, 0u, 8u});
+        b->build(bpx_batch_config{2u, 0u, 0u, 0u, 8u, 1u});
         auto h = std::make_unique<bpx_batch_s>();
         h->impl = std::move(b);
         *out = h.release();
@@ -319,26 +410,26 @@ bpx_status bpx_batch_tune(bpx_batch batch, int32_t reps, bpx_batch_config* chose
         BatchImpl& b = *batch->impl;
         b.wait();
         reps = std::max(reps, 1);
-        bpx_batch_config best{2u, 0u, 0u, 0u, 8u};
+        bpx_batch_config best{2u, 0u, 0u, 0u, 8u, 1u};
         double best_ms = INFINITY;
-        const uint32_t depths[2] = {8u, 16u};
-        for (uint32_t k : {2u, 4u})
-            for (uint32_t kz : depths) {
-                if (kz == 16u && format_complex(b.format)) continue;   // complex formats: depth 8
-                const int resident = batch_queue_resident_groups((int)k, (int)kz, b.format);
-                for (uint32_t frac : {1u, 2u})
-                    for (uint32_t order : {0u, 1u}) {
-                        const bpx_batch_config c{k, resident > 0 ? std::max(1u, (uint32_t)resident / frac) : 0u,
-                                                 order, 0u, kz};
-                        b.build(c);
-                        (void)run_timed(b, 1);  // warm-up
-                        const double ms = run_timed(b, reps);
-                        if (ms < best_ms) {
-                            best_ms = ms;
-                            best = b.cfg;
+        for (uint32_t launches : {1u, 2u})
+            for (uint32_t k : {2u, 4u})
+                for (uint32_t kz : {8u, 16u}) {
+                    if (kz == 16u && format_complex(b.format)) continue;   // complex formats: depth 8
+                    const int resident = batch_queue_resident_groups((int)k, (int)kz, b.format, (int)launches);
+                    for (uint32_t frac : {1u, 2u})
+                        for (uint32_t order : {0u, 1u, 2u}) {
+                            const bpx_batch_config c{k, resident > 0 ? std::max(1u, (uint32_t)resident / frac) : 0u,
+                                                     order, 0u, kz, launches};
+                            b.build(c);
+                            (void)run_timed(b, 1);  // warm-up
+                            const double ms = run_timed(b, reps);
+                            if (ms < best_ms) {
+                                best_ms = ms;
+                                best = b.cfg;
+                            }
                         }
-                    }
-            }
+                }
         b.build(best);
         if (chosen) *chosen = b.cfg;
     });

@@ -54,7 +54,7 @@ struct ConfocalArgs {
 // Factorised exhaustive-grid kernel parameters (threshold table in path-length space).
 // Batched confocal captures: one persistent launch over a queue of staging, LDS-staged and
 // texture-path items (batch_queue_kernel).
-constexpr uint32_t kBatchStageBlocks = 16;   // 256-element fused-row blocks per staging item
+constexpr uint32_t kBatchStageBlocks = 256;  // 256-element fused-row blocks per staging item
 enum : uint32_t { kBatchItemStage = 0, kBatchItemLds = 1, kBatchItemTex = 2 };
 struct StageJob;
 struct BatchQueueArgs {
@@ -64,7 +64,7 @@ struct BatchQueueArgs {
     const uint2* items;         // x = capture | type << 28; y = tile, or first row block (stage)
     uint32_t n_items;
     uint32_t spin_limit;        // polls of a capture's staging counter before giving up
-    uint32_t* next;             // queue head (0 at launch)
+    uint32_t* next;             // queue head (0 at launch; the prologue also clears next[1])
     uint32_t* staged;           // per capture: staging items not finished
     uint32_t* remaining;        // per capture: gather items not finished
     uint64_t* done;             // per capture: wall clock when its last gather item finished
@@ -254,9 +254,14 @@ hipError_t launch_pairs_raw(const PairsArgs& a, hipStream_t stream);
 // format): resident work-groups, launch, the prologue (path-length tables, queue reset; init =
 // [staging items, gather items] per capture), and whether / in how many tiles a capture runs
 // LDS-staged.
-int batch_queue_resident_groups(int K, int kz, int format);
+// launches 1: the queue serves every kind; 2: LDS-staged tiles only, beside the wave-granular
+// texture-path queue (batch_tex_kernel, items (capture, tile * 4 + quarter), head next[1]).
+int batch_queue_resident_groups(int K, int kz, int format, int launches);
 int batch_lds_depth(int kz, int format);   // the depth the queue kernel uses for this format
-hipError_t launch_batch_queue(const BatchQueueArgs& b, int K, int kz, int format, uint32_t groups, hipStream_t s);
+hipError_t launch_batch_queue(const BatchQueueArgs& b, int K, int kz, int format, int launches, uint32_t groups,
+                              hipStream_t s);
+int batch_tex_resident_groups(int K, int format);
+hipError_t launch_batch_tex(const BatchQueueArgs& b, int K, int format, uint32_t groups, hipStream_t s);
 hipError_t launch_batch_prologue(const StageJob* jobs, uint32_t n_jobs, uint32_t z_blocks, const BatchQueueArgs& b,
                                  const uint32_t* init, uint32_t n, uint64_t* start, hipStream_t s);
 bool batch_lds_applies(const Grid& g, float inv2, int format, bool sep, bool flat, int kz);

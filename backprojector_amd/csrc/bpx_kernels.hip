@@ -664,7 +664,7 @@ struct is_component : std::false_type {};
 template <class T>
 struct is_component<T, std::void_t<decltype(T::kComponent)>> : std::true_type {};
 
-template <class Fz, int KZ, int NW, bool BOUNDED = false>
+template <class Fz, int KZ, int NW, bool BOUNDED = false, bool UNROLL = true>
 __device__ __forceinline__ void lds_tile(const ConfocalArgs& a, uint32_t tile) {
     using acc_t = typename Fz::acc_t;
     // EG: bytes of a row element in HBM; E: bytes staged per element (its leading part)
@@ -826,7 +826,7 @@ __device__ __forceinline__ void lds_tile(const ConfocalArgs& a, uint32_t tile) {
         // once into scalar registers and the pair loop unrolled.  Deeper tiles (16 voxels per
         // lane) and complex accumulators keep the loop: unrolled, they need more registers than
         // 4 waves per SIMD allow.
-        constexpr bool kUnrollStage = KZ == 8 && std::is_same<acc_t, float>::value;
+        constexpr bool kUnrollStage = UNROLL && KZ == 8 && std::is_same<acc_t, float>::value;
         if (kUnrollStage && p0 + NW <= a.P) {
             uint32_t start[NW];
             float4 prs[NW];
@@ -876,10 +876,12 @@ __global__ __launch_bounds__(NW * 64) void confocal_lds_kernel(ConfocalArgs a) {
 //   texture -- one 8 x 8 x 4K tile of any other capture (the 4 waves take its quarters)
 // Work-groups take items in queue order (one atomic by thread 0, broadcast through LDS between
 // two barriers, so every wave of a work-group sees the same item and they leave together).  A
-// capture's gather items wait until its staging items are done; the queue places a capture's
-// staging one capture ahead of its gathers, so the next capture's rows are built while the
-// current one is gathered.  Staging items are taken before the gathers that wait for them and
-// never wait themselves, so every wait ends; it is also bounded (spin_limit polls, then the
+// capture's gather items wait until its staging items are done; the queue holds every staging
+// item first (in capture order) and then the gathers in the same order, so the first captures'
+// gathers start while the last captures are still being staged, and a wait is rare.  Staging
+// items are large (64K row elements: one atomic per item on the shared queue head).  Staging
+// items are taken before the gathers that wait for them and never wait themselves, so every
+// wait ends; it is also bounded (spin_limit polls, then the
 // error word is set and the work-group goes on), and the item loop is bounded by the item
 // count, so the launch always drains.  Each tile runs the single-capture kernels' tile body:
 // every volume is bit-identical to its own plan's.  The work-group finishing a capture's last
@@ -897,7 +899,9 @@ __device__ __forceinline__ T wave_uniform_load(const T* p) {
     return out;
 }
 
-template <class Fz, class Src, int K, int KZ>
+// KINDS: the item kinds the instantiation serves (bit 0 staging, bit 1 LDS-staged, bit 2 texture
+// path); an LDS-only queue compiles without the other paths and needs fewer registers.
+template <class Fz, class Src, int K, int KZ, int KINDS>
 __global__ __launch_bounds__(256) void batch_queue_kernel(BatchQueueArgs b) {
     __shared__ uint32_t s_item;
     const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -911,7 +915,7 @@ __global__ __launch_bounds__(256) void batch_queue_kernel(BatchQueueArgs b) {
         const uint32_t c = __builtin_amdgcn_readfirstlane(item.x & 0x0fffffffu);
         const uint32_t type = __builtin_amdgcn_readfirstlane(item.x >> 28);
         const uint32_t y = __builtin_amdgcn_readfirstlane(item.y);
-        if (type == kBatchItemStage) {
+        if ((KINDS & 1) && type == kBatchItemStage) {
             const StageJob jb = wave_uniform_load(b.jobs + c);
             const uint32_t total = jb.rows * ((jb.lde + 255u) / 256u);
             const uint32_t end = min(y + kBatchStageBlocks, total);
@@ -921,7 +925,7 @@ __global__ __launch_bounds__(256) void batch_queue_kernel(BatchQueueArgs b) {
             if (threadIdx.x == 0) atomicSub(&b.staged[c], 1u);
             continue;
         }
-        if (threadIdx.x == 0) {
+        if ((KINDS & 1) && threadIdx.x == 0) {
             uint32_t polls = 0;
             while (__hip_atomic_load(&b.staged[c], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
                 if (++polls > b.spin_limit) {
@@ -934,9 +938,9 @@ __global__ __launch_bounds__(256) void batch_queue_kernel(BatchQueueArgs b) {
         __syncthreads();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");   // the staged rows, for every wave
         const ConfocalArgs a = wave_uniform_load(b.caps + c);
-        if (type == kBatchItemLds) {
-            lds_tile<Fz, KZ, 4, true>(a, y);
-        } else {
+        if ((KINDS & 2) && type == kBatchItemLds) {
+            lds_tile<Fz, KZ, 4, true, false>(a, y);   // rolled stage loop: fewer registers, more work-groups
+        } else if (KINDS & 4) {
             const uint32_t f = __builtin_amdgcn_readfirstlane(b.flags[c]);
             if (f == 3u) confocal_tile<Src, K, true, true>(a, y, w);
             else if (f == 1u) confocal_tile<Src, K, true, false>(a, y, w);
@@ -952,9 +956,40 @@ __global__ __launch_bounds__(256) void batch_queue_kernel(BatchQueueArgs b) {
     }
 }
 
+// Texture-path tiles served by WAVES, for a batch run as two concurrent persistent launches (the
+// LDS-staged tiles in batch_queue_kernel<.., KINDS = 2>, these beside it on a second stream, the
+// rows staged beforehand): every wave takes the next (capture, tile * 4 + quarter) item.  The queue
+// head and the completion counters are updated by every lane with an increment that is 1 on lane
+// 0 and 0 elsewhere, never under a lane-divergent branch (a branch on "lane 0" around the head
+// update lets the compiler split the wave into lane 0 and the rest, which then re-read a stale
+// head and never leave); there is no work-group barrier, and the item loop is bounded by the
+// item count.
+template <int K, class Src>
+__global__ __launch_bounds__(256) void batch_tex_kernel(BatchQueueArgs b) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t one = lane == 0u ? 1u : 0u;
+    for (uint32_t taken = 0; taken <= b.n_items; ++taken) {
+        // all lanes active: the first active lane is lane 0, whose old value is the item
+        const uint32_t it = __builtin_amdgcn_readfirstlane(atomicAdd(b.next, one));
+        if (it >= b.n_items) return;
+        const uint2 item = b.items[it];
+        const uint32_t c = __builtin_amdgcn_readfirstlane(item.x & 0x0fffffffu);
+        const uint32_t q = __builtin_amdgcn_readfirstlane(item.y);  // tile * 4 + quarter
+        const ConfocalArgs a = wave_uniform_load(b.caps + c);
+        const uint32_t f = __builtin_amdgcn_readfirstlane(b.flags[c]);
+        if (f == 3u) confocal_tile<Src, K, true, true>(a, q >> 2, q & 3u);
+        else if (f == 1u) confocal_tile<Src, K, true, false>(a, q >> 2, q & 3u);
+        else confocal_tile<Src, K, false, false>(a, q >> 2, q & 3u);
+        __threadfence();
+        const uint32_t left = atomicSub(&b.remaining[c], one);
+        const unsigned long long stamp = (one && left == 1u) ? (unsigned long long)wall_clock64() : 0ull;
+        atomicMax(reinterpret_cast<unsigned long long*>(&b.done[c]), stamp);
+    }
+}
+
 // Before the queue: every capture's path-length table (blocks < z_blocks, as stage_z_kernel)
-// and, in the last block, the queue reset: head and error 0, per-capture counters from their
-// initial values, completion stamps 0, the start stamp.
+// and, in the last block, the queue reset: both heads and the error word 0, per-capture
+// counters from their initial values, completion stamps 0, the start stamp.
 __global__ __launch_bounds__(256) void batch_prologue_kernel(const StageJob* __restrict__ jobs, uint32_t n_jobs,
                                                              uint32_t z_blocks, BatchQueueArgs b,
                                                              const uint32_t* __restrict__ init, uint32_t n,
@@ -971,33 +1006,51 @@ __global__ __launch_bounds__(256) void batch_prologue_kernel(const StageJob* __r
         b.done[i] = 0ull;
     }
     if (threadIdx.x == 0) {
-        *b.next = 0u;
+        b.next[0] = 0u;   // the queue head, and the texture-path head of a two-launch batch
+        b.next[1] = 0u;
         *b.error = 0u;
         *start = wall_clock64();
     }
 }
 
-template <class Fz, class Src, int K, int KZ>
+template <class Fz, class Src, int K, int KZ, int KINDS>
 static const void* batch_queue_fn() {
-    return reinterpret_cast<const void*>(batch_queue_kernel<Fz, Src, K, KZ>);
+    return reinterpret_cast<const void*>(batch_queue_kernel<Fz, Src, K, KZ, KINDS>);
 }
-template <int K, int KZ>
+template <int K, int KZ, int KINDS>
 static const void* batch_queue_of(int format) {
     switch (format) {
-        case kF32: return batch_queue_fn<FusedF32, FusedSource<FusedF32>, K, KZ>();
-        case kF16: return batch_queue_fn<FusedH16<false>, FusedSource<FusedH16<false>>, K, KZ>();
-        case kBF16: return batch_queue_fn<FusedH16<true>, FusedSource<FusedH16<true>>, K, KZ>();
-        case kC64: return batch_queue_fn<FusedC64, FusedSource<FusedC64>, K, KZ>();
-        case kCF16: return batch_queue_fn<FusedC16<false>, FusedSource<FusedC16<false>>, K, KZ>();
-        case kCBF16: return batch_queue_fn<FusedC16<true>, FusedSource<FusedC16<true>>, K, KZ>();
+        case kF32: return batch_queue_fn<FusedF32, FusedSource<FusedF32>, K, KZ, KINDS>();
+        case kF16: return batch_queue_fn<FusedH16<false>, FusedSource<FusedH16<false>>, K, KZ, KINDS>();
+        case kBF16: return batch_queue_fn<FusedH16<true>, FusedSource<FusedH16<true>>, K, KZ, KINDS>();
+        case kC64: return batch_queue_fn<FusedC64, FusedSource<FusedC64>, K, KZ, KINDS>();
+        case kCF16: return batch_queue_fn<FusedC16<false>, FusedSource<FusedC16<false>>, K, KZ, KINDS>();
+        case kCBF16: return batch_queue_fn<FusedC16<true>, FusedSource<FusedC16<true>>, K, KZ, KINDS>();
         default: return nullptr;
     }
 }
-static const void* batch_queue(int K, int kz, int format) {
+// launches 1: every kind in one kernel; 2: the LDS-staged tiles alone (K unused)
+static const void* batch_queue(int K, int kz, int format, int launches) {
     if (format_complex(format)) kz = 8;   // two accumulators per voxel: 8 voxels per lane
-    if (K == 2) return kz == 16 ? batch_queue_of<2, 16>(format) : batch_queue_of<2, 8>(format);
-    if (K == 4) return kz == 16 ? batch_queue_of<4, 16>(format) : batch_queue_of<4, 8>(format);
+    if (launches == 2) return kz == 16 ? batch_queue_of<2, 16, 2>(format) : batch_queue_of<2, 8, 2>(format);
+    if (K == 2) return kz == 16 ? batch_queue_of<2, 16, 7>(format) : batch_queue_of<2, 8, 7>(format);
+    if (K == 4) return kz == 16 ? batch_queue_of<4, 16, 7>(format) : batch_queue_of<4, 8, 7>(format);
     return nullptr;
+}
+template <int K>
+static const void* batch_tex_of(int format) {
+    switch (format) {
+        case kF32: return reinterpret_cast<const void*>(batch_tex_kernel<K, FusedSource<FusedF32>>);
+        case kF16: return reinterpret_cast<const void*>(batch_tex_kernel<K, FusedSource<FusedH16<false>>>);
+        case kBF16: return reinterpret_cast<const void*>(batch_tex_kernel<K, FusedSource<FusedH16<true>>>);
+        case kC64: return reinterpret_cast<const void*>(batch_tex_kernel<K, FusedSource<FusedC64>>);
+        case kCF16: return reinterpret_cast<const void*>(batch_tex_kernel<K, FusedSource<FusedC16<false>>>);
+        case kCBF16: return reinterpret_cast<const void*>(batch_tex_kernel<K, FusedSource<FusedC16<true>>>);
+        default: return nullptr;
+    }
+}
+static const void* batch_tex(int K, int format) {
+    return K == 4 ? batch_tex_of<4>(format) : K == 2 ? batch_tex_of<2>(format) : nullptr;
 }
 
 static int resident_of(const void* fn) {
@@ -1014,11 +1067,22 @@ static int resident_of(const void* fn) {
     return cus * per_cu;
 }
 
-int batch_queue_resident_groups(int K, int kz, int format) { return resident_of(batch_queue(K, kz, format)); }
+int batch_queue_resident_groups(int K, int kz, int format, int launches) {
+    return resident_of(batch_queue(K, kz, format, launches));
+}
+int batch_tex_resident_groups(int K, int format) { return resident_of(batch_tex(K, format)); }
+
+hipError_t launch_batch_tex(const BatchQueueArgs& b, int K, int format, uint32_t groups, hipStream_t s) {
+    const void* fn = batch_tex(K, format);
+    if (!fn || groups == 0u) return hipErrorInvalidValue;
+    void* args[] = {const_cast<BatchQueueArgs*>(&b)};
+    return hipLaunchKernel(fn, dim3(groups), dim3(256), args, 0, s);
+}
 int batch_lds_depth(int kz, int format) { return format_complex(format) ? 8 : kz; }
 
-hipError_t launch_batch_queue(const BatchQueueArgs& b, int K, int kz, int format, uint32_t groups, hipStream_t s) {
-    const void* fn = batch_queue(K, kz, format);
+hipError_t launch_batch_queue(const BatchQueueArgs& b, int K, int kz, int format, int launches, uint32_t groups,
+                              hipStream_t s) {
+    const void* fn = batch_queue(K, kz, format, launches);
     if (!fn || groups == 0u) return hipErrorInvalidValue;
     void* args[] = {const_cast<BatchQueueArgs*>(&b)};
     return hipLaunchKernel(fn, dim3(groups), dim3(256), args, 0, s);

@@ -55,6 +55,8 @@ def parse():
                     help="timed one-shot nlosbpy.backproject calls (0 = skip)")
     ap.add_argument("--batch", type=int, default=64,
                     help="uneven captures of the batched comparison (0 = skip)")
+    ap.add_argument("--batch-serial", type=int, default=1,
+                    help="also time one plan execution per capture of the batch (0 = skip)")
     ap.add_argument("--counters", choices=["headline", "all", "off"], default="headline",
                     help="rocprofv3 --pmc passes run by this bench before its timed run: over the headline "
                          "gather only (default), over every workload of this run, or none")
@@ -163,9 +165,14 @@ def batch_comparison(bp, count, seed=1004, serial=True, ctx=None):
         out["one_launch_per_capture_ms"] = ms
         out["speedup_vs_one_launch_per_capture"] = ms / t["per_exec_ms"]
     if ctx is not None and ctx.get("counters"):
+        # the persistent queue kernel of the chosen configuration, and the prologue
+        k, d = cfg["lane_voxels"], cfg["lds_depth"]
+        names = [f"batch_queue_kernel<FusedF32, FusedSource<FusedF32>, {k}, {d}, {7 if cfg['launches'] == 1 else 2}>",
+                 "batch_prologue_kernel"]
+        if cfg["launches"] == 2:
+            names += [f"batch_tex_kernel<{k}, FusedSource<FusedF32>>", "stage_rows_kernel"]
         out["kernel_rooflines"] = {
-            name: roofline_of(ctx["counters"], name, None, None, ctx["fingerprint"], pick="max")
-            for name in ("batch_lds_kernel<FusedF32>", "batch_confocal_kernel<", "batch_kernel<")}
+            name: roofline_of(ctx["counters"], name, None, None, ctx["fingerprint"], pick="max") for name in names}
     return out
 
 
@@ -292,7 +299,7 @@ def counter_passes(args):
         child += ["--octree", "0", "--configs", "0", "--batch", "0"]
     else:
         child += ["--octree", str(args.octree), "--octree-tau", str(args.octree_tau), "--configs", str(args.configs),
-                  "--batch", str(args.batch)]
+                  "--batch", str(args.batch), "--batch-serial", "0"]
         if args.octree_proposal is not None:
             child += ["--octree-proposal", str(args.octree_proposal)]
         if args.octree_weights:
@@ -510,7 +517,7 @@ def main():
         log(f"octree: {octree}")
     batch = None
     if args.batch > 0:
-        batch = batch_comparison(bp, args.batch, ctx=ctx)
+        batch = batch_comparison(bp, args.batch, serial=bool(args.batch_serial), ctx=ctx)
         log(f"batch: {batch}")
 
     # ---- the gather kernel against its rooflines: counters from the rocprofv3 --pmc passes of this
@@ -525,7 +532,7 @@ def main():
                 "valu_frac": rl_block["valu"]["frac"], "valu_busy": rl_block.get("valu_busy"),
                 "sqrt_frac": rl_block.get("sqrt", {}).get("frac"), "hbm_frac": rl_block.get("hbm", {}).get("frac"),
                 "lds_bank_conflict_frac": rl_block.get("lds_bank_conflict_frac"),
-                "valu_instructions_per_update": rl_block["valu"]["instructions_per_dispatch"] / updates,
+                "valu_instructions_per_update": rl_block["valu"]["instructions_per_dispatch"] * 64 / updates,
                 "effective_gather_GBps": achieved, "algorithmic_bytes_per_update": sample_bytes,
                 "kernel": rl_block["kernels"][0], "kernel_ms": gather_ms,
                 "table_kernel_ms": timing["table_kernel_ms"],

@@ -379,11 +379,16 @@ typedef struct bpx_capture_desc {
 typedef struct bpx_batch_config {
     uint32_t lane_voxels;  /* voxels per lane: 2 or 4                                   */
     uint32_t groups;       /* persistent work-groups (0 = all that are resident at once) */
-    uint32_t order;        /* queue: 0 = captures with most pairs first, 1 = given order */
+    uint32_t order;        /* queue: 0 = most work first, 1 = given order, 2 = most work
+                              first with texture-path tiles spread among LDS-staged ones */
     uint32_t lds_captures; /* out (get/tune): captures run as LDS-staged tiles (those whose
                               bin windows fit the LDS segment); ignored by set_config   */
     uint32_t lds_depth;    /* LDS-staged tile depth: 8 (16 x 16 x 8) or 16 (16^3); 0 = 8;
                               complex formats always use 8                              */
+    uint32_t launches;     /* 1 (or 0): one persistent launch serves the rows' staging, the
+                              LDS-staged and the texture-path tiles from one queue; 2: rows
+                              staged first, then the LDS-staged queue and a wave-granular
+                              texture-path queue as two concurrent persistent launches  */
 } bpx_batch_config;
 typedef struct bpx_batch_timing {
     double per_exec_ms;    /* mean device time of one batch execution (HIP events)       */

@@ -55,13 +55,15 @@ def test_batch_lds_staged_queue(gpu, oracle):
     with gpu.Batch(caps) as b:
         cfg = b.config()
         assert 3 <= cfg["lds_captures"] < len(caps), cfg
-        for depth in (8, 16, 8):     # 16 x 16 x 8 and 16^3 LDS tiles, staging in the same queue
-            b.set_config(lane_voxels=2, groups=0, order=0, lds_depth=depth)
+        # 16 x 16 x 8 and 16^3 LDS tiles, queue orders, one launch or two concurrent ones
+        for depth, order, launches in ((8, 0, 1), (16, 2, 1), (8, 2, 1), (8, 0, 2), (16, 1, 2)):
+            b.set_config(lane_voxels=2, groups=0, order=order, lds_depth=depth, launches=launches)
+            assert b.config()["launches"] == launches
             for rep in range(2):
                 b.execute()
                 b.synchronize()
                 for i, ref in enumerate(alone):
-                    assert np.array_equal(b.volume(i), ref), (depth, rep, i)
+                    assert np.array_equal(b.volume(i), ref), (depth, launches, rep, i)
             assert np.all(b.latencies() > 0)
     for i in (0, 2, 4):
         H, pr, Nv, zero, inc, t0, dT = caps[i]
@@ -76,12 +78,15 @@ def mixed(seed=7):
     return [capture(rng, *sp, coded=(i % 2 == 0)) for i, sp in enumerate(spec)]
 
 
-def test_batch_execute_on_caller_stream(gpu, hip_streams):
-    """Both persistent queues (the graph's fork/join branch) replayed on two caller streams."""
+@pytest.mark.parametrize("launches", [1, 2])
+def test_batch_execute_on_caller_stream(gpu, hip_streams, launches):
+    """The batch graph (with two launches: its fork/join branch of two concurrent queues)
+    replayed on two caller streams."""
     caps = mixed()
     alone = [gpu.backprojection(*c) for c in caps]
     s1, s2 = hip_streams
     with gpu.Batch(caps) as b:
+        b.set_config(lane_voxels=2, groups=0, order=0, launches=launches)
         cfg = b.config()
         assert 1 <= cfg["lds_captures"] <= len(caps) - 1, cfg
         for rep in range(2):
@@ -135,7 +140,11 @@ def test_batch_complex_captures_equal_single_captures(gpu, storage):
                                  dict(lane_voxels=2, groups=1, order=0),
                                  dict(lane_voxels=4, groups=0, order=0),
                                  dict(lane_voxels=2, groups=0, order=1, lds_depth=16),
-                                 dict(lane_voxels=4, groups=3, order=0, lds_depth=16)])
+                                 dict(lane_voxels=4, groups=3, order=0, lds_depth=16),
+                                 dict(lane_voxels=2, groups=0, order=2),
+                                 dict(lane_voxels=4, groups=5, order=2, lds_depth=16),
+                                 dict(lane_voxels=2, groups=0, order=0, launches=2),
+                                 dict(lane_voxels=4, groups=3, order=1, lds_depth=16, launches=2)])
 def test_batch_configs_identical(gpu, cfg):
     caps = uneven(1)
     alone = [gpu.backprojection(*c) for c in caps]
@@ -143,7 +152,7 @@ def test_batch_configs_identical(gpu, cfg):
         b.set_config(**cfg)
         got = b.config()
         assert got["lane_voxels"] == cfg["lane_voxels"] and got["order"] == cfg["order"]
-        assert got["lds_depth"] == cfg.get("lds_depth", 8)
+        assert got["lds_depth"] == cfg.get("lds_depth", 8) and got["launches"] == cfg.get("launches", 1)
         b.execute()
         for i, ref in enumerate(alone):
             assert np.array_equal(b.volume(i), ref), (cfg, i)

@@ -86,3 +86,45 @@ def test_octree_plan_reuse_and_rejects(gpu):
     ex["camera_point"][:, 0] += 0.01
     with pytest.raises(gpu.BpxError):
         gpu.octree_backprojection(H, ex, N, zero, inc, 0.0, 0.001)
+
+
+def test_octree_pair_stride_and_full_resolution_fallback(gpu):
+    """Coarse-level pair subset and the low-confidence fallback, driven by a net whose weights are
+    all zero (every child's probability is exactly 0.5): computed voxels stay bit-identical to the
+    dense reconstruction whatever the options; the subset is used only with a net (the bound rule
+    alone keeps every pair); with the probability band (0.1, 0.9] every child of a level-0 block
+    that passes the bound takes the full-resolution fallback at once (all 512 of its fine voxels,
+    exactly), so with every pair the occupied voxels are all computed, as by the sound rule."""
+    H, pr, N, zero, inc = confocal_setup(side=16, N=64, T=4096, seed=5)
+    dense = gpu.backprojection(H, pr, N, zero, inc, 0.0, 0.001)
+    must = dense >= 0.25 * dense.max()
+    net = gpu.Net(seed=0, params=np.zeros(gpu.Net.param_count(), np.float32))
+    with gpu.Plan.octree(H, pr, N, zero, inc, 0.0, 0.001, levels=4, threshold=0.25) as op:
+        op.set_octree_options(0.25, 0.5, coarse_pair_stride=2)      # no net: every pair
+        op.execute()
+        st, vol = op.octree_stats(), op.volume()
+        assert st["coarse_pairs"] == len(pr) and st["fallback_voxels"] == 0
+        assert np.all((vol != 0)[must]) and np.array_equal(vol[vol != 0], dense[vol != 0])
+        bound_children = st["active"][1]      # children of level-0 blocks that pass the bound
+        op.attach_net(net)
+        # every child takes the fallback at level 0
+        op.set_octree_options(0.25, 0.9, fallback_threshold=0.1, coarse_pair_stride=1)
+        op.execute()
+        st, vol = op.octree_stats(), op.volume()
+        computed = vol != 0
+        assert st["active"][1:4] == [0, 0, 0]
+        assert st["fallback_voxels"] == bound_children * 8 ** 3 == st["active"][4]
+        assert np.all(computed[must]) and np.array_equal(vol[computed], dense[computed])
+        # every child refined (0.5 > 0.4), features from a quarter of the pairs
+        op.set_octree_options(0.25, 0.4, coarse_pair_stride=2)
+        op.execute()
+        st, vol = op.octree_stats(), op.volume()
+        computed = vol != 0
+        assert st["coarse_pairs"] == 64 and st["fallback_voxels"] == 0 and st["active"][4] > 0
+        assert np.array_equal(vol[computed], dense[computed])
+        # every child vetoed (0.5 <= 0.6), no fallback band: nothing below level 0
+        op.set_octree_options(0.25, 0.6, coarse_pair_stride=4)
+        op.execute()
+        st, vol = op.octree_stats(), op.volume()
+        assert st["coarse_pairs"] == 16 and st["active"][1:] == [0, 0, 0, 0] and not vol.any()
+    net.close()
