@@ -166,10 +166,25 @@ def largest_dispatch(dispatches):
     return dict(max(dispatches, key=lambda d: d.get("SQ_INSTS_VALU", 0.0))) if dispatches else {}
 
 
+def summarize(counts: dict, min_ns: float = 0.0) -> dict:
+    """{kernel: {"mean": ..., "largest": ..., "dispatches": n}} of a collect() result (kernels
+    whose largest dispatch lasts min_ns or more): the compact form a committed summary keeps."""
+    out = {}
+    for k, ds in counts.items():
+        big = largest_dispatch(ds)
+        if big.get("duration_ns", 0.0) >= min_ns:
+            out[k] = {"mean": mean_dispatch(ds), "largest": big, "dispatches": len(ds)}
+    return out
+
+
 def find_signature(counts: dict, prefix: str, pick: str = "mean"):
     """(full name, counters per dispatch) of the kernel whose signature starts with `prefix`:
-    the mean over its dispatches, or its largest dispatch (pick="max")."""
-    agg = largest_dispatch if pick == "max" else mean_dispatch
+    the mean over its dispatches, or its largest dispatch (pick="max").  `counts` is a collect()
+    result or its summarize() form."""
+    def agg(v):
+        if isinstance(v, dict):
+            return dict(v["largest" if pick == "max" else "mean"])
+        return largest_dispatch(v) if pick == "max" else mean_dispatch(v)
     hits = {k: agg(v) for k, v in counts.items() if signature(k).startswith(prefix)}
     if not hits:
         return None, None

@@ -167,8 +167,11 @@ def batch_comparison(bp, count, seed=1004, serial=True, ctx=None):
     if ctx is not None and ctx.get("counters"):
         # the persistent queue kernel of the chosen configuration, and the prologue
         k, d = cfg["lane_voxels"], cfg["lds_depth"]
-        names = [f"batch_queue_kernel<FusedF32, FusedSource<FusedF32>, {k}, {d}, {7 if cfg['launches'] == 1 else 2}>",
-                 "batch_prologue_kernel"]
+        if cfg["launches"] == 1:
+            queue = f"batch_queue_kernel<FusedF32, FusedSource<FusedF32>, {k}, {d}, 7>"
+        else:   # the LDS-staged tiles' queue (its voxels per lane are the LDS tile's)
+            queue = f"batch_queue_kernel<FusedF32, FusedSource<FusedF32>, 2, {d}, 2>"
+        names = [queue, "batch_prologue_kernel"]
         if cfg["launches"] == 2:
             names += [f"batch_tex_kernel<{k}, FusedSource<FusedF32>>", "stage_rows_kernel"]
         out["kernel_rooflines"] = {

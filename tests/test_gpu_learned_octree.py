@@ -9,7 +9,9 @@ from backprojector_amd import proposals, scenes
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("seed", [1000, 60009])   # the benchmark capture, a held-out test capture
+# the benchmark capture and held-out test captures: 60004 is the shipped operating point's worst
+# (rel. L2 1.91 % over occupied voxels), 60016 the worst under the previous weights
+@pytest.mark.parametrize("seed", [1000, 60004, 60009, 60016])
 def test_learned_proposals_gain(gpu, seed):
     cap = scenes.metric_capture(seed=seed)
     args = proposals.plan_args(cap, 256)
