@@ -173,7 +173,7 @@ def batch_comparison(bp, count, seed=1004, serial=True, ctx=None):
             queue = f"batch_queue_kernel<FusedF32, FusedSource<FusedF32>, 2, {d}, 2>"
         names = [queue, "batch_prologue_kernel"]
         if cfg["launches"] == 2:
-            names += [f"batch_tex_kernel<{k}, FusedSource<FusedF32>>", "stage_rows_kernel"]
+            names += [f"batch_tex_kernel<{k}, FusedSource<FusedF32>", "stage_rows_kernel"]
         out["kernel_rooflines"] = {
             name: roofline_of(ctx["counters"], name, None, None, ctx["fingerprint"], pick="max") for name in names}
     return out
