@@ -127,7 +127,7 @@ def octree_rooflines(ctx):
     if ctx is None or not ctx.get("counters"):
         return None
     out = {}
-    for name in ("oct_bound_kernel<1>", "oct_bound_kernel<2>", "oct_fine_kernel<1>", "oct_fine_kernel<2>",
+    for name in ("oct_bound_kernel<1", "oct_bound_kernel<2", "oct_fine_kernel<1", "oct_fine_kernel<2",
                  "oct_tile_kernel", "winmax_fused_kernel", "conv3_fwd_kernel"):
         r = roofline_of(ctx["counters"], name, None, None, ctx["fingerprint"], pick="max")
         if r is not None:
