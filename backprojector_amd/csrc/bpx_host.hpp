@@ -82,6 +82,10 @@ struct OctreeState {
     std::vector<uint32_t> level_active;   // blocks evaluated per level (+ fine voxels last)
     std::vector<double> level_stats;      // 5 FP64 values per level (see oct_finish_kernel)
     NetImpl* net = nullptr;               // learned proposals (not owned), or nullptr
+    // second stream (default priority) for the fallback voxels' gather, which runs beside the
+    // finer levels' bounds; joined into the plan's stream before the execution ends
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
     ~OctreeState();
 };
 

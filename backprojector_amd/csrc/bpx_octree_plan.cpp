@@ -13,6 +13,10 @@ namespace bpx {
 static uint32_t round_up32(uint64_t x, uint32_t m) { return (uint32_t)((x + m - 1) / m * m); }
 
 OctreeState::~OctreeState() {
+    if (side) (void)hipStreamSynchronize(side);
+    if (fork) (void)hipEventDestroy(fork);
+    if (join) (void)hipEventDestroy(join);
+    if (side) (void)hipStreamDestroy(side);
     if (host_total) (void)hipHostFree(host_total);
     if (host_stats) (void)hipHostFree(host_stats);
 }
@@ -147,6 +151,9 @@ PlanImpl* make_plan_octree(const CaptureIn& c, const uint32_t n[3], const float 
     st->partial.alloc(4 * sizeof(double) * 1024);
     st->bound_part_u.alloc(4 * OctreeState::kBoundPartCapacity);
     st->bound_part_v.alloc(4 * OctreeState::kBoundPartCapacity);
+    hip_check(hipStreamCreateWithFlags(&st->side, hipStreamNonBlocking), "hipStreamCreate");
+    hip_check(hipEventCreateWithFlags(&st->fork, hipEventDisableTiming), "hipEventCreate");
+    hip_check(hipEventCreateWithFlags(&st->join, hipEventDisableTiming), "hipEventCreate");
     hip_check(hipHostMalloc(&st->host_total, sizeof(uint32_t) * 2, hipHostMallocDefault), "hipHostMalloc");
     hip_check(hipHostMalloc(&st->host_stats, sizeof(double) * 8, hipHostMallocDefault), "hipHostMalloc");
     pl->oct = std::move(st);
@@ -223,9 +230,23 @@ void execute_octree(PlanImpl& pl, hipStream_t s) {
                              "x" + std::to_string(l0.n[2]) + ")";
     hip_check(launch_oct_all_blocks(l0.n[0], l0.n[1], l0.n[2], static_cast<uint32_t*>(l0.list.ptr), s),
               what.c_str());
-    // full-resolution fallback voxels gathered over the levels (disjoint from the refined ones)
+    // full-resolution fallback voxels gathered over the levels (disjoint from the refined ones);
+    // each level's are gathered on the side stream as soon as they are known, beside the finer
+    // levels' bounds and the refined voxels' gather (different voxels of the same volume)
     uint32_t* fb_voxels = static_cast<uint32_t*>(st.fb_voxels.ptr);
     uint64_t fb_total = 0;
+    bool side_used = false;
+    OctFineArgs fine{};
+    fine.E = E;
+    fine.pairs = static_cast<const float4*>(pl.pair4.ptr);
+    fine.out = static_cast<float*>(pl.vol.ptr);
+    fine.P = pl.P;
+    fine.Tc = pl.T;
+    fine.lde = pl.ldh;
+    fine.e_bytes = e_bytes;
+    fine.kahan_block = pl.kahan_block;
+    fine.g = pl.g;
+    fine.inv2 = pl.inv2;
     for (uint32_t l = 0; l < st.L; ++l) {
         OctreeState::Level& lv = st.levels[l];
         lv.count = n;
@@ -301,34 +322,32 @@ void execute_octree(PlanImpl& pl, hipStream_t s) {
             hip_check(launch_oct_expand(static_cast<const uint32_t*>(st.fb_list.ptr), st.host_total[1], Bc,
                                         fb_voxels + fb_total, s),
                       "octree fallback expansion");
+            if (!st.bounds_only) {
+                hip_check(hipEventRecord(st.fork, s), "hipEventRecord");
+                hip_check(hipStreamWaitEvent(st.side, st.fork, 0), "hipStreamWaitEvent");
+                OctFineArgs f = fine;
+                f.voxels = fb_voxels + fb_total;
+                f.n = (uint32_t)add;
+                hip_check(launch_oct_fine(f, st.side), "octree fine kernel (fallback voxels)");
+                side_used = true;
+            }
             fb_total += add;
             st.fallback_voxels[l] = (uint32_t)add;
         }
     }
     if (st.bounds_only) fb_total = 0;
     if ((uint64_t)n + fb_total > nvox) fail(BPX_ERR_DEVICE, "octree fine list overflow");
-    if (fb_total)
-        hip_check(hipMemcpyAsync(static_cast<uint32_t*>(st.fine_list.ptr) + n, fb_voxels, 4 * fb_total,
-                                 hipMemcpyDeviceToDevice, s),
-                  "hipMemcpyAsync(fallback voxels)");
-    n += (uint32_t)fb_total;
-    st.level_active[st.L] = st.bounds_only ? 0u : n;
+    st.level_active[st.L] = st.bounds_only ? 0u : n + (uint32_t)fb_total;
     hip_check(hipEventRecord(pl.ev[2], s), "hipEventRecord");
     if (!st.bounds_only) {
-        OctFineArgs f{};
-        f.E = E;
-        f.pairs = static_cast<const float4*>(pl.pair4.ptr);
+        OctFineArgs f = fine;
         f.voxels = static_cast<const uint32_t*>(st.fine_list.ptr);
         f.n = n;
-        f.out = static_cast<float*>(pl.vol.ptr);
-        f.P = pl.P;
-        f.Tc = pl.T;
-        f.lde = pl.ldh;
-        f.e_bytes = e_bytes;
-        f.kahan_block = pl.kahan_block;
-        f.g = pl.g;
-        f.inv2 = pl.inv2;
         hip_check(launch_oct_fine(f, s), "octree fine kernel");
+    }
+    if (side_used) {
+        hip_check(hipEventRecord(st.join, st.side), "hipEventRecord");
+        hip_check(hipStreamWaitEvent(s, st.join, 0), "hipStreamWaitEvent");
     }
     hip_check(hipEventRecord(pl.ev[3], s), "hipEventRecord");
 }

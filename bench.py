@@ -174,8 +174,9 @@ def batch_comparison(bp, count, seed=1004, serial=True, ctx=None):
         names = [queue, "batch_prologue_kernel"]
         if cfg["launches"] == 2:
             names += [f"batch_tex_kernel<{k}, FusedSource<FusedF32>", "stage_rows_kernel"]
-        out["kernel_rooflines"] = {
-            name: roofline_of(ctx["counters"], name, None, None, ctx["fingerprint"], pick="max") for name in names}
+        roofs = {name: roofline_of(ctx["counters"], name, None, None, ctx["fingerprint"], pick="max")
+                 for name in names}
+        out["kernel_rooflines"] = {k: v for k, v in roofs.items() if v is not None}
     return out
 
 
