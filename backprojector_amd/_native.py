@@ -143,6 +143,7 @@ EXPORTED = [
     "bpx_last_error", "bpx_version", "bpx_abi_version", "bpx_device_count",
     "bpx_backprojection", "bpx_complex_backprojection", "bpx_backprojection_ex", "bpx_backprojection_noalloc",
     "bpx_window_backprojection", "bpx_phasor_pulse", "bpx_nlos_backproject", "bpx_nlos_phasor", "bpx_nlos_phasor_multi",
+    "bpx_nlos_phasor_output",
     "bpx_plan_create", "bpx_nlos_plan_create", "bpx_plan_execute", "bpx_plan_synchronize", "bpx_plan_read_volume",
     "bpx_plan_info", "bpx_plan_kernel_name", "bpx_plan_read_thresholds", "bpx_plan_benchmark", "bpx_plan_destroy", "bpx_render_transient",
     "bpx_copy_benchmark", "bpx_octree_backprojection", "bpx_octree_plan_create",
@@ -187,6 +188,9 @@ def lib():
     L.bpx_nlos_phasor_multi.argtypes = [c_p, c_p, ctypes.c_int, c_p, c_p, c_p, c_p, c_p, c_p, f32, f32,
                                         ctypes.c_int, c_p, c_p, c_p, c_p, u32, ctypes.c_int, ctypes.c_int,
                                         c_p]
+    L.bpx_nlos_phasor_output.argtypes = [c_p, c_p, ctypes.c_int, c_p, c_p, c_p, c_p, c_p, c_p, f32, f32,
+                                         ctypes.c_int, c_p, c_p, c_p, c_p, u32, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_int, c_p]
     L.bpx_plan_create.argtypes = [c_p, u64, u32, c_p, u32, c_p, c_p, c_p, f32, f32, ctypes.c_int,
                                   c_p, ctypes.POINTER(c_p)]
     L.bpx_nlos_plan_create.argtypes = [c_p, c_p, ctypes.c_int, c_p, c_p, c_p, c_p, c_p, c_p, f32,

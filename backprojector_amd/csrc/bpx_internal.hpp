@@ -268,6 +268,9 @@ bool batch_lds_applies(const Grid& g, float inv2, int format, bool sep, bool fla
 uint32_t batch_lds_tiles(const Grid& g, int kz);
 hipError_t launch_window(const WindowArgs& a, int complex_samples, hipStream_t stream);
 hipError_t launch_abs_max(const float* src, uint64_t n, float* result_dev, hipStream_t stream);
+// |v| and atan2(im, re) of n interleaved complex values (either output may be null).
+hipError_t launch_complex_post(const void* complex_volume, uint64_t n, float* mag, float* phase,
+                               hipStream_t stream);
 hipError_t launch_copy(void* dst, const void* src, uint64_t bytes, hipStream_t stream);
 
 // Voxels per lane (K) of each tuned variant; variant -1 picks the default.

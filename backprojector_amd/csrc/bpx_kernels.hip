@@ -457,6 +457,33 @@ hipError_t launch_abs_max(const float* src, uint64_t n, float* result_dev, hipSt
     return hipGetLastError();
 }
 
+// Magnitude and phase of a complex volume (interleaved re, im): |v| = sqrt(re^2 + im^2) with
+// the squares and the sum in double and one rounding to float (the double-precision hypot the
+// host's std::abs(std::complex<float>) evaluates; infinite if either part is), phase =
+// atan2(im, re) in double rounded to float.  Either output may be null.
+__global__ __launch_bounds__(256) void complex_post_kernel(const float2* __restrict__ v, uint64_t n,
+                                                           float* __restrict__ mag, float* __restrict__ phase) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const float2 c = v[i];
+        const double re = (double)c.x, im = (double)c.y;
+        if (mag) {
+            const bool inf = __builtin_isinf(c.x) || __builtin_isinf(c.y);
+            mag[i] = inf ? __builtin_inff() : (float)__builtin_sqrt(re * re + im * im);
+        }
+        if (phase) phase[i] = (float)atan2(im, re);
+    }
+}
+
+hipError_t launch_complex_post(const void* complex_volume, uint64_t n, float* mag, float* phase,
+                               hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    const uint64_t blocks = std::min<uint64_t>((n + 255u) / 256u, 1u << 16);
+    hipLaunchKernelGGL(complex_post_kernel, dim3((uint32_t)blocks), dim3(256), 0, stream,
+                       static_cast<const float2*>(complex_volume), n, mag, phase);
+    return hipGetLastError();
+}
+
 // =================================================================== gather kernels
 Tiles make_tiles(const uint32_t n[3], int K) {
     Tiles t;

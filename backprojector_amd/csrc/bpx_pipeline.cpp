@@ -575,9 +575,25 @@ bpx_status bpx_nlos_phasor_multi(const float* transient_data, const int64_t* dat
                                  const float volume_position[3], const float volume_size[3],
                                  const uint32_t voxels_per_side[3], const float* wavelengths, uint32_t count,
                                  int assume_row_major, int use_cpu, float* complex_volumes_out) {
+    return bpx_nlos_phasor_output(transient_data, data_shape, data_ndim, camera_grid_positions, camera_grid_shape,
+                                  laser_grid_positions, laser_grid_shape, camera_position, laser_position, t0,
+                                  deltaT, is_confocal, volume_position, volume_size, voxels_per_side, wavelengths,
+                                  count, assume_row_major, use_cpu, BPX_PHASOR_COMPLEX, complex_volumes_out);
+}
+
+bpx_status bpx_nlos_phasor_output(const float* transient_data, const int64_t* data_shape, int data_ndim,
+                                  const float* camera_grid_positions, const int64_t* camera_grid_shape,
+                                  const float* laser_grid_positions, const int64_t* laser_grid_shape,
+                                  const float camera_position[3], const float laser_position[3],
+                                  float t0, float deltaT, int is_confocal,
+                                  const float volume_position[3], const float volume_size[3],
+                                  const uint32_t voxels_per_side[3], const float* wavelengths, uint32_t count,
+                                  int assume_row_major, int use_cpu, int output, float* volumes_out) {
     clear_error();
     try {
-        if (!complex_volumes_out) fail(BPX_ERR_INVALID_ARGUMENT, "NULL output");
+        if (!volumes_out) fail(BPX_ERR_INVALID_ARGUMENT, "NULL output");
+        if (output < BPX_PHASOR_COMPLEX || output > BPX_PHASOR_MAGNITUDE_PHASE)
+            fail(BPX_ERR_INVALID_ARGUMENT, "output must be a bpx_phasor_output value");
         if (count == 0) return BPX_OK;
         if (!wavelengths) fail(BPX_ERR_INVALID_ARGUMENT, "NULL wavelengths");
         PipelineGeometry g = prepare(transient_data, data_shape, data_ndim, camera_grid_positions,
@@ -607,6 +623,10 @@ bpx_status bpx_nlos_phasor_multi(const float* transient_data, const int64_t* dat
         const float chunked_t0 = t0 + imin * deltaT;  // :1011
         const size_t nvox = (size_t)voxels_per_side[0] * voxels_per_side[1] * voxels_per_side[2];
         std::unique_ptr<PlanImpl> plan;
+        // floats per voxel and wavelength in the output, and the device buffer of the real outputs
+        const size_t per = output == BPX_PHASOR_COMPLEX || output == BPX_PHASOR_MAGNITUDE_PHASE ? 2 : 1;
+        DeviceBuffer post;
+        if (output != BPX_PHASOR_COMPLEX) post.alloc(4 * nvox * per);
         for (uint32_t i = 0; i < count; ++i) {
             DeviceBuffer rows = phasor_rows_device(chunk, g.rows, w.Tc, cs[i], sn[i]);
             if (!plan) {
@@ -626,8 +646,17 @@ bpx_status bpx_nlos_phasor_multi(const float* transient_data, const int64_t* dat
                 plan_replace_rows(*plan, rows);
             }
             plan->execute(plan->stream);
-            hip_check(hipMemcpyAsync(complex_volumes_out + 2 * nvox * i, plan->vol.ptr, nvox * 8,
-                                     hipMemcpyDeviceToHost, plan->stream),
+            const void* src = plan->vol.ptr;
+            if (output != BPX_PHASOR_COMPLEX) {
+                float* mag = static_cast<float*>(post.ptr);
+                float* ph = output == BPX_PHASOR_MAGNITUDE_PHASE ? mag + nvox : mag;
+                hip_check(launch_complex_post(plan->vol.ptr, nvox, output == BPX_PHASOR_PHASE ? nullptr : mag,
+                                              output == BPX_PHASOR_MAGNITUDE ? nullptr : ph, plan->stream),
+                          "complex post-processing kernel");
+                src = post.ptr;
+            }
+            hip_check(hipMemcpyAsync(volumes_out + per * nvox * i, src, 4 * per * nvox, hipMemcpyDeviceToHost,
+                                     plan->stream),
                       "hipMemcpyAsync(volume)");
             hip_check(hipStreamSynchronize(plan->stream), "hipStreamSynchronize");
         }

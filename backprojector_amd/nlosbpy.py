@@ -17,6 +17,11 @@ distances, backproject.hpp:138-216), still computed on the GPU.  `assume_row_maj
 means the arrays are column-major (Fortran-order) transposes of the row-major layout; they
 are transposed before the pipeline runs (the reference's column-major branch reads out of
 bounds, backproject.hpp:384-385, 405).
+
+`phasor` and `phasor_multi` also take `output=` (keyword, default "complex": the reference's
+result): "magnitude" (|v|, what the reference CLI writes, main.cpp:162-166), "phase"
+(atan2(im, re)) or "magnitude_phase" (a (magnitude, phase) pair), computed on the device so
+that only the real volumes are copied back.
 """
 from __future__ import annotations
 
@@ -106,38 +111,66 @@ def prepare(transient_data, camera_grid_positions, laser_grid_positions, camera_
         lib().bpx_nlos_prepared_free(C.byref(out))
 
 
-def phasor(transient_data, camera_grid_positions, laser_grid_positions, camera_position,
-           laser_position, t0, deltaT, is_confocal, volume_position, volume_size,
-           voxels_per_side, wavelength, assume_row_major=False, use_cpu=False):
-    """bp::phasor_reconstruction (backproject.hpp:932-1039) -> complex64 volume."""
-    H, cg, lg, cp, lp, vp, vs, N = _prep(transient_data, camera_grid_positions,
-                                         laser_grid_positions, camera_position, laser_position,
-                                         volume_position, volume_size, voxels_per_side)
-    out = np.zeros(tuple(int(x) for x in N), np.complex64)
-    dshape, cshape, lshape = _shape(H), _shape(cg), _shape(lg)
-    check(lib().bpx_nlos_phasor(ptr(H), ptr(dshape), H.ndim, ptr(cg), ptr(cshape), ptr(lg),
-                                ptr(lshape), ptr(cp), ptr(lp), float(t0), float(deltaT),
-                                1 if is_confocal else 0, ptr(vp), ptr(vs), ptr(N),
-                                float(wavelength), 1 if assume_row_major else 0,
-                                1 if use_cpu else 0, ptr(out)))
-    return out
+_OUTPUTS = {"complex": 0, "magnitude": 1, "phase": 2, "magnitude_phase": 3}
 
 
-def phasor_multi(transient_data, camera_grid_positions, laser_grid_positions, camera_position,
-                 laser_position, t0, deltaT, is_confocal, volume_position, volume_size,
-                 voxels_per_side, wavelengths, assume_row_major=False, use_cpu=False):
-    """phasor() at several wavelengths of one capture -> complex64 [len(wavelengths), N0, N1, N2]
-    (an extension with no counterpart in the reference module; volume i equals
-    phasor(..., wavelengths[i], ...)).  The capture is copied to the GPU and cropped once."""
+def _phasor_volumes(transient_data, camera_grid_positions, laser_grid_positions, camera_position,
+                    laser_position, t0, deltaT, is_confocal, volume_position, volume_size,
+                    voxels_per_side, wavelengths, assume_row_major, use_cpu, output):
+    """Volumes of bpx_nlos_phasor_output, shaped [count, ...] per the output kind."""
+    if output not in _OUTPUTS:
+        raise ValueError(f"output must be one of {sorted(_OUTPUTS)}, not {output!r}")
     H, cg, lg, cp, lp, vp, vs, N = _prep(transient_data, camera_grid_positions,
                                          laser_grid_positions, camera_position, laser_position,
                                          volume_position, volume_size, voxels_per_side)
     wl = np.ascontiguousarray(np.asarray(wavelengths, dtype=np.float32).reshape(-1))
-    out = np.zeros((len(wl),) + tuple(int(x) for x in N), np.complex64)
+    shape = tuple(int(x) for x in N)
+    if output == "complex":
+        out = np.zeros((len(wl),) + shape, np.complex64)
+    elif output == "magnitude_phase":
+        out = np.zeros((len(wl), 2) + shape, np.float32)
+    else:
+        out = np.zeros((len(wl),) + shape, np.float32)
     dshape, cshape, lshape = _shape(H), _shape(cg), _shape(lg)
-    check(lib().bpx_nlos_phasor_multi(ptr(H), ptr(dshape), H.ndim, ptr(cg), ptr(cshape), ptr(lg),
-                                      ptr(lshape), ptr(cp), ptr(lp), float(t0), float(deltaT),
-                                      1 if is_confocal else 0, ptr(vp), ptr(vs), ptr(N), ptr(wl),
-                                      len(wl), 1 if assume_row_major else 0, 1 if use_cpu else 0,
-                                      ptr(out)))
+    check(lib().bpx_nlos_phasor_output(ptr(H), ptr(dshape), H.ndim, ptr(cg), ptr(cshape), ptr(lg),
+                                       ptr(lshape), ptr(cp), ptr(lp), float(t0), float(deltaT),
+                                       1 if is_confocal else 0, ptr(vp), ptr(vs), ptr(N), ptr(wl),
+                                       len(wl), 1 if assume_row_major else 0, 1 if use_cpu else 0,
+                                       _OUTPUTS[output], ptr(out)))
     return out
+
+
+def phasor(transient_data, camera_grid_positions, laser_grid_positions, camera_position,
+           laser_position, t0, deltaT, is_confocal, volume_position, volume_size,
+           voxels_per_side, wavelength, assume_row_major=False, use_cpu=False, *, output="complex"):
+    """bp::phasor_reconstruction (backproject.hpp:932-1039) -> complex64 volume (or, with
+    output=, its magnitude, phase or both as float32 volumes)."""
+    if output == "complex":
+        H, cg, lg, cp, lp, vp, vs, N = _prep(transient_data, camera_grid_positions,
+                                             laser_grid_positions, camera_position, laser_position,
+                                             volume_position, volume_size, voxels_per_side)
+        out = np.zeros(tuple(int(x) for x in N), np.complex64)
+        dshape, cshape, lshape = _shape(H), _shape(cg), _shape(lg)
+        check(lib().bpx_nlos_phasor(ptr(H), ptr(dshape), H.ndim, ptr(cg), ptr(cshape), ptr(lg),
+                                    ptr(lshape), ptr(cp), ptr(lp), float(t0), float(deltaT),
+                                    1 if is_confocal else 0, ptr(vp), ptr(vs), ptr(N),
+                                    float(wavelength), 1 if assume_row_major else 0,
+                                    1 if use_cpu else 0, ptr(out)))
+        return out
+    v = _phasor_volumes(transient_data, camera_grid_positions, laser_grid_positions, camera_position,
+                        laser_position, t0, deltaT, is_confocal, volume_position, volume_size,
+                        voxels_per_side, [wavelength], assume_row_major, use_cpu, output)[0]
+    return (v[0], v[1]) if output == "magnitude_phase" else v
+
+
+def phasor_multi(transient_data, camera_grid_positions, laser_grid_positions, camera_position,
+                 laser_position, t0, deltaT, is_confocal, volume_position, volume_size,
+                 voxels_per_side, wavelengths, assume_row_major=False, use_cpu=False, *, output="complex"):
+    """phasor() at several wavelengths of one capture -> complex64 [len(wavelengths), N0, N1, N2]
+    (an extension with no counterpart in the reference module; volume i equals
+    phasor(..., wavelengths[i], ...)).  The capture is copied to the GPU and cropped once.
+    output= as for phasor(): float32 [len, N0, N1, N2], or [len, 2, N0, N1, N2] for
+    "magnitude_phase" (magnitude first)."""
+    return _phasor_volumes(transient_data, camera_grid_positions, laser_grid_positions, camera_position,
+                           laser_position, t0, deltaT, is_confocal, volume_position, volume_size,
+                           voxels_per_side, wavelengths, assume_row_major, use_cpu, output)

@@ -200,6 +200,29 @@ bpx_status bpx_nlos_phasor_multi(const float* transient_data, const int64_t* dat
                                  const uint32_t voxels_per_side[3], const float* wavelengths, uint32_t count,
                                  int assume_row_major, int use_cpu, float* complex_volumes_out);
 
+/* Output of bpx_nlos_phasor_output: the complex volume, or what the reference's callers derive
+ * from it on the host -- the CLI writes std::abs of every voxel (main.cpp:162-166). */
+typedef enum bpx_phasor_output {
+    BPX_PHASOR_COMPLEX = 0,         /* interleaved complex, 2 * N0*N1*N2 floats per wavelength  */
+    BPX_PHASOR_MAGNITUDE = 1,       /* |v|, N0*N1*N2 floats per wavelength                       */
+    BPX_PHASOR_PHASE = 2,           /* atan2(im, re) in [-pi, pi], N0*N1*N2 floats per wavelength */
+    BPX_PHASOR_MAGNITUDE_PHASE = 3  /* per wavelength: the magnitude volume, then the phase one  */
+} bpx_phasor_output;
+
+/* bpx_nlos_phasor_multi whose volumes leave the device as `output` says (no reference
+ * counterpart: the reference returns the complex volume, backproject.hpp:932-1039, and its
+ * CLI takes the magnitude on the host, main.cpp:162-166).  Magnitude = sqrt(re^2 + im^2)
+ * evaluated in double and rounded once to float; phase = atan2(im, re) in double rounded to
+ * float.  volumes_out holds count x (2, 1, 1 or 2) x N0*N1*N2 floats. */
+bpx_status bpx_nlos_phasor_output(const float* transient_data, const int64_t* data_shape, int data_ndim,
+                                  const float* camera_grid_positions, const int64_t* camera_grid_shape,
+                                  const float* laser_grid_positions, const int64_t* laser_grid_shape,
+                                  const float camera_position[3], const float laser_position[3],
+                                  float t0, float deltaT, int is_confocal,
+                                  const float volume_position[3], const float volume_size[3],
+                                  const uint32_t voxels_per_side[3], const float* wavelengths, uint32_t count,
+                                  int assume_row_major, int use_cpu, int output, float* volumes_out);
+
 /* Intermediates of the reference pipeline up to the device call (backproject.hpp:779-847):
  * the transient chunk [rows][Tc], the pair table, the GPU-path basis and chunked t0.  Arrays
  * are allocated by the library; release them with bpx_nlos_prepared_free. */

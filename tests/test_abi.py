@@ -106,12 +106,17 @@ def test_top_level_nlosbpy_module_is_the_reference_name(bpx):
     assert nlosbpy.phasor is bpx.nlosbpy.phasor
     assert {"backproject", "phasor"} <= set(nlosbpy.__all__)
     sig = inspect.signature(nlosbpy.phasor)
-    assert list(sig.parameters) == [
+    positional = [n for n, p in sig.parameters.items() if p.kind is p.POSITIONAL_OR_KEYWORD]
+    assert positional == [
         "transient_data", "camera_grid_positions", "laser_grid_positions", "camera_position",
         "laser_position", "t0", "deltaT", "is_confocal", "volume_position", "volume_size",
         "voxels_per_side", "wavelength", "assume_row_major", "use_cpu"]
     assert sig.parameters["assume_row_major"].default is False
     assert sig.parameters["use_cpu"].default is False
+    # the one addition is keyword-only with the reference's behaviour as its default
+    extra = {n: p for n, p in sig.parameters.items() if p.kind is not p.POSITIONAL_OR_KEYWORD}
+    assert list(extra) == ["output"] and extra["output"].kind is extra["output"].KEYWORD_ONLY
+    assert extra["output"].default == "complex"
 
 
 def test_nlosbpy_shape_errors(bpx):

@@ -617,6 +617,50 @@ def test_nlosbpy_phasor_multi_equals_single_calls(gpu, use_cpu, nan_pair):
     assert np.abs(multi).max() > 0
 
 
+@pytest.mark.parametrize("use_cpu", [False, True])
+def test_nlosbpy_phasor_magnitude_and_phase(gpu, oracle, use_cpu):
+    """output= "magnitude" / "phase" / "magnitude_phase": the complex volume's |v| (what the
+    reference CLI writes, std::abs, main.cpp:162-166) and atan2(im, re), computed on the device
+    in double and rounded once; against the same formulas in float64 numpy on the same call's
+    complex volume (within 1 ulp), numpy's float32 abs and angle (within 2: each is within about
+    one ulp of the exact value), and the magnitude against |oracle phasor_reconstruction| at the
+    complex volume's tolerance."""
+
+    def exact(c):
+        re, im = c.real.astype(np.float64), c.imag.astype(np.float64)
+        return np.sqrt(re * re + im * im).astype(f32), np.arctan2(im, re).astype(f32)
+
+    cap = scenes.render(scenes.config_scene(1), 12, 2048, t0=2.0, seed=23)
+    N = (10, 12, 9)
+    args = list(cap.nlos_args(N))
+    wls = [0.004, 0.04]
+    cv = gpu.nlosbpy.phasor(*args, wls[0], assume_row_major=True, use_cpu=use_cpu)
+    mag = gpu.nlosbpy.phasor(*args, wls[0], assume_row_major=True, use_cpu=use_cpu, output="magnitude")
+    ph = gpu.nlosbpy.phasor(*args, wls[0], assume_row_major=True, use_cpu=use_cpu, output="phase")
+    m2, p2 = gpu.nlosbpy.phasor(*args, wls[0], assume_row_major=True, use_cpu=use_cpu, output="magnitude_phase")
+    assert mag.dtype == np.float32 and mag.shape == N and ph.shape == N and np.abs(cv).max() > 0
+    em, ep = exact(cv)
+    np.testing.assert_array_max_ulp(mag, em, maxulp=1)
+    np.testing.assert_array_max_ulp(ph, ep, maxulp=1)
+    np.testing.assert_array_max_ulp(mag, np.abs(cv), maxulp=2)
+    np.testing.assert_array_max_ulp(ph, np.angle(cv).astype(f32), maxulp=2)
+    assert np.array_equal(m2, mag) and np.array_equal(p2, ph)
+    multi = gpu.nlosbpy.phasor_multi(*args, wls, assume_row_major=True, use_cpu=use_cpu, output="magnitude_phase")
+    cmulti = gpu.nlosbpy.phasor_multi(*args, wls, assume_row_major=True, use_cpu=use_cpu)
+    assert multi.shape == (2, 2) + N
+    for i in range(2):
+        em, ep = exact(cmulti[i])
+        np.testing.assert_array_max_ulp(multi[i, 0], em, maxulp=1)
+        np.testing.assert_array_max_ulp(multi[i, 1], ep, maxulp=1)
+    if not use_cpu:
+        cref, _ = oracle.phasor_pipeline(cap.transient, cap.camera_grid, cap.laser_grid,
+                                         cap.camera_position, cap.laser_position, cap.t0, cap.deltaT,
+                                         True, cap.volume_position, cap.volume_size, N, wls[0])
+        assert rel_l2(mag, np.abs(cref)) <= 1e-5
+    with pytest.raises(ValueError):
+        gpu.nlosbpy.phasor(*args, wls[0], assume_row_major=True, output="real")
+
+
 def test_nlosbpy_column_major_equals_row_major(gpu):
     cap = scenes.render(scenes.config_scene(1), 16, 2048, t0=2.0, seed=3)
     N = (12, 12, 12)
