@@ -26,16 +26,17 @@ struct BatchImpl {
     hipStream_t stream = nullptr;
     hipEvent_t ev[2] = {};
     hipEvent_t done = nullptr, ready = nullptr;   // as PlanImpl::done / ready
-    DeviceBuffer caps, flags, items, items_tex, jobs, rjobs, init, next, staged, remaining, error;
+    DeviceBuffer caps, flags, items, items_tex, items_wide, jobs, init, next, staged, remaining, error;
     DeviceBuffer stamps;   // uint64 [1 + captures]: batch start, then each capture's completion
     DeviceBuffer zjobs;    // StageJob per capture with pairs (path-length tables, prologue)
     uint32_t n_zjobs = 0, z_blocks = 0, row_blocks = 0;
-    uint32_t n_items = 0, n_tex_items = 0;
-    uint32_t tex_groups = 0;
-    // two-launch form: the texture-path queue runs beside the LDS-staged one on a second stream
-    // (default priority), forked from and joined back into the captured sequence
-    hipStream_t side = nullptr;
-    hipEvent_t fork = nullptr, join = nullptr;
+    uint32_t n_items = 0, n_tex_items = 0, n_wide_items = 0;
+    uint32_t tex_groups = 0, wide_groups = 0;
+    // two- and three-launch forms: the texture-path queue (and the 16^3-tile queue) run beside
+    // the LDS-staged one on other streams (default priority), forked from and joined back into
+    // the captured sequence
+    hipStream_t side = nullptr, side2 = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr, join2 = nullptr;
     int format = kF32;     // storage of every capture (fused rows)
     bool cplx = false;     // complex captures and volumes
     bpx_batch_config cfg{2u, 0u, 0u, 0u, 8u, 1u};
@@ -52,7 +53,9 @@ struct BatchImpl {
         if (ready) (void)hipEventDestroy(ready);
         if (fork) (void)hipEventDestroy(fork);
         if (join) (void)hipEventDestroy(join);
+        if (join2) (void)hipEventDestroy(join2);
         if (side) (void)hipStreamDestroy(side);
+        if (side2) (void)hipStreamDestroy(side2);
         if (stream) (void)hipStreamDestroy(stream);
     }
     void build(const bpx_batch_config& c);
@@ -73,16 +76,18 @@ void BatchImpl::build(const bpx_batch_config& c) {
     const uint32_t kz_req = c.lds_depth ? c.lds_depth : 8u;
     if (kz_req != 8u && kz_req != 16u) fail(BPX_ERR_INVALID_ARGUMENT, "lds_depth must be 8 or 16");
     const uint32_t launches = c.launches ? c.launches : 1u;
-    if (launches > 2u) fail(BPX_ERR_INVALID_ARGUMENT, "launches must be 1 or 2");
+    if (launches > 3u) fail(BPX_ERR_INVALID_ARGUMENT, "launches must be 1, 2 or 3");
+    const bool three = launches == 3u;
+    if (three && format_complex(format)) fail(BPX_ERR_INVALID_ARGUMENT, "launches = 3 needs real samples");
     const int K = (int)c.lane_voxels;
-    const int kz = batch_lds_depth((int)kz_req, format);
-    const bool two = launches == 2u;
+    const int kz = three ? 8 : batch_lds_depth((int)kz_req, format);
+    const bool two = launches >= 2u;   // rows staged first, the texture-path tiles on their own queue
     const uint32_t n = (uint32_t)plans.size();
     // per-capture arguments, tiles and staging jobs for this configuration
     std::vector<ConfocalArgs> args(n);
     std::vector<StageJob> sj(std::max<uint32_t>(n, 1));
     std::vector<uint32_t> fl(n), nt(n), ns(n);
-    std::vector<char> lds(n, 0);
+    std::vector<char> lds(n, 0), wide(n, 0);
     for (uint32_t i = 0; i < n; ++i) {
         const PlanImpl& pl = *plans[i];
         ConfocalArgs& a = args[i];
@@ -102,9 +107,14 @@ void BatchImpl::build(const bpx_batch_config& c) {
         fl[i] = (pl.sep ? 1u : 0u) | (pl.sep && pl.flat ? 2u : 0u);
         // captures whose bin windows fit the LDS segment run LDS-staged tiles, the others
         // texture-path tiles
-        lds[i] = pl.P && batch_lds_applies(pl.g, pl.inv2, format, pl.sep, pl.flat, kz);
-        // (two launches: a texture-path tile is 4 wave items, one per quarter)
-        nt[i] = !pl.P ? 0u : lds[i] ? batch_lds_tiles(pl.g, kz) : (two ? 4u : 1u) * a.t.count;
+        // (three launches: those that fit 16^3 tiles take the 512-lane 16^3-tile queue)
+        wide[i] = three && pl.P && batch_lds_applies(pl.g, pl.inv2, format, pl.sep, pl.flat, 16);
+        lds[i] = pl.P && (wide[i] || batch_lds_applies(pl.g, pl.inv2, format, pl.sep, pl.flat, kz));
+        // (two or three launches: a texture-path tile is 4 wave items, one per quarter)
+        nt[i] = !pl.P ? 0u
+                : wide[i] ? batch_lds_tiles(pl.g, 16)
+                : lds[i]  ? batch_lds_tiles(pl.g, kz)
+                          : (two ? 4u : 1u) * a.t.count;
         if (pl.P) {
             sj[i] = stage_job(pl);
             ns[i] = two ? 0u : (stage_row_blocks(sj[i]) + kBatchStageBlocks - 1u) / kBatchStageBlocks;
@@ -158,12 +168,14 @@ void BatchImpl::build(const bpx_batch_config& c) {
     } else {
         for (uint32_t k = 0; k < n; ++k) push_gather(order[k]);
     }
-    // two launches: the texture-path items go to their own (wave-granular) queue
-    std::vector<uint32_t> itx;
+    // two launches: the texture-path items go to their own (wave-granular) queue; three: the
+    // 16^3 tiles to theirs as well
+    std::vector<uint32_t> itx, itw;
     if (two) {
         std::vector<uint32_t> keep;
         for (size_t k = 0; k < it.size(); k += 2) {
-            std::vector<uint32_t>& q = (it[k] >> 28) == kBatchItemTex ? itx : keep;
+            const uint32_t ci = it[k] & 0x0fffffffu;
+            std::vector<uint32_t>& q = (it[k] >> 28) == kBatchItemTex ? itx : wide[ci] ? itw : keep;
             q.push_back(it[k]);
             q.push_back(it[k + 1]);
         }
@@ -171,6 +183,7 @@ void BatchImpl::build(const bpx_batch_config& c) {
     }
     n_items = (uint32_t)(it.size() / 2);
     n_tex_items = (uint32_t)(itx.size() / 2);
+    n_wide_items = (uint32_t)(itw.size() / 2);
     // path-length tables of every capture in the prologue: each job's blocks follow the previous one's
     std::vector<StageJob> zj;
     uint64_t zb = 0, rb = 0;
@@ -199,6 +212,7 @@ void BatchImpl::build(const bpx_batch_config& c) {
     flags.alloc(4u * std::max<uint32_t>(n, 1));
     items.alloc(8u * std::max<uint32_t>(n_items, 1));
     items_tex.alloc(8u * std::max<uint32_t>(n_tex_items, 1));
+    items_wide.alloc(8u * std::max<uint32_t>(n_wide_items, 1));
     if (n_zjobs)
         hip_check(hipMemcpy(zjobs.ptr, zj.data(), sizeof(StageJob) * n_zjobs, hipMemcpyHostToDevice), "hipMemcpy");
     hip_check(hipMemcpy(jobs.ptr, sj.data(), sizeof(StageJob) * sj.size(), hipMemcpyHostToDevice), "hipMemcpy");
@@ -210,10 +224,19 @@ void BatchImpl::build(const bpx_batch_config& c) {
     if (n_items) hip_check(hipMemcpy(items.ptr, it.data(), 8u * n_items, hipMemcpyHostToDevice), "hipMemcpy");
     if (n_tex_items)
         hip_check(hipMemcpy(items_tex.ptr, itx.data(), 8u * n_tex_items, hipMemcpyHostToDevice), "hipMemcpy");
+    if (n_wide_items)
+        hip_check(hipMemcpy(items_wide.ptr, itw.data(), 8u * n_wide_items, hipMemcpyHostToDevice), "hipMemcpy");
     const int resident = batch_queue_resident_groups(K, kz, format, (int)launches);
     if (resident <= 0) fail(BPX_ERR_DEVICE, "cannot query the occupancy of the batch kernel");
     groups = c.groups ? std::min<uint32_t>(c.groups, (uint32_t)resident) : (uint32_t)resident;
     groups = std::max<uint32_t>(1u, std::min<uint32_t>(groups, std::max<uint32_t>(n_items, 1u)));
+    wide_groups = 0;
+    if (n_wide_items) {
+        const int rw = batch_wide_resident_groups(format);
+        if (rw <= 0) fail(BPX_ERR_DEVICE, "cannot query the occupancy of the batch 16^3-tile kernel");
+        wide_groups = c.groups ? std::min<uint32_t>(c.groups, (uint32_t)rw) : (uint32_t)rw;
+        wide_groups = std::max<uint32_t>(1u, std::min<uint32_t>(wide_groups, n_wide_items));
+    }
     if (two && n_tex_items) {
         const int rt = batch_tex_resident_groups(K, format);
         if (rt <= 0) fail(BPX_ERR_DEVICE, "cannot query the occupancy of the batch texture-path kernel");
@@ -224,7 +247,7 @@ void BatchImpl::build(const bpx_batch_config& c) {
     cfg.groups = groups;
     cfg.lds_depth = (uint32_t)kz;
     cfg.launches = launches;
-    cfg.lds_captures = (uint32_t)std::count(lds.begin(), lds.end(), (char)1);
+    cfg.lds_captures = (uint32_t)std::count(lds.begin(), lds.end(), (char)1);   // 16^3 tiles included
     BatchQueueArgs q{};
     q.caps = static_cast<const ConfocalArgs*>(caps.ptr);
     q.flags = static_cast<const uint32_t*>(flags.ptr);
@@ -246,8 +269,12 @@ void BatchImpl::build(const bpx_batch_config& c) {
     qt.items = static_cast<const uint2*>(items_tex.ptr);
     qt.n_items = n_tex_items;
     qt.next = static_cast<uint32_t*>(next.ptr) + 1;
+    BatchQueueArgs qw = q;   // the 16^3-tile queue of the three-launch form
+    qw.items = static_cast<const uint2*>(items_wide.ptr);
+    qw.n_items = n_wide_items;
+    qw.next = static_cast<uint32_t*>(next.ptr) + 2;
     hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-    bool side_joined = false;   // the side stream has joined the capture and not yet left it
+    bool side_joined = false, side2_joined = false;   // side streams in the capture, not yet joined back
     try {
         for (auto& pl : plans) {
             if (pl->P != 0) continue;
@@ -263,20 +290,37 @@ void BatchImpl::build(const bpx_batch_config& c) {
             // every row staged first (no waits in either queue), then both queues side by side
             hip_check(launch_stage(static_cast<const StageJob*>(zjobs.ptr), n_zjobs, 0u, row_blocks, stream),
                       "batch row staging kernel");
-            const bool fork_side = n_items && n_tex_items;
+            // every queue with items on its own stream: the 16 x 16 x kz one on the graph's
+            // stream, the texture-path one on `side`, the 16^3 one on `side2`
+            const bool fork_side = n_tex_items && (n_items || n_wide_items);
+            const bool fork_side2 = n_wide_items && (n_items || n_tex_items);
+            if (fork_side || fork_side2) hip_check(hipEventRecord(fork, stream), "hipEventRecord");
             if (fork_side) {
-                hip_check(hipEventRecord(fork, stream), "hipEventRecord");
                 hip_check(hipStreamWaitEvent(side, fork, 0), "hipStreamWaitEvent");
                 side_joined = true;
             }
-            if (n_items) hip_check(launch_batch_queue(q, K, kz, format, 2, groups, stream), "batch LDS-staged queue");
+            if (fork_side2) {
+                hip_check(hipStreamWaitEvent(side2, fork, 0), "hipStreamWaitEvent");
+                side2_joined = true;
+            }
+            if (n_items)
+                hip_check(launch_batch_queue(q, K, kz, format, (int)launches, groups, stream),
+                          "batch LDS-staged queue");
             if (n_tex_items)
                 hip_check(launch_batch_tex(qt, K, format, tex_groups, fork_side ? side : stream),
                           "batch texture-path queue");
+            if (n_wide_items)
+                hip_check(launch_batch_wide(qw, format, wide_groups, fork_side2 ? side2 : stream),
+                          "batch 16^3-tile queue");
             if (fork_side) {
                 hip_check(hipEventRecord(join, side), "hipEventRecord");
                 hip_check(hipStreamWaitEvent(stream, join, 0), "hipStreamWaitEvent");
                 side_joined = false;
+            }
+            if (fork_side2) {
+                hip_check(hipEventRecord(join2, side2), "hipEventRecord");
+                hip_check(hipStreamWaitEvent(stream, join2, 0), "hipStreamWaitEvent");
+                side2_joined = false;
             }
         } else if (n_items) {
             hip_check(launch_batch_queue(q, K, kz, format, 1, groups, stream), "batch queue kernel");
@@ -285,6 +329,7 @@ void BatchImpl::build(const bpx_batch_config& c) {
         // a side stream left in the capture would make the capture fail as unjoined and stay in
         // capture mode: join it back first, then end (and drop) the capture
         if (side_joined && hipEventRecord(join, side) == hipSuccess) (void)hipStreamWaitEvent(stream, join, 0);
+        if (side2_joined && hipEventRecord(join2, side2) == hipSuccess) (void)hipStreamWaitEvent(stream, join2, 0);
         hipGraph_t g = nullptr;
         (void)hipStreamEndCapture(stream, &g);
         if (g) (void)hipGraphDestroy(g);
@@ -351,6 +396,8 @@ bpx_status bpx_batch_create(const bpx_capture_desc* captures, uint32_t count, co
         hip_check(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking), "hipStreamCreate");
         hip_check(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming), "hipEventCreate");
         hip_check(hipEventCreateWithFlags(&b->join, hipEventDisableTiming), "hipEventCreate");
+        hip_check(hipStreamCreateWithFlags(&b->side2, hipStreamNonBlocking), "hipStreamCreate");
+        hip_check(hipEventCreateWithFlags(&b->join2, hipEventDisableTiming), "hipEventCreate");
         for (auto& e : b->ev) hip_check(hipEventCreate(&e), "hipEventCreate");
         hip_check(hipEventCreateWithFlags(&b->done, hipEventDisableTiming), "hipEventCreate");
         hip_check(hipEventCreateWithFlags(&b->ready, hipEventDisableTiming), "hipEventCreate");
@@ -369,7 +416,7 @@ bpx_status bpx_batch_create(const bpx_capture_desc* captures, uint32_t count, co
             b->plans.push_back(std::move(pl));
         }
         const uint32_t n = std::max<uint32_t>(count, 1);
-        b->next.alloc(8);   // queue heads: the (LDS-staged) queue, the texture-path queue
+        b->next.alloc(12);   // queue heads: the (LDS-staged) queue, texture-path, 16^3-tile
         b->error.alloc(4);
         hip_check(hipMemset(b->error.ptr, 0, 4), "hipMemset");
         b->staged.alloc(4u * n);
@@ -412,10 +459,11 @@ bpx_status bpx_batch_tune(bpx_batch batch, int32_t reps, bpx_batch_config* chose
         reps = std::max(reps, 1);
         bpx_batch_config best{2u, 0u, 0u, 0u, 8u, 1u};
         double best_ms = INFINITY;
-        for (uint32_t launches : {1u, 2u})
+        for (uint32_t launches : {1u, 2u, 3u})
             for (uint32_t k : {2u, 4u})
                 for (uint32_t kz : {8u, 16u}) {
                     if (kz == 16u && format_complex(b.format)) continue;   // complex formats: depth 8
+                    if (launches == 3u && (kz == 16u || format_complex(b.format))) continue;   // both depths
                     const int resident = batch_queue_resident_groups((int)k, (int)kz, b.format, (int)launches);
                     for (uint32_t frac : {1u, 2u})
                         for (uint32_t order : {0u, 1u, 2u}) {

@@ -64,7 +64,7 @@ struct BatchQueueArgs {
     const uint2* items;         // x = capture | type << 28; y = tile, or first row block (stage)
     uint32_t n_items;
     uint32_t spin_limit;        // polls of a capture's staging counter before giving up
-    uint32_t* next;             // queue head (0 at launch; the prologue also clears next[1])
+    uint32_t* next;             // queue head (0 at launch; the prologue clears next[0..2])
     uint32_t* staged;           // per capture: staging items not finished
     uint32_t* remaining;        // per capture: gather items not finished
     uint64_t* done;             // per capture: wall clock when its last gather item finished
@@ -261,6 +261,9 @@ int batch_lds_depth(int kz, int format);   // the depth the queue kernel uses fo
 hipError_t launch_batch_queue(const BatchQueueArgs& b, int K, int kz, int format, int launches, uint32_t groups,
                               hipStream_t s);
 int batch_tex_resident_groups(int K, int format);
+// launches 3: the 16^3-tile LDS-staged queue of 512-lane work-groups (real formats; head next[2])
+int batch_wide_resident_groups(int format);
+hipError_t launch_batch_wide(const BatchQueueArgs& b, int format, uint32_t groups, hipStream_t s);
 hipError_t launch_batch_tex(const BatchQueueArgs& b, int K, int format, uint32_t groups, hipStream_t s);
 hipError_t launch_batch_prologue(const StageJob* jobs, uint32_t n_jobs, uint32_t z_blocks, const BatchQueueArgs& b,
                                  const uint32_t* init, uint32_t n, uint64_t* start, hipStream_t s);

@@ -927,9 +927,12 @@ __device__ __forceinline__ T wave_uniform_load(const T* p) {
 }
 
 // KINDS: the item kinds the instantiation serves (bit 0 staging, bit 1 LDS-staged, bit 2 texture
-// path); an LDS-only queue compiles without the other paths and needs fewer registers.
-template <class Fz, class Src, int K, int KZ, int KINDS>
-__global__ __launch_bounds__(256) void batch_queue_kernel(BatchQueueArgs b) {
+// path); an LDS-only queue compiles without the other paths and needs fewer registers.  NW:
+// waves per work-group -- 4, or 8 for an LDS-only queue of 16^3 tiles with 8 voxels per lane
+// (the single-capture 512-lane kernel's shape and unrolled stage loop).
+template <class Fz, class Src, int K, int KZ, int KINDS, int NW = 4>
+__global__ __launch_bounds__(NW * 64) void batch_queue_kernel(BatchQueueArgs b) {
+    static_assert(NW == 4 || KINDS == 2, "8-wave work-groups serve LDS-staged tiles only");
     __shared__ uint32_t s_item;
     const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     for (uint32_t taken = 0; taken <= b.n_items; ++taken) {
@@ -966,7 +969,8 @@ __global__ __launch_bounds__(256) void batch_queue_kernel(BatchQueueArgs b) {
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");   // the staged rows, for every wave
         const ConfocalArgs a = wave_uniform_load(b.caps + c);
         if ((KINDS & 2) && type == kBatchItemLds) {
-            lds_tile<Fz, KZ, 4, true, false>(a, y);   // rolled stage loop: fewer registers, more work-groups
+            // 4 waves: rolled stage loop (fewer registers, more work-groups)
+            lds_tile<Fz, KZ, NW, true, NW == 8>(a, y);
         } else if (KINDS & 4) {
             const uint32_t f = __builtin_amdgcn_readfirstlane(b.flags[c]);
             if (f == 3u) confocal_tile<Src, K, true, true>(a, y, w);
@@ -1033,16 +1037,26 @@ __global__ __launch_bounds__(256) void batch_prologue_kernel(const StageJob* __r
         b.done[i] = 0ull;
     }
     if (threadIdx.x == 0) {
-        b.next[0] = 0u;   // the queue head, and the texture-path head of a two-launch batch
-        b.next[1] = 0u;
+        b.next[0] = 0u;   // the queue head; the texture-path and 16^3-tile heads of the
+        b.next[1] = 0u;   // two- and three-launch forms
+        b.next[2] = 0u;
         *b.error = 0u;
         *start = wall_clock64();
     }
 }
 
-template <class Fz, class Src, int K, int KZ, int KINDS>
+template <class Fz, class Src, int K, int KZ, int KINDS, int NW = 4>
 static const void* batch_queue_fn() {
-    return reinterpret_cast<const void*>(batch_queue_kernel<Fz, Src, K, KZ, KINDS>);
+    return reinterpret_cast<const void*>(batch_queue_kernel<Fz, Src, K, KZ, KINDS, NW>);
+}
+// the 16^3-tile queue of 512-lane work-groups (real formats)
+static const void* batch_wide_of(int format) {
+    switch (format) {
+        case kF32: return batch_queue_fn<FusedF32, FusedSource<FusedF32>, 2, 8, 2, 8>();
+        case kF16: return batch_queue_fn<FusedH16<false>, FusedSource<FusedH16<false>>, 2, 8, 2, 8>();
+        case kBF16: return batch_queue_fn<FusedH16<true>, FusedSource<FusedH16<true>>, 2, 8, 2, 8>();
+        default: return nullptr;
+    }
 }
 template <int K, int KZ, int KINDS>
 static const void* batch_queue_of(int format) {
@@ -1056,10 +1070,11 @@ static const void* batch_queue_of(int format) {
         default: return nullptr;
     }
 }
-// launches 1: every kind in one kernel; 2: the LDS-staged tiles alone (K unused)
+// launches 1: every kind in one kernel; 2 and 3: the (16 x 16 x kz) LDS-staged tiles alone
+// (K unused; 3 keeps depth 8 beside the 16^3-tile queue)
 static const void* batch_queue(int K, int kz, int format, int launches) {
-    if (format_complex(format)) kz = 8;   // two accumulators per voxel: 8 voxels per lane
-    if (launches == 2) return kz == 16 ? batch_queue_of<2, 16, 2>(format) : batch_queue_of<2, 8, 2>(format);
+    if (format_complex(format) || launches == 3) kz = 8;   // complex: two accumulators per voxel
+    if (launches >= 2) return kz == 16 ? batch_queue_of<2, 16, 2>(format) : batch_queue_of<2, 8, 2>(format);
     if (K == 2) return kz == 16 ? batch_queue_of<2, 16, 7>(format) : batch_queue_of<2, 8, 7>(format);
     if (K == 4) return kz == 16 ? batch_queue_of<4, 16, 7>(format) : batch_queue_of<4, 8, 7>(format);
     return nullptr;
@@ -1080,14 +1095,14 @@ static const void* batch_tex(int K, int format) {
     return K == 4 ? batch_tex_of<4>(format) : K == 2 ? batch_tex_of<2>(format) : nullptr;
 }
 
-static int resident_of(const void* fn) {
+static int resident_of(const void* fn, int threads = 256) {
     int dev = 0, cus = 0, per_cu = 0;
     if (!fn || hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
         (void)hipGetLastError();
         return 0;
     }
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, 0) != hipSuccess) {
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, 0) != hipSuccess) {
         (void)hipGetLastError();
         return 0;
     }
@@ -1098,6 +1113,13 @@ int batch_queue_resident_groups(int K, int kz, int format, int launches) {
     return resident_of(batch_queue(K, kz, format, launches));
 }
 int batch_tex_resident_groups(int K, int format) { return resident_of(batch_tex(K, format)); }
+int batch_wide_resident_groups(int format) { return resident_of(batch_wide_of(format), 512); }
+hipError_t launch_batch_wide(const BatchQueueArgs& b, int format, uint32_t groups, hipStream_t s) {
+    const void* fn = batch_wide_of(format);
+    if (!fn || groups == 0u) return hipErrorInvalidValue;
+    void* args[] = {const_cast<BatchQueueArgs*>(&b)};
+    return hipLaunchKernel(fn, dim3(groups), dim3(512), args, 0, s);
+}
 
 hipError_t launch_batch_tex(const BatchQueueArgs& b, int K, int format, uint32_t groups, hipStream_t s) {
     const void* fn = batch_tex(K, format);

@@ -411,7 +411,10 @@ typedef struct bpx_batch_config {
     uint32_t launches;     /* 1 (or 0): one persistent launch serves the rows' staging, the
                               LDS-staged and the texture-path tiles from one queue; 2: rows
                               staged first, then the LDS-staged queue and a wave-granular
-                              texture-path queue as two concurrent persistent launches  */
+                              texture-path queue as two concurrent persistent launches;
+                              3 (real samples): as 2, with the captures that fit 16^3 tiles
+                              on a third concurrent queue of 512-lane work-groups (8
+                              voxels per lane) and the others' LDS tiles at depth 8    */
 } bpx_batch_config;
 typedef struct bpx_batch_timing {
     double per_exec_ms;    /* mean device time of one batch execution (HIP events)       */

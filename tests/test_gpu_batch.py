@@ -56,7 +56,7 @@ def test_batch_lds_staged_queue(gpu, oracle):
         cfg = b.config()
         assert 3 <= cfg["lds_captures"] < len(caps), cfg
         # 16 x 16 x 8 and 16^3 LDS tiles, queue orders, one launch or two concurrent ones
-        for depth, order, launches in ((8, 0, 1), (16, 2, 1), (8, 2, 1), (8, 0, 2), (16, 1, 2)):
+        for depth, order, launches in ((8, 0, 1), (16, 2, 1), (8, 2, 1), (8, 0, 2), (16, 1, 2), (8, 0, 3), (8, 1, 3)):
             b.set_config(lane_voxels=2, groups=0, order=order, lds_depth=depth, launches=launches)
             assert b.config()["launches"] == launches
             for rep in range(2):
@@ -78,7 +78,7 @@ def mixed(seed=7):
     return [capture(rng, *sp, coded=(i % 2 == 0)) for i, sp in enumerate(spec)]
 
 
-@pytest.mark.parametrize("launches", [1, 2])
+@pytest.mark.parametrize("launches", [1, 2, 3])
 def test_batch_execute_on_caller_stream(gpu, hip_streams, launches):
     """The batch graph (with two launches: its fork/join branch of two concurrent queues)
     replayed on two caller streams."""
@@ -144,7 +144,9 @@ def test_batch_complex_captures_equal_single_captures(gpu, storage):
                                  dict(lane_voxels=2, groups=0, order=2),
                                  dict(lane_voxels=4, groups=5, order=2, lds_depth=16),
                                  dict(lane_voxels=2, groups=0, order=0, launches=2),
-                                 dict(lane_voxels=4, groups=3, order=1, lds_depth=16, launches=2)])
+                                 dict(lane_voxels=4, groups=3, order=1, lds_depth=16, launches=2),
+                                 dict(lane_voxels=2, groups=0, order=0, launches=3),
+                                 dict(lane_voxels=4, groups=2, order=1, launches=3)])
 def test_batch_configs_identical(gpu, cfg):
     caps = uneven(1)
     alone = [gpu.backprojection(*c) for c in caps]
