@@ -168,11 +168,13 @@ def batch_comparison(bp, count, seed=1004, serial=True, ctx=None):
         # the persistent queue kernel of the chosen configuration, and the prologue
         k, d = cfg["lane_voxels"], cfg["lds_depth"]
         if cfg["launches"] == 1:
-            queue = f"batch_queue_kernel<FusedF32, FusedSource<FusedF32>, {k}, {d}, 7>"
+            queue = f"batch_queue_kernel<FusedF32, FusedSource<FusedF32>, {k}, {d}, 7, 4>"
         else:   # the LDS-staged tiles' queue (its voxels per lane are the LDS tile's)
-            queue = f"batch_queue_kernel<FusedF32, FusedSource<FusedF32>, 2, {d}, 2>"
+            queue = f"batch_queue_kernel<FusedF32, FusedSource<FusedF32>, 2, {d}, 2, 4>"
         names = [queue, "batch_prologue_kernel"]
-        if cfg["launches"] == 2:
+        if cfg["launches"] == 3:   # the 512-lane 16^3-tile queue
+            names.append("batch_queue_kernel<FusedF32, FusedSource<FusedF32>, 2, 8, 2, 8>")
+        if cfg["launches"] >= 2:
             names += [f"batch_tex_kernel<{k}, FusedSource<FusedF32>", "stage_rows_kernel"]
         roofs = {name: roofline_of(ctx["counters"], name, None, None, ctx["fingerprint"], pick="max")
                  for name in names}
