@@ -144,7 +144,7 @@ class Plan:
 
     def __init__(self, transient_chunk, pairs, voxels_per_side, volume_zero_pos, voxel_inc, t0,
                  deltaT, storage="f32", kahan_block=0, reference_pair_truncation=False,
-                 variant=-1):
+                 variant=-1, keep_f32_rows=False):
         cplx = np.iscomplexobj(transient_chunk)
         H, n = _samples(transient_chunk, cplx)
         pr = as_pairs(pairs)
@@ -153,6 +153,8 @@ class Plan:
         self.N = tuple(int(x) for x in _u32(voxels_per_side).reshape(3))
         self.complex = cplx
         flags = nat.FLAG_REFERENCE_PAIR_TRUNCATION if reference_pair_truncation else 0
+        if keep_f32_rows:   # no exact fp16 staging of integer samples (see bpx.h)
+            flags |= nat.FLAG_KEEP_F32_ROWS
         opt = nat.make_options(storage, kahan_block, flags, variant)
         h = ctypes.c_void_p()
         check(lib().bpx_plan_create(ptr(H), n, T, ptr(pr), P, ptr(_u32(self.N)),
@@ -261,8 +263,11 @@ class Plan:
         names = {0: "confocal", 1: "grid", 2: "pairs", 3: "window"}
         kname = ctypes.create_string_buffer(160)
         check(lib().bpx_plan_kernel_name(self._h, kname, len(kname)))
+        rows = ctypes.c_uint32()
+        check(lib().bpx_plan_row_storage(self._h, ctypes.byref(rows)))
         return dict(path=names.get(path.value, path.value), lane_voxels=k.value,
-                    pairs_used=used.value, Tc=tc.value, scale=sc.value, kernel=kname.value.decode())
+                    pairs_used=used.value, Tc=tc.value, scale=sc.value, kernel=kname.value.decode(),
+                    row_storage={0: "f32", 1: "f16", 2: "bf16"}.get(rows.value, rows.value))
 
     def benchmark(self, warmup=2, iterations=10):
         t = nat.Timing()

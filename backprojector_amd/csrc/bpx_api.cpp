@@ -298,9 +298,10 @@ bpx_status bpx_plan_read_thresholds(bpx_plan plan, float* out, uint64_t capacity
         if (n == 0) return;
         plan_wait(pl);
         const DeviceBuffer& src = pl.fused ? pl.H : pl.S;
-        const size_t stride = pl.fused ? (size_t)fused_elem_bytes(pl.format) / 4 : 1;
+        const int fmt = staged_format(pl);
+        const size_t stride = pl.fused ? (size_t)fused_elem_bytes(fmt) / 4 : 1;
         const size_t pitch = pl.fused ? (size_t)pl.ldh * stride : pl.lds;
-        const size_t off = pl.fused ? (size_t)fused_s_offset(pl.format) : 0;
+        const size_t off = pl.fused ? (size_t)fused_s_offset(fmt) : 0;
         std::vector<float> host(src.bytes / 4);
         hip_check(hipMemcpy(host.data(), src.ptr, host.size() * 4, hipMemcpyDeviceToHost), "hipMemcpy(thresholds)");
         for (uint64_t p = 0; p < pl.P; ++p)
@@ -318,6 +319,14 @@ bpx_status bpx_plan_info(bpx_plan plan, int32_t* path, int32_t* lane_voxels, uin
         if (pairs_used) *pairs_used = pl.P;
         if (Tc) *Tc = pl.T;
         if (scale) *scale = pl.scale;
+    });
+}
+
+bpx_status bpx_plan_row_storage(bpx_plan plan, uint32_t* storage) {
+    return guarded([&] {
+        if (!plan || !storage) fail(BPX_ERR_INVALID_ARGUMENT, "NULL argument");
+        const int f = staged_format(*plan->impl);
+        *storage = f == kF16 ? BPX_STORAGE_F16 : f == kBF16 ? BPX_STORAGE_BF16 : BPX_STORAGE_F32;
     });
 }
 

@@ -385,6 +385,7 @@ bpx_status bpx_batch_create(const bpx_capture_desc* captures, uint32_t count, co
         if (options) o = *options;
         if (o.storage > BPX_STORAGE_BF16) fail(BPX_ERR_INVALID_ARGUMENT, "unknown storage format");
         o.variant = 5;  // fused rows: the layout the batch kernel reads
+        o.flags |= BPX_FLAG_KEEP_F32_ROWS;   // the batch stages the rows itself, in one format
         auto b = std::make_unique<BatchImpl>();
         b->cplx = (o.flags & BPX_FLAG_COMPLEX_SAMPLES) != 0;
         o.flags &= ~BPX_FLAG_COMPLEX_SAMPLES;

@@ -101,6 +101,10 @@ struct PlanImpl {
     Tiles tiles{};
     bool sep = false, flat = false, fused = false;
     bool lds_tiles = false;              // the LDS-staged confocal kernel runs (16 voxels per lane)
+    // fp32 rows staged as exact fp16 (FusedH16) whenever every sample of the execution is an
+    // integer of magnitude <= 2048; fmt_flag (one word) = 0 then, 1 otherwise
+    bool rows16 = false;
+    DeviceBuffer fmt_flag;
     float lz = 0;
     uint32_t kahan_block = 32;
     float inv2 = 0, inv = 0, c1 = 0;
@@ -122,6 +126,8 @@ struct PlanImpl {
 
 // Waits for the plan's own stream and for its last execution on a caller's stream.
 void plan_wait(const PlanImpl& pl);
+// The storage the plan's last execution staged its rows in (waits for that execution).
+int staged_format(const PlanImpl& pl);
 
 void execute_octree(PlanImpl& pl, hipStream_t s);
 

@@ -49,6 +49,10 @@ struct ConfocalArgs {
     float scale;           // per-capture sample scale (power of two)
     float lz;              // common wall z when every pair shares it
     uint32_t kahan_block;
+    // rows whose format is decided on the device at each execution: the kernel runs only when
+    // *fmt_flag == fmt_want (nullptr: always)
+    const uint32_t* fmt_flag;
+    uint32_t fmt_want;
 };
 
 // Factorised exhaustive-grid kernel parameters (threshold table in path-length space).
@@ -217,11 +221,12 @@ struct StageJob {
     void* dst;            // fused rows [rows][lde] elements
     const float* pair8;   // pair table, 8 floats per pair
     float* Z;             // [Tc + 1] path-length thresholds (written by the first pass)
+    const uint32_t* fmt_flag;   // non-null: rows written in alt_format while *fmt_flag == 0
     uint32_t rows, Tc, lde;
     int32_t format;
     float inv_scale, t0, dT;
     uint32_t z_first, s_first;
-    uint32_t pad_;
+    int32_t alt_format;
 };
 // raw[r][c] = src[map[r]][from + c - cfrom] for c in [cfrom, cfrom + ncopy), else 0
 // (r < rows, c < Tc; src rows have pitch src_T; map NULL = identity).
@@ -271,6 +276,9 @@ bool batch_lds_applies(const Grid& g, float inv2, int format, bool sep, bool fla
 uint32_t batch_lds_tiles(const Grid& g, int kz);
 hipError_t launch_window(const WindowArgs& a, int complex_samples, hipStream_t stream);
 hipError_t launch_abs_max(const float* src, uint64_t n, float* result_dev, hipStream_t stream);
+// *flag |= 1 unless every one of n samples is an integer of magnitude <= 2048 (exact in fp16);
+// the caller clears *flag first.
+hipError_t launch_exact16_check(const float* src, uint64_t n, uint32_t* flag, hipStream_t stream);
 // |v| and atan2(im, re) of n interleaved complex values (either output may be null).
 hipError_t launch_complex_post(const void* complex_volume, uint64_t n, float* mag, float* phase,
                                hipStream_t stream);

@@ -247,12 +247,13 @@ __device__ __forceinline__ void stage_row_block(const StageJob& jb, uint32_t loc
     const uint32_t per_row = (jb.lde + 255u) / 256u;
     const uint32_t p = local / per_row, j = (local - p * per_row) * 256u + threadIdx.x;
     if (j >= jb.lde) return;
+    const int format = jb.fmt_flag && *jb.fmt_flag == 0u ? jb.alt_format : jb.format;
     const uint64_t e = (uint64_t)p * jb.lde + j;
     const int cplx = format_complex(jb.format) ? 1 : 0;
     const float* row = jb.src + (size_t)p * jb.Tc * (cplx ? 2u : 1u);
     float S = 0.0f;
     if (j <= jb.Tc) S = stable_threshold(jb.pair8[8u * p + 7u], jb.pair8[8u * p + 6u], jb.Z[j]);
-    store_fused(jb.dst, e, jb.format, row, j, jb.Tc, jb.inv_scale, S);
+    store_fused(jb.dst, e, format, row, j, jb.Tc, jb.inv_scale, S);
 }
 
 __global__ __launch_bounds__(256) void stage_z_kernel(const StageJob* __restrict__ jobs, uint32_t n_jobs,
@@ -481,6 +482,25 @@ hipError_t launch_complex_post(const void* complex_volume, uint64_t n, float* ma
     const uint64_t blocks = std::min<uint64_t>((n + 255u) / 256u, 1u << 16);
     hipLaunchKernelGGL(complex_post_kernel, dim3((uint32_t)blocks), dim3(256), 0, stream,
                        static_cast<const float2*>(complex_volume), n, mag, phase);
+    return hipGetLastError();
+}
+
+// Whether every sample is an integer that fp16 holds exactly (|x| <= 2048): one flag word,
+// set by any wave that finds a sample that is not (NaN and infinities included).
+__global__ __launch_bounds__(256) void exact16_check_kernel(const float* __restrict__ src, uint64_t n,
+                                                            uint32_t* flag) {
+    bool bad = false;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const float x = src[i];
+        bad = bad || !(__builtin_fabsf(x) <= 2048.0f && x == __builtin_rintf(x));
+    }
+    if (__ballot(bad) != 0ull && (threadIdx.x & 63u) == 0u) atomicOr(flag, 1u);
+}
+
+hipError_t launch_exact16_check(const float* src, uint64_t n, uint32_t* flag, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    const uint64_t blocks = std::min<uint64_t>((n + 255u) / 256u, 2048u);
+    hipLaunchKernelGGL(exact16_check_kernel, dim3((uint32_t)blocks), dim3(256), 0, stream, src, n, flag);
     return hipGetLastError();
 }
 
@@ -893,6 +913,8 @@ __device__ __forceinline__ void lds_tile(const ConfocalArgs& a, uint32_t tile) {
 
 template <class Fz, int KZ, int NW>
 __global__ __launch_bounds__(NW * 64) void confocal_lds_kernel(ConfocalArgs a) {
+    // rows staged in the other format this execution: the other instantiation runs instead
+    if (a.fmt_flag && __builtin_amdgcn_readfirstlane(*a.fmt_flag) != a.fmt_want) return;
     lds_tile<Fz, KZ, NW>(a, xcd_remap(blockIdx.x, gridDim.x));
 }
 

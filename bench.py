@@ -193,8 +193,9 @@ def _plan_line(pl, N, sample_bytes, workload, reps, storage="f32", cplx=False, c
             "updates_per_s": upd / (t["per_exec_ms"] * 1e-3), "gather_updates_per_s": upd / (g * 1e-3),
             "bytes_per_update": sample_bytes,
             "effective_gather_GBps": upd * sample_bytes / (g * 1e-3) / 1e9}
+    line["row_storage"] = info["row_storage"]
     if ctx is not None:
-        prefix = device_kernel(info, storage, cplx)
+        prefix = device_kernel(info, storage if cplx else info["row_storage"], cplx)
         if prefix.startswith("confocal_lds_kernel<FusedC64Part<0>"):   # two passes, one per component
             prefix = [prefix, prefix.replace("Part<0>", "Part<1>")]
         line["roofline"] = roofline_of(ctx["counters"], prefix, g, upd * sample_bytes, ctx["fingerprint"])
@@ -421,6 +422,7 @@ def main():
     timing = plan.benchmark(0, args.steps)      # K executions, HIP events on the plan's stream
     plan.synchronize()
     elapsed = time.perf_counter() - t0
+    info = plan.info()   # after the executions: the row storage they staged
     value = updates * args.steps / elapsed
     gather_ms = timing["gather_kernel_ms"]
     sample_bytes = {"f32": 4, "fp32": 4, "f16": 2, "fp16": 2, "bf16": 2}[args.storage]
@@ -527,8 +529,9 @@ def main():
         log(f"batch: {batch}")
 
     # ---- the gather kernel against its rooflines: counters from the rocprofv3 --pmc passes of this
-    # run (else the committed ones), kernel time from this run's HIP events
-    prefix = device_kernel(info, args.storage, False)
+    # run (else the committed ones), kernel time from this run's HIP events; the row storage the
+    # executions staged (exact fp16 rows for integer samples) names the instantiation that ran
+    prefix = device_kernel(info, info["row_storage"], False)
     rl_block = roofline_of(live if live else committed, prefix, gather_ms, updates * sample_bytes, fp)
     if rl_block is not None and "valu" in rl_block:
         roof = {"bound": rl_block["bound"],
@@ -570,7 +573,7 @@ def main():
         "config": {"workload": f"confocal {args.side}x{args.side} pairs -> {N[0]}^3 voxels, "
                                f"T={args.T} bins of 1 mm (chunk {info['Tc']}), single-tap GPU path",
                    "pairs": P, "voxels": N[0] * N[1] * N[2], "chunk_bins": info["Tc"],
-                   "storage": args.storage, "kernel_path": info["path"],
+                   "storage": args.storage, "row_storage": info["row_storage"], "kernel_path": info["path"],
                    "lane_voxels": info["lane_voxels"]},
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -113,6 +113,37 @@ def test_lds_staged_kernel_bit_exact(gpu, oracle, N, size, kahan, variant, kz):
     assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("variant", [6, 7, 8])
+def test_exact_fp16_rows_for_integer_samples(gpu, oracle, variant):
+    """An fp32 plan on the LDS-staged kernel with 8 voxels per lane (variants 7, 8) stages
+    integer samples within fp16's exact range (|x| <= 2048) as fp16 rows: row_storage says so,
+    and the volume equals the fp32 rows' (keep_f32_rows) and the oracle bit for bit; the
+    16-per-lane shape (variant 6) keeps fp32 rows.  One sample that fp16 does not hold exactly
+    (a fraction, or beyond 2048, or NaN) keeps the whole execution on fp32 rows."""
+    pr = confocal_pairs(16, jitter=0.01, seed=7)
+    T, N, t0 = 3000, (35, 33, 40), 1.2
+    H = np.random.default_rng(5).integers(-2048, 2049, size=(len(pr), T)).astype(f32)
+    zero, inc = grid_basis(N, pos=(0.05, -0.02, 0.7), size=(0.25, 0.22, 0.3))
+    ref = oracle.a1_volume_f64acc(H, pr, N, zero, inc, t0, 0.001)
+
+    def run(h, keep):
+        with gpu.Plan(h, pr, N, zero, inc, t0, 0.001, variant=variant, keep_f32_rows=keep) as plan:
+            plan.execute()
+            return plan.volume(), plan.info()["row_storage"]
+
+    v16, rows = run(H, False)
+    v32, rows32 = run(H, True)
+    assert rows == ("f32" if variant == 6 else "f16") and rows32 == "f32"
+    assert np.array_equal(v16, v32) and np.array_equal(v16, ref.astype(f32))
+    for bad in (0.5, 2049.0, -3000.0, np.nan):
+        Hb = H.copy()
+        Hb[3, 1234] = bad
+        vb, rows_b = run(Hb, False)
+        vk, _ = run(Hb, True)
+        assert rows_b == "f32", bad
+        assert np.array_equal(vb, vk, equal_nan=True), bad
+
+
 @pytest.mark.parametrize("storage,cplx", [("f16", False), ("bf16", False), ("f32", True), ("f16", True),
                                           ("bf16", True)])
 def test_lds_staged_kernel_formats(gpu, storage, cplx):
