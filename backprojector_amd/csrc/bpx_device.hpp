@@ -192,6 +192,33 @@ struct FusedH16 {
         const uint2 v = *reinterpret_cast<const uint2*>(e);
         return cvt(s >= __uint_as_float(v.x) ? (v.y >> 16) : (v.y & 0xffffu));
     }
+    // acc += lds_pick(e, s) in three VALU instructions: the compare, a select that also
+    // moves the chosen half into place (SDWA word selects), and one add.  fp16: the chosen
+    // half lands in the low word and v_fma_mix_f32 adds it as h * 1 + acc (h converts to
+    // fp32 exactly, one rounding: the convert-then-add sum).  bf16: it lands in the high
+    // word over a zero low word, which is its fp32 value.  The select reads VCC two wait
+    // states after the compare writes it, as gfx950 requires of a VALU-written lane mask.
+    // (ALU-only asm: the compiler neither forms these selects here nor keeps a written
+    // fma(h, 1, acc), which it folds back into convert + add.)
+    __device__ static __forceinline__ void lds_acc(float& acc, const char* e, float s) {
+        const uint2 v = *reinterpret_cast<const uint2*>(e);
+        uint32_t h;
+        if constexpr (BF) {
+            asm("v_cmp_ge_f32_e32 vcc, %1, %2\n\t"
+                "s_nop 1\n\t"
+                "v_cndmask_b32_sdwa %0, %3, %3, vcc dst_sel:WORD_1 dst_unused:UNUSED_PAD "
+                "src0_sel:WORD_0 src1_sel:WORD_1"
+                : "=v"(h) : "v"(s), "v"(v.x), "v"(v.y) : "vcc");
+            acc = acc + __uint_as_float(h);
+        } else {
+            asm("v_cmp_ge_f32_e32 vcc, %1, %2\n\t"
+                "s_nop 1\n\t"
+                "v_cndmask_b32_sdwa %0, %3, %3, vcc dst_sel:DWORD dst_unused:UNUSED_PAD "
+                "src0_sel:DWORD src1_sel:WORD_1"
+                : "=v"(h) : "v"(s), "v"(v.x), "v"(v.y) : "vcc");
+            asm("v_fma_mix_f32 %0, %1, 1.0, %0 op_sel_hi:[1,0,0]" : "+v"(acc) : "v"(h));
+        }
+    }
 };
 struct FusedC64 {
     static constexpr int kElemBytes = 16;
@@ -256,6 +283,16 @@ struct FusedC16 {
 // ------------------------------------------------------------------ accumulators
 __device__ __forceinline__ void acc_add(float& a, float x) { a = a + x; }
 __device__ __forceinline__ void acc_add(float2& a, float2 x) { a.x = a.x + x.x; a.y = a.y + x.y; }
+// acc += Fz::lds_pick(e, s), through Fz::lds_acc where the layout has one
+template <class Fz>
+__device__ __forceinline__ auto lds_accumulate(typename Fz::acc_t& a, const char* e, float s, int)
+    -> decltype(Fz::lds_acc(a, e, s)) {
+    Fz::lds_acc(a, e, s);
+}
+template <class Fz>
+__device__ __forceinline__ void lds_accumulate(typename Fz::acc_t& a, const char* e, float s, long) {
+    acc_add(a, Fz::lds_pick(e, s));
+}
 __device__ __forceinline__ void acc_zero(float& a) { a = 0.0f; }
 __device__ __forceinline__ void acc_zero(float2& a) { a = make_float2(0.0f, 0.0f); }
 

@@ -844,17 +844,26 @@ __device__ __forceinline__ void lds_tile(const ConfocalArgs& a, uint32_t tile) {
     commit(0u);
     __syncthreads();
     // One pair of a stage: slot = its LDS segment, start = its window start (wave-uniform).
+    // The LDS byte offset of element j of a pair's window is E (j - start + slot SEG).  The
+    // update computes it without a float-to-int conversion: y = fma(sqrt(s), inv2, W) with
+    // W = w + (slot SEG - start) + M and M = 2^23 / E lies in [M, 2M), where a unit in the last
+    // place is 1/E, so the mantissa bits of y are E (y - M) and masking their low log2(E)
+    // bits leaves E floor(y - M) = E (floor(q) - start + slot SEG) for the approximate
+    // quotient q = fma(sqrt(s), inv2, w).  W and y each round to 1/E, so floor(q) comes from
+    // a quotient within 1/E of the one the host bounds (bound < 1/4, 1/E <= 1/8: total < 1/2,
+    // which keeps the element index in {bin, bin + 1}, the two the threshold pick decides
+    // between).  The window (elements start .. ) begins two elements below the lowest such
+    // index, so y - M >= slot SEG + 1 > 0.
+    constexpr uint32_t kMant = 0x7fffffu & ~(E - 1u), kM = 1u << (E == 8 ? 20 : 19);
     auto pair_step = [&](const float4 pr, uint32_t slot, uint32_t start) {
-        // byte offset of element j of this pair's window = E j + base (unsigned wrap-around);
-        // the window start is the same for every lane, so base lives in a scalar register
-        const uint32_t base = (slot * SEG - start) * E;
+        const float W = pr.w + (float)(int)(slot * SEG - start + kM);
         const float dx = pr.x - vx, dy = pr.y - vy;
         const float axy = dx * dx + dy * dy;
 #pragma unroll
         for (int r = 0; r < KZ; ++r) {
             const float s = axy + dz2[r];
-            const int j = (int)__builtin_fmaf(__builtin_amdgcn_sqrtf(s), a.inv2, pr.w);
-            acc_add(blk[r], Fz::lds_pick(segb + ((uint32_t)j * E + base), s));
+            const float y = __builtin_fmaf(__builtin_amdgcn_sqrtf(s), a.inv2, W);
+            lds_accumulate<Fz>(blk[r], segb + (__float_as_uint(y) & kMant), s, 0);
         }
         if (++in_block == a.kahan_block) {
 #pragma unroll
