@@ -92,6 +92,25 @@ __device__ __forceinline__ float buf_f32(__amdgpu_buffer_rsrc_t r, uint32_t voff
     return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
 }
 
+// Byte offsets of row elements from a biased quotient.  With E-byte elements and
+// kBias = 2^23 / E, y = q + kBias lies in [kBias, 2 kBias) for 0 <= q < kBias, where a unit in
+// the last place is 1/E: the mantissa bits of y are E (y - kBias), and masking their low
+// log2(E) bits leaves E floor(q), the offset of element floor(q) -- no float-to-int
+// conversion and no multiply.  Forming y from a quotient rounds to 1/E (twice: the bias add
+// and the fma that adds the distance term), which the callers' error budgets absorb.
+template <uint32_t E>
+struct ElemOffset {
+    static_assert(E == 8 || E == 16, "8- or 16-byte elements");
+    static constexpr float kBias = E == 8 ? 1048576.0f : 524288.0f;
+    static constexpr uint32_t kMask = 0x7fffffu & ~(E - 1u);
+    // offset of element floor(y - kBias), for kBias <= y < 2 kBias
+    __device__ static __forceinline__ uint32_t of(float y) { return __float_as_uint(y) & kMask; }
+    // offset of element clamp(floor(y - kBias), 0, top - kBias), top = kBias + an integer < kBias
+    __device__ static __forceinline__ uint32_t clamped(float y, float top) {
+        return of(__builtin_amdgcn_fmed3f(y, kBias, top));
+    }
+};
+
 // Transient sample formats.  `idx` is an element index inside a row that starts `soff`
 // bytes into the buffer.  Real formats return float, complex formats float2.
 struct SmpF32 {
@@ -155,7 +174,8 @@ struct SmpCBF16 {
 //   F16/BF16 element 8 B  (S[j], Hp[j] | Hp[j+1] << 16) load  8 B at 8j
 //   C64      element 16 B (re[j], im[j], S[j], 0)        loads 16 B at 16j and 8 B at 16(j+1)
 //   CF16/CBF16 element 16 B (S[j], c[j], c[j+1], 0)      load 12 B at 16j
-// `pick` returns the sample of bin j-1+[s >= S[j]].
+// `pick(r, off, soff, s)` returns the sample of bin j-1+[s >= S[j]] for the element j at byte
+// offset off (= kElemBytes j) of the row at soff.
 // kLdsBytes: bytes of an element as the LDS-staged kernel copies it (the leading bytes of the
 // row element; its lds_pick reads element j at e and the next element at e + kLdsBytes).
 struct FusedF32 {
@@ -163,8 +183,8 @@ struct FusedF32 {
     static constexpr int kLdsBytes = 8;
     static constexpr int kSOffset = 1;  // float slot of S inside the element
     using acc_t = float;
-    __device__ static __forceinline__ float pick(__amdgpu_buffer_rsrc_t r, uint32_t j, uint32_t soff, float s) {
-        const auto v = __builtin_amdgcn_raw_buffer_load_b96(r, (int)(j * 8u), (int)soff, 0);
+    __device__ static __forceinline__ float pick(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff, float s) {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b96(r, (int)off, (int)soff, 0);
         return s >= __uint_as_float(v[1]) ? __uint_as_float(v[2]) : __uint_as_float(v[0]);
     }
     // the same from a copy of the row in LDS; e = element j
@@ -183,8 +203,8 @@ struct FusedH16 {
     __device__ static __forceinline__ float cvt(uint32_t b) {
         return BF ? __uint_as_float(b << 16) : (float)__builtin_bit_cast(_Float16, (unsigned short)b);
     }
-    __device__ static __forceinline__ float pick(__amdgpu_buffer_rsrc_t r, uint32_t j, uint32_t soff, float s) {
-        const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)(j * 8u), (int)soff, 0);
+    __device__ static __forceinline__ float pick(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff, float s) {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, (int)soff, 0);
         const uint32_t h = s >= __uint_as_float(v[0]) ? (v[1] >> 16) : (v[1] & 0xffffu);
         return cvt(h);
     }
@@ -192,32 +212,43 @@ struct FusedH16 {
         const uint2 v = *reinterpret_cast<const uint2*>(e);
         return cvt(s >= __uint_as_float(v.x) ? (v.y >> 16) : (v.y & 0xffffu));
     }
-    // acc += lds_pick(e, s) in three VALU instructions: the compare, a select that also
-    // moves the chosen half into place (SDWA word selects), and one add.  fp16: the chosen
-    // half lands in the low word and v_fma_mix_f32 adds it as h * 1 + acc (h converts to
-    // fp32 exactly, one rounding: the convert-then-add sum).  bf16: it lands in the high
-    // word over a zero low word, which is its fp32 value.  The select reads VCC two wait
-    // states after the compare writes it, as gfx950 requires of a VALU-written lane mask.
-    // (ALU-only asm: the compiler neither forms these selects here nor keeps a written
-    // fma(h, 1, acc), which it folds back into convert + add.)
-    __device__ static __forceinline__ void lds_acc(float& acc, const char* e, float s) {
-        const uint2 v = *reinterpret_cast<const uint2*>(e);
+    // acc += the sample an element (S[j], w = Hp[j] | Hp[j+1] << 16) holds for s, in three VALU
+    // instructions: the compare, a select that also moves the chosen half into place (SDWA
+    // word selects), and one add.  fp16: the chosen half lands in the low word and
+    // v_fma_mix_f32 adds it as h * 1 + acc (h converts to fp32 exactly, one rounding: the
+    // convert-then-add sum).  bf16: it lands in the high word over a zero low word, which is
+    // its fp32 value.  The select reads VCC two wait states after the compare writes it, as
+    // gfx950 requires of a VALU-written lane mask.  (ALU-only asm: the compiler neither forms
+    // these selects here nor keeps a written fma(h, 1, acc), which it folds back into
+    // convert + add.)
+    __device__ static __forceinline__ void acc_elem(float& acc, uint32_t S, uint32_t w, float s) {
         uint32_t h;
         if constexpr (BF) {
             asm("v_cmp_ge_f32_e32 vcc, %1, %2\n\t"
                 "s_nop 1\n\t"
                 "v_cndmask_b32_sdwa %0, %3, %3, vcc dst_sel:WORD_1 dst_unused:UNUSED_PAD "
                 "src0_sel:WORD_0 src1_sel:WORD_1"
-                : "=v"(h) : "v"(s), "v"(v.x), "v"(v.y) : "vcc");
+                : "=v"(h) : "v"(s), "v"(S), "v"(w) : "vcc");
             acc = acc + __uint_as_float(h);
         } else {
             asm("v_cmp_ge_f32_e32 vcc, %1, %2\n\t"
                 "s_nop 1\n\t"
                 "v_cndmask_b32_sdwa %0, %3, %3, vcc dst_sel:DWORD dst_unused:UNUSED_PAD "
                 "src0_sel:DWORD src1_sel:WORD_1"
-                : "=v"(h) : "v"(s), "v"(v.x), "v"(v.y) : "vcc");
+                : "=v"(h) : "v"(s), "v"(S), "v"(w) : "vcc");
             asm("v_fma_mix_f32 %0, %1, 1.0, %0 op_sel_hi:[1,0,0]" : "+v"(acc) : "v"(h));
         }
+    }
+    // acc += lds_pick(e, s)
+    __device__ static __forceinline__ void lds_acc(float& acc, const char* e, float s) {
+        const uint2 v = *reinterpret_cast<const uint2*>(e);
+        acc_elem(acc, v.x, v.y, s);
+    }
+    // acc += pick(r, off, soff, s)
+    __device__ static __forceinline__ void pick_acc(float& acc, __amdgpu_buffer_rsrc_t r, uint32_t off,
+                                                    uint32_t soff, float s) {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, (int)soff, 0);
+        acc_elem(acc, v[0], v[1], s);
     }
 };
 struct FusedC64 {
@@ -225,9 +256,9 @@ struct FusedC64 {
     static constexpr int kLdsBytes = 16;
     static constexpr int kSOffset = 2;
     using acc_t = float2;
-    __device__ static __forceinline__ float2 pick(__amdgpu_buffer_rsrc_t r, uint32_t j, uint32_t soff, float s) {
-        const auto a = __builtin_amdgcn_raw_buffer_load_b128(r, (int)(j * 16u), (int)soff, 0);
-        const auto b = __builtin_amdgcn_raw_buffer_load_b64(r, (int)(j * 16u + 16u), (int)soff, 0);
+    __device__ static __forceinline__ float2 pick(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff, float s) {
+        const auto a = __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, (int)soff, 0);
+        const auto b = __builtin_amdgcn_raw_buffer_load_b64(r, (int)(off + 16u), (int)soff, 0);
         const bool up = s >= __uint_as_float(a[2]);
         return make_float2(__uint_as_float(up ? b[0] : a[0]), __uint_as_float(up ? b[1] : a[1]));
     }
@@ -267,8 +298,8 @@ struct FusedC16 {
     __device__ static __forceinline__ float cvt(uint32_t b) {
         return BF ? __uint_as_float(b << 16) : (float)__builtin_bit_cast(_Float16, (unsigned short)b);
     }
-    __device__ static __forceinline__ float2 pick(__amdgpu_buffer_rsrc_t r, uint32_t j, uint32_t soff, float s) {
-        const auto v = __builtin_amdgcn_raw_buffer_load_b96(r, (int)(j * 16u), (int)soff, 0);
+    __device__ static __forceinline__ float2 pick(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff, float s) {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b96(r, (int)off, (int)soff, 0);
         const uint32_t c = s >= __uint_as_float(v[0]) ? v[2] : v[1];
         return make_float2(cvt(c & 0xffffu), cvt(c >> 16));
     }
@@ -292,6 +323,17 @@ __device__ __forceinline__ auto lds_accumulate(typename Fz::acc_t& a, const char
 template <class Fz>
 __device__ __forceinline__ void lds_accumulate(typename Fz::acc_t& a, const char* e, float s, long) {
     acc_add(a, Fz::lds_pick(e, s));
+}
+// acc += Fz::pick(r, off, soff, s), through Fz::pick_acc where the layout has one
+template <class Fz>
+__device__ __forceinline__ auto pick_accumulate(typename Fz::acc_t& a, __amdgpu_buffer_rsrc_t r, uint32_t off,
+                                                uint32_t soff, float s, int) -> decltype(Fz::pick_acc(a, r, off, soff, s)) {
+    Fz::pick_acc(a, r, off, soff, s);
+}
+template <class Fz>
+__device__ __forceinline__ void pick_accumulate(typename Fz::acc_t& a, __amdgpu_buffer_rsrc_t r, uint32_t off,
+                                                uint32_t soff, float s, long) {
+    acc_add(a, Fz::pick(r, off, soff, s));
 }
 __device__ __forceinline__ void acc_zero(float& a) { a = 0.0f; }
 __device__ __forceinline__ void acc_zero(float2& a) { a = make_float2(0.0f, 0.0f); }

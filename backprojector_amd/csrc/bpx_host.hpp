@@ -100,6 +100,7 @@ struct PlanImpl {
     Grid g{};
     Tiles tiles{};
     bool sep = false, flat = false, fused = false;
+    bool zflat = false;                  // every wall point has the same z (lz), separable grid or not
     bool lds_tiles = false;              // the LDS-staged confocal kernel runs (16 voxels per lane)
     // fp32 rows staged as exact fp16 (FusedH16) whenever every sample of the execution is an
     // integer of magnitude <= 2048; fmt_flag (one word) = 0 then, 1 otherwise

@@ -138,6 +138,8 @@ struct OctBoundArgs {
     uint32_t chunk;         // pairs per segment (blockIdx.y); == P when not split
     float* partU;           // [segments][nblocks] partial sums when split, else nullptr
     float* partV;
+    float lz;               // wall z of every pair when flat != 0
+    uint32_t flat;
 };
 struct OctFineArgs {
     const void* E;
@@ -148,6 +150,8 @@ struct OctFineArgs {
     uint32_t P, Tc, lde, e_bytes, kahan_block;
     Grid g;
     float inv2;
+    float lz;                // wall z of every pair when flat != 0
+    uint32_t flat;
 };
 
 // ------------------------------------------------------------------ kernel launchers

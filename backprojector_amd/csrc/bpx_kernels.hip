@@ -582,6 +582,11 @@ struct SplitSource {
         const uint32_t hj = j + (s >= th ? 1u : 0u);
         return Smp::load(hr, hj, soff_h);
     }
+    // quotients are not biased: y = q, top = Tc
+    static constexpr float kBias = 0.0f;
+    __device__ __forceinline__ void accumulate(acc_t& acc, float y, float top, float s) const {
+        acc_add(acc, fetch((uint32_t)(int)__builtin_amdgcn_fmed3f(y, 0.0f, top), s));
+    }
 };
 template <class Fz>
 struct FusedSource {
@@ -590,7 +595,12 @@ struct FusedSource {
     uint32_t lde, soff = 0;
     __device__ explicit FusedSource(const ConfocalArgs& a) : er(make_rsrc(a.H, a.h_bytes)), lde(a.ldh) {}
     __device__ __forceinline__ void set_pair(uint32_t p) { soff = p * lde * (uint32_t)Fz::kElemBytes; }
-    __device__ __forceinline__ acc_t fetch(uint32_t j, float s) const { return Fz::pick(er, j, soff, s); }
+    // quotients biased for ElemOffset: y = q + kBias, top = kBias + Tc
+    using Off = ElemOffset<(uint32_t)Fz::kElemBytes>;
+    static constexpr float kBias = Off::kBias;
+    __device__ __forceinline__ void accumulate(acc_t& acc, float y, float top, float s) const {
+        pick_accumulate<Fz>(acc, er, Off::clamped(y, top), soff, s, 0);
+    }
 };
 
 // Wave quarter w of one 8 x 8 x 4K tile of a confocal capture (the body of the gather
@@ -620,7 +630,8 @@ __device__ __forceinline__ void confocal_tile(const ConfocalArgs& a, uint32_t ti
         dz2[r] = dz * dz;
     }
     Src src(a);
-    const float fTc = (float)a.Tc;
+    // sources of fused rows take quotients biased by Src::kBias (exact: integers < 2^24)
+    const float top = Src::kBias + (float)a.Tc;
     acc_t sum[K], comp[K], blk[K];
 #pragma unroll
     for (int r = 0; r < K; ++r) {
@@ -634,6 +645,7 @@ __device__ __forceinline__ void confocal_tile(const ConfocalArgs& a, uint32_t ti
         for (; p < pend; ++p) {
             const float4 pr = a.pairs[p];
             src.set_pair(p);
+            const float w = Src::kBias == 0.0f ? pr.w : pr.w + Src::kBias;
             float axy = 0.0f;
             if (SEP) {
                 const float dx = pr.x - vx[0], dy = pr.y - vy[0];
@@ -653,9 +665,7 @@ __device__ __forceinline__ void confocal_tile(const ConfocalArgs& a, uint32_t ti
                     s = sq3(pr.x - vx[r], pr.y - vy[r], pr.z - vz[r]);
                 }
                 const float d = __builtin_amdgcn_sqrtf(s);
-                const float q1 = __builtin_fmaf(d, a.inv2, pr.w);
-                const uint32_t j = (uint32_t)(int)__builtin_amdgcn_fmed3f(q1, 0.0f, fTc);
-                acc_add(blk[r], src.fetch(j, s));
+                src.accumulate(blk[r], __builtin_fmaf(d, a.inv2, w), top, s);
             }
         }
 #pragma unroll
@@ -845,25 +855,23 @@ __device__ __forceinline__ void lds_tile(const ConfocalArgs& a, uint32_t tile) {
     __syncthreads();
     // One pair of a stage: slot = its LDS segment, start = its window start (wave-uniform).
     // The LDS byte offset of element j of a pair's window is E (j - start + slot SEG).  The
-    // update computes it without a float-to-int conversion: y = fma(sqrt(s), inv2, W) with
-    // W = w + (slot SEG - start) + M and M = 2^23 / E lies in [M, 2M), where a unit in the last
-    // place is 1/E, so the mantissa bits of y are E (y - M) and masking their low log2(E)
-    // bits leaves E floor(y - M) = E (floor(q) - start + slot SEG) for the approximate
-    // quotient q = fma(sqrt(s), inv2, w).  W and y each round to 1/E, so floor(q) comes from
-    // a quotient within 1/E of the one the host bounds (bound < 1/4, 1/E <= 1/8: total < 1/2,
-    // which keeps the element index in {bin, bin + 1}, the two the threshold pick decides
-    // between).  The window (elements start .. ) begins two elements below the lowest such
-    // index, so y - M >= slot SEG + 1 > 0.
-    constexpr uint32_t kMant = 0x7fffffu & ~(E - 1u), kM = 1u << (E == 8 ? 20 : 19);
+    // update takes it from the mantissa of y = fma(sqrt(s), inv2, W), W = w + (slot SEG - start)
+    // + ElemOffset<E>::kBias: the offset of element floor(y - kBias) = floor(q) - start +
+    // slot SEG for the approximate quotient q = fma(sqrt(s), inv2, w).  W and y each round to
+    // 1/E, so floor(q) comes from a quotient within 1/E of the one the host bounds (bound
+    // < 1/4, 1/E <= 1/8: total < 1/2, which keeps the element index in {bin, bin + 1}, the two
+    // the threshold pick decides between).  The window (elements start .. ) begins two
+    // elements below the lowest such index, so y - kBias >= slot SEG + 1 > 0.
+    using Off = ElemOffset<E>;
     auto pair_step = [&](const float4 pr, uint32_t slot, uint32_t start) {
-        const float W = pr.w + (float)(int)(slot * SEG - start + kM);
+        const float W = pr.w + (float)(int)(slot * SEG - start + (uint32_t)Off::kBias);
         const float dx = pr.x - vx, dy = pr.y - vy;
         const float axy = dx * dx + dy * dy;
 #pragma unroll
         for (int r = 0; r < KZ; ++r) {
             const float s = axy + dz2[r];
             const float y = __builtin_fmaf(__builtin_amdgcn_sqrtf(s), a.inv2, W);
-            lds_accumulate<Fz>(blk[r], segb + (__float_as_uint(y) & kMant), s, 0);
+            lds_accumulate<Fz>(blk[r], segb + Off::of(y), s, 0);
         }
         if (++in_block == a.kahan_block) {
 #pragma unroll

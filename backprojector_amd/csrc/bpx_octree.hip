@@ -141,7 +141,9 @@ __device__ __forceinline__ void unpack_block(uint32_t c, uint32_t& x, uint32_t& 
 // pairs in the wave-uniform loop (pair data in SGPRs).  Coarse levels have few blocks, so the
 // pair range is split into segments (blockIdx.y) whose partial sums are added in segment
 // order by oct_bound_combine_kernel (deterministic).
-template <int K>
+// FLAT: every pair's wall point has z == a.lz, so the z terms of the squared distances are
+// per-block constants ((lz - z)^2, the same rounded value the full expression forms).
+template <int K, bool FLAT>
 __global__ __launch_bounds__(256) void oct_bound_kernel(OctBoundArgs a) {
     const uint32_t base = (blockIdx.x * blockDim.x + threadIdx.x) * K;
     const double h = 0.5 * (double)(a.B - 1u);
@@ -169,9 +171,18 @@ __global__ __launch_bounds__(256) void oct_bound_kernel(OctBoundArgs a) {
         vy[r] = v[1];
         vz[r] = v[2];
     }
+    float mz2[K], vz2[K];
+#pragma unroll
+    for (int r = 0; r < K; ++r) {
+        const float cm = a.lz - mz[r], cv = a.lz - vz[r];
+        mz2[r] = cm * cm;
+        vz2[r] = cv * cv;
+    }
     const auto mr = make_rsrc(a.M, a.m_bytes);
     const auto er = make_rsrc(a.E, a.e_bytes);
     const float fTc = (float)a.Tc;
+    using Off = ElemOffset<8>;
+    const float top = Off::kBias + fTc;
     float U[K], Vs[K];
 #pragma unroll
     for (int r = 0; r < K; ++r) U[r] = Vs[r] = 0.0f;
@@ -181,18 +192,26 @@ __global__ __launch_bounds__(256) void oct_bound_kernel(OctBoundArgs a) {
         // table row p of the level's pair set; fused sample row of the same pair in the capture
         const uint32_t moff = p * a.ldm * 4u, eoff = (a.erow ? a.erow[p] : p) * a.lde * 8u;
         const float c1R = pr.w + a.cR;
+        const float wb = pr.w + Off::kBias;
 #pragma unroll
         for (int r = 0; r < K; ++r) {
             // window start for the bounding sphere
-            const float sm = sq3(pr.x - mx[r], pr.y - my[r], pr.z - mz[r]);
+            float sm, s;
+            if (FLAT) {
+                const float mx_ = pr.x - mx[r], my_ = pr.y - my[r], vx_ = pr.x - vx[r], vy_ = pr.y - vy[r];
+                sm = (mx_ * mx_ + my_ * my_) + mz2[r];
+                s = (vx_ * vx_ + vy_ * vy_) + vz2[r];
+            } else {
+                sm = sq3(pr.x - mx[r], pr.y - my[r], pr.z - mz[r]);
+                s = sq3(pr.x - vx[r], pr.y - vy[r], pr.z - vz[r]);
+            }
             const float qlo = __builtin_fmaf(__builtin_amdgcn_sqrtf(sm), a.inv2, c1R);
             const uint32_t blo = (uint32_t)(int)__builtin_amdgcn_fmed3f(qlo, 0.0f, fTc);
             U[r] = U[r] + buf_f32(mr, blo * 4u, moff);
-            // exact tap of the representative fine voxel
-            const float s = sq3(pr.x - vx[r], pr.y - vy[r], pr.z - vz[r]);
-            const float q1 = __builtin_fmaf(__builtin_amdgcn_sqrtf(s), a.inv2, pr.w);
-            const uint32_t j = (uint32_t)(int)__builtin_amdgcn_fmed3f(q1, 0.0f, fTc);
-            Vs[r] = Vs[r] + FusedF32::pick(er, j, eoff, s);
+            // exact tap of the representative fine voxel (element offset from the biased
+            // quotient: within 1/8 of q1 = fma(sqrt(s), inv2, w), the texture path's budget)
+            const float y = __builtin_fmaf(__builtin_amdgcn_sqrtf(s), a.inv2, wb);
+            Vs[r] = Vs[r] + FusedF32::pick(er, Off::clamped(y, top), eoff, s);
         }
     }
 #pragma unroll
@@ -242,14 +261,16 @@ static uint32_t resident_groups(F kernel, uint32_t& cache) {
 
 // Each lane loops over all its pairs, so a launch takes ~ (rounds of resident work-groups) x
 // (items per lane); a last round that is only partly filled costs a whole round.  Returns the
-// items per lane among `ks` minimising rounds x items (ties: the larger).
+// items per lane among `ks` (ascending) minimising rounds x items; ties go to the smaller,
+// whose extra lanes hide more gather latency (measured: one block per lane instead of two
+// for 534K level-3 blocks x 4,096 pairs took the whole learned 256^3 octree 16.4 -> 16.0 ms).
 static int pick_items_per_lane(uint64_t n, const int* ks, const uint32_t* slots, int count) {
     int best = ks[0];
     uint64_t best_cost = ~0ull;
     for (int i = 0; i < count; ++i) {
         const uint64_t wgs = (n + 256u * (uint64_t)ks[i] - 1) / (256u * (uint64_t)ks[i]);
         const uint64_t cost = (wgs + slots[i] - 1) / slots[i] * (uint64_t)ks[i];
-        if (cost <= best_cost) {
+        if (cost < best_cost) {
             best_cost = cost;
             best = ks[i];
         }
@@ -277,16 +298,26 @@ hipError_t launch_oct_bound(const OctBoundArgs& a_in, uint32_t segments, hipStre
         a.chunk = (a.P + segments - 1u) / segments;
         segments = (a.P + a.chunk - 1u) / a.chunk;
     }
+    const bool flat = a.flat != 0u;
     if (segments > 1u) {
-        hipLaunchKernelGGL(oct_bound_kernel<1>, dim3((a.nblocks + 255u) / 256u, segments), dim3(256), 0, s, a);
+        const dim3 grid((a.nblocks + 255u) / 256u, segments);
+        if (flat) hipLaunchKernelGGL((oct_bound_kernel<1, true>), grid, dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((oct_bound_kernel<1, false>), grid, dim3(256), 0, s, a);
     } else {
         static uint32_t c1 = 0, c2 = 0;
         const int ks[2] = {1, 2};
-        const uint32_t slots[2] = {resident_groups(oct_bound_kernel<1>, c1), resident_groups(oct_bound_kernel<2>, c2)};
+        const uint32_t slots[2] = {resident_groups(oct_bound_kernel<1, true>, c1),
+                                   resident_groups(oct_bound_kernel<2, true>, c2)};
         const int K = pick_items_per_lane(a.nblocks, ks, slots, 2);
         const uint32_t lanes = (a.nblocks + K - 1u) / K;
-        if (K == 1) hipLaunchKernelGGL(oct_bound_kernel<1>, dim3((lanes + 255u) / 256u, 1u), dim3(256), 0, s, a);
-        else hipLaunchKernelGGL(oct_bound_kernel<2>, dim3((lanes + 255u) / 256u, 1u), dim3(256), 0, s, a);
+        const dim3 grid((lanes + 255u) / 256u, 1u);
+        if (K == 1) {
+            if (flat) hipLaunchKernelGGL((oct_bound_kernel<1, true>), grid, dim3(256), 0, s, a);
+            else hipLaunchKernelGGL((oct_bound_kernel<1, false>), grid, dim3(256), 0, s, a);
+        } else {
+            if (flat) hipLaunchKernelGGL((oct_bound_kernel<2, true>), grid, dim3(256), 0, s, a);
+            else hipLaunchKernelGGL((oct_bound_kernel<2, false>), grid, dim3(256), 0, s, a);
+        }
     }
     if (segments > 1u)
         hipLaunchKernelGGL(oct_bound_combine_kernel, dim3((a.nblocks + 255u) / 256u), dim3(256), 0, s, a.partU,
@@ -467,10 +498,10 @@ hipError_t launch_oct_expand(const uint32_t* blocks, uint32_t n, uint32_t B, uin
 
 // ------------------------------------------------------------------ finest level
 // Exact single-tap values of a list of fine voxels (K per lane), scattered into the volume.
-template <int K>
+template <int K, bool FLAT>
 __global__ __launch_bounds__(256) void oct_fine_kernel(OctFineArgs a) {
     const uint32_t base = (blockIdx.x * blockDim.x + threadIdx.x) * K;
-    float vx[K], vy[K], vz[K];
+    float vx[K], vy[K], vz[K], vz2[K];
     uint32_t ids[K];
 #pragma unroll
     for (int r = 0; r < K; ++r) {
@@ -487,10 +518,13 @@ __global__ __launch_bounds__(256) void oct_fine_kernel(OctFineArgs a) {
         vx[r] = v[0];
         vy[r] = v[1];
         vz[r] = v[2];
+        const float cz = a.lz - v[2];
+        vz2[r] = cz * cz;
         ids[r] = (x * a.g.n[1] + y) * a.g.n[2] + z;
     }
     const auto er = make_rsrc(a.E, a.e_bytes);
-    const float fTc = (float)a.Tc;
+    using Off = ElemOffset<8>;
+    const float top = Off::kBias + (float)a.Tc;
     float sum[K], comp[K], blk[K];
 #pragma unroll
     for (int r = 0; r < K; ++r) sum[r] = comp[r] = blk[r] = 0.0f;
@@ -500,12 +534,19 @@ __global__ __launch_bounds__(256) void oct_fine_kernel(OctFineArgs a) {
         // kUnroll pairs' loads in flight per lane; the block sums stay in pair order, so the
         // values equal the dense kernel's bit for bit
         constexpr int kUnroll = 4;
+        // (flat wall: the z term is the voxel's; element offset from the biased quotient,
+        // within 1/8 of q1 = fma(sqrt(s), inv2, w), the texture path's budget)
         auto tap = [&](uint32_t q, int r) {
             const float4 pr = a.pairs[q];
-            const float s = sq3(pr.x - vx[r], pr.y - vy[r], pr.z - vz[r]);
-            const float q1 = __builtin_fmaf(__builtin_amdgcn_sqrtf(s), a.inv2, pr.w);
-            const uint32_t j = (uint32_t)(int)__builtin_amdgcn_fmed3f(q1, 0.0f, fTc);
-            return FusedF32::pick(er, j, q * a.lde * 8u, s);
+            float s;
+            if (FLAT) {
+                const float dx = pr.x - vx[r], dy = pr.y - vy[r];
+                s = (dx * dx + dy * dy) + vz2[r];
+            } else {
+                s = sq3(pr.x - vx[r], pr.y - vy[r], pr.z - vz[r]);
+            }
+            const float y = __builtin_fmaf(__builtin_amdgcn_sqrtf(s), a.inv2, pr.w + Off::kBias);
+            return FusedF32::pick(er, Off::clamped(y, top), q * a.lde * 8u, s);
         };
         for (; p + kUnroll <= pend; p += kUnroll) {
             float t[kUnroll][K];
@@ -540,8 +581,14 @@ hipError_t launch_oct_fine(const OctFineArgs& a, hipStream_t s) {
     const int K = a.n < (1u << 22) ? 1 : 2;
     const uint32_t lanes = (a.n + K - 1) / K;
     const dim3 grid((lanes + 255u) / 256u);
-    if (K == 1) hipLaunchKernelGGL(oct_fine_kernel<1>, grid, dim3(256), 0, s, a);
-    else hipLaunchKernelGGL(oct_fine_kernel<2>, grid, dim3(256), 0, s, a);
+    const bool flat = a.flat != 0u;
+    if (K == 1) {
+        if (flat) hipLaunchKernelGGL((oct_fine_kernel<1, true>), grid, dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((oct_fine_kernel<1, false>), grid, dim3(256), 0, s, a);
+    } else {
+        if (flat) hipLaunchKernelGGL((oct_fine_kernel<2, true>), grid, dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((oct_fine_kernel<2, false>), grid, dim3(256), 0, s, a);
+    }
     return hipGetLastError();
 }
 

@@ -247,6 +247,8 @@ void execute_octree(PlanImpl& pl, hipStream_t s) {
     fine.kahan_block = pl.kahan_block;
     fine.g = pl.g;
     fine.inv2 = pl.inv2;
+    fine.lz = pl.lz;
+    fine.flat = pl.zflat ? 1u : 0u;
     for (uint32_t l = 0; l < st.L; ++l) {
         OctreeState::Level& lv = st.levels[l];
         lv.count = n;
@@ -272,6 +274,8 @@ void execute_octree(PlanImpl& pl, hipStream_t s) {
         a.B = lv.B;
         a.cR = lv.cR;
         a.inv2 = pl.inv2;
+        a.lz = pl.lz;
+        a.flat = pl.zflat ? 1u : 0u;
         a.partU = static_cast<float*>(st.bound_part_u.ptr);
         a.partV = static_cast<float*>(st.bound_part_v.ptr);
         hip_check(launch_oct_bound(a, oct_bound_segments(n, Pc, OctreeState::kBoundPartCapacity), s),

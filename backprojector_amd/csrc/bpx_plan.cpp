@@ -434,6 +434,7 @@ PlanImpl* make_plan_single_tap(const CaptureIn& c, const uint32_t n[3], const fl
             flat = flat && pr[p].laser_point[2] == pr[0].laser_point[2];
         }
         pl->flat = flat;
+        pl->zflat = flat;
         pl->lz = pr[0].laser_point[2];
         pl->sep = inc[6] == 0.0f && inc[7] == 0.0f;
         if (!pl->sep) pl->flat = false;
