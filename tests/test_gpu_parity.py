@@ -192,6 +192,45 @@ def test_lds_staged_kernel_windows_before_the_row(gpu, oracle, variant):
     assert raw.min() < 0 and raw.max() >= T
 
 
+def _tap_error_bound(pr, N, zero, inc, t0, dT, T):
+    """The plan's precondition of the exact bin lookup: 16 x 2^-24 x Q with
+    Q = (max total path + |t0|) / deltaT + T + 2, the total over pairs and grid corners."""
+    idx = np.array([[i, j, k] for i in (0, N[0] - 1) for j in (0, N[1] - 1) for k in (0, N[2] - 1)], np.float64)
+    corners = zero.astype(np.float64) + idx @ inc.reshape(3, 3).astype(np.float64)
+    lp, cp = pr["laser_point"].astype(np.float64), pr["camera_point"].astype(np.float64)
+    dl = np.linalg.norm(lp[:, None, :] - corners[None], axis=2).max(1)
+    dc = np.linalg.norm(cp[:, None, :] - corners[None], axis=2).max(1)
+    tot = (np.abs(pr["laser_wall"]) + np.abs(pr["camera_wall"]) + dl + dc).max()
+    return 16.0 * 2.0 ** -24 * ((tot + abs(t0)) / dT + T + 2.0)
+
+
+def test_biased_quotient_offsets_near_the_error_bound(gpu, oracle):
+    """The gathers take each element's offset from the mantissa of the quotient plus a bias,
+    which rounds it to 1/8 twice on top of the host-bounded error of the quotient itself.  A
+    capture whose bound sits close to its 1/4 limit (20-um voxels, 64-um bins, long paths):
+    every LDS-staged shape, with fp32 and exact fp16 rows, and the texture path, bit-exact
+    against the oracle; bins below 0 and above Tc included."""
+    pr = confocal_pairs(8, jitter=0.01, seed=11)
+    N = (24, 20, 18)
+    pitch = 2e-5
+    zero, inc = grid_basis(N, pos=(0.05, -0.02, 0.7), size=tuple(pitch * (n - 1) for n in N))
+    dT, T, t0 = 6.4e-5, 8000, 4.1
+    bound = _tap_error_bound(pr, N, zero, inc, t0, dT, T)
+    assert 0.15 < bound < 0.25, bound
+    H = coded_samples(len(pr), T, seed=9)
+    ref = oracle.a1_volume_f64acc(H, pr, N, zero, inc, t0, dT)
+    assert np.count_nonzero(ref) > ref.size // 4          # most voxels see in-range bins
+    for variant in (5, 6, 7, 8):
+        for keep in (False, True):
+            with gpu.Plan(H, pr, N, zero, inc, t0, dT, variant=variant, keep_f32_rows=keep) as plan:
+                plan.execute()
+                vol = plan.volume()
+                info = plan.info()
+            assert ("lds" in info["kernel"]) == (variant != 5), info["kernel"]
+            assert info["row_storage"] == ("f16" if variant in (7, 8) and not keep else "f32"), (variant, keep, info)
+            assert np.array_equal(vol, ref.astype(f32)), (variant, keep)
+
+
 def test_lds_staged_kernel_falls_back(gpu, oracle):
     """Variant 6 where the LDS kernel does not apply (coarse pitch: windows too long; a wall
     that is not flat) runs the fused texture-path kernel and stays exact."""
