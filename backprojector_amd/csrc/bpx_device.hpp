@@ -309,6 +309,22 @@ struct FusedC16 {
         const uint32_t c = s >= __uint_as_float(v.x) ? c1 : v.y;
         return make_float2(cvt(c & 0xffffu), cvt(c >> 16));
     }
+    // acc += lds_pick(e, s).  fp16: each half added by one mixed-precision FMA that reads it in
+    // place (h x 1 + acc: exact conversion, one rounding -- the convert-then-add sum; op_sel
+    // picks the high half for the imaginary part), instead of two conversions and two adds.
+    __device__ static __forceinline__ void lds_acc(float2& acc, const char* e, float s) {
+        if constexpr (BF) {
+            const float2 x = lds_pick(e, s);
+            acc.x = acc.x + x.x;
+            acc.y = acc.y + x.y;
+        } else {
+            const uint2 v = *reinterpret_cast<const uint2*>(e);
+            const uint32_t c1 = *reinterpret_cast<const uint32_t*>(e + 12);
+            const uint32_t c = s >= __uint_as_float(v.x) ? c1 : v.y;
+            asm("v_fma_mix_f32 %0, %1, 1.0, %0 op_sel_hi:[1,0,0]" : "+v"(acc.x) : "v"(c));
+            asm("v_fma_mix_f32 %0, %1, 1.0, %0 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "+v"(acc.y) : "v"(c));
+        }
+    }
 };
 
 // ------------------------------------------------------------------ accumulators
