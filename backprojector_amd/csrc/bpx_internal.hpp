@@ -49,7 +49,7 @@ struct ConfocalArgs {
     float scale;           // per-capture sample scale (power of two)
     float lz;              // common wall z when every pair shares it
     uint32_t kahan_block;
-    // rows whose format a device-side check of the samples decided: the kernel runs only when
+    // rows whose format is decided on the device at each execution: the kernel runs only when
     // *fmt_flag == fmt_want (nullptr: always)
     const uint32_t* fmt_flag;
     uint32_t fmt_want;

@@ -241,19 +241,6 @@ static void adopt_device_rows(PlanImpl& pl, DeviceBuffer& rows, uint32_t rows_in
     hip_check(hipStreamSynchronize(pl.stream), "hipStreamSynchronize");
 }
 
-// rows16 plans: fmt_flag = 0 when every raw sample is an integer of magnitude <= 2048 (fp16
-// holds it exactly), else 1; run whenever the raw rows change (callers change them only through
-// the plan), before any later execution.
-void check_row_format(PlanImpl& pl) {
-    if (!pl.rows16) return;
-    hip_check(launch_fill(static_cast<float*>(pl.fmt_flag.ptr), 1, 0.0f, pl.stream), "fill kernel");
-    hip_check(launch_exact16_check(static_cast<const float*>(pl.raw.ptr), (uint64_t)pl.P * pl.T,
-                                   static_cast<uint32_t*>(pl.fmt_flag.ptr), pl.stream),
-              "exact fp16 check kernel");
-    // executions may run on other streams: the flag is final before this returns
-    hip_check(hipStreamSynchronize(pl.stream), "hipStreamSynchronize");
-}
-
 void plan_replace_rows(PlanImpl& pl, DeviceBuffer& rows) {
     if (pl.path == kPathOctree) fail(BPX_ERR_UNSUPPORTED, "octree plans keep their rows");
     if (pl.P == 0 || !pl.raw.ptr) return;   // nothing staged: every execution writes zeros
@@ -265,7 +252,6 @@ void plan_replace_rows(PlanImpl& pl, DeviceBuffer& rows) {
         const StageJob job = stage_job(pl);
         hip_check(hipMemcpy(pl.stage.ptr, &job, sizeof(StageJob), hipMemcpyHostToDevice), "hipMemcpy(staging job)");
     }
-    check_row_format(pl);
 }
 
 static void stage_raw(PlanImpl& pl, const CaptureIn& c, const std::vector<uint32_t>& idx) {
@@ -473,9 +459,8 @@ PlanImpl* make_plan_single_tap(const CaptureIn& c, const uint32_t n[3], const fl
                         confocal_lds_fits(inc, pl->inv2, lds_shape(pl->variant, pl->format).depth(), seg);
         // fp32 rows of integer samples within fp16's exact range are staged as fp16: the same
         // values, half the LDS bytes per lookup (one 8-B read instead of an 8-B and a 4-B one).
-        // The samples are checked on the device whenever the plan's rows are set (here and in
-        // plan_replace_rows); every execution stages and gathers in the format the check
-        // allows, so the volume is the fp32 rows' bit for bit either way.  For
+        // Each execution checks its samples on the device first and stages and gathers in the
+        // format the check allows, so the volume is the fp32 rows' bit for bit either way.  For
         // the shapes of 8 voxels per lane (gather ms at 16,384 pairs, fp32 -> fp16 rows: 128^3
         // 16^3 x 512 lanes 12.96 -> 11.67, 16 x 16 x 8 x 256 14.57 -> 13.92; 256^3 83.3 -> 83.2,
         // 91.7 -> 89.7); the 16-per-lane shape, bound by its arithmetic and the default where
@@ -484,7 +469,7 @@ PlanImpl* make_plan_single_tap(const CaptureIn& c, const uint32_t n[3], const fl
                      !(o.flags & BPX_FLAG_KEEP_F32_ROWS);
         if (pl->rows16) {
             pl->fmt_flag.alloc(4);
-            check_row_format(*pl);
+            hip_check(hipMemset(pl->fmt_flag.ptr, 0xff, 4), "hipMemset(row format)");   // fp32 until run
             const StageJob job = stage_job(*pl);
             hip_check(hipMemcpy(pl->stage.ptr, &job, sizeof(StageJob), hipMemcpyHostToDevice),
                       "hipMemcpy(staging job)");
@@ -608,6 +593,12 @@ void PlanImpl::execute(hipStream_t s) {
     hip_check(hipEventRecord(ev[1], s), "hipEventRecord");
     if (path == kPathConfocal) {
         if (fused) {
+            if (rows16) {
+                hip_check(launch_fill(static_cast<float*>(fmt_flag.ptr), 1, 0.0f, s), "fill kernel");
+                hip_check(launch_exact16_check(static_cast<const float*>(raw.ptr), (uint64_t)P * T,
+                                               static_cast<uint32_t*>(fmt_flag.ptr), s),
+                          "exact fp16 check kernel");
+            }
             stage_rows(*this, s);
         } else {
             hip_check(launch_ztable(0, t0, dT, 0.0f, T + 1, static_cast<float*>(Z.ptr), s),

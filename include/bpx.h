@@ -94,11 +94,10 @@ typedef struct bpx_options {
 /* bpx_batch_create: every capture's transient_chunk holds interleaved complex samples
  * (transient_size counts complex elements) and every volume is complex (2 floats per voxel). */
 #define BPX_FLAG_COMPLEX_SAMPLES 0x4u
-/* F32 storage on the LDS-staged confocal kernel: when every sample of the plan is an integer
- * of magnitude <= 2048 (photon counts, as captures hold), the rows are staged as fp16, which
- * holds those values exactly, and the gather reads half the LDS bytes per lookup; the volume is
- * the fp32 rows' bit for bit.  The samples are checked on the device when the plan's rows are
- * set (creation, and the row replacement of multi-wavelength phasor calls).
+/* F32 storage on the LDS-staged confocal kernel: when every sample of an execution is an
+ * integer of magnitude <= 2048 (photon counts, as captures hold), the rows are staged as fp16,
+ * which holds those values exactly, and the gather reads half the LDS bytes per lookup; the
+ * volume is the fp32 rows' bit for bit.  Each execution checks its samples on the device.
  * This flag keeps fp32 rows always (no reference counterpart: an internal layout choice). */
 #define BPX_FLAG_KEEP_F32_ROWS 0x8u
 
@@ -362,9 +361,9 @@ bpx_status bpx_plan_info(bpx_plan plan, int32_t* path, int32_t* lane_voxels, uin
  * profiles and reports), as a NUL-terminated string of at most `capacity` bytes.  No reference
  * counterpart (the reference has one kernel per path). */
 bpx_status bpx_plan_kernel_name(bpx_plan plan, char* out, uint64_t capacity);
-/* The row storage (bpx_storage) the plan's executions stage its samples in: F16 when an F32
- * plan's samples are all exact in fp16 (see BPX_FLAG_KEEP_F32_ROWS), else the plan's own.
- * Waits for the last execution.  No reference counterpart. */
+/* The row storage (bpx_storage) the plan's last execution staged its samples in: F16 when an
+ * F32 plan's samples were all exact in fp16 (see BPX_FLAG_KEEP_F32_ROWS), else the plan's own.
+ * Waits for that execution.  No reference counterpart. */
 bpx_status bpx_plan_row_storage(bpx_plan plan, uint32_t* storage);
 
 /* Copies the threshold table of a confocal plan's last execution to host: out[p][j],
