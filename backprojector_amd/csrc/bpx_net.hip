@@ -30,34 +30,38 @@ __device__ __forceinline__ float from_bf16(uint16_t b) { return __uint_as_float(
 
 // ------------------------------------------------------------------ forward / data gradient
 // act: 0 = none, 1 = ReLU.  Writes bf16 (yb) and/or fp32 (yf) outputs for couts < cout_valid.
+// The work-group copies the layer's weights (27 x 16 x 16 bf16, 13.5 KB) to LDS once; each
+// wave then issues its 27 input loads before the 27 MFMAs (same taps, same order, same
+// operands as a tap-by-tap loop: the sums are unchanged).
 __global__ __launch_bounds__(256) void conv3_fwd_kernel(const uint16_t* __restrict__ X,
                                                         const uint16_t* __restrict__ W,
                                                         const float* __restrict__ bias,
                                                         uint16_t* __restrict__ yb, float* __restrict__ yf,
                                                         uint32_t n0, uint32_t n1, uint32_t n2,
                                                         int act, uint32_t cout_valid) {
+    __shared__ v4s ws[27 * 16 * 4];   // [tap][cout][q]: cin 4q .. 4q + 3
+    const v4s* Wv = reinterpret_cast<const v4s*>(W);
+    for (uint32_t i = threadIdx.x; i < 27u * 16u * 4u; i += 256u) ws[i] = Wv[i];
+    __syncthreads();
     const uint32_t nzt = (n2 + 15u) / 16u;
     const uint32_t wid = blockIdx.x * 4u + (threadIdx.x >> 6);
     if (wid >= n0 * n1 * nzt) return;
     const uint32_t zt = wid % nzt, y = (wid / nzt) % n1, x = wid / (nzt * n1);
     const uint32_t l = threadIdx.x & 63u, q = l >> 4, r = l & 15u;
     const uint32_t z0 = zt * 16u;
-    v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
-    for (int dx = -1; dx <= 1; ++dx) {
-        const int xs = (int)x + dx;
-        for (int dy = -1; dy <= 1; ++dy) {
-            const int ys = (int)y + dy;
-            for (int dz = -1; dz <= 1; ++dz) {
-                const int t = (dx + 1) * 9 + (dy + 1) * 3 + (dz + 1);
-                const int zs = (int)(z0 + r) + dz;
-                v4s a = {0, 0, 0, 0};
-                if (xs >= 0 && xs < (int)n0 && ys >= 0 && ys < (int)n1 && zs >= 0 && zs < (int)n2)
-                    a = *reinterpret_cast<const v4s*>(X + (((size_t)xs * n1 + ys) * n2 + zs) * 16u + 4u * q);
-                const v4s b = *reinterpret_cast<const v4s*>(W + ((size_t)t * 16u + r) * 16u + 4u * q);
-                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
-            }
-        }
+    v4s a[27];
+#pragma unroll
+    for (int t = 0; t < 27; ++t) {
+        const int dx = t / 9 - 1, dy = (t / 3) % 3 - 1, dz = t % 3 - 1;
+        const int xs = (int)x + dx, ys = (int)y + dy, zs = (int)(z0 + r) + dz;
+        a[t] = v4s{0, 0, 0, 0};
+        if (xs >= 0 && xs < (int)n0 && ys >= 0 && ys < (int)n1 && zs >= 0 && zs < (int)n2)
+            a[t] = *reinterpret_cast<const v4s*>(X + (((size_t)xs * n1 + ys) * n2 + zs) * 16u + 4u * q);
     }
+    v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int t = 0; t < 27; ++t)
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[t], ws[(t * 16 + (int)r) * 4 + (int)q], acc, 0, 0, 0);
     const uint32_t cout = r;
     const bool valid = cout < cout_valid;
     const float bv = (bias && valid) ? bias[cout] : 0.0f;
