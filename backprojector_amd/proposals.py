@@ -48,7 +48,14 @@ CANDIDATES = [0.2, 0.3,
               {"p": 0.3, "stride": 2, "fallback": [0, 0, 0.1, 0]},
               {"p": 0.3, "stride": 2, "fallback": [0, 0, 0.15, 0]},
               {"p": [0.1, 0.15, 0.2, 0.3], "stride": 2, "fallback": [0, 0, 0.1, 0]},
-              {"p": [0.1, 0.2, 0.3, 0.25], "stride": 2, "fallback": [0, 0.1, 0.15, 0]}]
+              {"p": [0.1, 0.2, 0.3, 0.25], "stride": 2, "fallback": [0, 0.1, 0.15, 0]},
+              # narrower bands just below the proposal threshold
+              {"p": 0.2, "stride": 2, "fallback": [0, 0, 0.15, 0]},
+              {"p": 0.2, "stride": 2, "fallback": [0, 0, 0.17, 0]},
+              {"p": 0.22, "stride": 2, "fallback": [0, 0, 0.15, 0]},
+              {"p": 0.22, "stride": 2, "fallback": [0, 0, 0.18, 0]},
+              {"p": 0.25, "stride": 2, "fallback": [0, 0, 0.15, 0]},
+              {"p": 0.25, "stride": 2, "fallback": [0, 0, 0.2, 0]}]
 
 
 def log(*a):

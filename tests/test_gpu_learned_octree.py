@@ -10,7 +10,7 @@ pytestmark = pytest.mark.gpu
 
 
 # the benchmark capture and held-out test captures: 60004 is the shipped operating point's worst
-# (rel. L2 1.91 % over occupied voxels), 60016 the worst under the previous weights
+# (rel. L2 1.60 % over occupied voxels), 60016 the worst under earlier weights
 @pytest.mark.parametrize("seed", [1000, 60004, 60009, 60016])
 def test_learned_proposals_gain(gpu, seed):
     cap = scenes.metric_capture(seed=seed)
