@@ -304,8 +304,25 @@ bpx_status bpx_plan_read_thresholds(bpx_plan plan, float* out, uint64_t capacity
         const size_t off = pl.fused ? (size_t)fused_s_offset(fmt) : 0;
         std::vector<float> host(src.bytes / 4);
         hip_check(hipMemcpy(host.data(), src.ptr, host.size() * 4, hipMemcpyDeviceToHost), "hipMemcpy(thresholds)");
+        // rows staged over windows: the entries outside them come from the same search, run
+        // here over the whole table
+        std::vector<float> full;
+        if (pl.fused && !pl.win_host.empty()) {
+            DeviceBuffer tmp;
+            tmp.alloc(sizeof(float) * n);
+            hip_check(launch_stable(static_cast<const float*>(pl.pair8.ptr), pl.P, pl.T,
+                                    static_cast<const float*>(pl.Z.ptr), static_cast<float*>(tmp.ptr), pl.T + 1u, 1u,
+                                    0u, pl.stream),
+                      "threshold table kernel");
+            hip_check(hipStreamSynchronize(pl.stream), "hipStreamSynchronize");
+            full.resize(n);
+            hip_check(hipMemcpy(full.data(), tmp.ptr, sizeof(float) * n, hipMemcpyDeviceToHost), "hipMemcpy(thresholds)");
+        }
         for (uint64_t p = 0; p < pl.P; ++p)
-            for (uint64_t j = 0; j <= pl.T; ++j) out[p * (pl.T + 1u) + j] = host[p * pitch + j * stride + off];
+            for (uint64_t j = 0; j <= pl.T; ++j) {
+                const bool staged = full.empty() || (j >= pl.win_host[2 * p] && j < pl.win_host[2 * p + 1]);
+                out[p * (pl.T + 1u) + j] = staged ? host[p * pitch + j * stride + off] : full[p * (pl.T + 1u) + j];
+            }
     });
 }
 

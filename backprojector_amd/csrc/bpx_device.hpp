@@ -83,6 +83,27 @@ __device__ uint32_t min_true_key(const Pred& pred, uint32_t kguess, uint32_t klo
     return hi_true;
 }
 
+// The same search for a guess usually within 8 keys of the answer, in straight-line code: the
+// predicate at kguess - 8 (false) and kguess + 8 (true) brackets the answer, and four fixed
+// bisection steps find it (6 evaluations, no loop, no divergent branch).  Lanes whose answer
+// lies outside the bracket, or whose bracket leaves [klo, khi], take min_true_key.
+template <class Pred>
+__device__ __forceinline__ uint32_t min_true_key_near(const Pred& pred, uint32_t kguess, uint32_t klo, uint32_t khi) {
+    kguess = kguess < klo ? klo : (kguess > khi ? khi : kguess);
+    const bool inside = kguess - klo >= 8u && khi - kguess >= 8u;
+    uint32_t lo = inside ? kguess - 8u : klo, hi = inside ? kguess + 8u : khi;
+    const bool bracketed = inside && !pred(lo) && pred(hi);
+#pragma unroll
+    for (int step = 0; step < 4; ++step) {
+        const uint32_t mid = lo + ((hi - lo) >> 1);
+        const bool t = pred(mid);
+        hi = t ? mid : hi;
+        lo = t ? lo : mid;
+    }
+    if (!bracketed) hi = min_true_key(pred, kguess, klo, khi);
+    return hi;
+}
+
 // ------------------------------------------------------------------ buffer resources
 // Raw buffer descriptor (gfx9 dword3 = 0x00020000: 32-bit data format, no swizzle).
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* base, uint32_t bytes) {

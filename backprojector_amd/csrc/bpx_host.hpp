@@ -114,6 +114,10 @@ struct PlanImpl {
     bool all_confocal = false;
     DeviceBuffer raw, H, S, Z, Z2, pair8, pair4, lasers, spads, vol;
     DeviceBuffer stage;                  // one StageJob (fused confocal layouts)
+    // per-pair element windows [lo, hi) of the fused rows that the gathers can read (plain
+    // confocal plans; empty = every element staged exactly), device copy and host copy
+    DeviceBuffer win;
+    std::vector<uint32_t> win_host;
     std::vector<uint32_t> used;          // source rows of the staged pairs (dropped pairs skipped)
     uint32_t rows_in = 0;                // rows of the capture the plan was made from
     uint32_t stage_zb = 0, stage_sb = 0; // its blocks in the two staging passes

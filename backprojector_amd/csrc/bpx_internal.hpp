@@ -58,7 +58,8 @@ struct ConfocalArgs {
 // Factorised exhaustive-grid kernel parameters (threshold table in path-length space).
 // Batched confocal captures: one persistent launch over a queue of staging, LDS-staged and
 // texture-path items (batch_queue_kernel).
-constexpr uint32_t kBatchStageBlocks = 256;  // 256-element fused-row blocks per staging item
+constexpr uint32_t kStageSpan = 1024;        // fused-row elements per staging chunk (4 per lane)
+constexpr uint32_t kBatchStageBlocks = 32;   // fused rows per staging item
 enum : uint32_t { kBatchItemStage = 0, kBatchItemLds = 1, kBatchItemTex = 2 };
 struct StageJob;
 struct BatchQueueArgs {
@@ -217,15 +218,19 @@ hipError_t launch_stable(const float* pair8, uint32_t P, uint32_t Tc, const floa
                          hipStream_t stream);
 
 // One confocal capture's staging into a fused sample+threshold layout: Z[0..Tc] first, then
-// every row element (scaled samples and the pair's threshold slot) written once.  A launch
-// serves any number of jobs; each job owns a contiguous range of 256-thread blocks in each of
-// the two passes (z_first / s_first, ascending in job order).
+// every row element of the row's window (scaled samples and the pair's threshold slot)
+// written once.  A launch serves any number of jobs; each job owns a contiguous range of
+// blocks in each of the two passes (z_first: 256 table entries each; s_first: one row each),
+// ascending in job order.
 struct StageJob {
     const float* src;     // raw samples [rows][Tc] (interleaved re, im for complex formats)
     void* dst;            // fused rows [rows][lde] elements
     const float* pair8;   // pair table, 8 floats per pair
     float* Z;             // [Tc + 1] path-length thresholds (written by the first pass)
     const uint32_t* fmt_flag;   // non-null: rows written in alt_format while *fmt_flag == 0
+    // non-null: [rows][2] element windows [lo, hi) of each row that the gathers can read;
+    // only those elements are staged (the others keep the zeros the plan wrote once)
+    const uint32_t* win;
     uint32_t rows, Tc, lde;
     int32_t format;
     float inv_scale, t0, dT;

@@ -42,8 +42,10 @@ struct SqrtPred {
     __device__ bool operator()(uint32_t key) const { return exact_sqrt(fkey_inv(key)) >= d; }
 };
 
-// The first search starts from a guess in double, d ~ (z - lw - cw) / 2; a good guess leaves
-// the gallop two predicate evaluations and the search keeps the result exact whatever the guess.
+// The first search starts from a guess in double, d ~ (z - lw - cw) / 2, usually within a few
+// floats of d* (three fp32 additions round total by at most ~1.5 ulp of total), so the
+// straight-line bracket of 8 keys either side settles it; the general search keeps the result
+// exact for the lanes where it does not (very small d against lw + cw).
 //
 // The second needs no search for d* in [2^-60, 2^60]: with d- the float below d* and
 // m = (d- + d*) / 2 (exact in double: 26 significant bits), sqrt_rn(s) >= d* iff sqrt(s) > m, or
@@ -51,25 +53,22 @@ struct SqrtPred {
 // has at most 52 significant bits, so it is exact in double, and sqrt(s) > m iff s > m^2.  The
 // smallest such float is m^2 rounded up to a float, one float higher when m^2 is itself a float
 // and the tie goes to d- (then sqrt_rn(m^2) = d-).  Outside that range (0, tiny, huge or
-// infinite d*) the search below runs from the same guess.
+// infinite d*) the general search over s runs instead.
 __device__ __forceinline__ float stable_threshold(float lw, float cw, float z) {
     const double gd = 0.5 * (((double)z - (double)lw) - (double)cw);
     const float g = gd > 0.0 ? (float)gd : 0.0f;
-    const float dstar = fkey_inv(min_true_key(ChainPred{lw, cw, z}, fkey(g), fkey(0.0f), fkey(__builtin_inff())));
-    float sg = dstar * dstar;
-    if (dstar > 0.0f && dstar < __builtin_inff()) {
-        // neighbours of non-negative finite floats by their bit patterns
-        const float below = __uint_as_float(__float_as_uint(dstar) - 1u);
-        const double m = 0.5 * ((double)dstar + (double)below);
-        const double msq = m * m;
-        sg = (float)msq;
-        if ((double)sg < msq) sg = __uint_as_float(__float_as_uint(sg) + 1u);
-        if (dstar >= 0x1p-60f && dstar <= 0x1p60f) {
-            if ((double)sg == msq && (__float_as_uint(dstar) & 1u)) sg = __uint_as_float(__float_as_uint(sg) + 1u);
-            return sg;
-        }
-    }
-    return fkey_inv(min_true_key(SqrtPred{dstar}, fkey(sg), fkey(0.0f), fkey(__builtin_inff())));
+    const float dstar =
+        fkey_inv(min_true_key_near(ChainPred{lw, cw, z}, fkey(g), fkey(0.0f), fkey(__builtin_inff())));
+    // neighbours of non-negative finite floats by their bit patterns
+    const float below = __uint_as_float(__float_as_uint(dstar) - 1u);
+    const double m = 0.5 * ((double)dstar + (double)below);
+    const double msq = m * m;
+    float sg = (float)msq;
+    sg = (double)sg < msq ? __uint_as_float(__float_as_uint(sg) + 1u) : sg;
+    sg = (double)sg == msq && (__float_as_uint(dstar) & 1u) ? __uint_as_float(__float_as_uint(sg) + 1u) : sg;
+    if (!(dstar >= 0x1p-60f && dstar <= 0x1p60f))   // 0, tiny, huge or infinite d*
+        sg = fkey_inv(min_true_key(SqrtPred{dstar}, fkey(dstar * dstar), fkey(0.0f), fkey(__builtin_inff())));
+    return sg;
 }
 
 // S[p][j] is written at float index p*pitch + j*stride + offset (stride 1, offset 0 for a
@@ -186,37 +185,35 @@ __device__ __forceinline__ float padded_sample(const float* row, uint32_t i, uin
     return cplx ? row[2 * (i - 1u) + comp] * inv_scale : row[i - 1u] * inv_scale;
 }
 
-// Element e of a fused row: the sample slot(s) of padded index j and the threshold slot S.
-__device__ __forceinline__ void store_fused(void* dst, uint64_t e, int format, const float* row, uint32_t j,
-                                            uint32_t Tc, float inv_scale, float S) {
+// The sample slots of element j of a fused row of format F, before their conversion: v[0]
+// (the real sample, or re), v[1] (im, or the next padded index's sample for the 16-bit real
+// formats), v[2] and v[3] (re and im of the next padded index for the 16-bit complex formats).
+template <int F>
+__device__ __forceinline__ void fused_samples(const float* row, uint32_t j, uint32_t Tc, float inv_scale, float v[4]) {
+    constexpr int cplx = (F == kC64 || F == kCF16 || F == kCBF16) ? 1 : 0;
+    constexpr bool two = F != kF32 && F != kC64;   // 16-bit layouts hold two padded indices
+    v[0] = padded_sample(row, j, Tc, inv_scale, 0, cplx);
+    v[1] = cplx ? padded_sample(row, j, Tc, inv_scale, 1, 1) : two ? padded_sample(row, j + 1u, Tc, inv_scale, 0, 0) : 0.0f;
+    v[2] = cplx && two ? padded_sample(row, j + 1u, Tc, inv_scale, 0, 1) : 0.0f;
+    v[3] = cplx && two ? padded_sample(row, j + 1u, Tc, inv_scale, 1, 1) : 0.0f;
+}
+
+// Element e of a fused row of format F from its sample slots (fused_samples) and the
+// threshold slot S.
+template <int F>
+__device__ __forceinline__ void write_fused(void* dst, uint64_t e, const float v[4], float S) {
     const uint32_t sb = __float_as_uint(S);
-    switch (format) {
-        case kF32:
-            static_cast<float2*>(dst)[e] = make_float2(padded_sample(row, j, Tc, inv_scale, 0, 0), S);
-            break;
-        case kF16:
-        case kBF16: {
-            const float a = padded_sample(row, j, Tc, inv_scale, 0, 0);
-            const float b = padded_sample(row, j + 1u, Tc, inv_scale, 0, 0);
-            const uint32_t ha = format == kF16 ? f32_to_f16(a) : f32_to_bf16_rne(a);
-            const uint32_t hb = format == kF16 ? f32_to_f16(b) : f32_to_bf16_rne(b);
-            static_cast<uint2*>(dst)[e] = make_uint2(sb, ha | (hb << 16));
-            break;
-        }
-        case kC64:
-            static_cast<float4*>(dst)[e] = make_float4(padded_sample(row, j, Tc, inv_scale, 0, 1),
-                                                       padded_sample(row, j, Tc, inv_scale, 1, 1), S, 0.0f);
-            break;
-        default: {
-            const bool bf = format == kCBF16;
-            auto q = [&](float x) -> uint32_t { return bf ? f32_to_bf16_rne(x) : f32_to_f16(x); };
-            const uint32_t c0 = q(padded_sample(row, j, Tc, inv_scale, 0, 1)) |
-                                (q(padded_sample(row, j, Tc, inv_scale, 1, 1)) << 16);
-            const uint32_t c1 = q(padded_sample(row, j + 1u, Tc, inv_scale, 0, 1)) |
-                                (q(padded_sample(row, j + 1u, Tc, inv_scale, 1, 1)) << 16);
-            static_cast<uint4*>(dst)[e] = make_uint4(sb, c0, c1, 0u);
-            break;
-        }
+    if constexpr (F == kF32) {
+        static_cast<float2*>(dst)[e] = make_float2(v[0], S);
+    } else if constexpr (F == kF16 || F == kBF16) {
+        const uint32_t ha = F == kF16 ? f32_to_f16(v[0]) : f32_to_bf16_rne(v[0]);
+        const uint32_t hb = F == kF16 ? f32_to_f16(v[1]) : f32_to_bf16_rne(v[1]);
+        static_cast<uint2*>(dst)[e] = make_uint2(sb, ha | (hb << 16));
+    } else if constexpr (F == kC64) {
+        static_cast<float4*>(dst)[e] = make_float4(v[0], v[1], S, 0.0f);
+    } else {
+        auto q = [](float x) -> uint32_t { return F == kCBF16 ? f32_to_bf16_rne(x) : f32_to_f16(x); };
+        static_cast<uint4*>(dst)[e] = make_uint4(sb, q(v[0]) | (q(v[1]) << 16), q(v[2]) | (q(v[3]) << 16), 0u);
     }
 }
 
@@ -241,19 +238,53 @@ __device__ __forceinline__ void stage_z_entry(const StageJob& jb, uint32_t j) {
     jb.Z[j] = fkey_inv(min_true_key(pred, fkey(g), fkey(-__builtin_inff()), fkey(__builtin_inff())));
 }
 
-// Lane threadIdx.x of row block `local` of a job (256 elements of one fused row): the row and
-// segment come from block-uniform values (no per-lane division).
-__device__ __forceinline__ void stage_row_block(const StageJob& jb, uint32_t local) {
-    const uint32_t per_row = (jb.lde + 255u) / 256u;
-    const uint32_t p = local / per_row, j = (local - p * per_row) * 256u + threadIdx.x;
-    if (j >= jb.lde) return;
-    const int format = jb.fmt_flag && *jb.fmt_flag == 0u ? jb.alt_format : jb.format;
-    const uint64_t e = (uint64_t)p * jb.lde + j;
-    const int cplx = format_complex(jb.format) ? 1 : 0;
+// Row p of a job: the elements of its window (StageJob::win; the whole row without one) in
+// chunks of kStageSpan, element base + threadIdx.x + 256 k of a chunk for k < kStageSpan / 256.
+// The row, window and format come from block-uniform values (one body per format without
+// format branches), and every lane issues the loads of all its elements of a chunk before the
+// first threshold search.  Elements outside the window are never written: the plan clears
+// its rows once when it is made, and no gather reads them.
+template <int F>
+__device__ __forceinline__ void stage_row_elements(const StageJob& jb, uint32_t p) {
+    constexpr int kE = (int)(kStageSpan / 256u);
+    constexpr bool cplx = F == kC64 || F == kCF16 || F == kCBF16;
     const float* row = jb.src + (size_t)p * jb.Tc * (cplx ? 2u : 1u);
-    float S = 0.0f;
-    if (j <= jb.Tc) S = stable_threshold(jb.pair8[8u * p + 7u], jb.pair8[8u * p + 6u], jb.Z[j]);
-    store_fused(jb.dst, e, format, row, j, jb.Tc, jb.inv_scale, S);
+    const float lw = jb.pair8[8u * p + 7u], cw = jb.pair8[8u * p + 6u];
+    uint32_t lo = 0u, hi = jb.lde;
+    if (jb.win) {
+        lo = jb.win[2u * p];
+        hi = jb.win[2u * p + 1u];
+    }
+    for (uint32_t base = lo; base < hi; base += kStageSpan) {
+        float v[kE][4], z[kE];
+#pragma unroll
+        for (int k = 0; k < kE; ++k) {
+            const uint32_t j = base + threadIdx.x + 256u * (uint32_t)k;
+            fused_samples<F>(row, j, j < hi ? jb.Tc : 0u, jb.inv_scale, v[k]);
+            z[k] = j < hi && j <= jb.Tc ? jb.Z[j] : 0.0f;
+        }
+#pragma unroll
+        for (int k = 0; k < kE; ++k) {
+            const uint32_t j = base + threadIdx.x + 256u * (uint32_t)k;
+            if (j < hi) {
+                const float S = j <= jb.Tc ? stable_threshold(lw, cw, z[k]) : 0.0f;
+                write_fused<F>(jb.dst, (uint64_t)p * jb.lde + j, v[k], S);
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ void stage_row_block(const StageJob& jb, uint32_t p) {
+    // rows of a plan whose samples are checked per execution: fp16 while the check allows it
+    const int format = __builtin_amdgcn_readfirstlane(jb.fmt_flag && *jb.fmt_flag == 0u ? jb.alt_format : jb.format);
+    switch (format) {
+        case kF32: stage_row_elements<kF32>(jb, p); break;
+        case kF16: stage_row_elements<kF16>(jb, p); break;
+        case kBF16: stage_row_elements<kBF16>(jb, p); break;
+        case kC64: stage_row_elements<kC64>(jb, p); break;
+        case kCF16: stage_row_elements<kCF16>(jb, p); break;
+        default: stage_row_elements<kCBF16>(jb, p); break;
+    }
 }
 
 __global__ __launch_bounds__(256) void stage_z_kernel(const StageJob* __restrict__ jobs, uint32_t n_jobs,
@@ -265,11 +296,21 @@ __global__ __launch_bounds__(256) void stage_z_kernel(const StageJob* __restrict
     }
 }
 
+// Each work-group stages runs of kStageRun consecutive rows (row blocks of the pass, numbered
+// over all jobs): one job search per run (the chain of dependent job-table loads is paid once),
+// then the next job's first row is checked as the run crosses into it (jobs with no rows are
+// stepped over, as in the search).
+constexpr uint32_t kStageRun = 2;
 __global__ __launch_bounds__(256) void stage_rows_kernel(const StageJob* __restrict__ jobs, uint32_t n_jobs,
                                                          uint32_t blocks) {
-    for (uint32_t blk = blockIdx.x; blk < blocks; blk += gridDim.x) {
-        const StageJob& jb = jobs[stage_job_of<false>(jobs, n_jobs, blk)];
-        stage_row_block(jb, blk - jb.s_first);
+    for (uint32_t run = blockIdx.x * kStageRun; run < blocks; run += gridDim.x * kStageRun) {
+        uint32_t ji = stage_job_of<false>(jobs, n_jobs, run);
+        const uint32_t end = min(run + kStageRun, blocks);
+        for (uint32_t blk = run; blk < end; ++blk) {
+            while (ji + 1u < n_jobs && jobs[ji + 1u].s_first <= blk) ++ji;
+            const StageJob& jb = jobs[ji];
+            stage_row_block(jb, blk - jb.s_first);
+        }
     }
 }
 
@@ -298,7 +339,7 @@ hipError_t launch_crop_rows(const float* src, uint32_t src_T, const uint32_t* ma
 
 uint32_t stage_z_blocks(const StageJob& j) { return (j.Tc + 1u + 255u) / 256u; }
 uint32_t stage_row_blocks(const StageJob& j) {
-    return (uint32_t)((uint64_t)j.rows * ((j.lde + 255u) / 256u));   // whole blocks per row
+    return j.rows;   // one row block per row
 }
 
 hipError_t launch_stage(const StageJob* jobs, uint32_t n_jobs, uint32_t z_blocks, uint32_t row_blocks,
@@ -309,8 +350,8 @@ hipError_t launch_stage(const StageJob* jobs, uint32_t n_jobs, uint32_t z_blocks
         hipLaunchKernelGGL(stage_z_kernel, dim3(std::min(z_blocks, kMaxGrid)), dim3(256), 0, stream, jobs, n_jobs,
                            z_blocks);
     if (row_blocks)
-        hipLaunchKernelGGL(stage_rows_kernel, dim3(std::min(row_blocks, kMaxGrid)), dim3(256), 0, stream, jobs,
-                           n_jobs, row_blocks);
+        hipLaunchKernelGGL(stage_rows_kernel, dim3(std::min((row_blocks + kStageRun - 1u) / kStageRun, kMaxGrid)),
+                           dim3(256), 0, stream, jobs, n_jobs, row_blocks);
     return hipGetLastError();
 }
 
@@ -937,7 +978,7 @@ __global__ __launch_bounds__(NW * 64) void confocal_lds_kernel(ConfocalArgs a) {
 
 // ------------------------------------------------------------------- batched captures
 // One persistent launch serves every item of a batch of confocal captures from a single queue:
-//   stage   -- kBatchStageBlocks row blocks (256 fused-row elements each) of one capture
+//   stage   -- kBatchStageBlocks fused rows of one capture
 //   LDS     -- one 16 x 16 x KZ tile of a capture whose bin windows fit the LDS segment
 //   texture -- one 8 x 8 x 4K tile of any other capture (the 4 waves take its quarters)
 // Work-groups take items in queue order (one atomic by thread 0, broadcast through LDS between
@@ -986,7 +1027,7 @@ __global__ __launch_bounds__(NW * 64) void batch_queue_kernel(BatchQueueArgs b) 
         const uint32_t y = __builtin_amdgcn_readfirstlane(item.y);
         if ((KINDS & 1) && type == kBatchItemStage) {
             const StageJob jb = wave_uniform_load(b.jobs + c);
-            const uint32_t total = jb.rows * ((jb.lde + 255u) / 256u);
+            const uint32_t total = jb.rows;
             const uint32_t end = min(y + kBatchStageBlocks, total);
             for (uint32_t blk = y; blk < end; ++blk) stage_row_block(jb, blk);
             __threadfence();

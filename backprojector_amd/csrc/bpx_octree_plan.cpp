@@ -113,6 +113,14 @@ PlanImpl* make_plan_octree(const CaptureIn& c, const uint32_t n[3], const float 
             fail(BPX_ERR_UNSUPPORTED, "the octree path takes confocal captures");
     bpx_options po{BPX_STORAGE_F32, 32u, 0u, 5};  // fused layout, 2 voxels per lane
     std::unique_ptr<PlanImpl> pl(make_plan_single_tap(c, n, zero, inc, &po));
+    // the coarse levels' window-max tables span bins around every block's bounding sphere,
+    // past the bins the fine voxels read: every row element is staged exactly
+    if (pl->win.ptr) {
+        pl->win.release();
+        pl->win_host.clear();
+        const StageJob job = stage_job(*pl);
+        hip_check(hipMemcpy(pl->stage.ptr, &job, sizeof(StageJob), hipMemcpyHostToDevice), "hipMemcpy(staging job)");
+    }
     auto st = std::make_unique<OctreeState>();
     st->L = o.levels;
     octree_set_options(*st, o);

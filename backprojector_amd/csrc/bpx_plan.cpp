@@ -73,6 +73,54 @@ static void grid_corners(const uint32_t n[3], const float zero[3], const float i
     }
 }
 
+// Per-pair element windows [lo, hi) of a fused confocal row (lde elements): every element a
+// gather kernel can read for the pair and any voxel of the grid.  A voxel's distance d to the
+// wall point lies between the distances to the nearest and farthest points of the corners'
+// bounding box, the box widened by 1e-5 of the largest coordinate and the distances by a
+// relative 1e-5 (the kernels' fp32 coordinates and squared distances are a few ulps off the
+// exact ones); its bin is round((lw + 2 d + cw - t0) / deltaT) over that range, and the
+// kernels read element bin + 1 and its neighbours (the two candidate bins of the approximate
+// quotient, the second sample of a 16-bit element), so 4 elements of slack below and 5 above
+// cover every read.  Bins before the row or past its end clamp to its ends (elements 0 and
+// Tc, Tc + 1), so a window reaching past an end keeps that end's first 6 / last 6 elements,
+// also when the whole window lies beyond it.
+static std::vector<uint32_t> row_windows(const std::vector<bpx_pair>& pr, const double corners[8][3], double t0,
+                                         double dT, uint32_t Tc, uint32_t lde) {
+    double lo[3], hi[3], scale = 1.0;
+    for (int k = 0; k < 3; ++k) lo[k] = hi[k] = corners[0][k];
+    for (int m = 0; m < 8; ++m)
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = std::min(lo[k], corners[m][k]);
+            hi[k] = std::max(hi[k], corners[m][k]);
+            scale = std::max(scale, std::fabs(corners[m][k]));
+        }
+    for (const auto& p : pr)
+        for (int k = 0; k < 3; ++k) scale = std::max(scale, std::fabs((double)p.laser_point[k]));
+    for (int k = 0; k < 3; ++k) {
+        lo[k] -= 1e-5 * scale;
+        hi[k] += 1e-5 * scale;
+    }
+    std::vector<uint32_t> w(2 * pr.size());
+    for (size_t p = 0; p < pr.size(); ++p) {
+        double near2 = 0.0, far2 = 0.0;
+        for (int k = 0; k < 3; ++k) {
+            const double x = pr[p].laser_point[k];
+            const double c = x < lo[k] ? lo[k] - x : x > hi[k] ? x - hi[k] : 0.0;
+            const double f = std::max(std::fabs(x - lo[k]), std::fabs(x - hi[k]));
+            near2 += c * c;
+            far2 += f * f;
+        }
+        const double base = (double)pr[p].laser_wall + (double)pr[p].camera_wall - t0;
+        const double qlo = (base + 2.0 * std::sqrt(near2) * (1.0 - 1e-5)) / dT;
+        const double qhi = (base + 2.0 * std::sqrt(far2) * (1.0 + 1e-5)) / dT;
+        double elo = std::floor(qlo) + 1.0 - 4.0, ehi = std::ceil(qhi) + 1.0 + 5.0;
+        if (!std::isfinite(elo) || !std::isfinite(ehi)) elo = 0.0, ehi = (double)lde;
+        w[2 * p] = (uint32_t)std::clamp(elo, 0.0, (double)(Tc >= 4u ? Tc - 4u : 0u));
+        w[2 * p + 1] = (uint32_t)std::clamp(ehi, (double)std::min(lde, 6u), (double)lde);
+    }
+    return w;
+}
+
 static double max_total_path(const bpx_pair* pr, size_t count, const double corners[8][3]) {
     double mx = 0.0;
     for (size_t i = 0; i < count; ++i) {
@@ -400,6 +448,8 @@ PlanImpl* make_plan_single_tap(const CaptureIn& c, const uint32_t n[3], const fl
                             (pl->fused ? fused_elem_bytes(pl->format) : format_bytes(pl->format));
     if (hbytes >= (1ull << 32)) fail(BPX_ERR_UNSUPPORTED, "transient rows above 4 GiB per plan");
     pl->H.alloc(hbytes);
+    // fused rows staged over per-pair windows keep zeros outside them (written here only)
+    if (pl->fused) hip_check(hipMemset(pl->H.ptr, 0, hbytes), "hipMemset(fused rows)");
 
     std::vector<float> p8((size_t)pl->P * 8);
     std::memcpy(p8.data(), pr.data(), sizeof(float) * p8.size());
@@ -415,6 +465,11 @@ PlanImpl* make_plan_single_tap(const CaptureIn& c, const uint32_t n[3], const fl
             if (sbytes >= (1ull << 32)) fail(BPX_ERR_UNSUPPORTED, "threshold table above 4 GiB");
             pl->S.alloc(sbytes);
         } else {
+            pl->win_host = row_windows(pr, corners, c.t0, c.dT, c.T, pl->ldh);
+            pl->win.alloc(sizeof(uint32_t) * pl->win_host.size());
+            hip_check(hipMemcpy(pl->win.ptr, pl->win_host.data(), sizeof(uint32_t) * pl->win_host.size(),
+                                hipMemcpyHostToDevice),
+                      "hipMemcpy(row windows)");
             const StageJob job = stage_job(*pl);
             pl->stage_zb = stage_z_blocks(job);
             pl->stage_sb = stage_row_blocks(job);
@@ -528,6 +583,7 @@ StageJob stage_job(const PlanImpl& pl) {
     j.lde = pl.ldh;
     j.format = pl.format;
     j.fmt_flag = pl.rows16 ? static_cast<const uint32_t*>(pl.fmt_flag.ptr) : nullptr;
+    j.win = static_cast<const uint32_t*>(pl.win.ptr);
     j.alt_format = kF16;
     j.inv_scale = 1.0f / pl.scale;
     j.t0 = pl.t0;
