@@ -192,6 +192,39 @@ def test_lds_staged_kernel_windows_before_the_row(gpu, oracle, variant):
     assert raw.min() < 0 and raw.max() >= T
 
 
+@pytest.mark.parametrize("variant", [-1, 5, 7])
+def test_row_windows_entirely_before_or_past_the_row(gpu, oracle, variant):
+    """Rows are staged only over each pair's bin window (the elements a gather can read).  A
+    third of the pairs see every bin before the row (short laser/camera legs), a third every
+    bin past its end (long legs), the rest straddle both ends: their windows clamp to the
+    row's ends.  Volumes equal the oracle bin for bin, and the threshold readback is the full
+    exact table (entries outside the windows from the same search)."""
+    pr = confocal_pairs(12, jitter=0.01, seed=21)
+    legs = np.array([0.1, 2.2, 1.5], f32)[np.arange(len(pr)) % 3]
+    pr["laser_wall"] = legs
+    pr["camera_wall"] = legs
+    T, t0, dT = 1200, 4.0, 0.001
+    H = coded_samples(len(pr), T, seed=4)
+    N = (21, 19, 23)
+    zero, inc = grid_basis(N, pos=(0.0, 0.0, 0.6), size=(0.3, 0.3, 0.3))
+    ref = oracle.a1_volume_f64acc(H, pr, N, zero, inc, t0, dT)
+    with gpu.Plan(H, pr, N, zero, inc, t0, dT, variant=variant) as plan:
+        plan.execute()
+        vol = plan.volume()
+        got = plan.thresholds()
+    assert np.array_equal(vol, ref.astype(f32))
+    assert ref.max() > 0
+    want = oracle.confocal_thresholds(pr["laser_wall"], pr["camera_wall"], T + 1, t0, dT)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    # the geometry reaches all three cases: every bin before the row, every bin past it, both
+    vox = np.array([[i, j, k] for i in (0, N[0] - 1) for j in (0, N[1] - 1) for k in (0, N[2] - 1)], np.uint32)
+    _, raw = oracle.a1_bins(pr, np.tile(vox, (len(pr), 1)), np.repeat(np.arange(len(pr)), len(vox)),
+                            zero, inc, t0, dT, T)
+    raw = raw.reshape(len(pr), len(vox))
+    assert (raw.max(1) < 0).any() and (raw.min(1) >= T).any()
+    assert ((raw.min(1) < 0) & (raw.max(1) >= T)).any() or ((raw.min(1) >= 0) & (raw.max(1) < T)).any()
+
+
 def _tap_error_bound(pr, N, zero, inc, t0, dT, T):
     """The plan's precondition of the exact bin lookup: 16 x 2^-24 x Q with
     Q = (max total path + |t0|) / deltaT + T + 2, the total over pairs and grid corners."""
