@@ -528,19 +528,30 @@ hipError_t launch_complex_post(const void* complex_volume, uint64_t n, float* ma
 
 // Whether every sample is an integer that fp16 holds exactly (|x| <= 2048): one flag word,
 // set by any wave that finds a sample that is not (NaN and infinities included).
+__device__ __forceinline__ bool exact16_sample(float x) {
+    return __builtin_fabsf(x) <= 2048.0f && x == __builtin_rintf(x);
+}
+
+// Sets *flag when a sample is not an integer of magnitude <= 2048: 16-byte loads over the
+// aligned body (src 16-byte aligned), single samples for the tail.
 __global__ __launch_bounds__(256) void exact16_check_kernel(const float* __restrict__ src, uint64_t n,
                                                             uint32_t* flag) {
     bool bad = false;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
-        const float x = src[i];
-        bad = bad || !(__builtin_fabsf(x) <= 2048.0f && x == __builtin_rintf(x));
+    const uint64_t n4 = n / 4u, stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const float4* s4 = reinterpret_cast<const float4*>(src);
+    for (uint64_t i = tid; i < n4; i += stride) {
+        const float4 v = s4[i];
+        bad = bad || !(exact16_sample(v.x) && exact16_sample(v.y) && exact16_sample(v.z) && exact16_sample(v.w));
     }
+    for (uint64_t i = 4u * n4 + tid; i < n; i += stride) bad = bad || !exact16_sample(src[i]);
     if (__ballot(bad) != 0ull && (threadIdx.x & 63u) == 0u) atomicOr(flag, 1u);
 }
 
 hipError_t launch_exact16_check(const float* src, uint64_t n, uint32_t* flag, hipStream_t stream) {
     if (n == 0) return hipSuccess;
-    const uint64_t blocks = std::min<uint64_t>((n + 255u) / 256u, 2048u);
+    if (reinterpret_cast<uintptr_t>(src) % 16u) return hipErrorInvalidValue;
+    const uint64_t blocks = std::min<uint64_t>((n / 4u + 255u) / 256u + 1u, 8192u);
     hipLaunchKernelGGL(exact16_check_kernel, dim3((uint32_t)blocks), dim3(256), 0, stream, src, n, flag);
     return hipGetLastError();
 }

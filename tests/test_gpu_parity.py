@@ -144,6 +144,31 @@ def test_exact_fp16_rows_for_integer_samples(gpu, oracle, variant):
         assert np.array_equal(vb, vk, equal_nan=True), bad
 
 
+def test_exact_fp16_check_body_and_tail(gpu, oracle):
+    """The device check of the samples reads them 16 bytes at a time and the last P*T mod 4
+    one at a time (here 225 x 2001 = 1 mod 4): one non-integer sample at the first element,
+    inside, or at the very last element keeps the execution on fp32 rows."""
+    pr = confocal_pairs(15, jitter=0.01, seed=11)
+    T, N, t0 = 2001, (33, 35, 31), 3.3
+    assert len(pr) * T % 4 == 1
+    H = np.random.default_rng(8).integers(-2048, 2049, size=(len(pr), T)).astype(f32)
+    zero, inc = grid_basis(N, pos=(0.0, 0.05, 0.45), size=(0.25, 0.25, 0.25))
+
+    def run(h):
+        with gpu.Plan(h, pr, N, zero, inc, t0, 0.001, variant=7) as plan:
+            plan.execute()
+            return plan.volume(), plan.info()["row_storage"]
+
+    vol, rows = run(H)
+    assert rows == "f16"
+    assert np.array_equal(vol, oracle.a1_volume_f64acc(H, pr, N, zero, inc, t0, 0.001).astype(f32))
+    for p, b in [(0, 0), (100, 1000), (len(pr) - 1, T - 1)]:
+        Hb = H.copy()
+        Hb[p, b] = 0.5
+        _, rows_b = run(Hb)
+        assert rows_b == "f32", (p, b)
+
+
 @pytest.mark.parametrize("storage,cplx", [("f16", False), ("bf16", False), ("f32", True), ("f16", True),
                                           ("bf16", True)])
 def test_lds_staged_kernel_formats(gpu, storage, cplx):
