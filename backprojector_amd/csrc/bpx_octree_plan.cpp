@@ -112,15 +112,10 @@ PlanImpl* make_plan_octree(const CaptureIn& c, const uint32_t n[3], const float 
         if (c.pairs && std::memcmp(c.pairs[p].laser_point, c.pairs[p].camera_point, 12) != 0)
             fail(BPX_ERR_UNSUPPORTED, "the octree path takes confocal captures");
     bpx_options po{BPX_STORAGE_F32, 32u, 0u, 5};  // fused layout, 2 voxels per lane
+    // the fused rows keep the plain plan's bin windows: the octree reads them only at the bins
+    // of fine-grid voxels (fine gathers, fallback voxels, each bound block's representative
+    // voxel); the window-max tables of the coarse levels come from the raw rows
     std::unique_ptr<PlanImpl> pl(make_plan_single_tap(c, n, zero, inc, &po));
-    // the coarse levels' window-max tables span bins around every block's bounding sphere,
-    // past the bins the fine voxels read: every row element is staged exactly
-    if (pl->win.ptr) {
-        pl->win.release();
-        pl->win_host.clear();
-        const StageJob job = stage_job(*pl);
-        hip_check(hipMemcpy(pl->stage.ptr, &job, sizeof(StageJob), hipMemcpyHostToDevice), "hipMemcpy(staging job)");
-    }
     auto st = std::make_unique<OctreeState>();
     st->L = o.levels;
     octree_set_options(*st, o);
