@@ -5,8 +5,8 @@ include/bpx.h); this package is its Python front.
 from . import nlosbpy
 from ._native import BpxError, InvalidArgument, device_count
 from .core import (Batch, Net, Plan, as_pairs, backprojection, octree_backprojection, phasor_pulse,
-                   render_transient, window_backprojection, PAIR_DTYPE)
+                   render_transient, row_windows, window_backprojection, PAIR_DTYPE)
 
 __all__ = ["nlosbpy", "Batch", "Net", "Plan", "backprojection", "window_backprojection", "octree_backprojection",
-           "phasor_pulse", "render_transient", "as_pairs", "PAIR_DTYPE", "BpxError",
+           "phasor_pulse", "render_transient", "row_windows", "as_pairs", "PAIR_DTYPE", "BpxError",
            "InvalidArgument", "device_count"]

@@ -141,7 +141,7 @@ _lib = None
 
 # Every symbol include/bpx.h declares (checked by the CPU test suite).
 EXPORTED = [
-    "bpx_last_error", "bpx_version", "bpx_abi_version", "bpx_device_count",
+    "bpx_last_error", "bpx_version", "bpx_abi_version", "bpx_device_count", "bpx_row_windows",
     "bpx_backprojection", "bpx_complex_backprojection", "bpx_backprojection_ex", "bpx_backprojection_noalloc",
     "bpx_window_backprojection", "bpx_phasor_pulse", "bpx_nlos_backproject", "bpx_nlos_phasor", "bpx_nlos_phasor_multi",
     "bpx_nlos_phasor_output",
@@ -201,6 +201,7 @@ def lib():
     L.bpx_plan_synchronize.argtypes = [c_p]
     L.bpx_plan_read_volume.argtypes = [c_p, c_p]
     L.bpx_plan_read_thresholds.argtypes = [c_p, c_p, u64]
+    L.bpx_row_windows.argtypes = [c_p, ctypes.c_uint32, c_p, c_p, c_p, f32, f32, ctypes.c_uint32, c_p]
     L.bpx_plan_kernel_name.argtypes = [c_p, ctypes.c_char_p, ctypes.c_uint64]
     L.bpx_plan_row_storage.argtypes = [c_p, ctypes.POINTER(ctypes.c_uint32)]
     L.bpx_plan_info.argtypes = [c_p, ctypes.POINTER(i32), ctypes.POINTER(i32),

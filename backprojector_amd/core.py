@@ -109,6 +109,17 @@ def phasor_pulse(transient_data, wavelength, deltaT, times=4, sigma=0.0):
     return out
 
 
+def row_windows(pairs, voxels_per_side, volume_zero_pos, voxel_inc, t0, deltaT, Tc):
+    """[P][2] uint32 element windows [lo, hi) of the fused rows that a confocal plan of this
+    geometry stages at each execution (no reference counterpart; host only, no device call)."""
+    pr = as_pairs(pairs)
+    out = np.zeros((len(pr), 2), np.uint32)
+    check(lib().bpx_row_windows(ptr(pr), len(pr), ptr(_u32(voxels_per_side).reshape(3)),
+                                ptr(_vec3(volume_zero_pos, "volume_zero_pos")), ptr(_mat9(voxel_inc)),
+                                float(t0), float(deltaT), int(Tc), ptr(out)))
+    return out
+
+
 def render_transient(pairs, T, t0, deltaT, samples, peak_count=1000.0, poisson=True, seed=0):
     """Synthetic capture [P][T] of hidden samples [n][7] = (x, y, z, nx, ny, nz, area)."""
     pr = as_pairs(pairs)

@@ -1,5 +1,6 @@
 // extern "C" entry points of libbpx (declared in include/bpx.h).  Every entry point catches
 // all failures and converts them to a bpx_status plus a thread-local message.
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -284,6 +285,21 @@ bpx_status bpx_plan_read_volume(bpx_plan plan, float* voxel_volume) {
         plan_wait(pl);
         hip_check(hipMemcpy(voxel_volume, pl.vol.ptr, nvox * (pl.cplx ? 8 : 4), hipMemcpyDeviceToHost),
                   "hipMemcpy(volume)");
+    });
+}
+
+bpx_status bpx_row_windows(const bpx_pair* pairs, uint32_t num_pairs, const uint32_t voxels_per_side[3],
+                           const float volume_zero_pos[3], const float voxel_inc[9], float t0, float deltaT,
+                           uint32_t Tc, uint32_t* out) {
+    return guarded([&] {
+        if ((num_pairs && (!pairs || !out)) || !voxels_per_side || !volume_zero_pos || !voxel_inc)
+            fail(BPX_ERR_INVALID_ARGUMENT, "NULL argument");
+        for (int k = 0; k < 3; ++k)
+            if (voxels_per_side[k] == 0) fail(BPX_ERR_INVALID_ARGUMENT, "voxels_per_side must be positive");
+        if (!(deltaT > 0.0f)) fail(BPX_ERR_INVALID_ARGUMENT, "deltaT must be positive");
+        const std::vector<uint32_t> w =
+            confocal_row_windows(pairs, num_pairs, voxels_per_side, volume_zero_pos, voxel_inc, t0, deltaT, Tc);
+        std::copy(w.begin(), w.end(), out);
     });
 }
 

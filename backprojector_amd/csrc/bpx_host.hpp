@@ -27,6 +27,11 @@ bool device_ready();  // a gfx950 device is visible and selected
 // lookup is exact below 0.25 (allocates nothing).
 double single_tap_error_bound(const bpx_pair* pairs, uint32_t P, const uint32_t n[3], const float zero[3],
                               const float inc[9], float t0, float dT, uint32_t T);
+// Per-pair element windows [lo, hi) of a fused confocal plan's rows (2 values per pair), as
+// the plan computes them for its staging.
+std::vector<uint32_t> confocal_row_windows(const bpx_pair* pairs, uint32_t P, const uint32_t n[3],
+                                           const float zero[3], const float inc[9], float t0, float dT,
+                                           uint32_t Tc);
 
 // ------------------------------------------------------------------ plan
 enum Path : int { kPathConfocal = 0, kPathGrid = 1, kPathPairs = 2, kPathWindow = 3, kPathOctree = 4 };
@@ -114,8 +119,8 @@ struct PlanImpl {
     bool all_confocal = false;
     DeviceBuffer raw, H, S, Z, Z2, pair8, pair4, lasers, spads, vol;
     DeviceBuffer stage;                  // one StageJob (fused confocal layouts)
-    // per-pair element windows [lo, hi) of the fused rows that the gathers can read (plain
-    // confocal plans; empty = every element staged exactly), device copy and host copy
+    // per-pair element windows [lo, hi) of the fused rows that the gathers can read (fused
+    // confocal plans, octree plans included; empty = every element staged), device and host copy
     DeviceBuffer win;
     std::vector<uint32_t> win_host;
     std::vector<uint32_t> used;          // source rows of the staged pairs (dropped pairs skipped)

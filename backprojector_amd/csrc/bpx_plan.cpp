@@ -121,6 +121,14 @@ static std::vector<uint32_t> row_windows(const std::vector<bpx_pair>& pr, const 
     return w;
 }
 
+std::vector<uint32_t> confocal_row_windows(const bpx_pair* pairs, uint32_t P, const uint32_t n[3],
+                                           const float zero[3], const float inc[9], float t0, float dT,
+                                           uint32_t Tc) {
+    double corners[8][3];
+    grid_corners(n, zero, inc, corners);
+    return row_windows(std::vector<bpx_pair>(pairs, pairs + P), corners, t0, dT, Tc, round_up((uint64_t)Tc + 2, 16));
+}
+
 static double max_total_path(const bpx_pair* pr, size_t count, const double corners[8][3]) {
     double mx = 0.0;
     for (size_t i = 0; i < count; ++i) {

@@ -107,6 +107,15 @@ const char* bpx_version(void);
 int bpx_abi_version(void);
 /* Number of usable gfx950 devices (0 when none); never fails. */
 int bpx_device_count(void);
+/* The per-pair element windows [lo, hi) of the fused rows that a confocal plan of this
+ * geometry stages at each execution: every element a gather kernel can read for the pair and
+ * any voxel of the grid (element j of a row holds bin j - 1's sample and bin j's threshold).
+ * No reference counterpart: exposed so that the windows can be checked against the bins on
+ * the host.  pairs: confocal (laser point == camera point); Tc: bins of the chunk; out:
+ * 2 * num_pairs uint32 (lo, hi), within [0, roundup(Tc + 2, 16)].  Host only (no device call). */
+bpx_status bpx_row_windows(const bpx_pair* pairs, uint32_t num_pairs, const uint32_t voxels_per_side[3],
+                           const float volume_zero_pos[3], const float voxel_inc[9], float t0, float deltaT,
+                           uint32_t Tc, uint32_t* out);
 
 /* ------------------------------------------------------------------ reference-equivalent
  * Host buffers in, host volume out (overwritten), synchronous.
