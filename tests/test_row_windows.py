@@ -65,3 +65,24 @@ def test_row_windows_hold_every_oracle_bin(bpx, oracle, name):
     assert np.all((hi == lde) | (hi - 1 - e >= 3)), name
     # the windows are tight enough to matter: well below the whole row on average
     assert (w[:, 1] - w[:, 0]).mean() < 0.8 * lde, name
+
+
+@pytest.mark.parametrize("index", [2, 7])
+def test_row_windows_hold_the_batch_workload_bins(bpx, oracle, index):
+    """Captures of the 64-capture uneven batch (SeedSequence(1004)), prepared by the reference
+    pipeline (time limits, chunk, chunked t0): every sampled voxel-pair bin inside its window."""
+    from backprojector_amd import proposals
+    cap, N = scenes.batch_captures(index + 1, 1004)[index]
+    chunk, pr, Nv, zero, inc, t0, dT = proposals.plan_args(cap, N)
+    Tc = np.asarray(chunk).shape[1]
+    zero, inc = np.asarray(zero, f32), np.asarray(inc, f32)
+    lde = -(-(Tc + 2) // 16) * 16
+    w = bpx.row_windows(pr, Nv, zero, inc, t0, dT, Tc).astype(np.int64)
+    rng = np.random.default_rng(index)
+    vox = np.stack([rng.integers(0, Nv[k], 300) for k in range(3)], 1).astype(np.uint32)
+    pi = np.repeat(np.arange(len(pr)), len(vox)).astype(np.uint32)
+    _, raw = oracle.a1_bins(pr, np.tile(vox, (len(pr), 1)), pi, zero, inc, t0, dT, Tc)
+    e = np.clip(raw.astype(np.int64) + 1, 0, Tc + 1)
+    lo, hi = w[pi, 0], w[pi, 1]
+    assert np.all((lo == 0) | (e - lo >= 3)) and np.all((hi == lde) | (hi - 1 - e >= 3))
+    assert (w[:, 1] - w[:, 0]).mean() < 0.6 * lde   # ~42 % of the row on this workload
