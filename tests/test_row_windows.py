@@ -86,3 +86,15 @@ def test_row_windows_hold_the_batch_workload_bins(bpx, oracle, index):
     lo, hi = w[pi, 0], w[pi, 1]
     assert np.all((lo == 0) | (e - lo >= 3)) and np.all((hi == lde) | (hi - 1 - e >= 3))
     assert (w[:, 1] - w[:, 0]).mean() < 0.6 * lde   # ~42 % of the row on this workload
+
+
+def test_row_windows_rejects_invalid_arguments(bpx):
+    """Empty grid axes, a non-positive bin width and malformed vectors are argument errors."""
+    pr, N, zero, inc, t0, dT, Tc = CASES["bins past the row end"]
+    with pytest.raises(bpx.InvalidArgument):
+        bpx.row_windows(pr, (0, N[1], N[2]), zero, inc, t0, dT, Tc)
+    with pytest.raises(bpx.InvalidArgument):
+        bpx.row_windows(pr, N, zero, inc, t0, 0.0, Tc)
+    with pytest.raises(ValueError):
+        bpx.row_windows(pr, N, zero[:2], inc, t0, dT, Tc)
+    assert bpx.row_windows(pr[:0], N, zero, inc, t0, dT, Tc).shape == (0, 2)
