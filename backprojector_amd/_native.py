@@ -146,7 +146,7 @@ EXPORTED = [
     "bpx_window_backprojection", "bpx_phasor_pulse", "bpx_nlos_backproject", "bpx_nlos_phasor", "bpx_nlos_phasor_multi",
     "bpx_nlos_phasor_output",
     "bpx_plan_create", "bpx_nlos_plan_create", "bpx_plan_execute", "bpx_plan_synchronize", "bpx_plan_read_volume",
-    "bpx_plan_info", "bpx_plan_kernel_name", "bpx_plan_row_storage", "bpx_plan_read_thresholds", "bpx_plan_benchmark", "bpx_plan_destroy", "bpx_render_transient",
+    "bpx_plan_info", "bpx_plan_kernel_name", "bpx_plan_row_storage", "bpx_plan_read_thresholds", "bpx_plan_benchmark", "bpx_plan_tune", "bpx_plan_destroy", "bpx_render_transient",
     "bpx_copy_benchmark", "bpx_octree_backprojection", "bpx_octree_plan_create",
     "bpx_octree_plan_stats", "bpx_nlos_prepare", "bpx_nlos_prepared_free",
     "bpx_net_param_count", "bpx_net_create", "bpx_net_destroy", "bpx_net_get_params",
@@ -201,6 +201,7 @@ def lib():
     L.bpx_plan_synchronize.argtypes = [c_p]
     L.bpx_plan_read_volume.argtypes = [c_p, c_p]
     L.bpx_plan_read_thresholds.argtypes = [c_p, c_p, u64]
+    L.bpx_plan_tune.argtypes = [c_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]
     L.bpx_row_windows.argtypes = [c_p, ctypes.c_uint32, c_p, c_p, c_p, f32, f32, ctypes.c_uint32, c_p]
     L.bpx_plan_kernel_name.argtypes = [c_p, ctypes.c_char_p, ctypes.c_uint64]
     L.bpx_plan_row_storage.argtypes = [c_p, ctypes.POINTER(ctypes.c_uint32)]

@@ -287,6 +287,14 @@ class Plan:
                     table_kernel_ms=t.table_kernel_ms, gather_kernel_ms=t.gather_kernel_ms,
                     iterations=t.iterations)
 
+    def tune(self, reps=3):
+        """bpx_plan_tune: a confocal plan with fused rows times every gather variant that applies
+        to it on its own data and keeps the fastest (the same volume bit for bit); returns the
+        chosen variant and its kernel.  Other plans keep their variant."""
+        v = ctypes.c_int32()
+        check(lib().bpx_plan_tune(self._h, int(reps), ctypes.byref(v)))
+        return dict(variant=int(v.value), kernel=self.info()["kernel"])
+
     def close(self):
         if getattr(self, "_h", None):
             lib().bpx_plan_destroy(self._h)

@@ -411,6 +411,52 @@ bpx_status bpx_plan_benchmark(bpx_plan plan, int32_t warmup, int32_t iterations,
     });
 }
 
+bpx_status bpx_plan_tune(bpx_plan plan, int32_t reps, int32_t* variant) {
+    return guarded([&] {
+        if (!plan) fail(BPX_ERR_INVALID_ARGUMENT, "NULL plan");
+        PlanImpl& pl = *plan->impl;
+        plan_wait(pl);
+        if (pl.path == kPathConfocal && pl.fused && pl.P) {
+            reps = std::max(reps, 1);
+            hipEvent_t t0 = nullptr, t1 = nullptr;
+            hip_check(hipEventCreate(&t0), "hipEventCreate");
+            if (hipEventCreate(&t1) != hipSuccess) {
+                (void)hipEventDestroy(t0);
+                fail(BPX_ERR_DEVICE, "hipEventCreate failed");
+            }
+            const int initial = pl.variant;
+            int best = initial;
+            float best_ms = INFINITY;
+            try {
+                for (int v = 3; v < variant_count(); ++v) {
+                    set_fused_variant(pl, v);
+                    if (v >= kLdsVariant && !pl.lds_tiles) continue;   // would run as a texture-path shape
+                    pl.execute(pl.stream);                             // warm-up
+                    hip_check(hipEventRecord(t0, pl.stream), "hipEventRecord");
+                    for (int r = 0; r < reps; ++r) pl.execute(pl.stream);
+                    hip_check(hipEventRecord(t1, pl.stream), "hipEventRecord");
+                    hip_check(hipEventSynchronize(t1), "hipEventSynchronize");
+                    float ms = 0;
+                    hip_check(hipEventElapsedTime(&ms, t0, t1), "hipEventElapsedTime");
+                    if (ms < best_ms) {
+                        best_ms = ms;
+                        best = v;
+                    }
+                }
+                set_fused_variant(pl, best);
+            } catch (...) {
+                (void)hipEventDestroy(t0);
+                (void)hipEventDestroy(t1);
+                set_fused_variant(pl, initial);
+                throw;
+            }
+            (void)hipEventDestroy(t0);
+            (void)hipEventDestroy(t1);
+        }
+        if (variant) *variant = pl.variant;
+    });
+}
+
 void bpx_plan_destroy(bpx_plan plan) {
     if (!plan) return;
     (void)hipStreamSynchronize(plan->impl->stream);

@@ -110,6 +110,7 @@ struct PlanImpl {
     // fp32 rows staged as exact fp16 (FusedH16) whenever every sample of the execution is an
     // integer of magnitude <= 2048; fmt_flag (one word) = 0 then, 1 otherwise
     bool rows16 = false;
+    bool keep_f32 = false;               // BPX_FLAG_KEEP_F32_ROWS: rows16 never applies
     DeviceBuffer fmt_flag;
     float lz = 0;
     uint32_t kahan_block = 32;
@@ -145,6 +146,10 @@ void execute_octree(PlanImpl& pl, hipStream_t s);
 // device copy stages the plan's rows.
 StageJob stage_job(const PlanImpl& pl);
 void stage_rows(PlanImpl& pl, hipStream_t s);
+// Sets the gather variant of a fused confocal plan (3-8: the same row layout) and what follows
+// from it: voxels per lane, the LDS-staged kernel where it applies, exact fp16 rows for its
+// shapes of 8 voxels per lane (fp32 storage), and the staging descriptor.
+void set_fused_variant(PlanImpl& pl, int variant);
 
 // Proposal net (bpx_net.hip kernels, bpx_net.cpp orchestration).  Four 3x3x3 conv layers of
 // 16 channels (inputs zero-padded to 16, 8 output logits = children), ReLU between layers.

@@ -518,24 +518,12 @@ PlanImpl* make_plan_single_tap(const CaptureIn& c, const uint32_t n[3], const fl
             else if (!cplx && t16 >= kLdsMinTiles && fits16) pl->variant = kLdsVariant + 2;
             else if (t8 >= kLdsMinTiles && fits8) pl->variant = kLdsVariant + 1;
         }
-        pl->lds_tiles = pl->variant >= kLdsVariant && pl->fused && pl->sep && pl->flat &&
-                        confocal_lds_fits(inc, pl->inv2, lds_shape(pl->variant, pl->format).depth(), seg);
-        // fp32 rows of integer samples within fp16's exact range are staged as fp16: the same
-        // values, half the LDS bytes per lookup (one 8-B read instead of an 8-B and a 4-B one).
-        // Each execution checks its samples on the device first and stages and gathers in the
-        // format the check allows, so the volume is the fp32 rows' bit for bit either way.  For
-        // the shapes of 8 voxels per lane (gather ms at 16,384 pairs, fp32 -> fp16 rows: 128^3
-        // 16^3 x 512 lanes 12.96 -> 11.67, 16 x 16 x 8 x 256 14.57 -> 13.92; 256^3 83.3 -> 83.2,
-        // 91.7 -> 89.7); the 16-per-lane shape, bound by its arithmetic and the default where
-        // it applies (256^3), measured slower with fp16 rows (84.5 -> 86.4), so it keeps fp32.
-        pl->rows16 = pl->lds_tiles && pl->format == kF32 && lds_shape(pl->variant, pl->format).kz == 8 &&
-                     !(o.flags & BPX_FLAG_KEEP_F32_ROWS);
-        if (pl->rows16) {
-            pl->fmt_flag.alloc(4);
-            hip_check(hipMemset(pl->fmt_flag.ptr, 0xff, 4), "hipMemset(row format)");   // fp32 until run
-            const StageJob job = stage_job(*pl);
-            hip_check(hipMemcpy(pl->stage.ptr, &job, sizeof(StageJob), hipMemcpyHostToDevice),
-                      "hipMemcpy(staging job)");
+        pl->keep_f32 = (o.flags & BPX_FLAG_KEEP_F32_ROWS) != 0;
+        if (pl->fused) {
+            set_fused_variant(*pl, pl->variant);
+        } else {
+            pl->lds_tiles = false;
+            pl->rows16 = false;
         }
         pl->pair4.alloc(sizeof(float) * p4.size());
         hip_check(hipMemcpy(pl->pair4.ptr, p4.data(), sizeof(float) * p4.size(), hipMemcpyHostToDevice),
@@ -597,6 +585,32 @@ StageJob stage_job(const PlanImpl& pl) {
     j.t0 = pl.t0;
     j.dT = pl.dT;
     return j;
+}
+
+void set_fused_variant(PlanImpl& pl, int variant) {
+    pl.variant = variant;
+    pl.K = variant_k(variant);
+    pl.tiles = make_tiles(pl.g.n, pl.K);
+    pl.lds_tiles = variant >= kLdsVariant && pl.sep && pl.flat &&
+                   confocal_lds_fits(pl.g.inc, pl.inv2, lds_shape(variant, pl.format).depth(),
+                                     lds_seg_elems(pl.format));
+    // fp32 rows of integer samples within fp16's exact range are staged as fp16: the same
+    // values, half the LDS bytes per lookup (one 8-B read instead of an 8-B and a 4-B one).
+    // Each execution checks its samples on the device first and stages and gathers in the
+    // format the check allows, so the volume is the fp32 rows' bit for bit either way.  For
+    // the shapes of 8 voxels per lane (gather ms at 16,384 pairs, fp32 -> fp16 rows: 128^3
+    // 16^3 x 512 lanes 12.96 -> 11.67, 16 x 16 x 8 x 256 14.57 -> 13.92; 256^3 83.3 -> 83.2,
+    // 91.7 -> 89.7); the 16-per-lane shape, bound by its arithmetic and the default where
+    // it applies (256^3), measured slower with fp16 rows (84.5 -> 86.4), so it keeps fp32.
+    pl.rows16 = pl.lds_tiles && pl.format == kF32 && lds_shape(variant, pl.format).kz == 8 && !pl.keep_f32;
+    if (pl.rows16 && !pl.fmt_flag.ptr) {
+        pl.fmt_flag.alloc(4);
+        hip_check(hipMemset(pl.fmt_flag.ptr, 0xff, 4), "hipMemset(row format)");   // fp32 until run
+    }
+    if (pl.stage.ptr) {   // the staging descriptor names the format flag (or none)
+        const StageJob job = stage_job(pl);
+        hip_check(hipMemcpy(pl.stage.ptr, &job, sizeof(StageJob), hipMemcpyHostToDevice), "hipMemcpy(staging job)");
+    }
 }
 
 void stage_rows(PlanImpl& pl, hipStream_t s) {

@@ -39,6 +39,8 @@ def parse():
     ap.add_argument("--voxels", type=int, default=128)
     ap.add_argument("--storage", default="f32")
     ap.add_argument("--variant", type=int, default=-1)
+    ap.add_argument("--tune", type=int, default=1,
+                    help="with --variant -1: time every gather variant on the capture first (bpx_plan_tune)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="target CPU time of the bounded cpu_baseline sample (0 = skip)")
     ap.add_argument("--check-voxels", type=int, default=1024)
@@ -74,6 +76,7 @@ def octree_comparison(bp, cap, side, tau, proposal=None, reps=3, ctx=None, weigh
     N = (side,) * 3
     args = proposals.plan_args(cap, side)
     with bp.Plan(*args) as plain:
+        plain.tune(reps=2)   # the fastest plain kernel for this capture (same volume)
         tp = plain.benchmark(1, reps)
         vp = plain.volume()
         plain_kernel = plain.info()["kernel"]
@@ -410,6 +413,9 @@ def main():
     N = (args.voxels,) * 3
     plan = bp.Plan.from_nlos(*cap.nlos_args(N), wall_normal=(0, 0, 0), assume_row_major=True,
                              storage=args.storage, variant=args.variant)
+    if args.variant < 0 and args.tune:   # plan set-up, before the timed region
+        tuned = plan.tune(reps=3)
+        log(f"autotuned gather variant: {tuned}")
     info = plan.info()
     P = info["pairs_used"]
     updates = P * N[0] * N[1] * N[2]
@@ -574,7 +580,9 @@ def main():
                                f"T={args.T} bins of 1 mm (chunk {info['Tc']}), single-tap GPU path",
                    "pairs": P, "voxels": N[0] * N[1] * N[2], "chunk_bins": info["Tc"],
                    "storage": args.storage, "row_storage": info["row_storage"], "kernel_path": info["path"],
-                   "lane_voxels": info["lane_voxels"]},
+                   "lane_voxels": info["lane_voxels"],
+                   "kernel_choice": "autotuned (bpx_plan_tune)" if args.variant < 0 and args.tune
+                   else f"variant {args.variant}"},
         "roofline": roof,
         "cpu_baseline": cpu,
         "check": check,

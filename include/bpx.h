@@ -392,6 +392,13 @@ typedef struct bpx_timing {
 /* Runs `warmup` untimed then `iterations` timed executions on the plan's stream, timing
  * each kernel with HIP events recorded on that stream. */
 bpx_status bpx_plan_benchmark(bpx_plan plan, int32_t warmup, int32_t iterations, bpx_timing* out);
+/* Autotuner of a confocal plan with fused rows (no reference counterpart; it replaces the
+ * occupancy-only launch policy of backproject_cuda.cu:406): executes the plan once plus `reps`
+ * times in every gather variant 3-8 that applies to it (the texture-path shapes of 4, 8 and 2
+ * voxels per lane; the LDS-staged tile shapes whose bin windows fit) on its own stream and keeps
+ * the fastest; every variant gives the same volume bit for bit.  *variant (may be NULL)
+ * receives the chosen one; other plans keep theirs. */
+bpx_status bpx_plan_tune(bpx_plan plan, int32_t reps, int32_t* variant);
 void bpx_plan_destroy(bpx_plan plan);
 
 /* Copies `bytes` (rounded down to 16) between two device buffers with a plain streaming copy

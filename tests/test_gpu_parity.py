@@ -347,6 +347,41 @@ def test_round_half_away_ties_on_gpu(gpu, oracle):
         assert vol[0, 0, 0] == ref[0, 0, 0] == want, k
 
 
+def test_plan_tune_keeps_the_volume(gpu, oracle):
+    """bpx_plan_tune times the gather variants that apply to a confocal plan (the texture-path
+    shapes 3-5, the LDS-staged shapes 6-8 whose bin windows fit -- here all of them, the 8-voxel
+    ones with exact fp16 rows) and keeps the fastest: its kernel then runs, and the volume equals
+    the default kernel's and the oracle's bit for bit.  A plan of an exhaustive grid keeps its
+    variant."""
+    pr = confocal_pairs(16, jitter=0.01, seed=7)
+    T, N, t0 = 3000, (35, 33, 40), 1.2
+    H = coded_samples(len(pr), T, seed=3)
+    zero, inc = grid_basis(N, pos=(0.05, -0.02, 0.7), size=(0.25, 0.22, 0.3))
+    ref = oracle.a1_volume_f64acc(H, pr, N, zero, inc, t0, 0.001).astype(f32)
+    with gpu.Plan(H, pr, N, zero, inc, t0, 0.001) as plan:
+        plan.execute()
+        before = plan.volume()
+        got = plan.tune(reps=2)
+        assert 3 <= got["variant"] <= 8, got
+        assert got["kernel"].startswith("confocal_lds_kernel") == (got["variant"] >= 6), got
+        plan.execute()
+        after = plan.volume()
+        assert plan.info()["kernel"] == got["kernel"]
+    assert np.array_equal(before, ref) and np.array_equal(after, ref)
+    lg = scenes.wall_grid(8)
+    cg = (scenes.wall_grid(6) * f32(0.9)).astype(f32)
+    prg = oracle.precompute_pairs(scenes.ORIGIN, lg, scenes.ORIGIN, cg, confocal=False)
+    Hg = coded_samples(len(prg), T, seed=2)
+    with gpu.Plan(Hg, prg, (16, 12, 20), *grid_basis((16, 12, 20)), 2.2, 0.001) as plan:
+        k0 = plan.info()["kernel"]
+        assert plan.info()["path"] == "grid"
+        plan.tune(reps=1)
+        assert plan.info()["kernel"] == k0
+        plan.execute()
+        assert np.array_equal(plan.volume(), oracle.a1_volume_f64acc(Hg, prg, (16, 12, 20), *grid_basis((16, 12, 20)),
+                                                                      2.2, 0.001).astype(f32))
+
+
 def test_exhaustive_grid_bins_bit_exact(gpu, oracle):
     lg = scenes.wall_grid(8)
     cg = (scenes.wall_grid(6) * f32(0.9)).astype(f32)
